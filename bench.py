@@ -1,0 +1,140 @@
+"""Benchmark of the FDTD time loop on one MI355X.
+
+Workload (default): the layered orthorhombic 384x384x192 FP32 box with
+24-cell separable damping on all faces, a lune-uniform moment tensor,
+Ricker 3 Hz and 64 three-component receivers; one step = one stress pass +
+one velocity pass over all 28.3M cells.  Inputs are resident in HBM before
+the timed region.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps 3000] [--warmup 50] [--config 1]
+                  [--cpu-steps 20] [--no-cpu]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def _traffic(kernel, workload):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration), from profiles/."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(p) as f:
+            d = json.load(f)
+        return d[workload][kernel]["bytes_per_launch"]
+    except Exception:
+        return None
+
+
+def cpu_baseline(wl_builder, steps, threads):
+    """The FP64 oracle (CPU restatement of the reference step, OpenMP over z
+    planes) on a bounded number of steps of the same workload."""
+    from elanfmt_amd import workloads as W
+    from oracle import Oracle
+
+    wl = wl_builder()
+    o = Oracle(wl.nx, wl.ny, wl.nz, wl.dx, wl.dy, wl.dz, wl.dt, wl.nt, mode=wl.mode, pml=wl.pml,
+               free_surface=wl.free_surface)
+    W.apply(o, wl)
+    o.run(2, threads)  # first touch
+    t0 = time.perf_counter()
+    o.run(steps, threads)
+    el = time.perf_counter() - t0
+    o.close()
+    return wl.cells * steps / el / 1e9, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(a z-slab decomposition is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from elanfmt_amd import fwe, workloads as W
+
+    if fwe.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    steps = args.steps
+    build = lambda: W.config(args.config, nt=max(steps, 1))
+    wl = build()
+    s = W.make_solver(wl)
+    s.run(args.warmup)
+    s.synchronize()
+    s.reset()
+    s.synchronize()
+    t0 = time.perf_counter()
+    ms_s, ms_v, ms_w = s.run_timed(steps)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    tr = s.traces()
+    finite = bool(__import__("numpy").isfinite(tr).all())
+    s.close()
+
+    cells = wl.cells
+    bs, bv = W.alg_bytes(wl.mode, wl.pml, wl.nbatch)
+    gcells = cells * wl.nbatch * steps / el / 1e9
+    dom, ms_dom, b_dom = ("stress", ms_s, bs) if ms_s >= ms_v else ("velocity", ms_v, bv)
+    per_launch_s = ms_dom / steps * 1e-3
+    ach = b_dom * cells / per_launch_s / 1e9
+    step_gbs = (bs + bv) * cells * steps / el / 1e9
+    wname = f"{wl.name}"
+    traffic = _traffic(dom, wname)
+    out = {
+        "metric": "Gcell-updates/s + % of 8 TB/s HBM; batched-6-MT vs 6 plain runs at equal error",
+        "value": round(gcells, 3),
+        "unit": "Gcell/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{wname}: layered orthorhombic FP32, {wl.info.get('pml_cells')}-cell separable damping, "
+                        f"lune MT, Ricker {wl.info.get('f0')} Hz, {len(wl.receivers)} receiver traces"
+            if wl.mode == fwe.MODE_ORTHO else f"{wname}: isotropic homogeneous FP32",
+            "grid": [wl.nx, wl.ny, wl.nz], "wavefields": wl.nbatch, "nt": wl.nt,
+            "parallelism": "single GPU",
+        },
+        "hbm_pct_step": round(100 * step_gbs / PEAK_HBM_GBS, 2),
+        "kernels": {
+            "stress": {"ms_per_launch": round(ms_s / steps, 5), "alg_bytes_per_cell": bs,
+                       "GBs": round(bs * cells / (ms_s / steps * 1e-3) / 1e9, 1)},
+            "velocity": {"ms_per_launch": round(ms_v / steps, 5), "alg_bytes_per_cell": bv,
+                         "GBs": round(bv * cells / (ms_v / steps * 1e-3) / 1e9, 1)},
+        },
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": traffic},
+        "traces_finite": finite,
+    }
+    if not args.no_cpu and args.cpu_steps > 0:
+        threads = min(16, os.cpu_count() or 1)
+        v, el_cpu = cpu_baseline(lambda: W.config(args.config, nt=args.cpu_steps + 2), args.cpu_steps, threads)
+        out["cpu_baseline"] = {"value": round(v, 5), "unit": "Gcell/s", "cores": threads, "kind": "port",
+                               "sample": f"FP64 oracle, {args.cpu_steps} steps of {wname} "
+                                         f"({el_cpu:.1f} s), OpenMP over z planes"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

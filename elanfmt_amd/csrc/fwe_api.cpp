@@ -1,0 +1,790 @@
+// fwe_api.cpp -- host implementation of the C ABI in include/fwe.h.
+//
+// Replaces the host half of the reference solver's main()
+// (OK4.cu:390-559, ANISO.cu:392-608): device set-up, input loading, the
+// time loop, timing and output writing.  The reference's file formats are
+// kept byte for byte; the device layout is this library's own (one padded
+// x-fastest layout shared by all staggered arrays).
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/fwe.h"
+#include "fwe_internal.h"
+
+using fwe::PointEntry;
+using fwe::StepArgs;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local char g_err[512];
+
+static fwe_status fail(fwe_status st, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return st;
+}
+
+#define HIPCHK(expr)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(FWE_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                             \
+  } while (0)
+
+extern "C" const char* fwe_last_error(void) { return g_err; }
+
+extern "C" const char* fwe_version(void) {
+  static char v[64];
+  snprintf(v, sizeof(v), "fwe %d gfx950", FWE_API_VERSION);
+  return v;
+}
+
+extern "C" fwe_status fwe_device_count(int32_t* count) {
+  if (!count) return fail(FWE_ERR_ARG, "count is NULL");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return FWE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct fwe_ctx {
+  fwe_config cfg;
+  int64_t sy = 0, sz = 0, nelem = 0;
+  float* fields = nullptr;  // nb * 9 * nelem
+  float* medium = nullptr;  // 12 * nelem (ortho) or 2 * nelem (iso)
+  float* w3 = nullptr;      // nelem (W3D)
+  float* prof = nullptr;    // wx wy wz whx why whz (profile / taper tables)
+  float* stf = nullptr;     // nb * nstf
+  float* traces = nullptr;  // nrec * nt
+  int32_t* inj_off = nullptr;
+  PointEntry* inj = nullptr;
+  int32_t* rec_off = nullptr;
+  PointEntry* rec = nullptr;
+  int32_t nstf = 0, ninj = 0, nrec = 0;
+  std::vector<float> stf_host;  // batch 0, for the legacy Src output
+  bool have_medium = false, have_damp = false;
+  float c66dt = 0, bxdt = 0, bydt = 0, bzdt = 0;
+  int ty = 4, kchunk = 8, ntx = 0, nty = 0, nkc = 0;
+  int it = 0;
+  hipStream_t stream = nullptr;
+  std::vector<hipEvent_t> events;
+};
+
+static int64_t round_up(int64_t a, int64_t m) { return (a + m - 1) / m * m; }
+
+static void field_shape(const fwe_config& c, int f, int64_t d[3]) {
+  const int64_t nx = c.nx, ny = c.ny, nz = c.nz;
+  switch (f) {
+    case FWE_F_VX: d[0] = nx + 1; d[1] = ny; d[2] = nz; break;
+    case FWE_F_VY: d[0] = nx; d[1] = ny + 1; d[2] = nz; break;
+    case FWE_F_VZ: d[0] = nx; d[1] = ny; d[2] = nz + 1; break;
+    case FWE_F_SXY: d[0] = nx + 1; d[1] = ny + 1; d[2] = nz; break;
+    case FWE_F_SXZ: d[0] = nx + 1; d[1] = ny; d[2] = nz + 1; break;
+    case FWE_F_SYZ: d[0] = nx; d[1] = ny + 1; d[2] = nz + 1; break;
+    default: d[0] = nx; d[1] = ny; d[2] = nz; break;
+  }
+}
+
+// reference shape (ex,ey,ez) -> padded layout, scaled
+static void pack(const fwe_ctx* x, const float* src, const int64_t d[3], double scale,
+                 std::vector<float>& dst) {
+  dst.assign((size_t)x->nelem, 0.f);
+  for (int64_t k = 0; k < d[2]; ++k)
+    for (int64_t j = 0; j < d[1]; ++j) {
+      const float* s = src + j * d[0] + k * d[0] * d[1];
+      float* o = dst.data() + j * x->sy + k * x->sz;
+      if (scale == 1.0)
+        memcpy(o, s, sizeof(float) * (size_t)d[0]);
+      else
+        for (int64_t i = 0; i < d[0]; ++i) o[i] = (float)((double)s[i] * scale);
+    }
+}
+
+static void unpack(const fwe_ctx* x, const std::vector<float>& src, const int64_t d[3], float* dst) {
+  for (int64_t k = 0; k < d[2]; ++k)
+    for (int64_t j = 0; j < d[1]; ++j)
+      memcpy(dst + j * d[0] + k * d[0] * d[1], src.data() + j * x->sy + k * x->sz,
+             sizeof(float) * (size_t)d[0]);
+}
+
+static void choose_tiling(fwe_ctx* x) {
+  const fwe_config& c = x->cfg;
+  const char* ety = getenv("FWE_TILE_Y");
+  const char* ekc = getenv("FWE_KCHUNK");
+  int ty = ety ? atoi(ety) : 4;
+  if (ty != 2 && ty != 4 && ty != 8) ty = 4;
+  x->ty = ty;
+  x->ntx = (c.nx + fwe::TILE_X - 1) / fwe::TILE_X;
+  x->nty = (c.ny + ty - 1) / ty;
+  int kc;
+  if (ekc) {
+    kc = atoi(ekc);
+  } else {
+    // enough workgroups to fill 256 CUs several times over
+    const int64_t cols = (int64_t)x->ntx * x->nty;
+    int64_t nkc = (4096 + cols - 1) / cols;
+    if (nkc < 1) nkc = 1;
+    kc = (int)((c.nz + nkc - 1) / nkc);
+    if (kc < 8) kc = 8;
+  }
+  if (kc < 3) kc = 3;  // BC1 needs planes 1 and 2 in one chunk
+  if (kc > c.nz) kc = c.nz;
+  x->kchunk = kc;
+  x->nkc = (c.nz + kc - 1) / kc;
+}
+
+extern "C" fwe_status fwe_config_default(fwe_config* cfg, int32_t mode) {
+  if (!cfg) return fail(FWE_ERR_ARG, "cfg is NULL");
+  if (mode != FWE_MODE_ISO && mode != FWE_MODE_ORTHO) return fail(FWE_ERR_ARG, "bad mode %d", mode);
+  memset(cfg, 0, sizeof(*cfg));
+  cfg->mode = mode;
+  cfg->pml = mode == FWE_MODE_ISO ? FWE_PML_PROFILE : FWE_PML_WEIGHTS3D;
+  cfg->free_surface = 1;
+  cfg->src_after_bc = mode == FWE_MODE_ISO ? 0 : 1;
+  cfg->nbatch = 1;
+  return FWE_OK;
+}
+
+static fwe_status check_cfg(const fwe_config* c) {
+  if (c->nx < 3 || c->ny < 3 || c->nz < 3) return fail(FWE_ERR_ARG, "grid %dx%dx%d too small", c->nx, c->ny, c->nz);
+  if ((int64_t)c->nx * c->ny * c->nz > (int64_t)1 << 33) return fail(FWE_ERR_ARG, "grid too large");
+  if (c->nt < 0) return fail(FWE_ERR_ARG, "nt < 0");
+  if (c->mode != FWE_MODE_ISO && c->mode != FWE_MODE_ORTHO) return fail(FWE_ERR_ARG, "bad mode");
+  if (c->nbatch < 1 || c->nbatch > FWE_MAX_BATCH) return fail(FWE_ERR_ARG, "nbatch %d out of 1..%d", c->nbatch, FWE_MAX_BATCH);
+  if (!(c->dx > 0 && c->dy > 0 && c->dz > 0 && c->dt > 0)) return fail(FWE_ERR_ARG, "dx,dy,dz,dt must be > 0");
+  if (c->mode == FWE_MODE_ISO && c->pml != FWE_PML_NONE && c->pml != FWE_PML_PROFILE)
+    return fail(FWE_ERR_UNSUPPORTED, "iso law takes PROFILE or NONE damping");
+  if (c->mode == FWE_MODE_ORTHO && c->pml == FWE_PML_PROFILE)
+    return fail(FWE_ERR_UNSUPPORTED, "ortho law takes WEIGHTS3D, SEPARABLE or NONE damping");
+  if (c->pml < 0 || c->pml > 3) return fail(FWE_ERR_ARG, "bad pml");
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_create(const fwe_config* cfg, fwe_ctx** out) {
+  if (!cfg || !out) return fail(FWE_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  fwe_status st = check_cfg(cfg);
+  if (st != FWE_OK) return st;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(FWE_ERR_HIP, "no HIP device available (libfwe has no CPU path)");
+  if (cfg->device < 0 || cfg->device >= ndev) return fail(FWE_ERR_ARG, "device %d of %d", cfg->device, ndev);
+  HIPCHK(hipSetDevice(cfg->device));
+  fwe_ctx* x = new (std::nothrow) fwe_ctx();
+  if (!x) return fail(FWE_ERR_NOMEM, "host allocation");
+  x->cfg = *cfg;
+  x->sy = round_up(cfg->nx + 1, 32);
+  x->sz = x->sy * (cfg->ny + 1);
+  x->nelem = x->sz * (cfg->nz + 1);
+  choose_tiling(x);
+  const size_t fbytes = sizeof(float) * (size_t)x->nelem * 9 * cfg->nbatch;
+  const int nmed = cfg->mode == FWE_MODE_ORTHO ? 12 : 2;
+  hipError_t e = hipMalloc(&x->fields, fbytes);
+  if (e == hipSuccess) e = hipMalloc(&x->medium, sizeof(float) * (size_t)x->nelem * nmed);
+  if (e == hipSuccess && cfg->pml == FWE_PML_WEIGHTS3D) e = hipMalloc(&x->w3, sizeof(float) * (size_t)x->nelem);
+  if (e == hipSuccess) {
+    const size_t np = 2 * ((size_t)cfg->nx + cfg->ny + cfg->nz + 3);
+    e = hipMalloc(&x->prof, sizeof(float) * np);
+  }
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMemsetAsync(x->fields, 0, fbytes, x->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(x->medium, 0, sizeof(float) * (size_t)x->nelem * nmed, x->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
+  if (e != hipSuccess) {
+    fwe_destroy(x);
+    return fail(e == hipErrorOutOfMemory ? FWE_ERR_NOMEM : FWE_ERR_HIP, "create: %s", hipGetErrorString(e));
+  }
+  // default: no injections, no receivers (empty tile lists)
+  fwe_set_injections(x, 0, nullptr);
+  fwe_set_receivers(x, 0, nullptr);
+  *out = x;
+  g_err[0] = 0;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_destroy(fwe_ctx* x) {
+  if (!x) return FWE_OK;
+  hipSetDevice(x->cfg.device);
+  if (x->stream) hipStreamSynchronize(x->stream);
+  for (auto ev : x->events) hipEventDestroy(ev);
+  hipFree(x->fields); hipFree(x->medium); hipFree(x->w3); hipFree(x->prof);
+  hipFree(x->stf); hipFree(x->traces); hipFree(x->inj_off); hipFree(x->inj);
+  hipFree(x->rec_off); hipFree(x->rec);
+  if (x->stream) hipStreamDestroy(x->stream);
+  delete x;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_get_config(const fwe_ctx* x, fwe_config* out) {
+  if (!x || !out) return fail(FWE_ERR_ARG, "NULL argument");
+  *out = x->cfg;
+  return FWE_OK;
+}
+
+static fwe_status upload(fwe_ctx* x, float* dst, const std::vector<float>& h) {
+  HIPCHK(hipSetDevice(x->cfg.device));
+  HIPCHK(hipMemcpyAsync(dst, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return FWE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// medium and damping
+// ---------------------------------------------------------------------------
+extern "C" fwe_status fwe_set_material_iso(fwe_ctx* x, const float* c11, const float* c12, float c66,
+                                           float bx, float by, float bz) {
+  if (!x || !c11 || !c12) return fail(FWE_ERR_ARG, "NULL argument");
+  if (x->cfg.mode != FWE_MODE_ISO) return fail(FWE_ERR_STATE, "context is not iso");
+  const double dt = x->cfg.dt;
+  int64_t d[3];
+  field_shape(x->cfg, FWE_F_SXX, d);
+  std::vector<float> h;
+  pack(x, c11, d, dt, h);
+  fwe_status st = upload(x, x->medium, h);
+  if (st) return st;
+  pack(x, c12, d, dt, h);
+  st = upload(x, x->medium + x->nelem, h);
+  if (st) return st;
+  x->c66dt = (float)((double)c66 * dt);
+  x->bxdt = (float)((double)bx * dt);
+  x->bydt = (float)((double)by * dt);
+  x->bzdt = (float)((double)bz * dt);
+  x->have_medium = true;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_set_material_ortho(fwe_ctx* x, const float* const cij[9],
+                                             const float* const buoy[3]) {
+  if (!x || !cij || !buoy) return fail(FWE_ERR_ARG, "NULL argument");
+  if (x->cfg.mode != FWE_MODE_ORTHO) return fail(FWE_ERR_STATE, "context is not ortho");
+  for (int c = 0; c < 9; ++c) if (!cij[c]) return fail(FWE_ERR_ARG, "cij[%d] NULL", c);
+  for (int c = 0; c < 3; ++c) if (!buoy[c]) return fail(FWE_ERR_ARG, "buoy[%d] NULL", c);
+  const double dt = x->cfg.dt;
+  static const int shape_of[12] = {FWE_F_SXX, FWE_F_SXX, FWE_F_SXX, FWE_F_SXX, FWE_F_SXX, FWE_F_SXX,
+                                   FWE_F_SYZ, FWE_F_SXZ, FWE_F_SXY, FWE_F_VX,  FWE_F_VY,  FWE_F_VZ};
+  std::vector<float> h;
+  for (int c = 0; c < 12; ++c) {
+    int64_t d[3];
+    field_shape(x->cfg, shape_of[c], d);
+    pack(x, c < 9 ? cij[c] : buoy[c - 9], d, dt, h);
+    fwe_status st = upload(x, x->medium + (int64_t)c * x->nelem, h);
+    if (st) return st;
+  }
+  x->have_medium = true;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_set_pml_profiles(fwe_ctx* x, const float* wx, const float* wy, const float* wz) {
+  if (!x || !wx || !wy || !wz) return fail(FWE_ERR_ARG, "NULL argument");
+  const fwe_config& c = x->cfg;
+  if (c.pml != FWE_PML_PROFILE && c.pml != FWE_PML_SEPARABLE)
+    return fail(FWE_ERR_STATE, "context damping is not PROFILE or SEPARABLE");
+  const int add = c.pml == FWE_PML_PROFILE ? 1 : 0;
+  const int n[3] = {c.nx, c.ny, c.nz};
+  const float* w[3] = {wx, wy, wz};
+  // device table: [wx (nx+1)][wy (ny+1)][wz (nz+1)][whx (nx+1)][why][whz]
+  std::vector<float> h(2 * ((size_t)c.nx + c.ny + c.nz + 3), 1.f);
+  size_t o = 0;
+  for (int a = 0; a < 3; ++a) {
+    for (int q = 0; q < n[a] + add; ++q) h[o + q] = w[a][q];
+    o += n[a] + 1;
+  }
+  for (int a = 0; a < 3; ++a) {
+    if (add)  // OK4.cu:307-309: (DAT)0.5*(w[i+1]+w[i])
+      for (int q = 0; q < n[a]; ++q) h[o + q] = 0.5f * (w[a][q + 1] + w[a][q]);
+    o += n[a] + 1;
+  }
+  HIPCHK(hipSetDevice(c.device));
+  HIPCHK(hipMemcpy(x->prof, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+  x->have_damp = true;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_set_pml_weights(fwe_ctx* x, const float* w) {
+  if (!x || !w) return fail(FWE_ERR_ARG, "NULL argument");
+  if (x->cfg.pml != FWE_PML_WEIGHTS3D) return fail(FWE_ERR_STATE, "context damping is not WEIGHTS3D");
+  int64_t d[3];
+  field_shape(x->cfg, FWE_F_SXX, d);
+  std::vector<float> h;
+  pack(x, w, d, 1.0, h);
+  fwe_status st = upload(x, x->w3, h);
+  if (st) return st;
+  x->have_damp = true;
+  return FWE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// source and receivers
+// ---------------------------------------------------------------------------
+extern "C" fwe_status fwe_set_stf(fwe_ctx* x, int32_t batch, const float* stf, int32_t n) {
+  if (!x || (!stf && n > 0) || n < 0) return fail(FWE_ERR_ARG, "bad stf arguments");
+  if (batch < 0 || batch >= x->cfg.nbatch) return fail(FWE_ERR_ARG, "batch %d out of range", batch);
+  HIPCHK(hipSetDevice(x->cfg.device));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  if (n > x->nstf) {
+    // grow, keeping the other batches' functions
+    float* nb = nullptr;
+    HIPCHK(hipMalloc(&nb, sizeof(float) * (size_t)n * x->cfg.nbatch));
+    HIPCHK(hipMemset(nb, 0, sizeof(float) * (size_t)n * x->cfg.nbatch));
+    for (int b = 0; b < x->cfg.nbatch && x->stf; ++b)
+      HIPCHK(hipMemcpy(nb + (size_t)b * n, x->stf + (size_t)b * x->nstf, sizeof(float) * x->nstf,
+                       hipMemcpyDeviceToDevice));
+    hipFree(x->stf);
+    x->stf = nb;
+    x->nstf = n;
+  }
+  std::vector<float> h((size_t)x->nstf, 0.f);
+  if (n) memcpy(h.data(), stf, sizeof(float) * n);
+  HIPCHK(hipMemcpy(x->stf + (size_t)batch * x->nstf, h.data(), sizeof(float) * h.size(),
+                   hipMemcpyHostToDevice));
+  if (batch == 0) x->stf_host.assign(stf, stf + n);
+  return FWE_OK;
+}
+
+// sort point entries by xy tile and upload with their offsets
+static fwe_status upload_points(fwe_ctx* x, std::vector<PointEntry>& v, int32_t** off, PointEntry** dev) {
+  const int ntile = x->ntx * x->nty;
+  std::vector<int32_t> counts(ntile + 1, 0);
+  auto tile_of = [&](const PointEntry& p) { return p.i / fwe::TILE_X + (p.j / x->ty) * x->ntx; };
+  std::stable_sort(v.begin(), v.end(), [&](const PointEntry& a, const PointEntry& b) {
+    return tile_of(a) < tile_of(b);
+  });
+  for (auto& p : v) counts[tile_of(p) + 1]++;
+  for (int t = 0; t < ntile; ++t) counts[t + 1] += counts[t];
+  HIPCHK(hipSetDevice(x->cfg.device));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  hipFree(*off);
+  hipFree(*dev);
+  *off = nullptr;
+  *dev = nullptr;
+  HIPCHK(hipMalloc(off, sizeof(int32_t) * counts.size()));
+  HIPCHK(hipMemcpy(*off, counts.data(), sizeof(int32_t) * counts.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(dev, sizeof(PointEntry) * std::max<size_t>(1, v.size())));
+  if (!v.empty())
+    HIPCHK(hipMemcpy(*dev, v.data(), sizeof(PointEntry) * v.size(), hipMemcpyHostToDevice));
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_set_injections(fwe_ctx* x, int32_t n, const fwe_injection* inj) {
+  if (!x || n < 0 || (n > 0 && !inj)) return fail(FWE_ERR_ARG, "bad injection arguments");
+  const fwe_config& c = x->cfg;
+  std::vector<PointEntry> v;
+  for (int q = 0; q < n; ++q) {
+    const fwe_injection& s = inj[q];
+    if (s.comp < FWE_SXX || s.comp > FWE_SYZ) return fail(FWE_ERR_ARG, "injection %d: bad component %d", q, s.comp);
+    if (s.batch < 0 || s.batch >= c.nbatch) return fail(FWE_ERR_ARG, "injection %d: bad batch", q);
+    // the stress kernel covers cells i<nx, j<ny, k<nz of every component
+    if (s.ix < 0 || s.iy < 0 || s.iz < 0 || s.ix >= c.nx || s.iy >= c.ny || s.iz >= c.nz)
+      return fail(FWE_ERR_ARG, "injection %d at (%d,%d,%d) outside %dx%dx%d", q, s.ix, s.iy, s.iz, c.nx, c.ny, c.nz);
+    v.push_back(PointEntry{s.ix, s.iy, s.iz, s.comp, s.batch, s.coef, -1, 0});
+  }
+  fwe_status st = upload_points(x, v, &x->inj_off, &x->inj);
+  if (st) return st;
+  x->ninj = n;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_set_receivers(fwe_ctx* x, int32_t n, const fwe_receiver* rec) {
+  if (!x || n < 0 || (n > 0 && !rec)) return fail(FWE_ERR_ARG, "bad receiver arguments");
+  const fwe_config& c = x->cfg;
+  std::vector<PointEntry> v;
+  for (int r = 0; r < n; ++r) {
+    const fwe_receiver& s = rec[r];
+    if (s.comp < 0 || s.comp > 2) return fail(FWE_ERR_ARG, "receiver %d: bad component", r);
+    if (s.batch < 0 || s.batch >= c.nbatch) return fail(FWE_ERR_ARG, "receiver %d: bad batch", r);
+    if (s.ix < 0 || s.iy < 0 || s.iz < 0 || s.ix >= c.nx || s.iy >= c.ny || s.iz >= c.nz)
+      return fail(FWE_ERR_ARG, "receiver %d at (%d,%d,%d) outside %dx%dx%d", r, s.ix, s.iy, s.iz, c.nx, c.ny, c.nz);
+    v.push_back(PointEntry{s.ix, s.iy, s.iz, s.comp, s.batch, 0.f, r, 0});
+  }
+  fwe_status st = upload_points(x, v, &x->rec_off, &x->rec);
+  if (st) return st;
+  hipFree(x->traces);
+  x->traces = nullptr;
+  x->nrec = n;
+  const size_t tb = sizeof(float) * std::max<size_t>(1, (size_t)n * std::max(1, c.nt));
+  HIPCHK(hipMalloc(&x->traces, tb));
+  HIPCHK(hipMemset(x->traces, 0, tb));
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_mt_injections(int32_t layout, int32_t xs, int32_t ys, int32_t zs,
+                                        const float m[6], int32_t batch, fwe_injection out[6]) {
+  if (!m || !out) return fail(FWE_ERR_ARG, "NULL argument");
+  // offsets of Sxx Syy Szz Sxy Sxz Syz relative to (xs,ys,zs)
+  static const int ok4[6][3] = {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}, {3, 3, 1}, {3, 3, 1}, {2, 2, 1}};
+  static const int an[6][3] = {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}, {2, 2, 1}, {-1, 1, 1}, {3, 2, 1}};
+  if (layout != FWE_SRC_LAYOUT_OK4 && layout != FWE_SRC_LAYOUT_ANISO) return fail(FWE_ERR_ARG, "bad layout");
+  const int(*o)[3] = layout == FWE_SRC_LAYOUT_OK4 ? ok4 : an;
+  for (int c = 0; c < 6; ++c) {
+    out[c].comp = c;
+    out[c].ix = xs + o[c][0];
+    out[c].iy = ys + o[c][1];
+    out[c].iz = zs + o[c][2];
+    out[c].batch = batch;
+    out[c].coef = m[c];
+  }
+  return FWE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// time loop
+// ---------------------------------------------------------------------------
+static StepArgs make_args(fwe_ctx* x) {
+  const fwe_config& c = x->cfg;
+  StepArgs a;
+  memset(&a, 0, sizeof(a));
+  a.nx = c.nx; a.ny = c.ny; a.nz = c.nz; a.nb = c.nbatch;
+  a.sy = x->sy; a.sz = x->sz; a.fstride = 9 * x->nelem;
+  for (int q = 0; q < 3; ++q) a.V[q] = x->fields + (int64_t)q * x->nelem;
+  for (int q = 0; q < 6; ++q) a.S[q] = x->fields + (int64_t)(3 + q) * x->nelem;
+  if (c.mode == FWE_MODE_ORTHO) {
+    for (int q = 0; q < 9; ++q) a.C[q] = x->medium + (int64_t)q * x->nelem;
+    for (int q = 0; q < 3; ++q) a.B[q] = x->medium + (int64_t)(9 + q) * x->nelem;
+  } else {
+    a.C[0] = x->medium;
+    a.C[1] = x->medium + x->nelem;
+  }
+  a.c66dt = x->c66dt; a.bxdt = x->bxdt; a.bydt = x->bydt; a.bzdt = x->bzdt;
+  const size_t ox = 0, oy = c.nx + 1, oz = oy + c.ny + 1, oh = oz + c.nz + 1;
+  a.wx = x->prof + ox; a.wy = x->prof + oy; a.wz = x->prof + oz;
+  a.whx = x->prof + oh + ox; a.why = x->prof + oh + oy; a.whz = x->prof + oh + oz;
+  a.w3 = x->w3;
+  a.rdx = (float)(1.0 / c.dx); a.rdy = (float)(1.0 / c.dy); a.rdz = (float)(1.0 / c.dz);
+  a.dtdx = (float)(c.dt / c.dx); a.dtdy = (float)(c.dt / c.dy); a.dtdz = (float)(c.dt / c.dz);
+  a.ty_rows = x->ty; a.ntx = x->ntx; a.nty = x->nty; a.nkc = x->nkc; a.kchunk = x->kchunk;
+  a.inj_off = x->inj_off; a.inj = x->inj; a.stf = x->stf; a.nstf = x->nstf;
+  a.src_after_bc = c.src_after_bc;
+  a.rec_off = x->rec_off; a.rec = x->rec; a.traces = x->traces; a.nt = c.nt;
+  return a;
+}
+
+static fwe_status ready(fwe_ctx* x) {
+  if (!x->have_medium) return fail(FWE_ERR_STATE, "medium not set");
+  if (x->cfg.pml != FWE_PML_NONE && !x->have_damp) return fail(FWE_ERR_STATE, "damping not set");
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_reset(fwe_ctx* x) {
+  if (!x) return fail(FWE_ERR_ARG, "NULL ctx");
+  HIPCHK(hipSetDevice(x->cfg.device));
+  HIPCHK(hipMemsetAsync(x->fields, 0, sizeof(float) * (size_t)x->nelem * 9 * x->cfg.nbatch, x->stream));
+  if (x->traces)
+    HIPCHK(hipMemsetAsync(x->traces, 0, sizeof(float) * std::max<size_t>(1, (size_t)x->nrec * std::max(1, x->cfg.nt)), x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  x->it = 0;
+  return FWE_OK;
+}
+
+static fwe_status run_impl(fwe_ctx* x, int32_t nsteps, double* ms_s, double* ms_v, double* ms_w) {
+  if (!x || nsteps < 0) return fail(FWE_ERR_ARG, "bad run arguments");
+  fwe_status st = ready(x);
+  if (st) return st;
+  HIPCHK(hipSetDevice(x->cfg.device));
+  const bool timed = ms_s || ms_v || ms_w;
+  if (timed && (int)x->events.size() < 2 * nsteps + 1) {
+    while ((int)x->events.size() < 2 * nsteps + 1) {
+      hipEvent_t ev;
+      HIPCHK(hipEventCreate(&ev));
+      x->events.push_back(ev);
+    }
+  }
+  StepArgs a = make_args(x);
+  const fwe_config& c = x->cfg;
+  if (timed) HIPCHK(hipEventRecord(x->events[0], x->stream));
+  for (int s = 0; s < nsteps; ++s) {
+    a.it = x->it;
+    HIPCHK(fwe::launch_stress(a, c.mode, c.pml, c.free_surface, x->stream));
+    if (timed) HIPCHK(hipEventRecord(x->events[2 * s + 1], x->stream));
+    HIPCHK(fwe::launch_velocity(a, c.mode, c.pml, x->stream));
+    if (timed) HIPCHK(hipEventRecord(x->events[2 * s + 2], x->stream));
+    x->it++;
+  }
+  if (timed) {
+    HIPCHK(hipEventSynchronize(x->events[2 * nsteps]));
+    double ts = 0, tv = 0;
+    for (int s = 0; s < nsteps; ++s) {
+      float a1 = 0, a2 = 0;
+      HIPCHK(hipEventElapsedTime(&a1, x->events[2 * s], x->events[2 * s + 1]));
+      HIPCHK(hipEventElapsedTime(&a2, x->events[2 * s + 1], x->events[2 * s + 2]));
+      ts += a1;
+      tv += a2;
+    }
+    float wall = 0;
+    HIPCHK(hipEventElapsedTime(&wall, x->events[0], x->events[2 * nsteps]));
+    if (ms_s) *ms_s = ts;
+    if (ms_v) *ms_v = tv;
+    if (ms_w) *ms_w = wall;
+  }
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_run(fwe_ctx* x, int32_t nsteps) { return run_impl(x, nsteps, nullptr, nullptr, nullptr); }
+
+extern "C" fwe_status fwe_run_timed(fwe_ctx* x, int32_t nsteps, double* ms_stress, double* ms_velocity,
+                                    double* ms_wall) {
+  double a = 0, b = 0, w = 0;
+  fwe_status st = run_impl(x, nsteps, &a, &b, &w);
+  if (st) return st;
+  if (ms_stress) *ms_stress = a;
+  if (ms_velocity) *ms_velocity = b;
+  if (ms_wall) *ms_wall = w;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_synchronize(fwe_ctx* x) {
+  if (!x) return fail(FWE_ERR_ARG, "NULL ctx");
+  HIPCHK(hipSetDevice(x->cfg.device));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_get_step(const fwe_ctx* x, int32_t* it) {
+  if (!x || !it) return fail(FWE_ERR_ARG, "NULL argument");
+  *it = x->it;
+  return FWE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// results
+// ---------------------------------------------------------------------------
+extern "C" fwe_status fwe_get_traces(fwe_ctx* x, float* out) {
+  if (!x || (!out && x->nrec > 0)) return fail(FWE_ERR_ARG, "NULL argument");
+  if (x->nrec == 0 || x->cfg.nt == 0) return FWE_OK;
+  HIPCHK(hipSetDevice(x->cfg.device));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  const size_t n = (size_t)x->nrec * x->cfg.nt;
+  HIPCHK(hipMemcpy(out, x->traces, sizeof(float) * n, hipMemcpyDeviceToHost));
+  for (size_t q = 0; q < n; ++q)
+    if (!isfinite(out[q])) return fail(FWE_ERR_NONFINITE, "trace sample %zu is not finite", q);
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_get_field(fwe_ctx* x, int32_t batch, int32_t field, float* out) {
+  if (!x || !out) return fail(FWE_ERR_ARG, "NULL argument");
+  if (batch < 0 || batch >= x->cfg.nbatch || field < 0 || field > 8) return fail(FWE_ERR_ARG, "bad batch/field");
+  HIPCHK(hipSetDevice(x->cfg.device));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  std::vector<float> h((size_t)x->nelem);
+  HIPCHK(hipMemcpy(h.data(), x->fields + ((int64_t)batch * 9 + field) * x->nelem, sizeof(float) * h.size(),
+                   hipMemcpyDeviceToHost));
+  int64_t d[3];
+  field_shape(x->cfg, field, d);
+  unpack(x, h, d, out);
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_set_field(fwe_ctx* x, int32_t batch, int32_t field, const float* in) {
+  if (!x || !in) return fail(FWE_ERR_ARG, "NULL argument");
+  if (batch < 0 || batch >= x->cfg.nbatch || field < 0 || field > 8) return fail(FWE_ERR_ARG, "bad batch/field");
+  int64_t d[3];
+  field_shape(x->cfg, field, d);
+  std::vector<float> h;
+  pack(x, in, d, 1.0, h);
+  return upload(x, x->fields + ((int64_t)batch * 9 + field) * x->nelem, h);
+}
+
+// ---------------------------------------------------------------------------
+// legacy file contract (OK4.cu:153-199, 401-457, 521-532)
+// ---------------------------------------------------------------------------
+static std::string path_of(const char* dir, const char* name) {
+  std::string p = (dir && dir[0]) ? dir : ".";
+  if (p.back() != '/') p += '/';
+  return p + name;
+}
+
+// read exactly n float32 values; FWE_ERR_IO on a missing or short file
+// (the reference would fread(NULL) and crash, OK4.cu:190)
+static fwe_status read_f32(const char* dir, const char* name, size_t n, std::vector<float>& out, size_t* got) {
+  const std::string p = path_of(dir, name);
+  FILE* f = fopen(p.c_str(), "rb");
+  if (!f) return fail(FWE_ERR_IO, "cannot open %s", p.c_str());
+  out.assign(n, 0.f);
+  const size_t r = n ? fread(out.data(), sizeof(float), n, f) : 0;
+  fclose(f);
+  if (got) *got = r;
+  if (!got && r != n) return fail(FWE_ERR_IO, "%s: read %zu of %zu float32 values", p.c_str(), r, n);
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_read_legacy_params(const char* dir, fwe_legacy_params* out) {
+  if (!out) return fail(FWE_ERR_ARG, "NULL argument");
+  memset(out, 0, sizeof(*out));
+  std::vector<float> v;
+  size_t got = 0;
+  fwe_status st = read_f32(dir, "pa1.dat", 10, v, &got);
+  if (st) return st;
+  if (got < 4) return fail(FWE_ERR_IO, "pa1.dat holds %zu values, need >= 4", got);
+  out->n_pa1 = (int)got;
+  for (size_t q = 0; q < got; ++q) out->pa1[q] = v[q];
+  st = read_f32(dir, "pa2.dat", 13, v, &got);
+  if (st) return st;
+  out->n_pa2 = (int)got;
+  for (size_t q = 0; q < got; ++q) out->pa2[q] = v[q];
+  st = read_f32(dir, "pa3.dat", 12, v, &got);
+  if (st) return st;
+  if (got < 3) return fail(FWE_ERR_IO, "pa3.dat holds %zu values, need >= 3", got);
+  out->n_pa3 = (int)got;
+  for (size_t q = 0; q < got; ++q) out->pa3[q] = v[q];
+  if (read_f32(dir, "MTS.dat", 6, v, &got) == FWE_OK && got == 6) {
+    out->has_mts = 1;
+    for (int q = 0; q < 6; ++q) out->mts[q] = v[q];
+  }
+  g_err[0] = 0;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_load_legacy_inputs(fwe_ctx* x, const char* dir, const fwe_legacy_params* p) {
+  if (!x || !p) return fail(FWE_ERR_ARG, "NULL argument");
+  const fwe_config& c = x->cfg;
+  const size_t nc = (size_t)c.nx * c.ny * c.nz;
+  std::vector<float> a, b;
+  fwe_status st;
+  if (c.mode == FWE_MODE_ISO) {
+    if ((st = read_f32(dir, "c11u_het.dat", nc, a, nullptr))) return st;
+    if ((st = read_f32(dir, "c12u_het.dat", nc, b, nullptr))) return st;
+    if (p->n_pa2 < 8) return fail(FWE_ERR_IO, "pa2.dat lacks c66u (index 7)");
+    st = fwe_set_material_iso(x, a.data(), b.data(), (float)p->pa2[7], (float)p->pa3[0], (float)p->pa3[1],
+                              (float)p->pa3[2]);
+    if (st) return st;
+    if (c.pml == FWE_PML_PROFILE) {
+      std::vector<float> wx, wy, wz;
+      if ((st = read_f32(dir, "weights_x.dat", c.nx + 1, wx, nullptr))) return st;
+      if ((st = read_f32(dir, "weights_y.dat", c.ny + 1, wy, nullptr))) return st;
+      if ((st = read_f32(dir, "weights_z.dat", c.nz + 1, wz, nullptr))) return st;
+      if ((st = fwe_set_pml_profiles(x, wx.data(), wy.data(), wz.data()))) return st;
+    }
+  } else {
+    static const char* cn[9] = {"C11m.dat", "C22m.dat", "C33m.dat", "C12m.dat", "C13m.dat",
+                                "C23m.dat", "C44myz.dat", "C55mxz.dat", "C66mxy.dat"};
+    static const int shp[9] = {FWE_F_SXX, FWE_F_SXX, FWE_F_SXX, FWE_F_SXX, FWE_F_SXX,
+                               FWE_F_SXX, FWE_F_SYZ, FWE_F_SXZ, FWE_F_SXY};
+    static const char* bn[3] = {"vrho_x_11_hc.dat", "vrho_y_11_hc.dat", "vrho_z_11_hc.dat"};
+    std::vector<std::vector<float>> cv(9), bv(3);
+    const float* cp[9];
+    const float* bp[3];
+    for (int q = 0; q < 9; ++q) {
+      int64_t d[3];
+      field_shape(c, shp[q], d);
+      if ((st = read_f32(dir, cn[q], (size_t)(d[0] * d[1] * d[2]), cv[q], nullptr))) return st;
+      cp[q] = cv[q].data();
+    }
+    for (int q = 0; q < 3; ++q) {
+      int64_t d[3];
+      field_shape(c, q, d);
+      if ((st = read_f32(dir, bn[q], (size_t)(d[0] * d[1] * d[2]), bv[q], nullptr))) return st;
+      bp[q] = bv[q].data();
+    }
+    if ((st = fwe_set_material_ortho(x, cp, bp))) return st;
+    if (c.pml == FWE_PML_WEIGHTS3D) {
+      if ((st = read_f32(dir, "weights.dat", nc, a, nullptr))) return st;
+      if ((st = fwe_set_pml_weights(x, a.data()))) return st;
+    } else if (c.pml == FWE_PML_SEPARABLE) {
+      std::vector<float> wx, wy, wz;
+      if ((st = read_f32(dir, "taper_x.dat", c.nx, wx, nullptr))) return st;
+      if ((st = read_f32(dir, "taper_y.dat", c.ny, wy, nullptr))) return st;
+      if ((st = read_f32(dir, "taper_z.dat", c.nz, wz, nullptr))) return st;
+      if ((st = fwe_set_pml_profiles(x, wx.data(), wy.data(), wz.data()))) return st;
+    }
+  }
+  if ((st = read_f32(dir, "Src.dat", (size_t)c.nt, a, nullptr))) return st;
+  for (int b2 = 0; b2 < c.nbatch; ++b2)
+    if ((st = fwe_set_stf(x, b2, a.data(), c.nt))) return st;
+  g_err[0] = 0;
+  return FWE_OK;
+}
+
+static fwe_status write_f32(const char* dir, const char* name, const float* a, size_t n) {
+  // "%d_%d_%s.res" with isave = 0 and rank 0 (OK4.cu:159-164)
+  char fn[256];
+  snprintf(fn, sizeof(fn), "0_0_%s.res", name);
+  const std::string p = path_of(dir, fn);
+  FILE* f = fopen(p.c_str(), "wb");
+  if (!f) return fail(FWE_ERR_IO, "cannot write %s", p.c_str());
+  const size_t w = n ? fwrite(a, sizeof(float), n, f) : 0;
+  fclose(f);
+  if (w != n) return fail(FWE_ERR_IO, "short write %s", p.c_str());
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_write_legacy_outputs(fwe_ctx* x, const char* dir, const char* const* rec_names,
+                                               int32_t n_names) {
+  if (!x) return fail(FWE_ERR_ARG, "NULL ctx");
+  if (n_names < 0 || (n_names > 0 && !rec_names) || n_names > x->nrec)
+    return fail(FWE_ERR_ARG, "rec_names must name at most the %d receivers", x->nrec);
+  const fwe_config& c = x->cfg;
+  {
+    // save_info (OK4.cu:153-157): PRECIS nx ny nz NB_PARAMS ceil(niter/nout)
+    const std::string p = path_of(dir, "0_infos.inf");
+    FILE* f = fopen(p.c_str(), "w");
+    if (!f) return fail(FWE_ERR_IO, "cannot write %s", p.c_str());
+    fprintf(f, "%d %d %d %d %d %d", 4, c.nx, c.ny, c.nz, 1, (int)ceil(10.0f / 200.0f));
+    fclose(f);
+  }
+  fwe_status st;
+  std::vector<float> src((size_t)c.nt, 0.f);
+  for (size_t q = 0; q < x->stf_host.size() && q < src.size(); ++q) src[q] = x->stf_host[q];
+  if ((st = write_f32(dir, "Src", src.data(), src.size()))) return st;
+  if (x->nrec > 0 && c.nt > 0) {
+    std::vector<float> tr((size_t)x->nrec * c.nt);
+    st = fwe_get_traces(x, tr.data());
+    if (st && st != FWE_ERR_NONFINITE) return st;
+    for (int r = 0; r < n_names; ++r)
+      if ((st = write_f32(dir, rec_names[r], tr.data() + (size_t)r * c.nt, (size_t)c.nt))) return st;
+  }
+  static const char* fn[3] = {"Vx", "Vy", "Vz"};
+  for (int f = 0; f < 3; ++f) {
+    int64_t d[3];
+    field_shape(c, f, d);
+    std::vector<float> h((size_t)(d[0] * d[1] * d[2]));
+    if ((st = fwe_get_field(x, 0, f, h.data()))) return st;
+    if ((st = write_f32(dir, fn[f], h.data(), h.size()))) return st;
+  }
+  return FWE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// measurement
+// ---------------------------------------------------------------------------
+extern "C" fwe_status fwe_copy_probe(int32_t device, int64_t bytes, int32_t reps, double* ms_per_copy) {
+  if (bytes < 1024 || reps < 1 || !ms_per_copy) return fail(FWE_ERR_ARG, "bad probe arguments");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+    return fail(FWE_ERR_HIP, "no HIP device %d", device);
+  HIPCHK(hipSetDevice(device));
+  const int64_t n = bytes / 16 * 4;
+  float *a = nullptr, *b = nullptr;
+  HIPCHK(hipMalloc(&a, sizeof(float) * n));
+  HIPCHK(hipMalloc(&b, sizeof(float) * n));
+  hipStream_t s;
+  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  HIPCHK(hipMemsetAsync(a, 0, sizeof(float) * n, s));
+  HIPCHK(fwe::launch_copy(a, b, n, s));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r) HIPCHK(fwe::launch_copy(a, b, n, s));
+  HIPCHK(hipEventRecord(e1, s));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *ms_per_copy = ms / reps;
+  hipEventDestroy(e0); hipEventDestroy(e1);
+  hipStreamDestroy(s);
+  hipFree(a); hipFree(b);
+  return FWE_OK;
+}

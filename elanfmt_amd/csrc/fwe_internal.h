@@ -1,0 +1,70 @@
+// fwe_internal.h -- shared definitions between the C-ABI host code and the
+// gfx950 kernels of libfwe.so.  Not part of the public ABI.
+#pragma once
+#include <stdint.h>
+
+#include <hip/hip_runtime.h>
+
+namespace fwe {
+
+// Tile of one workgroup in the xy plane: 64 lanes along x (one wavefront
+// row, coalesced x-fastest access), TILE_Y rows along y.  Each workgroup
+// marches a chunk of z planes.
+constexpr int TILE_X = 64;
+
+enum Mode { MODE_ISO = 0, MODE_ORTHO = 1 };
+enum Pml { PML_NONE = 0, PML_PROFILE = 1, PML_W3D = 2, PML_SEP = 3 };
+
+// One injection or receiver, sorted by xy tile on the host.
+struct PointEntry {
+  int32_t i, j, k;
+  int32_t comp;
+  int32_t batch;
+  float coef;  // injection coefficient (unused for receivers)
+  int32_t slot;  // receiver index (trace row) or -1
+  int32_t pad;
+};
+
+// Everything one step needs.  Fields of wavefield b live at base + b*fstride.
+// All 3-D arrays share one padded layout: element (i,j,k) at
+// i + j*sy + k*sz with sy >= nx+1 (rounded to 32 floats) and sz = sy*(ny+1).
+struct StepArgs {
+  int32_t nx, ny, nz, nb;
+  int64_t sy, sz, fstride;
+  float* V[3];        // Vx Vy Vz (batch 0)
+  float* S[6];        // Sxx Syy Szz Sxy Sxz Syz (batch 0)
+  // ortho medium, pre-multiplied by dt: C11 C22 C33 C12 C13 C23 C44yz C55xz C66xy
+  const float* C[9];
+  // ortho buoyancy pre-multiplied by dt: bx by bz (staggered)
+  const float* B[3];
+  // iso medium: c11*dt, c12*dt arrays, scalars c66*dt and b*dt
+  float c66dt, bxdt, bydt, bzdt;
+  // damping: PROFILE -> w (n+1) and mean-of-neighbours wh (n) per axis;
+  // SEP -> tapers (n); W3D -> w3 in the padded layout.
+  const float* wx; const float* wy; const float* wz;
+  const float* whx; const float* why; const float* whz;
+  const float* w3;
+  float rdx, rdy, rdz;   // 1/dx ...
+  float dtdx, dtdy, dtdz; // dt/dx ... (iso stress)
+  // tiling
+  int32_t ty_rows;      // TILE_Y
+  int32_t ntx, nty, nkc, kchunk;
+  // sources
+  const int32_t* inj_off;   // ntx*nty+1
+  const PointEntry* inj;
+  const float* stf;         // [nb][nstf]
+  int32_t nstf;
+  int32_t src_after_bc;
+  // receivers
+  const int32_t* rec_off;
+  const PointEntry* rec;
+  float* traces;            // [nrec][nt]
+  int32_t nt;
+  int32_t it;
+};
+
+hipError_t launch_stress(const StepArgs& a, int mode, int pml, int fs, hipStream_t s);
+hipError_t launch_velocity(const StepArgs& a, int mode, int pml, hipStream_t s);
+hipError_t launch_copy(const float* src, float* dst, int64_t n, hipStream_t s);
+
+}  // namespace fwe

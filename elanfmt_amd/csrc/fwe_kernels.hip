@@ -1,0 +1,328 @@
+// fwe_kernels.hip -- gfx950 kernels of the staggered-grid velocity-stress
+// FDTD step (stress + damping + BC1 + injection, velocity + damping +
+// receivers).  Semantics follow the reference kernels
+//   compute_StressPrf / update_Qxft   OK4.cu:285-388   (iso law)
+//   compute_StressOrtho / update_VelocityFromStress / BC1 / source / record
+//                                      ANISO.cu:228-389 (ortho law)
+// Layout and schedule are MI355X-first:
+//  * every 3-D array (fields, medium, 3-D weights) shares one padded layout,
+//    x fastest, row pitch a multiple of 32 floats (128 B lines), so one index
+//    serves all staggered arrays and a 64-lane wavefront reads 256 B
+//    contiguous per array;
+//  * a workgroup owns a 64 x TY column tile and marches a chunk of z planes,
+//    carrying the z-neighbours it needs in registers;
+//  * workgroup ids are remapped so that neighbouring tiles run on the same
+//    XCD and share its L2 for the x/y halo re-reads;
+//  * the reference's separate source/BC1/record launches are fused into the
+//    two stencil kernels (per-tile injection and receiver lists).
+// The stencil is HBM-bound (~0.8 flop/B), so there is no MFMA here.
+#include "fwe_internal.h"
+
+namespace fwe {
+
+// Workgroup w is dispatched to XCD (w mod 8).  Give each XCD a contiguous
+// range of logical tiles so x/y neighbours share an L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7;
+  const int x = b & 7, w = b >> 3;
+  return x * q + (x < r ? x : r) + w;
+}
+
+__device__ __forceinline__ float ld(const float* p) { return __builtin_nontemporal_load(p); }
+
+// ---------------------------------------------------------------------------
+// Stress pass.  For cell (i,j,k) of the column tile:
+//   normal:  S_ii <- damp * S_ii + sum_j C_ij * dt * dV_j/dx_j
+//   shear:   S_xy <- damp * S_xy + C66 * dt * (dVy/dx + dVx/dy)   (i,j >= 1)
+//            S_xz ... (i,k >= 1),  S_yz ... (j,k >= 1)
+// iso law (OK4.cu:296-329): damp multiplies (old + increment); ortho law
+// (ANISO.cu:306-352): damp multiplies the old value only.
+// Injection and BC1 are applied in the reference order (see StepArgs).
+// ---------------------------------------------------------------------------
+template <int MODE, int PML, bool FS, int TY>
+__global__ __launch_bounds__(64 * TY) void k_stress(const StepArgs a) {
+  const int lane = threadIdx.x, row = threadIdx.y;
+  int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = lb % a.ntx;
+  lb /= a.ntx;
+  const int tyi = lb % a.nty;
+  const int kc = lb / a.nty;
+  const int i = tx * TILE_X + lane, j = tyi * TY + row;
+  if (i >= a.nx || j >= a.ny) return;
+  const int k0 = kc * a.kchunk;
+  const int k1 = min(k0 + a.kchunk, a.nz);
+  const int64_t sy = a.sy, sz = a.sz;
+  const int tile = tx + tyi * a.ntx;
+  const int e0 = a.inj_off[tile], e1 = a.inj_off[tile + 1];
+  const bool im = i >= 1, jm = j >= 1;
+  const float third = 1.0f / 3.0f;
+
+  float fxh = 1.f, fyh = 1.f, wxi = 1.f, wyj = 1.f;
+  if (PML == PML_PROFILE) { fxh = a.whx[i]; fyh = a.why[j]; }
+  if (PML == PML_PROFILE || PML == PML_SEP) { wxi = a.wx[i]; wyj = a.wy[j]; }
+
+  for (int k = k0; k < k1; ++k) {
+    const int64_t c = (int64_t)i + j * sy + k * sz;
+    const bool km = k >= 1;
+    float wzk = 1.f, fzh = 1.f;
+    if (PML == PML_PROFILE) { wzk = a.wz[k]; fzh = a.whz[k]; }
+    if (PML == PML_SEP) wzk = a.wz[k];
+    float w = 1.f;
+    if (PML == PML_W3D) w = ld(a.w3 + c);
+    if (PML == PML_SEP) w = wxi * wyj * wzk;
+    // medium at this cell, shared by all wavefields of the batch
+    float m0, m1, m2, m3, m4, m5, m6, m7, m8;
+    if (MODE == MODE_ORTHO) {
+      m0 = ld(a.C[0] + c); m1 = ld(a.C[1] + c); m2 = ld(a.C[2] + c);
+      m3 = ld(a.C[3] + c); m4 = ld(a.C[4] + c); m5 = ld(a.C[5] + c);
+      m6 = ld(a.C[6] + c); m7 = ld(a.C[7] + c); m8 = ld(a.C[8] + c);
+    } else {
+      m0 = ld(a.C[0] + c); m1 = ld(a.C[1] + c);
+      m2 = m3 = m4 = m5 = m6 = m7 = m8 = 0.f;
+    }
+    for (int b = 0; b < a.nb; ++b) {
+      const int64_t fo = (int64_t)b * a.fstride;
+      const float* Vx = a.V[0] + fo;
+      const float* Vy = a.V[1] + fo;
+      const float* Vz = a.V[2] + fo;
+      const float vx = Vx[c], vy = Vy[c], vz = Vz[c];
+      const float exx = (Vx[c + 1] - vx), eyy = (Vy[c + sy] - vy), ezz = (Vz[c + sz] - vz);
+      const float vx_jm = jm ? Vx[c - sy] : 0.f;
+      const float vx_km = km ? Vx[c - sz] : 0.f;
+      const float vy_im = im ? Vy[c - 1] : 0.f;
+      const float vy_km = km ? Vy[c - sz] : 0.f;
+      const float vz_im = im ? Vz[c - 1] : 0.f;
+      const float vz_jm = jm ? Vz[c - sy] : 0.f;
+      float* Sxx = a.S[0] + fo; float* Syy = a.S[1] + fo; float* Szz = a.S[2] + fo;
+      float* Sxy = a.S[3] + fo; float* Sxz = a.S[4] + fo; float* Syz = a.S[5] + fo;
+      float sxx = ld(Sxx + c), syy = ld(Syy + c), szz = ld(Szz + c);
+      float sxy = (im && jm) ? ld(Sxy + c) : 0.f;
+      float sxz = (im && km) ? ld(Sxz + c) : 0.f;
+      float syz = (jm && km) ? ld(Syz + c) : 0.f;
+      if (MODE == MODE_ISO) {
+        // OK4.cu:307-309,323-329 with c11,c12 arrays holding c*dt
+        const float ex = exx * a.rdx, ey = eyy * a.rdy, ez = ezz * a.rdz;
+        sxx = fxh * (sxx + (m0 * ex + m1 * ey + m1 * ez));
+        syy = fyh * (syy + (m1 * ex + m0 * ey + m1 * ez));
+        szz = fzh * (szz + (m1 * ex + m1 * ey + m0 * ez));
+        sxy = (wzk * wyj * wxi) * (sxy + a.c66dt * ((vy - vy_im) * a.rdx + (vx - vx_jm) * a.rdy));
+        sxz = (wyj * wzk * wxi) * (sxz + a.c66dt * ((vz - vz_im) * a.rdx + (vx - vx_km) * a.rdz));
+        syz = (wxi * wyj * wzk) * (syz + a.c66dt * ((vy - vy_km) * a.rdz + (vz - vz_jm) * a.rdy));
+      } else {
+        // ANISO.cu:306-352 with Cij holding Cij*dt
+        const float ex = exx * a.rdx, ey = eyy * a.rdy, ez = ezz * a.rdz;
+        sxx = sxx * w + (m0 * ex + m3 * ey + m4 * ez);
+        syy = syy * w + (m3 * ex + m1 * ey + m5 * ez);
+        szz = szz * w + (m4 * ex + m5 * ey + m2 * ez);
+        sxy = sxy * w + m8 * ((vy - vy_im) * a.rdx + (vx - vx_jm) * a.rdy);
+        sxz = sxz * w + m7 * ((vz - vz_im) * a.rdx + (vx - vx_km) * a.rdz);
+        syz = syz * w + m6 * ((vy - vy_km) * a.rdz + (vz - vz_jm) * a.rdy);
+      }
+      // shear cells outside the reference update ranges stay zero
+      if (!(im && jm)) sxy = 0.f;
+      if (!(im && km)) sxz = 0.f;
+      if (!(jm && km)) syz = 0.f;
+      // point injections of this cell (OK4.cu:336-342 / ANISO.cu:228-252)
+      float src[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      float src_xz1 = 0.f, src_yz1 = 0.f;  // injections at (i,j,1), for BC1
+      if (e1 > e0) {
+        const float amp = (a.it < a.nstf) ? a.stf[(int64_t)b * a.nstf + a.it] : 0.f;
+        for (int e = e0; e < e1; ++e) {
+          const PointEntry p = a.inj[e];
+          if (p.i == i && p.j == j && p.batch == b) {
+            if (p.k == k) src[p.comp] += p.coef * amp;
+            if (FS && k == 2 && p.k == 1) {
+              if (p.comp == 4) src_xz1 += p.coef * amp;
+              if (p.comp == 5) src_yz1 += p.coef * amp;
+            }
+          }
+        }
+      }
+      if (!a.src_after_bc) {
+        sxx += src[0]; syy += src[1]; szz += src[2];
+        sxy += src[3]; sxz += src[4]; syz += src[5];
+      }
+      if (FS) {
+        // BC1 (OK4.cu:279-282): Szz(k=0)=0, Sxz/Syz(k=1) = Sxz/Syz(k=2)/3
+        if (k == 0) szz = 0.f;
+        if (k == 2) {
+          if (im) Sxz[c - sz] = third * sxz + (a.src_after_bc ? src_xz1 : 0.f);
+          if (jm) Syz[c - sz] = third * syz + (a.src_after_bc ? src_yz1 : 0.f);
+        }
+      }
+      if (a.src_after_bc) {
+        sxx += src[0]; syy += src[1]; szz += src[2];
+        sxy += src[3]; sxz += src[4]; syz += src[5];
+      }
+      Sxx[c] = sxx; Syy[c] = syy; Szz[c] = szz;
+      Sxy[c] = sxy; Sxz[c] = sxz; Syz[c] = syz;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Velocity pass (OK4.cu:351-378 iso, ANISO.cu:374-386 ortho) and receiver
+// sampling after the update (OK4.cu:383-386).  Ranges:
+//   Vx 1<=i<=nx-1, 1<=j<=ny-2, 1<=k<=nz-2
+//   Vy 1<=i<=nx-2, 1<=j<=ny-1, 1<=k<=nz-2
+//   Vz 1<=i<=nx-2, 1<=j<=ny-2, 1<=k<=nz-1
+// ---------------------------------------------------------------------------
+template <int MODE, int PML, int TY>
+__global__ __launch_bounds__(64 * TY) void k_velocity(const StepArgs a) {
+  const int lane = threadIdx.x, row = threadIdx.y;
+  int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = lb % a.ntx;
+  lb /= a.ntx;
+  const int tyi = lb % a.nty;
+  const int kc = lb / a.nty;
+  const int i = tx * TILE_X + lane, j = tyi * TY + row;
+  if (i >= a.nx || j >= a.ny) return;
+  const int k0 = kc * a.kchunk;
+  const int k1 = min(k0 + a.kchunk, a.nz);
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int64_t sy = a.sy, sz = a.sz;
+  const int tile = tx + tyi * a.ntx;
+  const int r0 = a.rec_off[tile], r1 = a.rec_off[tile + 1];
+  const bool xin = i >= 1 && j >= 1;  // all three need i>=1, j>=1, k>=1
+  const bool ux_ij = xin && j <= ny - 2;
+  const bool uy_ij = xin && i <= nx - 2;
+  const bool uz_ij = xin && i <= nx - 2 && j <= ny - 2;
+
+  float wxm = 1.f, wym = 1.f, txi = 1.f, tyj = 1.f;
+  if (PML == PML_PROFILE) {
+    wxm = i >= 1 ? a.wx[i - 1] : 1.f;
+    wym = j >= 1 ? a.wy[j - 1] : 1.f;
+  }
+  if (PML == PML_SEP) { txi = a.wx[i]; tyj = a.wy[j]; }
+
+  for (int k = k0; k < k1; ++k) {
+    const int64_t c = (int64_t)i + j * sy + k * sz;
+    const bool ux = ux_ij && k >= 1 && k <= nz - 2;
+    const bool uy = uy_ij && k >= 1 && k <= nz - 2;
+    const bool uz = uz_ij && k >= 1;
+    float w = 1.f;
+    if (PML == PML_W3D) w = ld(a.w3 + c);
+    if (PML == PML_SEP) w = txi * tyj * a.wz[k];
+    float bx = a.bxdt, by = a.bydt, bz = a.bzdt;
+    if (MODE == MODE_ORTHO) {
+      bx = ld(a.B[0] + c); by = ld(a.B[1] + c); bz = ld(a.B[2] + c);
+    }
+    float wzm = 1.f;
+    if (PML == PML_PROFILE) wzm = k >= 1 ? a.wz[k - 1] : 1.f;
+    for (int b = 0; b < a.nb; ++b) {
+      const int64_t fo = (int64_t)b * a.fstride;
+      float* Vx = a.V[0] + fo; float* Vy = a.V[1] + fo; float* Vz = a.V[2] + fo;
+      const float* Sxx = a.S[0] + fo; const float* Syy = a.S[1] + fo;
+      const float* Szz = a.S[2] + fo; const float* Sxy = a.S[3] + fo;
+      const float* Sxz = a.S[4] + fo; const float* Syz = a.S[5] + fo;
+      float vx = ld(Vx + c), vy = ld(Vy + c), vz = ld(Vz + c);
+      if (ux) {
+        const float dS = (Sxx[c] - Sxx[c - 1]) * a.rdx + (Sxy[c + sy] - Sxy[c]) * a.rdy +
+                         (Sxz[c + sz] - Sxz[c]) * a.rdz;
+        vx = (MODE == MODE_ISO) ? wxm * (vx + bx * dS) : vx * w + bx * dS;
+      }
+      if (uy) {
+        const float dS = (Syy[c] - Syy[c - sy]) * a.rdy + (Sxy[c + 1] - Sxy[c]) * a.rdx +
+                         (Syz[c + sz] - Syz[c]) * a.rdz;
+        vy = (MODE == MODE_ISO) ? wym * (vy + by * dS) : vy * w + by * dS;
+      }
+      if (uz) {
+        const float dS = (Szz[c] - Szz[c - sz]) * a.rdz + (Sxz[c + 1] - Sxz[c]) * a.rdx +
+                         (Syz[c + sy] - Syz[c]) * a.rdy;
+        vz = (MODE == MODE_ISO) ? wzm * (vz + bz * dS) : vz * w + bz * dS;
+      }
+      if (ux) Vx[c] = vx;
+      if (uy) Vy[c] = vy;
+      if (uz) Vz[c] = vz;
+      for (int r = r0; r < r1; ++r) {
+        const PointEntry p = a.rec[r];
+        if (p.i == i && p.j == j && p.k == k && p.batch == b && a.it < a.nt)
+          a.traces[(int64_t)p.slot * a.nt + a.it] = p.comp == 0 ? vx : (p.comp == 1 ? vy : vz);
+      }
+    }
+  }
+}
+
+// Plain vector copy, the memory-copy probe (Performance/PerfMC.m).
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src,
+                                              float4* __restrict__ dst, int64_t n4) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n4; t += stride)
+    dst[t] = src[t];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <int MODE, int PML, bool FS>
+static hipError_t go_stress(const StepArgs& a, hipStream_t s) {
+  const int nblk = a.ntx * a.nty * a.nkc;
+  if (a.ty_rows == 4) {
+    hipLaunchKernelGGL((k_stress<MODE, PML, FS, 4>), dim3(nblk), dim3(64, 4), 0, s, a);
+  } else if (a.ty_rows == 8) {
+    hipLaunchKernelGGL((k_stress<MODE, PML, FS, 8>), dim3(nblk), dim3(64, 8), 0, s, a);
+  } else if (a.ty_rows == 2) {
+    hipLaunchKernelGGL((k_stress<MODE, PML, FS, 2>), dim3(nblk), dim3(64, 2), 0, s, a);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int MODE, int PML>
+static hipError_t go_vel(const StepArgs& a, hipStream_t s) {
+  const int nblk = a.ntx * a.nty * a.nkc;
+  if (a.ty_rows == 4) {
+    hipLaunchKernelGGL((k_velocity<MODE, PML, 4>), dim3(nblk), dim3(64, 4), 0, s, a);
+  } else if (a.ty_rows == 8) {
+    hipLaunchKernelGGL((k_velocity<MODE, PML, 8>), dim3(nblk), dim3(64, 8), 0, s, a);
+  } else if (a.ty_rows == 2) {
+    hipLaunchKernelGGL((k_velocity<MODE, PML, 2>), dim3(nblk), dim3(64, 2), 0, s, a);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_stress(const StepArgs& a, int mode, int pml, int fs, hipStream_t s) {
+  if (mode == MODE_ISO) {
+    if (pml == PML_PROFILE) return fs ? go_stress<MODE_ISO, PML_PROFILE, true>(a, s)
+                                      : go_stress<MODE_ISO, PML_PROFILE, false>(a, s);
+    if (pml == PML_NONE) return fs ? go_stress<MODE_ISO, PML_NONE, true>(a, s)
+                                   : go_stress<MODE_ISO, PML_NONE, false>(a, s);
+    return hipErrorInvalidValue;
+  }
+  switch (pml) {
+    case PML_NONE: return fs ? go_stress<MODE_ORTHO, PML_NONE, true>(a, s)
+                             : go_stress<MODE_ORTHO, PML_NONE, false>(a, s);
+    case PML_W3D: return fs ? go_stress<MODE_ORTHO, PML_W3D, true>(a, s)
+                            : go_stress<MODE_ORTHO, PML_W3D, false>(a, s);
+    case PML_SEP: return fs ? go_stress<MODE_ORTHO, PML_SEP, true>(a, s)
+                            : go_stress<MODE_ORTHO, PML_SEP, false>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_velocity(const StepArgs& a, int mode, int pml, hipStream_t s) {
+  if (mode == MODE_ISO) {
+    if (pml == PML_PROFILE) return go_vel<MODE_ISO, PML_PROFILE>(a, s);
+    if (pml == PML_NONE) return go_vel<MODE_ISO, PML_NONE>(a, s);
+    return hipErrorInvalidValue;
+  }
+  switch (pml) {
+    case PML_NONE: return go_vel<MODE_ORTHO, PML_NONE>(a, s);
+    case PML_W3D: return go_vel<MODE_ORTHO, PML_W3D>(a, s);
+    case PML_SEP: return go_vel<MODE_ORTHO, PML_SEP>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_copy(const float* src, float* dst, int64_t n, hipStream_t s) {
+  const int64_t n4 = n / 4;
+  const int blocks = 256 * 16;
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, s,
+                     reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst), n4);
+  return hipGetLastError();
+}
+
+}  // namespace fwe

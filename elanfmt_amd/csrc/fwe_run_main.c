@@ -1,0 +1,204 @@
+/*
+ * fwe_run -- drop-in replacement for the reference's `a.exe`
+ * (main() of OK4.cu:390-559 and ANISO.cu:392-608).
+ *
+ * Run in the directory that holds the legacy inputs.  The reference is
+ * configured at compile time (-DNBX= -DNBY= -DNBZ= -DOVERX= -DOVERY= -DOVERZ=
+ * -DNPARS1..3, OK4.m:432-442); this binary takes the same names as KEY=VALUE
+ * arguments at run time, so the MATLAB driver's run_cmd only needs its
+ * command string changed:
+ *
+ *   fwe_run NBX=12 NBY=12 NBZ=7 OVERX=0 OVERY=0 OVERZ=0 NPARS1=10 NPARS2=13 NPARS3=12
+ *
+ * Further keys: nx= ny= nz= (instead of the NB and OVER keys), nt= (default 3000 iso,
+ * 4500 ortho as OK4.cu:43 / ANISO.cu:43), mode=iso|ortho, pml=profile|
+ * weights|taper|none, fs=0|1 (BC1), xsrc= ysrc= zsrc=, rec=C:i:j:k:name[,..]
+ * (C in x|y|z), dir=<path>, device=<n>, nout=<n> (progress every n steps).
+ * Outputs: 0_infos.inf and 0_0_{Src,Rec,Rec_x,Rec_y,Rec_y2,Vx,Vy,Vz}.res.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "../../include/fwe.h"
+
+#define MAXREC 1024
+
+static int usage(void) {
+  fprintf(stderr,
+          "usage: fwe_run [NBX=.. NBY=.. NBZ=.. OVERX=.. OVERY=.. OVERZ=..] [nx= ny= nz=] [nt=]\n"
+          "               [mode=iso|ortho] [pml=profile|weights|taper|none] [fs=0|1]\n"
+          "               [xsrc= ysrc= zsrc=] [rec=x:i:j:k:name,...] [dir=.] [device=0] [nout=200]\n");
+  return 2;
+}
+
+static double now_s(void) {
+  struct timespec t;
+  clock_gettime(CLOCK_MONOTONIC, &t);
+  return t.tv_sec + 1e-9 * t.tv_nsec;
+}
+
+#define CHECK(call)                                                                     \
+  do {                                                                                  \
+    fwe_status s_ = (call);                                                             \
+    if (s_ != FWE_OK) {                                                                 \
+      fprintf(stderr, "fwe_run: %s -> status %d: %s\n", #call, (int)s_, fwe_last_error()); \
+      return 1;                                                                         \
+    }                                                                                   \
+  } while (0)
+
+int main(int argc, char** argv) {
+  int nbx = -1, nby = -1, nbz = -1, ovx = 0, ovy = 0, ovz = 0, nx = -1, ny = -1, nz = -1;
+  int nt = -1, mode = FWE_MODE_ISO, pml = -1, fs = 1, device = 0, nout = 200;
+  int xsrc = -1, ysrc = -1, zsrc = -1;
+  int npars[3] = {-1, -1, -1};
+  const char* dir = ".";
+  const char* recspec = NULL;
+  for (int a = 1; a < argc; ++a) {
+    const char* eq = strchr(argv[a], '=');
+    if (!eq) return usage();
+    const size_t kl = (size_t)(eq - argv[a]);
+    const char* v = eq + 1;
+#define KEY(s) (kl == strlen(s) && strncmp(argv[a], s, kl) == 0)
+    if (KEY("NBX")) nbx = atoi(v);
+    else if (KEY("NBY")) nby = atoi(v);
+    else if (KEY("NBZ")) nbz = atoi(v);
+    else if (KEY("OVERX")) ovx = atoi(v);
+    else if (KEY("OVERY")) ovy = atoi(v);
+    else if (KEY("OVERZ")) ovz = atoi(v);
+    else if (KEY("NPARS1")) npars[0] = atoi(v);
+    else if (KEY("NPARS2")) npars[1] = atoi(v);
+    else if (KEY("NPARS3")) npars[2] = atoi(v);
+    else if (KEY("nx")) nx = atoi(v);
+    else if (KEY("ny")) ny = atoi(v);
+    else if (KEY("nz")) nz = atoi(v);
+    else if (KEY("nt")) nt = atoi(v);
+    else if (KEY("mode")) mode = strcmp(v, "ortho") == 0 ? FWE_MODE_ORTHO : FWE_MODE_ISO;
+    else if (KEY("pml")) pml = strcmp(v, "profile") == 0 ? FWE_PML_PROFILE
+                            : strcmp(v, "weights") == 0 ? FWE_PML_WEIGHTS3D
+                            : strcmp(v, "taper") == 0   ? FWE_PML_SEPARABLE
+                                                        : FWE_PML_NONE;
+    else if (KEY("fs")) fs = atoi(v);
+    else if (KEY("xsrc")) xsrc = atoi(v);
+    else if (KEY("ysrc")) ysrc = atoi(v);
+    else if (KEY("zsrc")) zsrc = atoi(v);
+    else if (KEY("rec")) recspec = v;
+    else if (KEY("dir")) dir = v;
+    else if (KEY("device")) device = atoi(v);
+    else if (KEY("nout")) nout = atoi(v);
+    else { fprintf(stderr, "fwe_run: unknown key in '%s'\n", argv[a]); return usage(); }
+#undef KEY
+  }
+  /* OK4.cu:26-41: nx = GRID_X*BLOCKS_X - OVERX = 64*NBX - OVERX */
+  if (nx < 0 && nbx > 0) nx = 64 * nbx - ovx;
+  if (ny < 0 && nby > 0) ny = 64 * nby - ovy;
+  if (nz < 0 && nbz > 0) nz = 64 * nbz - ovz;
+  if (nx < 3 || ny < 3 || nz < 3) { fprintf(stderr, "fwe_run: grid size not given\n"); return usage(); }
+  if (nt < 0) nt = mode == FWE_MODE_ISO ? 3000 : 4500;
+
+  fwe_legacy_params p;
+  CHECK(fwe_read_legacy_params(dir, &p));
+  if (npars[0] > 0 && npars[0] != p.n_pa1) fprintf(stderr, "fwe_run: note: NPARS1=%d, pa1.dat holds %d\n", npars[0], p.n_pa1);
+  if (npars[1] > 0 && npars[1] != p.n_pa2) fprintf(stderr, "fwe_run: note: NPARS2=%d, pa2.dat holds %d\n", npars[1], p.n_pa2);
+  if (npars[2] > 0 && npars[2] != p.n_pa3) fprintf(stderr, "fwe_run: note: NPARS3=%d, pa3.dat holds %d\n", npars[2], p.n_pa3);
+
+  fwe_config cfg;
+  CHECK(fwe_config_default(&cfg, mode));
+  cfg.nx = nx; cfg.ny = ny; cfg.nz = nz; cfg.nt = nt;
+  cfg.dx = p.pa1[0]; cfg.dy = p.pa1[1]; cfg.dz = p.pa1[2]; cfg.dt = p.pa1[3];
+  cfg.free_surface = fs;
+  cfg.device = device;
+  if (pml >= 0) cfg.pml = pml;
+  fwe_ctx* ctx = NULL;
+  CHECK(fwe_create(&cfg, &ctx));
+  CHECK(fwe_load_legacy_inputs(ctx, dir, &p));
+
+  /* source: OK4.cu:450-452 (234, round(Ly/2/dy), 100) and MTS.dat;
+   * ANISO.cu:483-485 (180, round(Ly/2/dy), 200) with the coefficients of
+   * ANISO.cu:233-247 unless MTS.dat is present */
+  const double Ly = p.n_pa1 > 5 ? p.pa1[5] : ny * cfg.dy;
+  if (xsrc < 0) xsrc = mode == FWE_MODE_ISO ? 234 : 180;
+  if (ysrc < 0) ysrc = (int)lround((Ly - Ly / 2.0) / cfg.dy);
+  if (zsrc < 0) zsrc = mode == FWE_MODE_ISO ? 100 : 200;
+  float m[6];
+  if (p.has_mts) {
+    for (int c = 0; c < 6; ++c) m[c] = (float)p.mts[c];
+  } else if (mode == FWE_MODE_ORTHO) {
+    const float an[6] = {-0.6282f, 0.9437f, 0.2439f, 0.4455f, -0.2709f, -0.2362f};
+    memcpy(m, an, sizeof(m));
+  } else {
+    fprintf(stderr, "fwe_run: MTS.dat missing (required in iso mode, OK4.cu:410)\n");
+    fwe_destroy(ctx);
+    return 1;
+  }
+  fwe_injection inj[6];
+  CHECK(fwe_mt_injections(mode == FWE_MODE_ISO ? FWE_SRC_LAYOUT_OK4 : FWE_SRC_LAYOUT_ANISO, xsrc, ysrc,
+                          zsrc, m, 0, inj));
+  CHECK(fwe_set_injections(ctx, 6, inj));
+
+  /* receivers: OK4.cu:383-386 or ANISO.cu:264-267 unless rec= is given */
+  static fwe_receiver rec[MAXREC];
+  static char names[MAXREC][64];
+  const char* np[MAXREC];
+  int nrec = 0;
+  if (recspec) {
+    const char* s = recspec;
+    while (*s && nrec < MAXREC) {
+      char c;
+      int i, j, k, used = 0;
+      if (sscanf(s, "%c:%d:%d:%d:%63[^,]%n", &c, &i, &j, &k, names[nrec], &used) != 5) {
+        fprintf(stderr, "fwe_run: bad rec= entry at '%s'\n", s);
+        return usage();
+      }
+      rec[nrec].comp = c == 'x' ? FWE_REC_VX : c == 'y' ? FWE_REC_VY : FWE_REC_VZ;
+      rec[nrec].ix = i; rec[nrec].iy = j; rec[nrec].iz = k; rec[nrec].batch = 0;
+      ++nrec;
+      s += used;
+      if (*s == ',') ++s;
+    }
+  } else if (mode == FWE_MODE_ISO) {
+    const int t[4][4] = {{FWE_REC_VZ, 504, 385, 1}, {FWE_REC_VX, 504, 385, 1}, {FWE_REC_VY, 504, 388, 1},
+                         {FWE_REC_VY, 504, 387, 1}};
+    const char* nm[4] = {"Rec", "Rec_x", "Rec_y", "Rec_y2"};
+    for (nrec = 0; nrec < 4; ++nrec) {
+      rec[nrec] = (fwe_receiver){t[nrec][0], t[nrec][1], t[nrec][2], t[nrec][3], 0};
+      snprintf(names[nrec], 64, "%s", nm[nrec]);
+    }
+  } else {
+    const int t[4] = {FWE_REC_VZ, FWE_REC_VX, FWE_REC_VY, FWE_REC_VY};
+    const char* nm[4] = {"Rec", "Rec_x", "Rec_y", "Rec_y2"};
+    for (nrec = 0; nrec < 4; ++nrec) {
+      rec[nrec] = (fwe_receiver){t[nrec], 631, 257, 1, 0};
+      snprintf(names[nrec], 64, "%s", nm[nrec]);
+    }
+  }
+  for (int r = 0; r < nrec; ++r) np[r] = names[r];
+  CHECK(fwe_set_receivers(ctx, nrec, rec));
+
+  printf("fwe_run: %s law, %dx%dx%d cells, %d steps, dt=%.6g, free surface %s (%s)\n",
+         mode == FWE_MODE_ISO ? "iso" : "ortho", nx, ny, nz, nt, cfg.dt, fs ? "on" : "off", fwe_version());
+  const double t0 = now_s();
+  int done = 0;
+  while (done < nt) {
+    const int chunk = (nout > 0 && nt - done > nout) ? nout : nt - done;
+    CHECK(fwe_run(ctx, chunk));
+    done += chunk;
+    if (nout > 0 && done < nt) {
+      CHECK(fwe_synchronize(ctx));
+      printf("it=%05d > \n", done);
+      fflush(stdout);
+    }
+  }
+  CHECK(fwe_synchronize(ctx));
+  const double el = now_s() - t0;
+  const double cells = (double)nx * ny * nz;
+  printf("time %.3f s after %d iterations, %.2f it/s, %.3f Gcell-updates/s\n", el, nt, nt / el,
+         cells * nt / el * 1e-9);
+  fwe_status st = fwe_write_legacy_outputs(ctx, dir, np, nrec);
+  if (st == FWE_ERR_NONFINITE) fprintf(stderr, "fwe_run: warning: %s\n", fwe_last_error());
+  else if (st != FWE_OK) { fprintf(stderr, "fwe_run: %s\n", fwe_last_error()); fwe_destroy(ctx); return 1; }
+  fwe_destroy(ctx);
+  return 0;
+}

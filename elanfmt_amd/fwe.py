@@ -1,0 +1,298 @@
+"""ctypes binding of libfwe.so (include/fwe.h and include/fwe_gen.h).
+
+A thin Python mirror of the C ABI, used by the tests and bench.py.  All
+compute happens in libfwe.so on the GPU; there is no CPU path here: when the
+library or a HIP device is missing, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfwe.so")
+RUN_PATH = os.path.join(_HERE, "fwe_run")
+
+# status codes (fwe_status)
+OK, ERR_ARG, ERR_IO, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_UNSUPPORTED, ERR_NONFINITE = range(8)
+MODE_ISO, MODE_ORTHO = 0, 1
+PML_NONE, PML_PROFILE, PML_WEIGHTS3D, PML_SEPARABLE = 0, 1, 2, 3
+SXX, SYY, SZZ, SXY, SXZ, SYZ = range(6)
+F_VX, F_VY, F_VZ, F_SXX, F_SYY, F_SZZ, F_SXY, F_SXZ, F_SYZ = range(9)
+REC_VX, REC_VY, REC_VZ = range(3)
+SRC_LAYOUT_OK4, SRC_LAYOUT_ANISO = 0, 1
+MAX_BATCH = 8
+
+FIELD_NAMES = ["Vx", "Vy", "Vz", "Sxx", "Syy", "Szz", "Sxy", "Sxz", "Syz"]
+
+
+class Config(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("nt", C.c_int32),
+                ("mode", C.c_int32), ("pml", C.c_int32), ("free_surface", C.c_int32),
+                ("src_after_bc", C.c_int32), ("nbatch", C.c_int32), ("device", C.c_int32),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double)]
+
+
+class Injection(C.Structure):
+    _fields_ = [("comp", C.c_int32), ("ix", C.c_int32), ("iy", C.c_int32), ("iz", C.c_int32),
+                ("batch", C.c_int32), ("coef", C.c_float)]
+
+
+class Receiver(C.Structure):
+    _fields_ = [("comp", C.c_int32), ("ix", C.c_int32), ("iy", C.c_int32), ("iz", C.c_int32),
+                ("batch", C.c_int32)]
+
+
+class LegacyParams(C.Structure):
+    _fields_ = [("pa1", C.c_double * 10), ("pa2", C.c_double * 13), ("pa3", C.c_double * 12),
+                ("mts", C.c_double * 6), ("n_pa1", C.c_int32), ("n_pa2", C.c_int32),
+                ("n_pa3", C.c_int32), ("has_mts", C.c_int32)]
+
+
+class RNG(C.Structure):
+    _fields_ = [("s", C.c_uint64)]
+
+
+class FweError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"fwe status {status}: {msg}")
+        self.status = status
+
+
+_lib = None
+
+_FP = C.POINTER(C.c_float)
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int32)
+
+# name -> (restype, argtypes); every symbol of include/fwe.h and fwe_gen.h
+SIGNATURES = {
+    "fwe_version": (C.c_char_p, []),
+    "fwe_last_error": (C.c_char_p, []),
+    "fwe_device_count": (C.c_int, [_IP]),
+    "fwe_config_default": (C.c_int, [C.POINTER(Config), C.c_int32]),
+    "fwe_create": (C.c_int, [C.POINTER(Config), C.POINTER(C.c_void_p)]),
+    "fwe_destroy": (C.c_int, [C.c_void_p]),
+    "fwe_get_config": (C.c_int, [C.c_void_p, C.POINTER(Config)]),
+    "fwe_set_material_iso": (C.c_int, [C.c_void_p, _FP, _FP, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "fwe_set_material_ortho": (C.c_int, [C.c_void_p, C.POINTER(_FP), C.POINTER(_FP)]),
+    "fwe_set_pml_profiles": (C.c_int, [C.c_void_p, _FP, _FP, _FP]),
+    "fwe_set_pml_weights": (C.c_int, [C.c_void_p, _FP]),
+    "fwe_set_stf": (C.c_int, [C.c_void_p, C.c_int32, _FP, C.c_int32]),
+    "fwe_set_injections": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Injection)]),
+    "fwe_set_receivers": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Receiver)]),
+    "fwe_mt_injections": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, _FP, C.c_int32,
+                                    C.POINTER(Injection)]),
+    "fwe_reset": (C.c_int, [C.c_void_p]),
+    "fwe_run": (C.c_int, [C.c_void_p, C.c_int32]),
+    "fwe_run_timed": (C.c_int, [C.c_void_p, C.c_int32, _DP, _DP, _DP]),
+    "fwe_synchronize": (C.c_int, [C.c_void_p]),
+    "fwe_get_step": (C.c_int, [C.c_void_p, _IP]),
+    "fwe_get_traces": (C.c_int, [C.c_void_p, _FP]),
+    "fwe_get_field": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _FP]),
+    "fwe_set_field": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _FP]),
+    "fwe_read_legacy_params": (C.c_int, [C.c_char_p, C.POINTER(LegacyParams)]),
+    "fwe_load_legacy_inputs": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(LegacyParams)]),
+    "fwe_write_legacy_outputs": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_char_p), C.c_int32]),
+    "fwe_copy_probe": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, _DP]),
+    # generators (fwe_gen.h)
+    "fwe_rng_seed": (None, [C.POINTER(RNG), C.c_uint64]),
+    "fwe_rng_next": (C.c_uint64, [C.POINTER(RNG)]),
+    "fwe_rng_uniform": (C.c_double, [C.POINTER(RNG)]),
+    "fwe_rng_normal": (C.c_double, [C.POINTER(RNG)]),
+    "fwe_gen_ricker": (None, [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, _FP]),
+    "fwe_gen_mt_sdr": (None, [C.c_double, C.c_double, C.c_double, C.c_double, _DP]),
+    "fwe_gen_lune_eigs": (None, [C.c_double, C.c_double, _DP]),
+    "fwe_gen_mt_from_eigs": (None, [_DP, _DP, C.c_double, _DP]),
+    "fwe_gen_random_rotation": (None, [C.POINTER(RNG), _DP]),
+    "fwe_gen_mt_lune_random": (None, [C.POINTER(RNG), C.c_double, _DP, _DP, _DP]),
+    "fwe_gen_pml_profile": (None, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                                   C.c_int32, _FP]),
+    "fwe_gen_cell_profile": (None, [C.c_int32, _FP, _FP]),
+    "fwe_gen_taper": (None, [C.c_int32, C.c_int32, C.c_int32, C.c_double, _FP]),
+    "fwe_gen_weights3d": (None, [C.c_int32] * 9 + [C.c_double, _FP]),
+    "fwe_gen_stagger_buoyancy": (None, [C.c_int32] * 3 + [_FP] * 4),
+    "fwe_gen_harmonic_edges": (None, [C.c_int32] * 3 + [_FP] * 6),
+    "fwe_gen_ortho_stiffness": (C.c_int, [C.c_double, C.c_double, C.c_double, _DP, _DP]),
+    "fwe_gen_layered_ortho": (C.c_int, [C.c_int32] * 4 + [C.c_uint64] + [C.c_double] * 7 +
+                              [C.POINTER(_FP), _FP, _IP, _DP]),
+    "fwe_gen_random_field": (None, [C.c_int32] * 3 + [C.c_double, C.c_uint64, _FP]),
+    "fwe_gen_cfl_dt": (C.c_double, [C.c_double] * 5),
+}
+
+
+def lib():
+    """Load libfwe.so (built by __graft_entry__.build()); raise if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FweError(ERR_STATE, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(st):
+    if st != OK:
+        raise FweError(st, lib().fwe_last_error().decode())
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_FP)
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_DP)
+
+
+def f32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32).ravel(order="F"))
+
+
+def device_count() -> int:
+    n = C.c_int32(0)
+    _check(lib().fwe_device_count(C.byref(n)))
+    return n.value
+
+
+def field_shape(nx, ny, nz, field):
+    """Reference shape (x, y, z) of a field id (include/fwe.h)."""
+    return [(nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1), (nx, ny, nz), (nx, ny, nz),
+            (nx, ny, nz), (nx + 1, ny + 1, nz), (nx + 1, ny, nz + 1), (nx, ny + 1, nz + 1)][field]
+
+
+class Solver:
+    """One fwe_ctx.  Arrays are numpy in the reference's x-fastest layout:
+    a 3-D numpy array of shape (nx', ny', nz') is passed Fortran-ordered."""
+
+    def __init__(self, nx, ny, nz, dx, dy, dz, dt, nt, mode=MODE_ISO, pml=None,
+                 free_surface=0, src_after_bc=None, nbatch=1, device=0):
+        L = lib()
+        cfg = Config()
+        _check(L.fwe_config_default(C.byref(cfg), mode))
+        cfg.nx, cfg.ny, cfg.nz, cfg.nt = nx, ny, nz, nt
+        cfg.dx, cfg.dy, cfg.dz, cfg.dt = dx, dy, dz, dt
+        if pml is not None:
+            cfg.pml = pml
+        cfg.free_surface = free_surface
+        if src_after_bc is not None:
+            cfg.src_after_bc = src_after_bc
+        cfg.nbatch = nbatch
+        cfg.device = device
+        self.cfg = cfg
+        self._h = C.c_void_p()
+        _check(L.fwe_create(C.byref(cfg), C.byref(self._h)))
+        self.nrec = 0
+
+    # context management
+    def close(self):
+        if self._h:
+            lib().fwe_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # medium and damping
+    def set_material_iso(self, c11, c12, c66, bx, by, bz):
+        a, b = f32(c11), f32(c12)
+        _check(lib().fwe_set_material_iso(self._h, fptr(a), fptr(b), c66, bx, by, bz))
+
+    def set_material_ortho(self, cij, buoy):
+        cs = [f32(c) for c in cij]
+        bs = [f32(b) for b in buoy]
+        carr = (_FP * 9)(*[fptr(c) for c in cs])
+        barr = (_FP * 3)(*[fptr(b) for b in bs])
+        _check(lib().fwe_set_material_ortho(self._h, carr, barr))
+
+    def set_pml_profiles(self, wx, wy, wz):
+        a, b, c = f32(wx), f32(wy), f32(wz)
+        _check(lib().fwe_set_pml_profiles(self._h, fptr(a), fptr(b), fptr(c)))
+
+    def set_pml_weights(self, w):
+        a = f32(w)
+        _check(lib().fwe_set_pml_weights(self._h, fptr(a)))
+
+    # source and receivers
+    def set_stf(self, stf, batch=0):
+        a = f32(stf)
+        _check(lib().fwe_set_stf(self._h, batch, fptr(a), len(a)))
+
+    def set_injections(self, inj):
+        """inj: list of (comp, ix, iy, iz, batch, coef)."""
+        arr = (Injection * max(1, len(inj)))(*[Injection(*t) for t in inj])
+        _check(lib().fwe_set_injections(self._h, len(inj), arr))
+
+    def set_receivers(self, rec):
+        """rec: list of (comp, ix, iy, iz, batch)."""
+        arr = (Receiver * max(1, len(rec)))(*[Receiver(*t) for t in rec])
+        _check(lib().fwe_set_receivers(self._h, len(rec), arr))
+        self.nrec = len(rec)
+
+    # time loop
+    def reset(self):
+        _check(lib().fwe_reset(self._h))
+
+    def run(self, nsteps):
+        _check(lib().fwe_run(self._h, nsteps))
+
+    def run_timed(self, nsteps):
+        a, b, w = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().fwe_run_timed(self._h, nsteps, C.byref(a), C.byref(b), C.byref(w)))
+        return a.value, b.value, w.value
+
+    def synchronize(self):
+        _check(lib().fwe_synchronize(self._h))
+
+    def step(self):
+        it = C.c_int32()
+        _check(lib().fwe_get_step(self._h, C.byref(it)))
+        return it.value
+
+    # results
+    def traces(self):
+        out = np.zeros(max(1, self.nrec * self.cfg.nt), np.float32)
+        st = lib().fwe_get_traces(self._h, fptr(out))
+        if st not in (OK, ERR_NONFINITE):
+            _check(st)
+        return out[: self.nrec * self.cfg.nt].reshape(self.nrec, self.cfg.nt)
+
+    def get_field(self, field, batch=0):
+        shp = field_shape(self.cfg.nx, self.cfg.ny, self.cfg.nz, field)
+        out = np.zeros(int(np.prod(shp)), np.float32)
+        _check(lib().fwe_get_field(self._h, batch, field, fptr(out)))
+        return out.reshape(shp, order="F")
+
+    def set_field(self, field, arr, batch=0):
+        a = f32(arr)
+        _check(lib().fwe_set_field(self._h, batch, field, fptr(a)))
+
+
+def mt_injections(layout, xs, ys, zs, m, batch=0):
+    out = (Injection * 6)()
+    mm = np.ascontiguousarray(np.asarray(m, np.float32))
+    _check(lib().fwe_mt_injections(layout, xs, ys, zs, fptr(mm), batch, out))
+    return [(o.comp, o.ix, o.iy, o.iz, o.batch, o.coef) for o in out]
+
+
+def copy_probe(nbytes, reps=20, device=0):
+    ms = C.c_double()
+    _check(lib().fwe_copy_probe(device, nbytes, reps, C.byref(ms)))
+    return ms.value

@@ -1,0 +1,240 @@
+/*
+ * fwe.h -- C ABI of the MI355X elastodynamic FDTD time loop (libfwe.so).
+ *
+ * The reference (FastWaveED_GPU3D, celsoa/ELANFMT) has no library API: its
+ * boundary is a process + file-system contract.  A MATLAB driver writes raw
+ * float32 x-fastest inputs (pa1/pa2/pa3/MTS/Src + material + PML weights),
+ * compiles the solver with -DNBX.. macros, runs `a.exe` in its working
+ * directory and reads back `0_infos.inf` and `0_0_<name>.res`
+ * (src/FastWaveED_GPU3D_v1_DC_Earthquake_FULL_OK4_final/
+ *  vizme_FastBiotED_GPU3D_OK_2024_final.m:348-467).
+ * Every entry point below replaces one step of that contract or one piece of
+ * the solver's main(); the comment on each names the reference code it
+ * stands in for.  Aliases used in citations:
+ *   OK4.cu   = src/FastWaveED_GPU3D_v1_DC_Earthquake_FULL_OK4_final/FastWaveED_GPU3D_v1.cu
+ *   ANISO.cu = src/FastWaveED_GPU3D_v1_9km_station_ANISO/FastWaveED_GPU3D_v1_9kmst_ANISO.cu
+ *   ANIS.m   = src/FastWaveED_GPU3D_v1_9km_station_ANISO/vizme_FastWaveED_GPU3D_9kmst_ANIS.m
+ *   OK4.m    = src/FastWaveED_GPU3D_v1_DC_Earthquake_FULL_OK4_final/vizme_FastBiotED_GPU3D_OK_2024_final.m
+ *
+ * Conventions
+ *  - Every call returns fwe_status; nothing in the library calls exit().
+ *    fwe_last_error() returns a thread-local message for the last failure.
+ *  - The caller owns every host array it passes; a context owns its device
+ *    memory.  Host arrays use the reference's staggered shapes, x fastest:
+ *      cell      nx*ny*nz            Vx, bx     (nx+1)*ny*nz
+ *      Vy, by    nx*(ny+1)*nz        Vz, bz     nx*ny*(nz+1)
+ *      Sxy,C66xy (nx+1)*(ny+1)*nz    Sxz,C55xz  (nx+1)*ny*(nz+1)
+ *      Syz,C44yz nx*(ny+1)*(nz+1)
+ *  - k = 0 is the top plane (the free surface when enabled).
+ *  - One context per host thread; calls on one context are serialised by
+ *    the caller.  All compute runs on one HIP stream owned by the context.
+ *  - There is no CPU fallback: when no HIP device is usable, fwe_create
+ *    returns FWE_ERR_HIP.
+ */
+#ifndef FWE_H
+#define FWE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FWE_API_VERSION 1
+#define FWE_MAX_BATCH 8
+
+typedef enum {
+  FWE_OK = 0,
+  FWE_ERR_ARG = 1,         /* invalid argument or size */
+  FWE_ERR_IO = 2,          /* missing/short input file, write failure */
+  FWE_ERR_HIP = 3,         /* HIP runtime error or no usable device */
+  FWE_ERR_STATE = 4,       /* call out of order (e.g. run before material) */
+  FWE_ERR_NOMEM = 5,       /* host or device allocation failed */
+  FWE_ERR_UNSUPPORTED = 6, /* combination not provided (e.g. iso + 3-D weights) */
+  FWE_ERR_NONFINITE = 7    /* NaN/Inf detected in the receiver traces */
+} fwe_status;
+
+/* Stress law.  ISO = OK4.cu:285-344 (cell-centred c11,c12 arrays, scalar c66,
+ * scalar buoyancy pa3[0..2]); ORTHO = ANISO.cu:287-389 (9 heterogeneous Cij,
+ * edge-staggered C44yz/C55xz/C66xy, staggered buoyancy arrays). */
+typedef enum { FWE_MODE_ISO = 0, FWE_MODE_ORTHO = 1 } fwe_mode;
+
+/* Damping.  PROFILE = OK4's three 1-D profiles of length n+1, applied as in
+ * OK4.cu:307-329,361-375 (iso only).  WEIGHTS3D = ANISO's cell-centred 3-D
+ * weights.dat multiplying the old value (ANISO.cu:306-352,374-386; ortho
+ * only).  SEPARABLE = the same ANISO law with w(i,j,k)=tx[i]*ty[j]*tz[k]
+ * rebuilt in the kernel from three 1-D cell-centred profiles of length n
+ * (ortho only; the ANISO taper of ANIS.m:466-512 is separable). */
+typedef enum {
+  FWE_PML_NONE = 0,
+  FWE_PML_PROFILE = 1,
+  FWE_PML_WEIGHTS3D = 2,
+  FWE_PML_SEPARABLE = 3
+} fwe_pml;
+
+/* Stress components (injection targets). */
+enum { FWE_SXX = 0, FWE_SYY = 1, FWE_SZZ = 2, FWE_SXY = 3, FWE_SXZ = 4, FWE_SYZ = 5 };
+/* Field ids for fwe_get_field / fwe_set_field. */
+enum {
+  FWE_F_VX = 0, FWE_F_VY = 1, FWE_F_VZ = 2,
+  FWE_F_SXX = 3, FWE_F_SYY = 4, FWE_F_SZZ = 5,
+  FWE_F_SXY = 6, FWE_F_SXZ = 7, FWE_F_SYZ = 8
+};
+/* Receiver components. */
+enum { FWE_REC_VX = 0, FWE_REC_VY = 1, FWE_REC_VZ = 2 };
+/* Injection offset layouts of the reference's moment-tensor source. */
+enum { FWE_SRC_LAYOUT_OK4 = 0, FWE_SRC_LAYOUT_ANISO = 1 };
+
+typedef struct fwe_config {
+  int32_t nx, ny, nz;     /* cells: OK4.cu:39-41 (nx = 64*NBX - OVERX ...) */
+  int32_t nt;             /* trace length in steps (OK4.cu:43) */
+  int32_t mode;           /* fwe_mode */
+  int32_t pml;            /* fwe_pml */
+  int32_t free_surface;   /* 1: apply BC1 every step (OK4.cu:271-283,490) */
+  int32_t src_after_bc;   /* 0: inject before BC1 (OK4 order, OK4.cu:336-342);
+                             1: after BC1 (ANISO order, ANISO.cu:536-538) */
+  int32_t nbatch;         /* wavefields sharing one medium, 1..FWE_MAX_BATCH */
+  int32_t device;         /* HIP device ordinal (OK4.cu:14 GPU_ID) */
+  double dx, dy, dz, dt;  /* pa1.dat[0..3] (OK4.cu:404) */
+} fwe_config;
+
+/* One point injection: S_comp(ix,iy,iz) of wavefield `batch` gains
+ * coef * stf_batch[it] after the stress update of step it.  The reference's
+ * moment-tensor source is six of these (OK4.cu:336-342, ANISO.cu:228-252). */
+typedef struct fwe_injection {
+  int32_t comp;           /* FWE_SXX..FWE_SYZ */
+  int32_t ix, iy, iz;     /* index in the component's own staggered array */
+  int32_t batch;
+  float coef;
+} fwe_injection;
+
+/* One receiver: trace[r][it] = V_comp(ix,iy,iz) of wavefield `batch` after
+ * the velocity update of step it (OK4.cu:383-386, ANISO.cu:254-268). */
+typedef struct fwe_receiver {
+  int32_t comp;           /* FWE_REC_VX..FWE_REC_VZ */
+  int32_t ix, iy, iz;
+  int32_t batch;
+} fwe_receiver;
+
+/* Scalars of the legacy parameter packs pa1/pa2/pa3 (OK4.cu:401-408). */
+typedef struct fwe_legacy_params {
+  double pa1[10];         /* dx dy dz dt Lx Ly Lz lamx lamy lamz */
+  double pa2[13];         /* c11u c33u c13u c12u c23u c44u c55u c66u a1 a2 a3 M1 c22u */
+  double pa3[12];         /* vrho_x_11 vrho_y_11 vrho_z_11 ... (only [0..2] used) */
+  double mts[6];          /* MTS.dat: Mxx Myy Mzz Mxy Mxz Myz (OK4.cu:410-411) */
+  int32_t n_pa1, n_pa2, n_pa3, has_mts;
+} fwe_legacy_params;
+
+typedef struct fwe_ctx fwe_ctx;
+
+/* ---- library ------------------------------------------------------------ */
+/* "fwe <api> <arch>". */
+const char* fwe_version(void);
+/* Message of the last failing call on this host thread ("" if none). */
+const char* fwe_last_error(void);
+/* Usable HIP devices; replaces the fixed device pick of set_up_gpu
+ * (OK4.cu:87-94).  Returns FWE_OK with 0 on a host without a GPU. */
+fwe_status fwe_device_count(int32_t* count);
+
+/* ---- context ------------------------------------------------------------ */
+/* Reference defaults for `mode`: free surface on (BC1 runs every step in both
+ * OK4.cu:490 and ANISO.cu:536), OK4 or ANISO injection order, PROFILE or
+ * WEIGHTS3D damping, nbatch 1, device 0. Sizes and spacings are zeroed. */
+fwe_status fwe_config_default(fwe_config* cfg, int32_t mode);
+/* Allocate zeroed device fields for cfg (OK4.cu:428-439) on cfg->device and
+ * a stream; replaces set_up_gpu and the zeros()/cudaMalloc blocks of main. */
+fwe_status fwe_create(const fwe_config* cfg, fwe_ctx** out);
+/* Free everything the context owns (OK4.cu:538-557). NULL is accepted. */
+fwe_status fwe_destroy(fwe_ctx* ctx);
+/* The configuration the context was created with. */
+fwe_status fwe_get_config(const fwe_ctx* ctx, fwe_config* out);
+
+/* ---- medium ------------------------------------------------------------- */
+/* Iso medium of OK4.cu:415-418 (c11u_het.dat, c12u_het.dat: cell arrays),
+ * with the scalar shear modulus pa2[7] (c66u, OK4.cu:323-329) and the scalar
+ * buoyancies pa3[0..2] (vrho_x_11.., OK4.cu:361-375). */
+fwe_status fwe_set_material_iso(fwe_ctx* ctx, const float* c11, const float* c12,
+                                float c66, float bx, float by, float bz);
+/* Ortho medium of ANISO.cu:411-457: cij[0..5] = C11 C22 C33 C12 C13 C23
+ * (cell), cij[6] = C44yz, cij[7] = C55xz, cij[8] = C66xy (edges);
+ * buoy[0..2] = vrho_{x,y,z}_11_hc (staggered with Vx, Vy, Vz). */
+fwe_status fwe_set_material_ortho(fwe_ctx* ctx, const float* const cij[9],
+                                  const float* const buoy[3]);
+/* PROFILE: weights_{x,y,z}.dat, lengths nx+1, ny+1, nz+1 (OK4.cu:419-424).
+ * SEPARABLE: cell-centred tapers of lengths nx, ny, nz. */
+fwe_status fwe_set_pml_profiles(fwe_ctx* ctx, const float* wx, const float* wy,
+                                const float* wz);
+/* WEIGHTS3D: weights.dat, nx*ny*nz (ANISO.cu:456-457). */
+fwe_status fwe_set_pml_weights(fwe_ctx* ctx, const float* w);
+
+/* ---- source and receivers ------------------------------------------------ */
+/* Source time function of wavefield `batch` (Src.dat, OK4.cu:454); steps at
+ * or beyond n inject nothing. */
+fwe_status fwe_set_stf(fwe_ctx* ctx, int32_t batch, const float* stf, int32_t n);
+/* Replace the injection list (the hard-coded guards of OK4.cu:336-342 and
+ * ANISO.cu:233-247 become data).  Indices are checked against the shapes. */
+fwe_status fwe_set_injections(fwe_ctx* ctx, int32_t n, const fwe_injection* inj);
+/* Replace the receiver list (the hard-coded guards of OK4.cu:383-386). */
+fwe_status fwe_set_receivers(fwe_ctx* ctx, int32_t n, const fwe_receiver* rec);
+/* The six injections of a moment tensor m = (Mxx Myy Mzz Mxy Mxz Myz) at base
+ * cell (xs,ys,zs) with the offsets of OK4.cu:336-342 (layout OK4) or
+ * ANISO.cu:233-247 (layout ANISO).  Pure host helper; out must hold 6. */
+fwe_status fwe_mt_injections(int32_t layout, int32_t xs, int32_t ys, int32_t zs,
+                             const float m[6], int32_t batch, fwe_injection out[6]);
+
+/* ---- time loop ---------------------------------------------------------- */
+/* Zero fields and traces and rewind the step counter (OK4.cu:428-439). */
+fwe_status fwe_reset(fwe_ctx* ctx);
+/* Advance nsteps steps of the hot loop (OK4.cu:485-505, ANISO.cu:519-555):
+ * stress + damping [+ BC1] [+ injection], velocity + damping + receivers.
+ * Asynchronous on the context stream; errors of earlier launches surface
+ * here or at the next synchronising call. */
+fwe_status fwe_run(fwe_ctx* ctx, int32_t nsteps);
+/* As fwe_run, and brackets every stress and every velocity launch with HIP
+ * events on the context stream; returns the summed kernel milliseconds and
+ * the wall milliseconds of the whole call (the cudaEvent pair of
+ * OK4.cu:477-518, per kernel).  Any pointer may be NULL. */
+fwe_status fwe_run_timed(fwe_ctx* ctx, int32_t nsteps, double* ms_stress,
+                         double* ms_velocity, double* ms_wall);
+/* Block until the context stream is idle. */
+fwe_status fwe_synchronize(fwe_ctx* ctx);
+/* Steps advanced since create/reset. */
+fwe_status fwe_get_step(const fwe_ctx* ctx, int32_t* it);
+
+/* ---- results ------------------------------------------------------------ */
+/* Receiver traces, out[r*nt + it] (Rec..Rec_y2 of OK4.cu:454-459,525-529).
+ * Returns FWE_ERR_NONFINITE (after copying) if any sample is NaN/Inf. */
+fwe_status fwe_get_traces(fwe_ctx* ctx, float* out);
+/* Copy one field of wavefield `batch` to/from a host array of its reference
+ * shape (gather/SaveArray, OK4.cu:75,164,530-532; read_data, OK4.cu:166-175). */
+fwe_status fwe_get_field(fwe_ctx* ctx, int32_t batch, int32_t field, float* out);
+fwe_status fwe_set_field(fwe_ctx* ctx, int32_t batch, int32_t field, const float* in);
+
+/* ---- legacy file contract ----------------------------------------------- */
+/* Read pa1.dat, pa2.dat, pa3.dat and (if present) MTS.dat from dir
+ * (load(pa1..), OK4.cu:401-411). */
+fwe_status fwe_read_legacy_params(const char* dir, fwe_legacy_params* out);
+/* Read the medium and damping files of the context's mode from dir:
+ * iso: c11u_het.dat c12u_het.dat weights_{x,y,z}.dat (OK4.cu:415-425);
+ * ortho: C11m..C23m, C44myz, C55mxz, C66mxy, vrho_{x,y,z}_11_hc.dat and
+ * weights.dat (ANISO.cu:411-457); plus Src.dat (nt values, OK4.cu:454).
+ * Scalars come from p (c66 = pa2[7], buoyancy = pa3[0..2]). */
+fwe_status fwe_load_legacy_inputs(fwe_ctx* ctx, const char* dir,
+                                  const fwe_legacy_params* p);
+/* Write 0_infos.inf ("4 nx ny nz 1 ceil(10/200)", OK4.cu:153-157) and
+ * 0_0_<name>.res for Src, each trace name in rec_names (one per receiver,
+ * in receiver order) and Vx, Vy, Vz of wavefield 0 (OK4.cu:521-532). */
+fwe_status fwe_write_legacy_outputs(fwe_ctx* ctx, const char* dir,
+                                    const char* const* rec_names, int32_t n_names);
+
+/* ---- measurement -------------------------------------------------------- */
+/* Stream-copy probe: copies `bytes` bytes device->device `reps` times with a
+ * plain vector copy kernel and returns the mean ms per copy (the memory-copy
+ * figure of Performance/PerfMC.m:1-15). */
+fwe_status fwe_copy_probe(int32_t device, int64_t bytes, int32_t reps, double* ms_per_copy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
