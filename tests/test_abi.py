@@ -1,0 +1,92 @@
+"""The C ABI library loads, exports every declared symbol, and its host-only
+helpers follow the reference (no GPU needed)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fwe_[a-z0-9_]+)\s*\(", txt)))
+
+
+@pytest.mark.parametrize("header", ["fwe.h", "fwe_gen.h"])
+def test_every_declared_symbol_is_exported_and_bound(fwe, header):
+    names = _declared(header)
+    assert len(names) > 10
+    L = fwe.lib()
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in fwe.SIGNATURES, f"{n} has no ctypes signature"
+
+
+def test_version_and_device_count(fwe):
+    assert fwe.lib().fwe_version().decode().startswith("fwe 1")
+    assert fwe.device_count() >= 0
+
+
+def test_create_without_device_fails_loudly(fwe):
+    if fwe.device_count() > 0:
+        pytest.skip("a GPU is present")
+    with pytest.raises(fwe.FweError) as e:
+        fwe.Solver(16, 16, 16, 1.0, 1.0, 1.0, 0.1, 10)
+    assert e.value.status == fwe.ERR_HIP
+
+
+def test_config_default_matches_reference_loop(fwe):
+    import ctypes as C
+
+    cfg = fwe.Config()
+    assert fwe.lib().fwe_config_default(C.byref(cfg), fwe.MODE_ISO) == 0
+    # BC1 runs every step in both reference loops (OK4.cu:490, ANISO.cu:536)
+    assert cfg.free_surface == 1 and cfg.src_after_bc == 0 and cfg.pml == fwe.PML_PROFILE
+    assert fwe.lib().fwe_config_default(C.byref(cfg), fwe.MODE_ORTHO) == 0
+    assert cfg.free_surface == 1 and cfg.src_after_bc == 1 and cfg.pml == fwe.PML_WEIGHTS3D
+    assert fwe.lib().fwe_config_default(C.byref(cfg), 7) == fwe.ERR_ARG
+
+
+def test_mt_injection_offsets_follow_reference_guards(fwe):
+    m = [1, 2, 3, 4, 5, 6]
+    inj = {t[0]: t for t in fwe.mt_injections(fwe.SRC_LAYOUT_OK4, 234, 384, 100, m)}
+    # OK4.cu:336-342: guards a < i < a+2 select i = a+1; Syz uses ys+1<iy<ys+3,
+    # xs+1<ix<xs+3 -> (xs+2, ys+2); Sxz and Sxy use +2 < . < +4 -> (xs+3, ys+3)
+    for c in (fwe.SXX, fwe.SYY, fwe.SZZ):
+        assert inj[c][1:4] == (235, 385, 101)
+    assert inj[fwe.SYZ][1:4] == (236, 386, 101)
+    assert inj[fwe.SXZ][1:4] == (237, 387, 101)
+    assert inj[fwe.SXY][1:4] == (237, 387, 101)
+    assert [inj[c][5] for c in range(6)] == m
+    an = {t[0]: t for t in fwe.mt_injections(fwe.SRC_LAYOUT_ANISO, 180, 256, 200, m)}
+    # ANISO.cu:233-247: Sxy ys+1<iy<ys+3, xs+1<ix<xs+3 -> (182, 258);
+    # Syz iy -> ys+2, xs+2<ix<xs+4 -> (183, 258); Sxz ys<iy<ys+2, xs-2<ix<xs -> (179, 257)
+    for c in (fwe.SXX, fwe.SYY, fwe.SZZ):
+        assert an[c][1:4] == (181, 257, 201)
+    assert an[fwe.SXY][1:4] == (182, 258, 201)
+    assert an[fwe.SYZ][1:4] == (183, 258, 201)
+    assert an[fwe.SXZ][1:4] == (179, 257, 201)
+
+
+def test_legacy_params_reader(fwe, tmp_path):
+    import ctypes as C
+
+    pa1 = np.arange(1, 11, dtype=np.float32)
+    pa2 = np.arange(20, 33, dtype=np.float32)
+    pa3 = np.arange(40, 52, dtype=np.float32)
+    for n, a in (("pa1", pa1), ("pa2", pa2), ("pa3", pa3)):
+        a.tofile(tmp_path / f"{n}.dat")
+    p = fwe.LegacyParams()
+    assert fwe.lib().fwe_read_legacy_params(str(tmp_path).encode(), C.byref(p)) == 0
+    assert (p.n_pa1, p.n_pa2, p.n_pa3, p.has_mts) == (10, 13, 12, 0)
+    assert list(p.pa1) == list(pa1) and p.pa2[7] == 27 and p.pa3[2] == 42
+    np.array([1, 2, 3, 4, 5, 6], np.float32).tofile(tmp_path / "MTS.dat")
+    assert fwe.lib().fwe_read_legacy_params(str(tmp_path).encode(), C.byref(p)) == 0
+    assert p.has_mts == 1 and list(p.mts) == [1, 2, 3, 4, 5, 6]
+    # a missing input is an error status, not a crash (OK4.cu:190 freads NULL)
+    os.remove(tmp_path / "pa2.dat")
+    assert fwe.lib().fwe_read_legacy_params(str(tmp_path).encode(), C.byref(p)) == fwe.ERR_IO
+    assert b"pa2.dat" in fwe.lib().fwe_last_error()
