@@ -82,7 +82,19 @@ struct fwe_ctx {
   int it = 0;
   hipStream_t stream = nullptr;
   std::vector<hipEvent_t> events;
+  // host copies of the point lists (re-sorted when the tiling changes)
+  std::vector<PointEntry> inj_host, rec_host;
+  // captured time loop: graph_steps steps + counter advance, replayed
+  int32_t* it_dev = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_steps = 0;
 };
+
+static void drop_graph(fwe_ctx* x) {
+  if (x->gexec) hipGraphExecDestroy(x->gexec);
+  x->gexec = nullptr;
+  x->graph_steps = 0;
+}
 
 static int64_t round_up(int64_t a, int64_t m) { return (a + m - 1) / m * m; }
 
@@ -200,6 +212,7 @@ extern "C" fwe_status fwe_create(const fwe_config* cfg, fwe_ctx** out) {
     const size_t np = 2 * ((size_t)cfg->nx + cfg->ny + cfg->nz + 3);
     e = hipMalloc(&x->prof, sizeof(float) * np);
   }
+  if (e == hipSuccess) e = hipMalloc(&x->it_dev, sizeof(int32_t));
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMemsetAsync(x->fields, 0, fbytes, x->stream);
   if (e == hipSuccess) e = hipMemsetAsync(x->medium, 0, sizeof(float) * (size_t)x->nelem * nmed, x->stream);
@@ -220,6 +233,8 @@ extern "C" fwe_status fwe_destroy(fwe_ctx* x) {
   if (!x) return FWE_OK;
   hipSetDevice(x->cfg.device);
   if (x->stream) hipStreamSynchronize(x->stream);
+  drop_graph(x);
+  hipFree(x->it_dev);
   for (auto ev : x->events) hipEventDestroy(ev);
   hipFree(x->fields); hipFree(x->medium); hipFree(x->w3); hipFree(x->prof);
   hipFree(x->stf); hipFree(x->traces); hipFree(x->inj_off); hipFree(x->inj);
@@ -346,6 +361,7 @@ extern "C" fwe_status fwe_set_stf(fwe_ctx* x, int32_t batch, const float* stf, i
     hipFree(x->stf);
     x->stf = nb;
     x->nstf = n;
+    drop_graph(x);
   }
   std::vector<float> h((size_t)x->nstf, 0.f);
   if (n) memcpy(h.data(), stf, sizeof(float) * n);
@@ -356,7 +372,8 @@ extern "C" fwe_status fwe_set_stf(fwe_ctx* x, int32_t batch, const float* stf, i
 }
 
 // sort point entries by xy tile and upload with their offsets
-static fwe_status upload_points(fwe_ctx* x, std::vector<PointEntry>& v, int32_t** off, PointEntry** dev) {
+static fwe_status upload_points(fwe_ctx* x, std::vector<PointEntry> v, int32_t** off, PointEntry** dev) {
+  drop_graph(x);
   const int ntile = x->ntx * x->nty;
   std::vector<int32_t> counts(ntile + 1, 0);
   auto tile_of = [&](const PointEntry& p) { return p.i / fwe::TILE_X + (p.j / x->ty) * x->ntx; };
@@ -394,6 +411,7 @@ extern "C" fwe_status fwe_set_injections(fwe_ctx* x, int32_t n, const fwe_inject
   }
   fwe_status st = upload_points(x, v, &x->inj_off, &x->inj);
   if (st) return st;
+  x->inj_host = v;
   x->ninj = n;
   return FWE_OK;
 }
@@ -412,6 +430,7 @@ extern "C" fwe_status fwe_set_receivers(fwe_ctx* x, int32_t n, const fwe_receive
   }
   fwe_status st = upload_points(x, v, &x->rec_off, &x->rec);
   if (st) return st;
+  x->rec_host = v;
   hipFree(x->traces);
   x->traces = nullptr;
   x->nrec = n;
@@ -504,6 +523,37 @@ static fwe_status run_impl(fwe_ctx* x, int32_t nsteps, double* ms_s, double* ms_
   }
   StepArgs a = make_args(x);
   const fwe_config& c = x->cfg;
+  if (!timed && x->graph_steps > 0 && nsteps >= x->graph_steps) {
+    // replay the captured G-step graph; the kernels read the step index
+    // from it_dev, which each replay advances by G
+    if (!x->gexec) {
+      StepArgs g = a;
+      g.it_base = x->it_dev;
+      hipGraph_t graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t e = hipSuccess;
+      for (int s = 0; s < x->graph_steps && e == hipSuccess; ++s) {
+        g.it = s;
+        e = fwe::launch_stress(g, c.mode, c.pml, c.free_surface, x->stream);
+        if (e == hipSuccess) e = fwe::launch_velocity(g, c.mode, c.pml, x->stream);
+      }
+      if (e == hipSuccess) e = fwe::launch_advance(x->it_dev, x->graph_steps, 0, x->stream);
+      hipError_t e2 = hipStreamEndCapture(x->stream, &graph);
+      if (e == hipSuccess) e = e2;
+      if (e == hipSuccess) e = hipGraphInstantiate(&x->gexec, graph, nullptr, nullptr, 0);
+      if (graph) hipGraphDestroy(graph);
+      if (e != hipSuccess) {
+        x->gexec = nullptr;
+        return fail(FWE_ERR_HIP, "time-loop graph capture: %s", hipGetErrorString(e));
+      }
+    }
+    HIPCHK(fwe::launch_advance(x->it_dev, x->it, 1, x->stream));
+    while (nsteps >= x->graph_steps) {
+      HIPCHK(hipGraphLaunch(x->gexec, x->stream));
+      x->it += x->graph_steps;
+      nsteps -= x->graph_steps;
+    }
+  }
   if (timed) HIPCHK(hipEventRecord(x->events[0], x->stream));
   for (int s = 0; s < nsteps; ++s) {
     a.it = x->it;
@@ -534,6 +584,64 @@ static fwe_status run_impl(fwe_ctx* x, int32_t nsteps, double* ms_s, double* ms_
 
 extern "C" fwe_status fwe_run(fwe_ctx* x, int32_t nsteps) { return run_impl(x, nsteps, nullptr, nullptr, nullptr); }
 
+extern "C" fwe_status fwe_set_graph_steps(fwe_ctx* x, int32_t g) {
+  if (!x || g < 0 || g > 4096) return fail(FWE_ERR_ARG, "graph steps must be in 0..4096");
+  drop_graph(x);
+  x->graph_steps = g;
+  return FWE_OK;
+}
+
+static fwe_status apply_tiling(fwe_ctx* x, int ty, int kchunk) {
+  const fwe_config& c = x->cfg;
+  if (ty != 2 && ty != 4 && ty != 8) return fail(FWE_ERR_ARG, "tile rows must be 2, 4 or 8");
+  if (kchunk < 3 || kchunk > c.nz) return fail(FWE_ERR_ARG, "z chunk must be in 3..nz");
+  x->ty = ty;
+  x->ntx = (c.nx + fwe::TILE_X - 1) / fwe::TILE_X;
+  x->nty = (c.ny + ty - 1) / ty;
+  x->kchunk = kchunk;
+  x->nkc = (c.nz + kchunk - 1) / kchunk;
+  fwe_status st = upload_points(x, x->inj_host, &x->inj_off, &x->inj);
+  if (st) return st;
+  return upload_points(x, x->rec_host, &x->rec_off, &x->rec);
+}
+
+extern "C" fwe_status fwe_set_tiling(fwe_ctx* x, int32_t tile_rows, int32_t zchunk) {
+  if (!x) return fail(FWE_ERR_ARG, "NULL ctx");
+  return apply_tiling(x, tile_rows, zchunk);
+}
+
+extern "C" fwe_status fwe_get_tiling(const fwe_ctx* x, int32_t* tile_rows, int32_t* zchunk) {
+  if (!x || !tile_rows || !zchunk) return fail(FWE_ERR_ARG, "NULL argument");
+  *tile_rows = x->ty;
+  *zchunk = x->kchunk;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_autotune(fwe_ctx* x, int32_t trial_steps, double* best_ms_per_step) {
+  if (!x || trial_steps < 1) return fail(FWE_ERR_ARG, "bad autotune arguments");
+  fwe_status st = ready(x);
+  if (st) return st;
+  const int nz = x->cfg.nz;
+  int best_ty = x->ty, best_kc = x->kchunk;
+  double best = 1e30;
+  static const int tys[3] = {2, 4, 8};
+  static const int kcs[8] = {4, 8, 12, 16, 24, 32, 48, 64};
+  for (int ty : tys) {
+    for (int kc : kcs) {
+      if (kc > nz) continue;
+      if ((st = apply_tiling(x, ty, kc))) return st;
+      if ((st = run_impl(x, 1, nullptr, nullptr, nullptr))) return st;  // warm-up
+      double a = 0, b = 0, w = 0;
+      if ((st = run_impl(x, trial_steps, &a, &b, &w))) return st;
+      const double per = (a + b) / trial_steps;
+      if (per < best) { best = per; best_ty = ty; best_kc = kc; }
+    }
+  }
+  if ((st = apply_tiling(x, best_ty, best_kc))) return st;
+  if (best_ms_per_step) *best_ms_per_step = best;
+  return fwe_reset(x);
+}
+
 extern "C" fwe_status fwe_run_timed(fwe_ctx* x, int32_t nsteps, double* ms_stress, double* ms_velocity,
                                     double* ms_wall) {
   double a = 0, b = 0, w = 0;
@@ -555,6 +663,12 @@ extern "C" fwe_status fwe_synchronize(fwe_ctx* x) {
 extern "C" fwe_status fwe_get_step(const fwe_ctx* x, int32_t* it) {
   if (!x || !it) return fail(FWE_ERR_ARG, "NULL argument");
   *it = x->it;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_set_step(fwe_ctx* x, int32_t it) {
+  if (!x || it < 0) return fail(FWE_ERR_ARG, "bad step");
+  x->it = it;
   return FWE_OK;
 }
 

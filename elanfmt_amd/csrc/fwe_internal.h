@@ -60,11 +60,16 @@ struct StepArgs {
   const PointEntry* rec;
   float* traces;            // [nrec][nt]
   int32_t nt;
+  // step index = it + (it_base ? *it_base : 0); a captured graph of G steps
+  // bakes it = 0..G-1 and advances *it_base by G after each replay
   int32_t it;
+  const int32_t* it_base;
 };
 
 hipError_t launch_stress(const StepArgs& a, int mode, int pml, int fs, hipStream_t s);
 hipError_t launch_velocity(const StepArgs& a, int mode, int pml, hipStream_t s);
+// *it_base = g (set) or *it_base += g
+hipError_t launch_advance(int32_t* it_base, int32_t g, int32_t set, hipStream_t s);
 hipError_t launch_copy(const float* src, float* dst, int64_t n, hipStream_t s);
 
 }  // namespace fwe

@@ -126,7 +126,8 @@ __global__ __launch_bounds__(64 * TY) void k_stress(const StepArgs a) {
       float src[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       float src_xz1 = 0.f, src_yz1 = 0.f;  // injections at (i,j,1), for BC1
       if (e1 > e0) {
-        const float amp = (a.it < a.nstf) ? a.stf[(int64_t)b * a.nstf + a.it] : 0.f;
+        const int it = a.it + (a.it_base ? *a.it_base : 0);
+        const float amp = (it < a.nstf) ? a.stf[(int64_t)b * a.nstf + it] : 0.f;
         for (int e = e0; e < e1; ++e) {
           const PointEntry p = a.inj[e];
           if (p.i == i && p.j == j && p.batch == b) {
@@ -236,11 +237,18 @@ __global__ __launch_bounds__(64 * TY) void k_velocity(const StepArgs a) {
       if (uz) Vz[c] = vz;
       for (int r = r0; r < r1; ++r) {
         const PointEntry p = a.rec[r];
-        if (p.i == i && p.j == j && p.k == k && p.batch == b && a.it < a.nt)
-          a.traces[(int64_t)p.slot * a.nt + a.it] = p.comp == 0 ? vx : (p.comp == 1 ? vy : vz);
+        const int it = a.it + (a.it_base ? *a.it_base : 0);
+        if (p.i == i && p.j == j && p.k == k && p.batch == b && it < a.nt)
+          a.traces[(int64_t)p.slot * a.nt + it] = p.comp == 0 ? vx : (p.comp == 1 ? vy : vz);
       }
     }
   }
+}
+
+// Step counter of a captured time-loop graph: one lane advances it after the
+// G steps of one replay (graph nodes run in order on one stream).
+__global__ void k_advance(int32_t* it_base, int32_t g, int32_t set) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *it_base = set ? g : *it_base + g;
 }
 
 // Plain vector copy, the memory-copy probe (Performance/PerfMC.m).
@@ -315,6 +323,11 @@ hipError_t launch_velocity(const StepArgs& a, int mode, int pml, hipStream_t s) 
     case PML_SEP: return go_vel<MODE_ORTHO, PML_SEP>(a, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_advance(int32_t* it_base, int32_t g, int32_t set, hipStream_t s) {
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, it_base, g, set);
+  return hipGetLastError();
 }
 
 hipError_t launch_copy(const float* src, float* dst, int64_t n, hipStream_t s) {

@@ -89,7 +89,12 @@ SIGNATURES = {
     "fwe_run": (C.c_int, [C.c_void_p, C.c_int32]),
     "fwe_run_timed": (C.c_int, [C.c_void_p, C.c_int32, _DP, _DP, _DP]),
     "fwe_synchronize": (C.c_int, [C.c_void_p]),
+    "fwe_set_graph_steps": (C.c_int, [C.c_void_p, C.c_int32]),
+    "fwe_set_tiling": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "fwe_get_tiling": (C.c_int, [C.c_void_p, _IP, _IP]),
+    "fwe_autotune": (C.c_int, [C.c_void_p, C.c_int32, _DP]),
     "fwe_get_step": (C.c_int, [C.c_void_p, _IP]),
+    "fwe_set_step": (C.c_int, [C.c_void_p, C.c_int32]),
     "fwe_get_traces": (C.c_int, [C.c_void_p, _FP]),
     "fwe_get_field": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _FP]),
     "fwe_set_field": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _FP]),
@@ -261,10 +266,29 @@ class Solver:
     def synchronize(self):
         _check(lib().fwe_synchronize(self._h))
 
+    def set_graph_steps(self, g):
+        _check(lib().fwe_set_graph_steps(self._h, g))
+
+    def set_tiling(self, tile_rows, zchunk):
+        _check(lib().fwe_set_tiling(self._h, tile_rows, zchunk))
+
+    def tiling(self):
+        a, b = C.c_int32(), C.c_int32()
+        _check(lib().fwe_get_tiling(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, trial_steps=5):
+        ms = C.c_double()
+        _check(lib().fwe_autotune(self._h, trial_steps, C.byref(ms)))
+        return ms.value
+
     def step(self):
         it = C.c_int32()
         _check(lib().fwe_get_step(self._h, C.byref(it)))
         return it.value
+
+    def set_step(self, it):
+        _check(lib().fwe_set_step(self._h, it))
 
     # results
     def traces(self):

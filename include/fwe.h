@@ -199,8 +199,23 @@ fwe_status fwe_run_timed(fwe_ctx* ctx, int32_t nsteps, double* ms_stress,
                          double* ms_velocity, double* ms_wall);
 /* Block until the context stream is idle. */
 fwe_status fwe_synchronize(fwe_ctx* ctx);
+/* Capture g steps of the loop (2g kernel launches + a step-counter advance)
+ * in one HIP graph that fwe_run replays; 0 = launch every kernel directly.
+ * Replaces the per-kernel launch + cudaDeviceSynchronize of OK4.cu:485-505.
+ * fwe_run_timed always launches directly. */
+fwe_status fwe_set_graph_steps(fwe_ctx* ctx, int32_t g);
+/* Workgroup tile: 64 x tile_rows cells (tile_rows 2, 4 or 8), marching
+ * zchunk planes (3..nz).  Replaces the fixed 32x2x8 block of OK4.cu:26-31. */
+fwe_status fwe_set_tiling(fwe_ctx* ctx, int32_t tile_rows, int32_t zchunk);
+fwe_status fwe_get_tiling(const fwe_ctx* ctx, int32_t* tile_rows, int32_t* zchunk);
+/* Time trial_steps steps for each tiling candidate, keep the fastest, and
+ * reset the context (fields, traces, step counter). */
+fwe_status fwe_autotune(fwe_ctx* ctx, int32_t trial_steps, double* best_ms_per_step);
 /* Steps advanced since create/reset. */
 fwe_status fwe_get_step(const fwe_ctx* ctx, int32_t* it);
+/* Set the step counter (restart from saved fields: the reference's unused
+ * read_data hook, OK4.cu:166-175, only restores fields). */
+fwe_status fwe_set_step(fwe_ctx* ctx, int32_t it);
 
 /* ---- results ------------------------------------------------------------ */
 /* Receiver traces, out[r*nt + it] (Rec..Rec_y2 of OK4.cu:454-459,525-529).

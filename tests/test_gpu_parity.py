@@ -51,3 +51,208 @@ def test_ortho_small_separable(gpu):
     assert rel_l2(g, r) < TOL
     for f in range(9):
         assert rel_l2(gf[f], rf[f]) < TOL, f
+
+
+def test_iso_free_surface_source_near_top(gpu):
+    """BC1 with injections in planes 1 and 2 (OK4 order: inject, then BC1)."""
+    from elanfmt_amd import workloads as W
+
+    wl = W.iso_homogeneous(n=36, L=5, nt=60, rec_depth=2, nrec_side=3, free_surface=1)
+    c = 18
+    wl.injections = [(0, c, c, 1, 0, 3e14), (4, c, c, 1, 0, 2e14), (4, c + 1, c, 2, 0, -1e14),
+                     (5, c, c + 2, 2, 0, 5e13), (2, c, c, 0, 0, 1e14), (3, c, c, 2, 0, 1e14)]
+    wl.receivers = [(q, c + a, c + b, 1, 0) for q in range(3) for a, b in ((0, 0), (3, 1), (-4, 2))]
+    g, r, gf, rf = _run_both(wl)
+    assert np.abs(r).max() > 0
+    assert rel_l2(g, r) < TOL
+    for f in range(9):
+        assert rel_l2(gf[f], rf[f]) < TOL, f
+
+
+@pytest.mark.parametrize("pml", [0, 2, 3])
+def test_ortho_free_surface_all_damping_forms(gpu, pml):
+    """ANISO order (BC1, then inject) with weights.dat, separable and no
+    damping; injections in planes 0..2 exercise the BC1/injection order."""
+    from elanfmt_amd import fwe, workloads as W
+
+    wl = W.ortho_layered(nx=70, ny=36, nz=30, L=5, nt=60, nrec_side=3,
+                         pml=fwe.PML_WEIGHTS3D if pml == 2 else fwe.PML_SEPARABLE)
+    if pml == 0:
+        wl.pml, wl.prof, wl.w3 = fwe.PML_NONE, None, None
+    wl.free_surface = 1
+    c = 17
+    wl.injections += [(4, c, c, 1, 0, 2e14), (5, c, c, 1, 0, -2e14), (2, c, c, 0, 0, 1e14),
+                      (4, c + 3, c, 2, 0, 1e14)]
+    g, r, gf, rf = _run_both(wl)
+    assert rel_l2(g, r) < TOL
+    for f in range(9):
+        assert rel_l2(gf[f], rf[f]) < TOL, f
+
+
+def test_batch_members_equal_single_runs(gpu):
+    """nbatch wavefields sharing one medium give exactly the traces of
+    independent single-wavefield runs."""
+    from elanfmt_amd import fwe, workloads as W
+
+    wl = W.ortho_layered(nx=66, ny=34, nz=28, L=5, nt=50, nrec_side=2)
+    ms = [[1e15, 0, 0, 0, 0, 0], [0, 0, 0, 3e14, -2e14, 0], [1e14, 2e14, -3e14, 0, 0, 5e14]]
+    singles = []
+    for m in ms:
+        wl.nbatch = 1
+        wl.injections = fwe.mt_injections(fwe.SRC_LAYOUT_OK4, 30, 15, 12, m)
+        rec0 = [t for t in wl.receivers if t[4] == 0]
+        wl.receivers = rec0
+        wl.stf = wl.stf[:1]
+        s = W.make_solver(wl)
+        s.run(wl.nt)
+        singles.append(s.traces())
+        s.close()
+    wl.nbatch = len(ms)
+    wl.injections = sum((fwe.mt_injections(fwe.SRC_LAYOUT_OK4, 30, 15, 12, m, batch=b) for b, m in enumerate(ms)), [])
+    wl.receivers = [t[:4] + (b,) for b in range(len(ms)) for t in rec0]
+    wl.stf = wl.stf * len(ms)
+    s = W.make_solver(wl)
+    s.run(wl.nt)
+    tb = s.traces()
+    s.close()
+    n0 = len(rec0)
+    for b in range(len(ms)):
+        assert np.array_equal(tb[b * n0:(b + 1) * n0], singles[b]), b
+
+
+def test_restart_and_field_roundtrip(gpu):
+    """Saving all nine fields after N steps, resetting, restoring them and
+    the step counter, and running N more equals 2N steps bit for bit."""
+    from elanfmt_amd import workloads as W
+
+    wl = W.iso_homogeneous(n=32, L=4, nt=40, rec_depth=2, nrec_side=2)
+    a = W.make_solver(wl)
+    a.run(40)
+    fa = [a.get_field(f) for f in range(9)]
+    a.close()
+    b = W.make_solver(wl)
+    b.run(20)
+    saved = [b.get_field(f) for f in range(9)]
+    b.reset()
+    assert b.step() == 0 and all(np.all(b.get_field(f) == 0) for f in range(9))
+    for f in range(9):
+        b.set_field(f, saved[f])
+        assert np.array_equal(b.get_field(f), saved[f])
+    b.set_step(20)
+    b.run(20)
+    for f in range(9):
+        assert np.array_equal(b.get_field(f), fa[f]), f
+    b.close()
+
+
+def test_error_statuses(gpu):
+    from elanfmt_amd import fwe
+
+    s = fwe.Solver(20, 20, 20, 1.0, 1.0, 1.0, 1e-3, 10, mode=fwe.MODE_ORTHO, pml=fwe.PML_NONE)
+    with pytest.raises(fwe.FweError) as e:
+        s.run(1)
+    assert e.value.status == fwe.ERR_STATE
+    with pytest.raises(fwe.FweError) as e:
+        s.set_receivers([(0, 20, 3, 3, 0)])
+    assert e.value.status == fwe.ERR_ARG
+    with pytest.raises(fwe.FweError) as e:
+        s.set_injections([(6, 3, 3, 3, 0, 1.0)])
+    assert e.value.status == fwe.ERR_ARG
+    s.close()
+    with pytest.raises(fwe.FweError) as e:
+        fwe.Solver(20, 20, 20, 1.0, 1.0, 1.0, 1e-3, 10, mode=fwe.MODE_ISO, pml=fwe.PML_WEIGHTS3D)
+    assert e.value.status == fwe.ERR_UNSUPPORTED
+
+
+def test_c0_full_size_parity(gpu):
+    """The 128^3 isotropic configuration, 600 steps: receiver traces within
+    1e-5 relative L2 of the FP64 oracle."""
+    from elanfmt_amd import workloads as W
+
+    wl = W.iso_homogeneous()
+    g, r, gf, rf = _run_both(wl)
+    err = rel_l2(g, r)
+    print(f"C0 128^3 x 600 rel-L2 traces = {err:.3e}", flush=True)
+    assert err < TOL
+
+
+def test_legacy_drop_in_binary(gpu, tmp_path):
+    """fwe_run reads the legacy .dat set in its working directory and writes
+    0_infos.inf and 0_0_*.res like the reference's a.exe."""
+    import subprocess
+
+    from elanfmt_amd import fwe, legacy, workloads as W
+    from oracle import Oracle
+
+    wl = W.iso_homogeneous(n=40, L=6, nt=70, free_surface=1)
+    m = np.array(wl.info["mt"], np.float32)
+    legacy.write_inputs(str(tmp_path), wl, mts=m)
+    xs = 18
+    wl.injections = fwe.mt_injections(fwe.SRC_LAYOUT_OK4, xs, xs, xs, m)
+    recs = [(2, 25, 21, 1, 0), (0, 25, 21, 1, 0), (1, 25, 24, 1, 0), (1, 25, 23, 1, 0)]
+    wl.receivers = recs
+    spec = ",".join(f"{'xyz'[c]}:{i}:{j}:{k}:{n}" for (c, i, j, k, _), n in
+                    zip(recs, ["Rec", "Rec_x", "Rec_y", "Rec_y2"]))
+    r = subprocess.run([fwe.RUN_PATH, "nx=40", "ny=40", "nz=40", "nt=70", "mode=iso", "fs=1",
+                        f"xsrc={xs}", f"ysrc={xs}", f"zsrc={xs}", f"rec={spec}", "NPARS1=10", "NPARS2=13",
+                        "NPARS3=12"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    print(r.stdout[-500:], r.stderr[-500:], flush=True)
+    assert r.returncode == 0
+    assert legacy.read_infos(str(tmp_path)) == (4, 40, 40, 40, 1, 1)
+    o = Oracle(40, 40, 40, wl.dx, wl.dy, wl.dz, np.float32(wl.dt), 70, mode=0, pml=1, free_surface=1)
+    W.apply(o, wl)
+    o.run(70)
+    ref = o.traces()
+    got = np.stack([legacy.read_res(str(tmp_path), n) for n in ["Rec", "Rec_x", "Rec_y", "Rec_y2"]])
+    assert rel_l2(got, ref) < TOL
+    assert np.array_equal(legacy.read_res(str(tmp_path), "Src"), wl.stf[0])
+    vx = legacy.read_res(str(tmp_path), "Vx", (41, 40, 40))
+    assert rel_l2(vx, o.get_field(0)) < TOL
+
+
+def test_graph_replay_equals_direct_launches(gpu):
+    """The captured time loop (kernels reading the step index from device
+    memory) gives the same traces and fields bit for bit, including a
+    remainder shorter than one replay and a second call."""
+    from elanfmt_amd import workloads as W
+
+    wl = W.iso_homogeneous(n=36, L=5, nt=75, rec_depth=2, nrec_side=2)
+    a = W.make_solver(wl)
+    a.run(75)
+    ta, fa = a.traces(), [a.get_field(f) for f in range(9)]
+    a.close()
+    b = W.make_solver(wl)
+    b.set_graph_steps(16)
+    b.run(40)   # 2 replays + 8 direct
+    b.run(35)   # 2 replays + 3 direct
+    assert b.step() == 75
+    tb, fb = b.traces(), [b.get_field(f) for f in range(9)]
+    b.close()
+    assert np.array_equal(ta, tb)
+    for f in range(9):
+        assert np.array_equal(fa[f], fb[f]), f
+
+
+def test_autotune_and_tilings_agree(gpu):
+    """Every tiling gives the same result bit for bit; autotune picks one of
+    them and leaves the context reset."""
+    from elanfmt_amd import workloads as W
+
+    wl = W.ortho_layered(nx=70, ny=30, nz=26, L=4, nt=30, nrec_side=2)
+    ref = None
+    for ty, kc in ((4, 8), (2, 3), (8, 26), (4, 5)):
+        s = W.make_solver(wl)
+        s.set_tiling(ty, kc)
+        assert s.tiling() == (ty, kc)
+        s.run(30)
+        t = s.traces()
+        s.close()
+        if ref is None:
+            ref = t
+        assert np.array_equal(t, ref), (ty, kc)
+    s = W.make_solver(wl)
+    ms = s.autotune(2)
+    assert ms > 0 and s.step() == 0
+    s.run(30)
+    assert np.array_equal(s.traces(), ref)
+    s.close()
