@@ -2,16 +2,15 @@
 
 Workload (default): the layered orthorhombic 384x384x192 FP32 box with
 24-cell separable damping on all faces, a lune-uniform moment tensor,
-Ricker 3 Hz and 64 three-component receivers; one step = one stress pass +
-one velocity pass over all 28.3M cells.  Inputs are resident in HBM before
-the timed region.  Prints one JSON line.
+Ricker 3 Hz and 64 three-component receivers; one step = one full
+velocity-stress update of all 28.3M cells.  Inputs are resident in HBM
+before the timed region.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps 3000] [--warmup 50] [--config 1]
-                  [--cpu-steps 20] [--no-cpu]
+                  [--kernel auto|two_pass|fused] [--cpu-steps 300] [--no-cpu]
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -22,14 +21,15 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 
 
-def _traffic(kernel, workload):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, scaled by the copy-kernel calibration), from profiles/."""
+def _traffic(kernel_name, workload):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, each scaled by the same counter on the copy probe), kept in
+    profiles/traffic.json."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p) as f:
             d = json.load(f)
-        return d[workload][kernel]["bytes_per_launch"]
+        return d[workload][kernel_name]["bytes_per_launch"]
     except Exception:
         return None
 
@@ -58,12 +58,14 @@ def main():
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--config", type=int, default=1)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--kernel", default="auto", choices=["auto", "two_pass", "fused"])
+    ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(a z-slab decomposition is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "(the z-slab decomposition it would use is described in DESIGN.md); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
 
     from elanfmt_amd import fwe, workloads as W
@@ -72,30 +74,50 @@ def main():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
     steps = args.steps
-    build = lambda: W.config(args.config, nt=max(steps, 1))
-    wl = build()
+    wl = W.config(args.config, nt=max(steps, 1))
+    kind = {"auto": fwe.KERNEL_AUTO, "two_pass": fwe.KERNEL_TWO_PASS, "fused": fwe.KERNEL_FUSED}[args.kernel]
+    # copy probe: achievable stream bandwidth on this device (1 GiB copy)
+    copy_ms = fwe.copy_probe(1 << 30, reps=10)
+    copy_gbs = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
+
     s = W.make_solver(wl)
+    s.set_kernel(kind)
+    fused = s.kernel() == fwe.KERNEL_FUSED
     s.run(args.warmup)
     s.synchronize()
     s.reset()
     s.synchronize()
     t0 = time.perf_counter()
-    ms_s, ms_v, ms_w = s.run_timed(steps)
+    ms_a, ms_b, ms_w = s.run_timed(steps)
     s.synchronize()
     el = time.perf_counter() - t0
     tr = s.traces()
     finite = bool(__import__("numpy").isfinite(tr).all())
+    tiling = s.tiling()
     s.close()
 
     cells = wl.cells
-    bs, bv = W.alg_bytes(wl.mode, wl.pml, wl.nbatch)
+    ba, bb = W.alg_bytes(wl.mode, wl.pml, wl.nbatch, fused=fused)
     gcells = cells * wl.nbatch * steps / el / 1e9
-    dom, ms_dom, b_dom = ("stress", ms_s, bs) if ms_s >= ms_v else ("velocity", ms_v, bv)
+    if fused:
+        kernels = {"fused": {"ms_per_step": round(ms_a / steps, 5), "alg_bytes_per_cell": ba,
+                             "GBs": round(ba * cells / (ms_a / steps * 1e-3) / 1e9, 1)}}
+        dom, ms_dom, b_dom = "fused", ms_a, ba
+        kname = f"k_fused<{wl.mode}, {wl.pml}, {tiling[0]}>"
+    else:
+        kernels = {
+            "stress": {"ms_per_launch": round(ms_a / steps, 5), "alg_bytes_per_cell": ba,
+                       "GBs": round(ba * cells / (ms_a / steps * 1e-3) / 1e9, 1)},
+            "velocity": {"ms_per_launch": round(ms_b / steps, 5), "alg_bytes_per_cell": bb,
+                         "GBs": round(bb * cells / (ms_b / steps * 1e-3) / 1e9, 1)},
+        }
+        dom, ms_dom, b_dom = ("stress", ms_a, ba) if ms_a >= ms_b else ("velocity", ms_b, bb)
+        kname = (f"k_stress<{wl.mode}, {wl.pml}, {'true' if wl.free_surface else 'false'}, {tiling[0]}>"
+                 if dom == "stress" else f"k_velocity<{wl.mode}, {wl.pml}, {tiling[0]}>")
     per_launch_s = ms_dom / steps * 1e-3
     ach = b_dom * cells / per_launch_s / 1e9
-    step_gbs = (bs + bv) * cells * steps / el / 1e9
-    wname = f"{wl.name}"
-    traffic = _traffic(dom, wname)
+    step_gbs = (ba + bb) * cells * steps / el / 1e9
+    wname = wl.name
     out = {
         "metric": "Gcell-updates/s + % of 8 TB/s HBM; batched-6-MT vs 6 plain runs at equal error",
         "value": round(gcells, 3),
@@ -110,21 +132,19 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"{wname}: layered orthorhombic FP32, {wl.info.get('pml_cells')}-cell separable damping, "
-                        f"lune MT, Ricker {wl.info.get('f0')} Hz, {len(wl.receivers)} receiver traces"
+            "workload": (f"{wname}: layered orthorhombic FP32, {wl.info.get('pml_cells')}-cell separable damping, "
+                         f"lune MT, Ricker {wl.info.get('f0')} Hz, {len(wl.receivers)} receiver traces")
             if wl.mode == fwe.MODE_ORTHO else f"{wname}: isotropic homogeneous FP32",
             "grid": [wl.nx, wl.ny, wl.nz], "wavefields": wl.nbatch, "nt": wl.nt,
+            "kernel": "fused single-pass" if fused else "two-pass", "tiling": list(tiling),
             "parallelism": "single GPU",
         },
         "hbm_pct_step": round(100 * step_gbs / PEAK_HBM_GBS, 2),
-        "kernels": {
-            "stress": {"ms_per_launch": round(ms_s / steps, 5), "alg_bytes_per_cell": bs,
-                       "GBs": round(bs * cells / (ms_s / steps * 1e-3) / 1e9, 1)},
-            "velocity": {"ms_per_launch": round(ms_v / steps, 5), "alg_bytes_per_cell": bv,
-                         "GBs": round(bv * cells / (ms_v / steps * 1e-3) / 1e9, 1)},
-        },
+        "kernels": kernels,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": traffic},
+                     "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 4),
+                     "traffic": _traffic(kname, wname)},
+        "copy_probe_GBs": round(copy_gbs, 1),
         "traces_finite": finite,
     }
     if not args.no_cpu and args.cpu_steps > 0:

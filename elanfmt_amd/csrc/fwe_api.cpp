@@ -64,7 +64,15 @@ extern "C" fwe_status fwe_device_count(int32_t* count) {
 struct fwe_ctx {
   fwe_config cfg;
   int64_t sy = 0, sz = 0, nelem = 0;
-  float* fields = nullptr;  // nb * 9 * nelem
+  // fields (nb * 9 * nelem): fbuf[cur] holds the current time level; the
+  // single-pass kernel writes the next level into fbuf[cur^1]
+  float* fbuf[2] = {nullptr, nullptr};
+  int cur = 0;
+  int kernel = FWE_KERNEL_TWO_PASS;
+  // per-tile injection lists grown by one cell (halo values of the
+  // single-pass kernel)
+  int32_t* injx_off = nullptr;
+  PointEntry* injx = nullptr;
   float* medium = nullptr;  // 12 * nelem (ortho) or 2 * nelem (iso)
   float* w3 = nullptr;      // nelem (W3D)
   float* prof = nullptr;    // wx wy wz whx why whz (profile / taper tables)
@@ -88,13 +96,16 @@ struct fwe_ctx {
   int32_t* it_dev = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_steps = 0;
+  int graph_from = 0;  // time-level buffer the captured graph starts from
 };
 
 static void drop_graph(fwe_ctx* x) {
   if (x->gexec) hipGraphExecDestroy(x->gexec);
   x->gexec = nullptr;
-  x->graph_steps = 0;
 }
+
+static float* fields(const fwe_ctx* x) { return x->fbuf[x->cur]; }
+static size_t field_bytes(const fwe_ctx* x) { return sizeof(float) * (size_t)x->nelem * 9 * x->cfg.nbatch; }
 
 static int64_t round_up(int64_t a, int64_t m) { return (a + m - 1) / m * m; }
 
@@ -205,7 +216,7 @@ extern "C" fwe_status fwe_create(const fwe_config* cfg, fwe_ctx** out) {
   choose_tiling(x);
   const size_t fbytes = sizeof(float) * (size_t)x->nelem * 9 * cfg->nbatch;
   const int nmed = cfg->mode == FWE_MODE_ORTHO ? 12 : 2;
-  hipError_t e = hipMalloc(&x->fields, fbytes);
+  hipError_t e = hipMalloc(&x->fbuf[0], fbytes);
   if (e == hipSuccess) e = hipMalloc(&x->medium, sizeof(float) * (size_t)x->nelem * nmed);
   if (e == hipSuccess && cfg->pml == FWE_PML_WEIGHTS3D) e = hipMalloc(&x->w3, sizeof(float) * (size_t)x->nelem);
   if (e == hipSuccess) {
@@ -214,7 +225,7 @@ extern "C" fwe_status fwe_create(const fwe_config* cfg, fwe_ctx** out) {
   }
   if (e == hipSuccess) e = hipMalloc(&x->it_dev, sizeof(int32_t));
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipMemsetAsync(x->fields, 0, fbytes, x->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(x->fbuf[0], 0, fbytes, x->stream);
   if (e == hipSuccess) e = hipMemsetAsync(x->medium, 0, sizeof(float) * (size_t)x->nelem * nmed, x->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
   if (e != hipSuccess) {
@@ -236,8 +247,9 @@ extern "C" fwe_status fwe_destroy(fwe_ctx* x) {
   drop_graph(x);
   hipFree(x->it_dev);
   for (auto ev : x->events) hipEventDestroy(ev);
-  hipFree(x->fields); hipFree(x->medium); hipFree(x->w3); hipFree(x->prof);
+  hipFree(x->fbuf[0]); hipFree(x->fbuf[1]); hipFree(x->medium); hipFree(x->w3); hipFree(x->prof);
   hipFree(x->stf); hipFree(x->traces); hipFree(x->inj_off); hipFree(x->inj);
+  hipFree(x->injx_off); hipFree(x->injx);
   hipFree(x->rec_off); hipFree(x->rec);
   if (x->stream) hipStreamDestroy(x->stream);
   delete x;
@@ -372,15 +384,21 @@ extern "C" fwe_status fwe_set_stf(fwe_ctx* x, int32_t batch, const float* stf, i
 }
 
 // sort point entries by xy tile and upload with their offsets
-static fwe_status upload_points(fwe_ctx* x, std::vector<PointEntry> v, int32_t** off, PointEntry** dev) {
+// (tile, entry) pairs -> per-tile CSR lists on the device
+static fwe_status upload_tiled(fwe_ctx* x, std::vector<std::pair<int, PointEntry>> tv, int32_t** off,
+                               PointEntry** dev) {
   drop_graph(x);
   const int ntile = x->ntx * x->nty;
   std::vector<int32_t> counts(ntile + 1, 0);
-  auto tile_of = [&](const PointEntry& p) { return p.i / fwe::TILE_X + (p.j / x->ty) * x->ntx; };
-  std::stable_sort(v.begin(), v.end(), [&](const PointEntry& a, const PointEntry& b) {
-    return tile_of(a) < tile_of(b);
-  });
-  for (auto& p : v) counts[tile_of(p) + 1]++;
+  std::stable_sort(tv.begin(), tv.end(),
+                   [](const std::pair<int, PointEntry>& a, const std::pair<int, PointEntry>& b) {
+                     return a.first < b.first;
+                   });
+  std::vector<PointEntry> v;
+  for (auto& p : tv) {
+    counts[p.first + 1]++;
+    v.push_back(p.second);
+  }
   for (int t = 0; t < ntile; ++t) counts[t + 1] += counts[t];
   HIPCHK(hipSetDevice(x->cfg.device));
   HIPCHK(hipStreamSynchronize(x->stream));
@@ -394,6 +412,30 @@ static fwe_status upload_points(fwe_ctx* x, std::vector<PointEntry> v, int32_t**
   if (!v.empty())
     HIPCHK(hipMemcpy(*dev, v.data(), sizeof(PointEntry) * v.size(), hipMemcpyHostToDevice));
   return FWE_OK;
+}
+
+// each entry in the list of the tile that owns its cell
+static fwe_status upload_points(fwe_ctx* x, const std::vector<PointEntry>& v, int32_t** off, PointEntry** dev) {
+  std::vector<std::pair<int, PointEntry>> tv;
+  for (auto& p : v) tv.push_back({p.i / fwe::TILE_X + (p.j / x->ty) * x->ntx, p});
+  return upload_tiled(x, tv, off, dev);
+}
+
+// each injection in the list of every tile whose one-cell-grown footprint
+// holds it (the single-pass kernel recomputes those halo stresses)
+static fwe_status upload_grown(fwe_ctx* x) {
+  std::vector<std::pair<int, PointEntry>> tv;
+  for (auto& p : x->inj_host) {
+    for (int tx = 0; tx < x->ntx; ++tx) {
+      const int lo = tx * fwe::TILE_X - 1, hi = tx * fwe::TILE_X + fwe::TILE_X;
+      if (p.i < lo || p.i > hi) continue;
+      for (int ty = 0; ty < x->nty; ++ty) {
+        const int jl = ty * x->ty - 1, jh = ty * x->ty + x->ty;
+        if (p.j >= jl && p.j <= jh) tv.push_back({tx + ty * x->ntx, p});
+      }
+    }
+  }
+  return upload_tiled(x, tv, &x->injx_off, &x->injx);
 }
 
 extern "C" fwe_status fwe_set_injections(fwe_ctx* x, int32_t n, const fwe_injection* inj) {
@@ -413,7 +455,7 @@ extern "C" fwe_status fwe_set_injections(fwe_ctx* x, int32_t n, const fwe_inject
   if (st) return st;
   x->inj_host = v;
   x->ninj = n;
-  return FWE_OK;
+  return upload_grown(x);
 }
 
 extern "C" fwe_status fwe_set_receivers(fwe_ctx* x, int32_t n, const fwe_receiver* rec) {
@@ -462,14 +504,14 @@ extern "C" fwe_status fwe_mt_injections(int32_t layout, int32_t xs, int32_t ys, 
 // ---------------------------------------------------------------------------
 // time loop
 // ---------------------------------------------------------------------------
-static StepArgs make_args(fwe_ctx* x) {
+static StepArgs make_args(fwe_ctx* x, int from) {
   const fwe_config& c = x->cfg;
   StepArgs a;
   memset(&a, 0, sizeof(a));
   a.nx = c.nx; a.ny = c.ny; a.nz = c.nz; a.nb = c.nbatch;
   a.sy = x->sy; a.sz = x->sz; a.fstride = 9 * x->nelem;
-  for (int q = 0; q < 3; ++q) a.V[q] = x->fields + (int64_t)q * x->nelem;
-  for (int q = 0; q < 6; ++q) a.S[q] = x->fields + (int64_t)(3 + q) * x->nelem;
+  for (int q = 0; q < 3; ++q) a.V[q] = x->fbuf[from] + (int64_t)q * x->nelem;
+  for (int q = 0; q < 6; ++q) a.S[q] = x->fbuf[from] + (int64_t)(3 + q) * x->nelem;
   if (c.mode == FWE_MODE_ORTHO) {
     for (int q = 0; q < 9; ++q) a.C[q] = x->medium + (int64_t)q * x->nelem;
     for (int q = 0; q < 3; ++q) a.B[q] = x->medium + (int64_t)(9 + q) * x->nelem;
@@ -491,6 +533,63 @@ static StepArgs make_args(fwe_ctx* x) {
   return a;
 }
 
+// single-pass arguments for wavefield b, time level in fbuf[from]
+static fwe::FusedArgs make_fused(fwe_ctx* x, int from, int b) {
+  const StepArgs s = make_args(x, from);
+  fwe::FusedArgs f;
+  memset(&f, 0, sizeof(f));
+  f.nx = s.nx; f.ny = s.ny; f.nz = s.nz; f.sy = s.sy; f.sz = s.sz;
+  const int64_t fo = (int64_t)b * 9 * x->nelem;
+  for (int q = 0; q < 3; ++q) {
+    f.Vi[q] = x->fbuf[from] + fo + (int64_t)q * x->nelem;
+    f.Vo[q] = x->fbuf[from ^ 1] + fo + (int64_t)q * x->nelem;
+  }
+  for (int q = 0; q < 6; ++q) {
+    f.Si[q] = x->fbuf[from] + fo + (int64_t)(3 + q) * x->nelem;
+    f.So[q] = x->fbuf[from ^ 1] + fo + (int64_t)(3 + q) * x->nelem;
+  }
+  for (int q = 0; q < 9; ++q) f.C[q] = s.C[q];
+  for (int q = 0; q < 3; ++q) f.B[q] = s.B[q];
+  f.c66dt = s.c66dt; f.bxdt = s.bxdt; f.bydt = s.bydt; f.bzdt = s.bzdt;
+  f.wx = s.wx; f.wy = s.wy; f.wz = s.wz; f.whx = s.whx; f.why = s.why; f.whz = s.whz; f.w3 = s.w3;
+  f.rdx = s.rdx; f.rdy = s.rdy; f.rdz = s.rdz;
+  f.ty_rows = s.ty_rows; f.ntx = s.ntx; f.nty = s.nty; f.nkc = s.nkc; f.kchunk = s.kchunk;
+  f.inj_off = x->injx_off; f.inj = x->injx;
+  f.stf = x->stf ? x->stf + (int64_t)b * x->nstf : nullptr;
+  f.nstf = x->nstf;
+  f.batch = b;
+  f.rec_off = s.rec_off; f.rec = s.rec; f.traces = s.traces; f.nt = s.nt;
+  return f;
+}
+
+static bool use_fused(const fwe_ctx* x) {
+  return x->kernel == FWE_KERNEL_FUSED || (x->kernel == FWE_KERNEL_AUTO && !x->cfg.free_surface);
+}
+
+// one step from time level fbuf[from]; step index it (+ *it_base); `mid`
+// (optional) is recorded between the stress and velocity passes
+static hipError_t step_launch(fwe_ctx* x, int from, int it, const int32_t* it_base, hipEvent_t mid) {
+  const fwe_config& c = x->cfg;
+  hipError_t e = hipSuccess;
+  if (use_fused(x)) {
+    for (int b = 0; b < c.nbatch && e == hipSuccess; ++b) {
+      fwe::FusedArgs f = make_fused(x, from, b);
+      f.it = it;
+      f.it_base = it_base;
+      e = fwe::launch_fused(f, c.mode, c.pml, x->stream);
+    }
+    if (e == hipSuccess && mid) e = hipEventRecord(mid, x->stream);
+  } else {
+    StepArgs a = make_args(x, from);
+    a.it = it;
+    a.it_base = it_base;
+    e = fwe::launch_stress(a, c.mode, c.pml, c.free_surface, x->stream);
+    if (e == hipSuccess && mid) e = hipEventRecord(mid, x->stream);
+    if (e == hipSuccess) e = fwe::launch_velocity(a, c.mode, c.pml, x->stream);
+  }
+  return e;
+}
+
 static fwe_status ready(fwe_ctx* x) {
   if (!x->have_medium) return fail(FWE_ERR_STATE, "medium not set");
   if (x->cfg.pml != FWE_PML_NONE && !x->have_damp) return fail(FWE_ERR_STATE, "damping not set");
@@ -500,7 +599,9 @@ static fwe_status ready(fwe_ctx* x) {
 extern "C" fwe_status fwe_reset(fwe_ctx* x) {
   if (!x) return fail(FWE_ERR_ARG, "NULL ctx");
   HIPCHK(hipSetDevice(x->cfg.device));
-  HIPCHK(hipMemsetAsync(x->fields, 0, sizeof(float) * (size_t)x->nelem * 9 * x->cfg.nbatch, x->stream));
+  for (int b = 0; b < 2; ++b)
+    if (x->fbuf[b]) HIPCHK(hipMemsetAsync(x->fbuf[b], 0, field_bytes(x), x->stream));
+  x->cur = 0;
   if (x->traces)
     HIPCHK(hipMemsetAsync(x->traces, 0, sizeof(float) * std::max<size_t>(1, (size_t)x->nrec * std::max(1, x->cfg.nt)), x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
@@ -513,6 +614,14 @@ static fwe_status run_impl(fwe_ctx* x, int32_t nsteps, double* ms_s, double* ms_
   fwe_status st = ready(x);
   if (st) return st;
   HIPCHK(hipSetDevice(x->cfg.device));
+  const bool fused = use_fused(x);
+  if (fused && x->cfg.free_surface)
+    return fail(FWE_ERR_UNSUPPORTED, "the single-pass kernel has no free surface; use the two-pass kernels");
+  if (fused && !x->fbuf[1]) {
+    // second time level, starting as a copy so cells no kernel updates agree
+    HIPCHK(hipMalloc(&x->fbuf[1], field_bytes(x)));
+    HIPCHK(hipMemcpyAsync(x->fbuf[1], x->fbuf[x->cur], field_bytes(x), hipMemcpyDeviceToDevice, x->stream));
+  }
   const bool timed = ms_s || ms_v || ms_w;
   if (timed && (int)x->events.size() < 2 * nsteps + 1) {
     while ((int)x->events.size() < 2 * nsteps + 1) {
@@ -521,23 +630,23 @@ static fwe_status run_impl(fwe_ctx* x, int32_t nsteps, double* ms_s, double* ms_
       x->events.push_back(ev);
     }
   }
-  StepArgs a = make_args(x);
-  const fwe_config& c = x->cfg;
-  if (!timed && x->graph_steps > 0 && nsteps >= x->graph_steps) {
-    // replay the captured G-step graph; the kernels read the step index
-    // from it_dev, which each replay advances by G
+  const int flip = fused ? 1 : 0;
+  int g = x->graph_steps;
+  if (fused && (g & 1)) ++g;  // a replay must return to the buffer it started from
+  if (!timed && g > 0 && nsteps >= g) {
+    // replay a captured G-step graph; the kernels read the step index from
+    // it_dev, which each replay advances by G
+    if (x->gexec && x->graph_from != x->cur) drop_graph(x);
     if (!x->gexec) {
-      StepArgs g = a;
-      g.it_base = x->it_dev;
       hipGraph_t graph = nullptr;
       HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
       hipError_t e = hipSuccess;
-      for (int s = 0; s < x->graph_steps && e == hipSuccess; ++s) {
-        g.it = s;
-        e = fwe::launch_stress(g, c.mode, c.pml, c.free_surface, x->stream);
-        if (e == hipSuccess) e = fwe::launch_velocity(g, c.mode, c.pml, x->stream);
+      int from = x->cur;
+      for (int s = 0; s < g && e == hipSuccess; ++s) {
+        e = step_launch(x, from, s, x->it_dev, nullptr);
+        from ^= flip;
       }
-      if (e == hipSuccess) e = fwe::launch_advance(x->it_dev, x->graph_steps, 0, x->stream);
+      if (e == hipSuccess) e = fwe::launch_advance(x->it_dev, g, 0, x->stream);
       hipError_t e2 = hipStreamEndCapture(x->stream, &graph);
       if (e == hipSuccess) e = e2;
       if (e == hipSuccess) e = hipGraphInstantiate(&x->gexec, graph, nullptr, nullptr, 0);
@@ -546,21 +655,20 @@ static fwe_status run_impl(fwe_ctx* x, int32_t nsteps, double* ms_s, double* ms_
         x->gexec = nullptr;
         return fail(FWE_ERR_HIP, "time-loop graph capture: %s", hipGetErrorString(e));
       }
+      x->graph_from = x->cur;
     }
     HIPCHK(fwe::launch_advance(x->it_dev, x->it, 1, x->stream));
-    while (nsteps >= x->graph_steps) {
+    while (nsteps >= g) {
       HIPCHK(hipGraphLaunch(x->gexec, x->stream));
-      x->it += x->graph_steps;
-      nsteps -= x->graph_steps;
+      x->it += g;
+      nsteps -= g;
     }
   }
   if (timed) HIPCHK(hipEventRecord(x->events[0], x->stream));
   for (int s = 0; s < nsteps; ++s) {
-    a.it = x->it;
-    HIPCHK(fwe::launch_stress(a, c.mode, c.pml, c.free_surface, x->stream));
-    if (timed) HIPCHK(hipEventRecord(x->events[2 * s + 1], x->stream));
-    HIPCHK(fwe::launch_velocity(a, c.mode, c.pml, x->stream));
+    HIPCHK(step_launch(x, x->cur, x->it, nullptr, timed ? x->events[2 * s + 1] : nullptr));
     if (timed) HIPCHK(hipEventRecord(x->events[2 * s + 2], x->stream));
+    x->cur ^= flip;
     x->it++;
   }
   if (timed) {
@@ -602,7 +710,25 @@ static fwe_status apply_tiling(fwe_ctx* x, int ty, int kchunk) {
   x->nkc = (c.nz + kchunk - 1) / kchunk;
   fwe_status st = upload_points(x, x->inj_host, &x->inj_off, &x->inj);
   if (st) return st;
+  if ((st = upload_grown(x))) return st;
   return upload_points(x, x->rec_host, &x->rec_off, &x->rec);
+}
+
+extern "C" fwe_status fwe_set_kernel(fwe_ctx* x, int32_t kind) {
+  if (!x) return fail(FWE_ERR_ARG, "NULL ctx");
+  if (kind != FWE_KERNEL_AUTO && kind != FWE_KERNEL_TWO_PASS && kind != FWE_KERNEL_FUSED)
+    return fail(FWE_ERR_ARG, "bad kernel kind %d", kind);
+  if (kind == FWE_KERNEL_FUSED && x->cfg.free_surface)
+    return fail(FWE_ERR_UNSUPPORTED, "the single-pass kernel has no free surface");
+  drop_graph(x);
+  x->kernel = kind;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_get_kernel(const fwe_ctx* x, int32_t* kind) {
+  if (!x || !kind) return fail(FWE_ERR_ARG, "NULL argument");
+  *kind = use_fused(x) ? FWE_KERNEL_FUSED : FWE_KERNEL_TWO_PASS;
+  return FWE_OK;
 }
 
 extern "C" fwe_status fwe_set_tiling(fwe_ctx* x, int32_t tile_rows, int32_t zchunk) {
@@ -693,7 +819,7 @@ extern "C" fwe_status fwe_get_field(fwe_ctx* x, int32_t batch, int32_t field, fl
   HIPCHK(hipSetDevice(x->cfg.device));
   HIPCHK(hipStreamSynchronize(x->stream));
   std::vector<float> h((size_t)x->nelem);
-  HIPCHK(hipMemcpy(h.data(), x->fields + ((int64_t)batch * 9 + field) * x->nelem, sizeof(float) * h.size(),
+  HIPCHK(hipMemcpy(h.data(), fields(x) + ((int64_t)batch * 9 + field) * x->nelem, sizeof(float) * h.size(),
                    hipMemcpyDeviceToHost));
   int64_t d[3];
   field_shape(x->cfg, field, d);
@@ -708,7 +834,13 @@ extern "C" fwe_status fwe_set_field(fwe_ctx* x, int32_t batch, int32_t field, co
   field_shape(x->cfg, field, d);
   std::vector<float> h;
   pack(x, in, d, 1.0, h);
-  return upload(x, x->fields + ((int64_t)batch * 9 + field) * x->nelem, h);
+  // both time-level buffers: cells no kernel updates must agree in both
+  for (int b = 0; b < 2; ++b)
+    if (x->fbuf[b]) {
+      fwe_status st = upload(x, x->fbuf[b] + ((int64_t)batch * 9 + field) * x->nelem, h);
+      if (st) return st;
+    }
+  return FWE_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,358 @@
+// fwe_fused.hip -- single-pass gfx950 kernel of one FDTD step.
+//
+// Same arithmetic as the two-pass kernels (and so as the reference's
+// compute_StressPrf/update_Qxft, OK4.cu:285-388, and compute_StressOrtho/
+// update_VelocityFromStress, ANISO.cu:287-389), scheduled differently:
+// a workgroup owns a 64 x TY column tile and marches its z chunk once per
+// step.  At plane m it
+//   1. loads V^n of plane m (plus Vz of m+1) and stages it in LDS,
+//   2. computes S^{n+1/2} of plane m for its cells, plus the few halo
+//      values its velocity update needs from neighbouring tiles
+//      (Syy of row j0-1, Sxy/Syz of row j0+TY by two extra waves; Sxx of
+//      column i0-1 by lane 0, Sxy/Sxz of column i0+64 by lane 63),
+//   3. computes V^{n+1} of plane m-1, whose stress neighbours are now all
+//      known: z from registers, y from LDS, x from wavefront shuffles.
+// Time level n is read from one buffer set and n+1 written to the other,
+// so tiles never race on shared halo cells.  Per cell and step the fields
+// are read once and written once (18 streams) and the medium is read once
+// (9 moduli + 3 buoyancies): 30 FP32 streams against 39 for two passes.
+#include "fwe_internal.h"
+
+namespace fwe {
+
+__device__ __forceinline__ int xcd_remap_f(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7;
+  const int x = b & 7, w = b >> 3;
+  return x * q + (x < r ? x : r) + w;
+}
+
+__device__ __forceinline__ float ldn(const float* p) { return __builtin_nontemporal_load(p); }
+
+// sum of the injections of one stress component at one cell this step
+__device__ __forceinline__ float src_at(const FusedArgs& a, int e0, int e1, float amp, int comp,
+                                        int i, int j, int k) {
+  float s = 0.f;
+  for (int e = e0; e < e1; ++e) {
+    const PointEntry p = a.inj[e];
+    if (p.comp == comp && p.i == i && p.j == j && p.k == k && p.batch == a.batch) s += p.coef * amp;
+  }
+  return s;
+}
+
+template <int MODE, int PML>
+__device__ __forceinline__ float damp3(const FusedArgs& a, int i, int j, int k, int64_t c) {
+  if (PML == PML_W3D) return ldn(a.w3 + c);
+  if (PML == PML_SEP) return a.wx[i] * a.wy[j] * a.wz[k];
+  return 1.f;
+}
+
+template <int MODE, int PML, int TY>
+__global__ __launch_bounds__(64 * (TY + 2)) void k_fused(const FusedArgs a) {
+  __shared__ float LV[2][3][TY + 2][64];  // Vx Vy Vz of plane m, rows j0-1..j0+TY
+  __shared__ float LS[2][3][TY + 2][64];  // Syy Sxy Syz of plane m
+  const int lane = threadIdx.x, w = threadIdx.y, r = w - 1;
+  int lb = xcd_remap_f(blockIdx.x, gridDim.x);
+  const int tx = lb % a.ntx;
+  lb /= a.ntx;
+  const int tyi = lb % a.nty;
+  const int kc = lb / a.nty;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int i0 = tx * TILE_X, j0 = tyi * TY;
+  const int i = i0 + lane, j = j0 + r;
+  const bool interior = r >= 0 && r < TY;
+  const bool halo_lo = r == -1, halo_hi = r == TY;
+  const bool jin = j >= 0 && j < ny;
+  const bool ok = i < nx && jin;              // the thread owns (or halo-computes) cell (i,j)
+  const bool lx = i <= nx && jin;             // Vx exists at (i,j)
+  const bool ly = i < nx && j >= 0 && j <= ny;  // Vy exists at (i,j)
+  const int jc = j < 0 ? 0 : (j > ny ? ny : j);
+  const int64_t sy = a.sy, sz = a.sz;
+  const int k0 = kc * a.kchunk;
+  const int k1 = min(k0 + a.kchunk, nz);
+  const int tile = tx + tyi * a.ntx;
+  const int e0 = a.inj_off[tile], e1 = a.inj_off[tile + 1];
+  const int q0 = a.rec_off[tile], q1 = a.rec_off[tile + 1];
+  const int it = a.it + (a.it_base ? *a.it_base : 0);
+  const float amp = (e1 > e0 && it < a.nstf) ? a.stf[it] : 0.f;
+  const bool have_src = e1 > e0;
+  const bool im = i >= 1, jm = j >= 1;
+  // x-halo duties (interior rows only)
+  const bool xl = interior && ok && lane == 0 && i0 >= 1;                 // Sxx(i0-1)
+  const bool xr = interior && ok && lane == 63 && i0 + TILE_X <= nx - 1;  // Sxy,Sxz(i0+64)
+  // velocity update masks (OK4.cu:357,364,371)
+  const bool ux_ij = interior && ok && im && jm && j <= ny - 2;
+  const bool uy_ij = interior && ok && im && i <= nx - 2 && jm;
+  const bool uz_ij = interior && ok && im && i <= nx - 2 && jm && j <= ny - 2;
+
+  const float* Vx = a.Vi[0];
+  const float* Vy = a.Vi[1];
+  const float* Vz = a.Vi[2];
+
+  // per-column damping factors
+  float fxh = 1.f, fyh = 1.f, wxi = 1.f, wyj = 1.f, wxm = 1.f, wym = 1.f;
+  if (PML == PML_PROFILE && ok) {
+    fxh = a.whx[i]; fyh = a.why[j]; wxi = a.wx[i]; wyj = a.wy[j];
+    wxm = im ? a.wx[i - 1] : 1.f;
+    wym = jm ? a.wy[j - 1] : 1.f;
+  }
+  if (PML == PML_SEP && ok) { wxi = a.wx[i]; wyj = a.wy[j]; }
+
+  const int mstart = k0 >= 1 ? k0 - 1 : 0;
+  // carried state: V^n of plane m-1 and Vz of plane m; S^{n+1/2} of m-1, Szz of m-2
+  float vx_p = 0.f, vy_p = 0.f, vz_p = 0.f, vz_c = 0.f;
+  float sp0 = 0.f, sp1 = 0.f, sp2 = 0.f, sp3 = 0.f, sp4 = 0.f, sp5 = 0.f, szz_pp = 0.f;
+  float sxxL_p = 0.f, sxyR_p = 0.f, sxzR_p = 0.f, w_p = 1.f;
+  {
+    const int64_t c = (int64_t)i + jc * sy + (int64_t)mstart * sz;
+    if (mstart >= 1) {
+      if (lx) vx_p = Vx[c - sz];
+      if (ly) vy_p = Vy[c - sz];
+    }
+    if (ok) vz_c = Vz[c];
+  }
+
+  for (int m = mstart; m <= k1; ++m) {
+    const int buf = m & 1;
+    const bool sm = m < nz;  // stress plane exists
+    const int64_t c = (int64_t)i + jc * sy + (int64_t)m * sz;
+    float vx_c = 0.f, vy_c = 0.f, vz_n = 0.f;
+    if (sm) {
+      if (lx) vx_c = Vx[c];
+      if (ly) vy_c = Vy[c];
+      if (ok) vz_n = Vz[c + sz];
+    }
+    LV[buf][0][w][lane] = vx_c;
+    LV[buf][1][w][lane] = vy_c;
+    LV[buf][2][w][lane] = vz_c;
+    __syncthreads();
+
+    // neighbours of V^n at plane m
+    float vx_ip = __shfl_down(vx_c, 1);
+    float vy_im = __shfl_up(vy_c, 1);
+    float vz_im = __shfl_up(vz_c, 1);
+    if (lane == 63 && sm && i + 1 <= nx && jin) vx_ip = Vx[c + 1];
+    if (lane == 0 && sm && im && jin) { vy_im = Vy[c - 1]; vz_im = Vz[c - 1]; }
+    const float vy_jp = (w <= TY) ? LV[buf][1][w + 1][lane] : 0.f;
+    const float vx_jm = (w >= 1) ? LV[buf][0][w - 1][lane] : 0.f;
+    const float vz_jm = (w >= 1) ? LV[buf][2][w - 1][lane] : 0.f;
+    const bool km = m >= 1;
+
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f;  // S^{n+1/2}(m), own cell
+    float w_c = 1.f;
+    float sxxL = 0.f, sxyR = 0.f, sxzR = 0.f;
+    if (sm && ok) {
+      const float ex = (vx_ip - vx_c) * a.rdx, ey = (vy_jp - vy_c) * a.rdy, ez = (vz_n - vz_c) * a.rdz;
+      const float dxy = (vy_c - vy_im) * a.rdx + (vx_c - vx_jm) * a.rdy;  // dVy/dx + dVx/dy
+      const float dxz = (vz_c - vz_im) * a.rdx + (vx_c - vx_p) * a.rdz;   // dVz/dx + dVx/dz
+      const float dyz = (vy_c - vy_p) * a.rdz + (vz_c - vz_jm) * a.rdy;   // dVy/dz + dVz/dy
+      float wzk = 1.f, fzh = 1.f;
+      if (PML == PML_PROFILE) { wzk = a.wz[m]; fzh = a.whz[m]; }
+      if (PML == PML_SEP) wzk = a.wz[m];
+      if (PML == PML_W3D) w_c = ldn(a.w3 + c);
+      if (PML == PML_SEP) w_c = wxi * wyj * wzk;
+      const float* const* Si = a.Si;
+      if (interior) {
+        // own cell, all six components (OK4.cu:307-329 / ANISO.cu:306-352)
+        float so0 = ldn(Si[0] + c), so1 = ldn(Si[1] + c), so2 = ldn(Si[2] + c);
+        float so3 = (im && jm) ? ldn(Si[3] + c) : 0.f;
+        float so4 = (im && km) ? ldn(Si[4] + c) : 0.f;
+        float so5 = (jm && km) ? ldn(Si[5] + c) : 0.f;
+        if (MODE == MODE_ISO) {
+          const float m0 = ldn(a.C[0] + c), m1 = ldn(a.C[1] + c);
+          s0 = fxh * (so0 + (m0 * ex + m1 * ey + m1 * ez));
+          s1 = fyh * (so1 + (m1 * ex + m0 * ey + m1 * ez));
+          s2 = fzh * (so2 + (m1 * ex + m1 * ey + m0 * ez));
+          s3 = (wzk * wyj * wxi) * (so3 + a.c66dt * dxy);
+          s4 = (wyj * wzk * wxi) * (so4 + a.c66dt * dxz);
+          s5 = (wxi * wyj * wzk) * (so5 + a.c66dt * dyz);
+        } else {
+          const float C11 = ldn(a.C[0] + c), C22 = ldn(a.C[1] + c), C33 = ldn(a.C[2] + c);
+          const float C12 = ldn(a.C[3] + c), C13 = ldn(a.C[4] + c), C23 = ldn(a.C[5] + c);
+          const float C44 = ldn(a.C[6] + c), C55 = ldn(a.C[7] + c), C66 = ldn(a.C[8] + c);
+          s0 = so0 * w_c + (C11 * ex + C12 * ey + C13 * ez);
+          s1 = so1 * w_c + (C12 * ex + C22 * ey + C23 * ez);
+          s2 = so2 * w_c + (C13 * ex + C23 * ey + C33 * ez);
+          s3 = so3 * w_c + C66 * dxy;
+          s4 = so4 * w_c + C55 * dxz;
+          s5 = so5 * w_c + C44 * dyz;
+        }
+        if (!(im && jm)) s3 = 0.f;
+        if (!(im && km)) s4 = 0.f;
+        if (!(jm && km)) s5 = 0.f;
+        if (have_src) {
+          s0 += src_at(a, e0, e1, amp, 0, i, j, m);
+          s1 += src_at(a, e0, e1, amp, 1, i, j, m);
+          s2 += src_at(a, e0, e1, amp, 2, i, j, m);
+          s3 += src_at(a, e0, e1, amp, 3, i, j, m);
+          s4 += src_at(a, e0, e1, amp, 4, i, j, m);
+          s5 += src_at(a, e0, e1, amp, 5, i, j, m);
+        }
+        if (m >= k0 && m < k1) {
+          a.So[0][c] = s0; a.So[1][c] = s1; a.So[2][c] = s2;
+          if (im && jm) a.So[3][c] = s3;
+          if (im && km) a.So[4][c] = s4;
+          if (jm && km) a.So[5][c] = s5;
+        }
+      } else if (halo_lo) {
+        // Syy of row j0-1, needed by the Vy update of row j0
+        const float so1 = ldn(Si[1] + c);
+        if (MODE == MODE_ISO) {
+          const float m0 = ldn(a.C[0] + c), m1 = ldn(a.C[1] + c);
+          s1 = fyh * (so1 + (m1 * ex + m0 * ey + m1 * ez));
+        } else {
+          const float C22 = ldn(a.C[1] + c), C12 = ldn(a.C[3] + c), C23 = ldn(a.C[5] + c);
+          s1 = so1 * w_c + (C12 * ex + C22 * ey + C23 * ez);
+        }
+        if (have_src) s1 += src_at(a, e0, e1, amp, 1, i, j, m);
+      } else {
+        // Sxy and Syz of row j0+TY, needed by the Vx and Vz updates of row j0+TY-1
+        const float so3 = im ? ldn(Si[3] + c) : 0.f;
+        const float so5 = km ? ldn(Si[5] + c) : 0.f;
+        if (MODE == MODE_ISO) {
+          s3 = (wzk * wyj * wxi) * (so3 + a.c66dt * dxy);
+          s5 = (wxi * wyj * wzk) * (so5 + a.c66dt * dyz);
+        } else {
+          s3 = so3 * w_c + ldn(a.C[8] + c) * dxy;
+          s5 = so5 * w_c + ldn(a.C[6] + c) * dyz;
+        }
+        if (!im) s3 = 0.f;
+        if (!km) s5 = 0.f;
+        if (have_src) {
+          s3 += src_at(a, e0, e1, amp, 3, i, j, m);
+          s5 += src_at(a, e0, e1, amp, 5, i, j, m);
+        }
+      }
+      if (xl) {
+        // Sxx of column i0-1 (Vx update of lane 0)
+        const int64_t cl = c - 1;
+        const float exl = (vx_c - Vx[cl]) * a.rdx;
+        const float eyl = (Vy[cl + sy] - Vy[cl]) * a.rdy;
+        const float ezl = (Vz[cl + sz] - Vz[cl]) * a.rdz;
+        const float so0 = ldn(a.Si[0] + cl);
+        if (MODE == MODE_ISO) {
+          const float m0 = ldn(a.C[0] + cl), m1 = ldn(a.C[1] + cl);
+          sxxL = a.whx[i - 1] * (so0 + (m0 * exl + m1 * eyl + m1 * ezl));
+        } else {
+          float wl = 1.f;
+          if (PML == PML_W3D) wl = ldn(a.w3 + cl);
+          if (PML == PML_SEP) wl = a.wx[i - 1] * wyj * wzk;
+          sxxL = so0 * wl + (ldn(a.C[0] + cl) * exl + ldn(a.C[3] + cl) * eyl + ldn(a.C[4] + cl) * ezl);
+        }
+        if (have_src) sxxL += src_at(a, e0, e1, amp, 0, i - 1, j, m);
+      }
+      if (xr) {
+        // Sxy and Sxz of column i0+64 (Vy and Vz updates of lane 63)
+        const int64_t cr = c + 1;
+        const float vxr = vx_ip;  // Vx(i0+64, j, m)
+        const float dxyr = (Vy[cr] - vy_c) * a.rdx + (vxr - (jm ? Vx[cr - sy] : 0.f)) * a.rdy;
+        const float dxzr = (Vz[cr] - vz_c) * a.rdx + (vxr - (km ? Vx[cr - sz] : 0.f)) * a.rdz;
+        const float so3 = jm ? ldn(a.Si[3] + cr) : 0.f;
+        const float so4 = km ? ldn(a.Si[4] + cr) : 0.f;
+        if (MODE == MODE_ISO) {
+          const float wxr = a.wx[i + 1];
+          sxyR = (wzk * wyj * wxr) * (so3 + a.c66dt * dxyr);
+          sxzR = (wyj * wzk * wxr) * (so4 + a.c66dt * dxzr);
+        } else {
+          float wr = 1.f;
+          if (PML == PML_W3D) wr = ldn(a.w3 + cr);
+          if (PML == PML_SEP) wr = a.wx[i + 1] * wyj * wzk;
+          sxyR = so3 * wr + ldn(a.C[8] + cr) * dxyr;
+          sxzR = so4 * wr + ldn(a.C[7] + cr) * dxzr;
+        }
+        if (!jm) sxyR = 0.f;
+        if (!km) sxzR = 0.f;
+        if (have_src) {
+          sxyR += src_at(a, e0, e1, amp, 3, i + 1, j, m);
+          sxzR += src_at(a, e0, e1, amp, 4, i + 1, j, m);
+        }
+      }
+    }
+    LS[buf][0][w][lane] = s1;
+    LS[buf][1][w][lane] = s3;
+    LS[buf][2][w][lane] = s5;
+
+    // ---- velocity of plane kv = m-1 (OK4.cu:351-378 / ANISO.cu:374-386) ----
+    const int kv = m - 1;
+    float sxx_im = __shfl_up(sp0, 1);
+    float sxy_ip = __shfl_down(sp3, 1);
+    float sxz_ip = __shfl_down(sp4, 1);
+    if (lane == 0) sxx_im = sxxL_p;
+    if (lane == 63) { sxy_ip = sxyR_p; sxz_ip = sxzR_p; }
+    if (interior && ok && kv >= k0 && kv < k1) {
+      const int pb = buf ^ 1;
+      const float syy_jm = LS[pb][0][w - 1][lane];
+      const float sxy_jp = LS[pb][1][w + 1][lane];
+      const float syz_jp = LS[pb][2][w + 1][lane];
+      const int64_t cv = c - sz;
+      const bool ux = ux_ij && kv >= 1 && kv <= nz - 2;
+      const bool uy = uy_ij && kv >= 1 && kv <= nz - 2;
+      const bool uz = uz_ij && kv >= 1;
+      float vx = vx_p, vy = vy_p, vz = vz_p;
+      float bx = a.bxdt, by = a.bydt, bz = a.bzdt;
+      if (MODE == MODE_ORTHO) {
+        bx = ux ? ldn(a.B[0] + cv) : 0.f;
+        by = uy ? ldn(a.B[1] + cv) : 0.f;
+        bz = uz ? ldn(a.B[2] + cv) : 0.f;
+      }
+      float wzm = 1.f;
+      if (PML == PML_PROFILE) wzm = kv >= 1 ? a.wz[kv - 1] : 1.f;
+      if (ux) {
+        const float dS = (sp0 - sxx_im) * a.rdx + (sxy_jp - sp3) * a.rdy + (s4 - sp4) * a.rdz;
+        vx = (MODE == MODE_ISO) ? wxm * (vx + bx * dS) : vx * w_p + bx * dS;
+        a.Vo[0][cv] = vx;
+      }
+      if (uy) {
+        const float dS = (sp1 - syy_jm) * a.rdy + (sxy_ip - sp3) * a.rdx + (s5 - sp5) * a.rdz;
+        vy = (MODE == MODE_ISO) ? wym * (vy + by * dS) : vy * w_p + by * dS;
+        a.Vo[1][cv] = vy;
+      }
+      if (uz) {
+        const float dS = (sp2 - szz_pp) * a.rdz + (sxz_ip - sp4) * a.rdx + (syz_jp - sp5) * a.rdy;
+        vz = (MODE == MODE_ISO) ? wzm * (vz + bz * dS) : vz * w_p + bz * dS;
+        a.Vo[2][cv] = vz;
+      }
+      for (int q = q0; q < q1; ++q) {
+        const PointEntry p = a.rec[q];
+        if (p.i == i && p.j == j && p.k == kv && p.batch == a.batch && it < a.nt)
+          a.traces[(int64_t)p.slot * a.nt + it] = p.comp == 0 ? vx : (p.comp == 1 ? vy : vz);
+      }
+    }
+    // advance the carried planes
+    szz_pp = sp2;
+    sp0 = s0; sp1 = s1; sp2 = s2; sp3 = s3; sp4 = s4; sp5 = s5;
+    sxxL_p = sxxL; sxyR_p = sxyR; sxzR_p = sxzR;
+    vx_p = vx_c; vy_p = vy_c; vz_p = vz_c; vz_c = vz_n;
+    w_p = w_c;
+  }
+}
+
+template <int MODE, int PML>
+static hipError_t go_fused(const FusedArgs& a, hipStream_t s) {
+  const int nblk = a.ntx * a.nty * a.nkc;
+  if (a.ty_rows == 4) {
+    hipLaunchKernelGGL((k_fused<MODE, PML, 4>), dim3(nblk), dim3(64, 6), 0, s, a);
+  } else if (a.ty_rows == 8) {
+    hipLaunchKernelGGL((k_fused<MODE, PML, 8>), dim3(nblk), dim3(64, 10), 0, s, a);
+  } else if (a.ty_rows == 2) {
+    hipLaunchKernelGGL((k_fused<MODE, PML, 2>), dim3(nblk), dim3(64, 4), 0, s, a);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_fused(const FusedArgs& a, int mode, int pml, hipStream_t s) {
+  if (mode == MODE_ISO) {
+    if (pml == PML_PROFILE) return go_fused<MODE_ISO, PML_PROFILE>(a, s);
+    if (pml == PML_NONE) return go_fused<MODE_ISO, PML_NONE>(a, s);
+    return hipErrorInvalidValue;
+  }
+  switch (pml) {
+    case PML_NONE: return go_fused<MODE_ORTHO, PML_NONE>(a, s);
+    case PML_W3D: return go_fused<MODE_ORTHO, PML_W3D>(a, s);
+    case PML_SEP: return go_fused<MODE_ORTHO, PML_SEP>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace fwe

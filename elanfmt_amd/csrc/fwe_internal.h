@@ -66,6 +66,39 @@ struct StepArgs {
   const int32_t* it_base;
 };
 
+// Single-pass step: stress of plane m and velocity of plane m-1 in one
+// z-march, reading time level n (Vi, Si) and writing n+1 (Vo, So).  Free
+// surface is not handled here (the two-pass kernels cover it).
+struct FusedArgs {
+  int32_t nx, ny, nz;
+  int64_t sy, sz;
+  const float* Vi[3];
+  const float* Si[6];
+  float* Vo[3];
+  float* So[6];
+  const float* C[9];
+  const float* B[3];
+  float c66dt, bxdt, bydt, bzdt;
+  const float* wx; const float* wy; const float* wz;
+  const float* whx; const float* why; const float* whz;
+  const float* w3;
+  float rdx, rdy, rdz;
+  int32_t ty_rows, ntx, nty, nkc, kchunk;
+  const int32_t* inj_off;   // per tile, entries within the tile grown by one cell
+  const PointEntry* inj;
+  const float* stf;         // this wavefield's source time function
+  int32_t nstf;
+  int32_t batch;            // wavefield index (selects injections and receivers)
+  const int32_t* rec_off;
+  const PointEntry* rec;
+  float* traces;
+  int32_t nt;
+  int32_t it;
+  const int32_t* it_base;
+};
+
+hipError_t launch_fused(const FusedArgs& a, int mode, int pml, hipStream_t s);
+
 hipError_t launch_stress(const StepArgs& a, int mode, int pml, int fs, hipStream_t s);
 hipError_t launch_velocity(const StepArgs& a, int mode, int pml, hipStream_t s);
 // *it_base = g (set) or *it_base += g

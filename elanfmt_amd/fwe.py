@@ -17,6 +17,7 @@ RUN_PATH = os.path.join(_HERE, "fwe_run")
 
 # status codes (fwe_status)
 OK, ERR_ARG, ERR_IO, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_UNSUPPORTED, ERR_NONFINITE = range(8)
+KERNEL_AUTO, KERNEL_TWO_PASS, KERNEL_FUSED = range(3)
 MODE_ISO, MODE_ORTHO = 0, 1
 PML_NONE, PML_PROFILE, PML_WEIGHTS3D, PML_SEPARABLE = 0, 1, 2, 3
 SXX, SYY, SZZ, SXY, SXZ, SYZ = range(6)
@@ -90,6 +91,8 @@ SIGNATURES = {
     "fwe_run_timed": (C.c_int, [C.c_void_p, C.c_int32, _DP, _DP, _DP]),
     "fwe_synchronize": (C.c_int, [C.c_void_p]),
     "fwe_set_graph_steps": (C.c_int, [C.c_void_p, C.c_int32]),
+    "fwe_set_kernel": (C.c_int, [C.c_void_p, C.c_int32]),
+    "fwe_get_kernel": (C.c_int, [C.c_void_p, _IP]),
     "fwe_set_tiling": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "fwe_get_tiling": (C.c_int, [C.c_void_p, _IP, _IP]),
     "fwe_autotune": (C.c_int, [C.c_void_p, C.c_int32, _DP]),
@@ -268,6 +271,14 @@ class Solver:
 
     def set_graph_steps(self, g):
         _check(lib().fwe_set_graph_steps(self._h, g))
+
+    def set_kernel(self, kind):
+        _check(lib().fwe_set_kernel(self._h, kind))
+
+    def kernel(self):
+        k = C.c_int32()
+        _check(lib().fwe_get_kernel(self._h, C.byref(k)))
+        return k.value
 
     def set_tiling(self, tile_rows, zchunk):
         _check(lib().fwe_set_tiling(self._h, tile_rows, zchunk))

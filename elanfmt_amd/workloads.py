@@ -268,11 +268,17 @@ def config(idx, **kw):
 
 # algorithmic bytes per cell-update of each kernel formulation: every
 # streamed 3-D array counted once per pass (read and/or write), FP32.
-def alg_bytes(mode, pml, nbatch=1):
-    """(stress_bytes, velocity_bytes) per cell per step, all wavefields."""
+def alg_bytes(mode, pml, nbatch=1, fused=False):
+    """(stress_bytes, velocity_bytes) per cell per step, all wavefields; for
+    the single-pass kernel the whole step is in the first entry."""
+    w = 1 if pml == fwe.PML_WEIGHTS3D else 0
+    if fused:
+        # fields R9 W9 per wavefield; medium once per launch (one launch per
+        # wavefield: iso c11 c12; ortho C9 b3 [w])
+        med = 2 if mode == fwe.MODE_ISO else 12 + w
+        return 4 * nbatch * (18 + med), 0
     if mode == fwe.MODE_ISO:
         # stress: R V3 S6 c11 c12, W S6 ; velocity: R S6 V3, W V3
         return 4 * (nbatch * 15 + 2), 4 * (nbatch * 12)
-    w = 1 if pml == fwe.PML_WEIGHTS3D else 0
     # stress: R V3 S6 C9 [w], W S6 ; velocity: R S6 V3 b3 [w], W V3
     return 4 * (nbatch * 15 + 9 + w), 4 * (nbatch * 12 + 3 + w)

@@ -85,6 +85,11 @@ enum {
 enum { FWE_REC_VX = 0, FWE_REC_VY = 1, FWE_REC_VZ = 2 };
 /* Injection offset layouts of the reference's moment-tensor source. */
 enum { FWE_SRC_LAYOUT_OK4 = 0, FWE_SRC_LAYOUT_ANISO = 1 };
+/* Kernel schedule: TWO_PASS = a stress launch and a velocity launch per step
+ * (in place, the reference's two-kernel structure); FUSED = one z-marching
+ * launch per step over two time-level buffers (no free surface); AUTO =
+ * FUSED unless the free surface is on. */
+enum { FWE_KERNEL_AUTO = 0, FWE_KERNEL_TWO_PASS = 1, FWE_KERNEL_FUSED = 2 };
 
 typedef struct fwe_config {
   int32_t nx, ny, nz;     /* cells: OK4.cu:39-41 (nx = 64*NBX - OVERX ...) */
@@ -204,6 +209,10 @@ fwe_status fwe_synchronize(fwe_ctx* ctx);
  * Replaces the per-kernel launch + cudaDeviceSynchronize of OK4.cu:485-505.
  * fwe_run_timed always launches directly. */
 fwe_status fwe_set_graph_steps(fwe_ctx* ctx, int32_t g);
+/* Select the kernel schedule (FWE_KERNEL_*); results agree to FP32
+ * round-off.  get returns the schedule fwe_run will use. */
+fwe_status fwe_set_kernel(fwe_ctx* ctx, int32_t kind);
+fwe_status fwe_get_kernel(const fwe_ctx* ctx, int32_t* kind);
 /* Workgroup tile: 64 x tile_rows cells (tile_rows 2, 4 or 8), marching
  * zchunk planes (3..nz).  Replaces the fixed 32x2x8 block of OK4.cu:26-31. */
 fwe_status fwe_set_tiling(fwe_ctx* ctx, int32_t tile_rows, int32_t zchunk);
