@@ -61,6 +61,7 @@ def main():
     ap.add_argument("--kernel", default="auto", choices=["auto", "two_pass", "fused"])
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -83,6 +84,10 @@ def main():
     s = W.make_solver(wl)
     s.set_kernel(kind)
     fused = s.kernel() == fwe.KERNEL_FUSED
+    tune_ms = None
+    if not args.no_autotune:
+        # library autotuner: times each tile shape, keeps the fastest, resets
+        tune_ms = s.autotune(3)
     s.run(args.warmup)
     s.synchronize()
     s.reset()
@@ -137,6 +142,7 @@ def main():
             if wl.mode == fwe.MODE_ORTHO else f"{wname}: isotropic homogeneous FP32",
             "grid": [wl.nx, wl.ny, wl.nz], "wavefields": wl.nbatch, "nt": wl.nt,
             "kernel": "fused single-pass" if fused else "two-pass", "tiling": list(tiling),
+            "autotune_ms_per_step": None if tune_ms is None else round(tune_ms, 4),
             "parallelism": "single GPU",
         },
         "hbm_pct_step": round(100 * step_gbs / PEAK_HBM_GBS, 2),

@@ -39,13 +39,6 @@ __device__ __forceinline__ float src_at(const FusedArgs& a, int e0, int e1, floa
   return s;
 }
 
-template <int MODE, int PML>
-__device__ __forceinline__ float damp3(const FusedArgs& a, int i, int j, int k, int64_t c) {
-  if (PML == PML_W3D) return ldn(a.w3 + c);
-  if (PML == PML_SEP) return a.wx[i] * a.wy[j] * a.wz[k];
-  return 1.f;
-}
-
 template <int MODE, int PML, int TY>
 __global__ __launch_bounds__(64 * (TY + 2)) void k_fused(const FusedArgs a) {
   __shared__ float LV[2][3][TY + 2][64];  // Vx Vy Vz of plane m, rows j0-1..j0+TY
@@ -115,50 +108,113 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_fused(const FusedArgs a) {
     const int buf = m & 1;
     const bool sm = m < nz;  // stress plane exists
     const int64_t c = (int64_t)i + jc * sy + (int64_t)m * sz;
+    const bool km = m >= 1;
+    const int kv = m - 1;
+    const bool vel = interior && ok && kv >= k0 && kv < k1;
+    const int64_t cv = c - sz;
+    const bool ux = vel && ux_ij && kv >= 1 && kv <= nz - 2;
+    const bool uy = vel && uy_ij && kv >= 1 && kv <= nz - 2;
+    const bool uz = vel && uz_ij && kv >= 1;
+
+    // ---- phase 1: issue every global load of this plane ----
     float vx_c = 0.f, vy_c = 0.f, vz_n = 0.f;
     if (sm) {
       if (lx) vx_c = Vx[c];
       if (ly) vy_c = Vy[c];
       if (ok) vz_n = Vz[c + sz];
     }
+    float vx_r = 0.f, vy_l = 0.f, vz_l = 0.f;  // Vx(i0+64), Vy(i0-1), Vz(i0-1)
+    if (lane == 63 && sm && i + 1 <= nx && jin) vx_r = Vx[c + 1];
+    if (lane == 0 && sm && im && jin) { vy_l = Vy[c - 1]; vz_l = Vz[c - 1]; }
+    // old stress and moduli of the cells this thread updates (by role)
+    float so0 = 0.f, so1 = 0.f, so2 = 0.f, so3 = 0.f, so4 = 0.f, so5 = 0.f;
+    float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f, m6 = 0.f, m7 = 0.f, m8 = 0.f;
+    float w_c = 1.f;
+    const bool st = sm && ok;
+    if (st) {
+      if (interior) {
+        so0 = ldn(a.Si[0] + c); so1 = ldn(a.Si[1] + c); so2 = ldn(a.Si[2] + c);
+        if (im && jm) so3 = ldn(a.Si[3] + c);
+        if (im && km) so4 = ldn(a.Si[4] + c);
+        if (jm && km) so5 = ldn(a.Si[5] + c);
+        m0 = ldn(a.C[0] + c); m1 = ldn(a.C[1] + c);
+        if (MODE == MODE_ORTHO) {
+          m2 = ldn(a.C[2] + c); m3 = ldn(a.C[3] + c); m4 = ldn(a.C[4] + c); m5 = ldn(a.C[5] + c);
+          m6 = ldn(a.C[6] + c); m7 = ldn(a.C[7] + c); m8 = ldn(a.C[8] + c);
+        }
+      } else if (halo_lo) {
+        so1 = ldn(a.Si[1] + c);
+        m1 = ldn(a.C[1] + c);  // c12 (iso) or C22 (ortho)
+        if (MODE == MODE_ISO) m0 = ldn(a.C[0] + c);
+        else { m3 = ldn(a.C[3] + c); m5 = ldn(a.C[5] + c); }
+      } else {
+        if (im) so3 = ldn(a.Si[3] + c);
+        if (km) so5 = ldn(a.Si[5] + c);
+        if (MODE == MODE_ORTHO) { m6 = ldn(a.C[6] + c); m8 = ldn(a.C[8] + c); }
+      }
+      if (PML == PML_W3D) w_c = ldn(a.w3 + c);
+    }
+    // x-halo inputs: Sxx(i0-1) on lane 0, Sxy/Sxz(i0+64) on lane 63
+    float xa0 = 0.f, xa1 = 0.f, xa2 = 0.f, xa3 = 0.f, xa4 = 0.f, xa5 = 0.f, xa6 = 0.f, xa7 = 0.f, xw = 1.f;
+    if (st && xl) {
+      const int64_t cl = c - 1;
+      xa0 = Vx[cl]; xa1 = Vy[cl]; xa2 = Vy[cl + sy]; xa3 = Vz[cl]; xa4 = Vz[cl + sz];
+      xa5 = ldn(a.Si[0] + cl);
+      xa6 = ldn(a.C[0] + cl);  // c11 (iso) or C11 (ortho)
+      if (MODE == MODE_ISO) {
+        xa7 = ldn(a.C[1] + cl);  // c12
+      } else {
+        xa7 = ldn(a.C[3] + cl);  // C12
+        xw = ldn(a.C[4] + cl);   // C13
+      }
+    }
+    float xb0 = 0.f, xb1 = 0.f, xb2 = 0.f, xb3 = 0.f, xb4 = 0.f, xb5 = 0.f, xb6 = 0.f, xb7 = 0.f, xbw = 1.f;
+    if (st && xr) {
+      const int64_t cr = c + 1;
+      xb0 = Vy[cr]; xb1 = jm ? Vx[cr - sy] : 0.f; xb2 = Vz[cr]; xb3 = km ? Vx[cr - sz] : 0.f;
+      xb4 = jm ? ldn(a.Si[3] + cr) : 0.f; xb5 = km ? ldn(a.Si[4] + cr) : 0.f;
+      if (MODE == MODE_ORTHO) { xb6 = ldn(a.C[8] + cr); xb7 = ldn(a.C[7] + cr); }
+      if (PML == PML_W3D) xbw = ldn(a.w3 + cr);
+    }
+    float xlw = 1.f;
+    if (PML == PML_W3D && st && xl) xlw = ldn(a.w3 + c - 1);
+    // buoyancy of the velocity plane m-1
+    float bx = a.bxdt, by = a.bydt, bz = a.bzdt;
+    if (MODE == MODE_ORTHO) {
+      bx = ux ? ldn(a.B[0] + cv) : 0.f;
+      by = uy ? ldn(a.B[1] + cv) : 0.f;
+      bz = uz ? ldn(a.B[2] + cv) : 0.f;
+    }
+
+    // ---- phase 2: stage V^n of plane m ----
     LV[buf][0][w][lane] = vx_c;
     LV[buf][1][w][lane] = vy_c;
     LV[buf][2][w][lane] = vz_c;
     __syncthreads();
 
-    // neighbours of V^n at plane m
     float vx_ip = __shfl_down(vx_c, 1);
     float vy_im = __shfl_up(vy_c, 1);
     float vz_im = __shfl_up(vz_c, 1);
-    if (lane == 63 && sm && i + 1 <= nx && jin) vx_ip = Vx[c + 1];
-    if (lane == 0 && sm && im && jin) { vy_im = Vy[c - 1]; vz_im = Vz[c - 1]; }
+    if (lane == 63) vx_ip = vx_r;
+    if (lane == 0) { vy_im = vy_l; vz_im = vz_l; }
     const float vy_jp = (w <= TY) ? LV[buf][1][w + 1][lane] : 0.f;
     const float vx_jm = (w >= 1) ? LV[buf][0][w - 1][lane] : 0.f;
     const float vz_jm = (w >= 1) ? LV[buf][2][w - 1][lane] : 0.f;
-    const bool km = m >= 1;
 
-    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f;  // S^{n+1/2}(m), own cell
-    float w_c = 1.f;
+    // ---- phase 3: S^{n+1/2} of plane m ----
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f;
     float sxxL = 0.f, sxyR = 0.f, sxzR = 0.f;
-    if (sm && ok) {
+    if (st) {
       const float ex = (vx_ip - vx_c) * a.rdx, ey = (vy_jp - vy_c) * a.rdy, ez = (vz_n - vz_c) * a.rdz;
       const float dxy = (vy_c - vy_im) * a.rdx + (vx_c - vx_jm) * a.rdy;  // dVy/dx + dVx/dy
       const float dxz = (vz_c - vz_im) * a.rdx + (vx_c - vx_p) * a.rdz;   // dVz/dx + dVx/dz
       const float dyz = (vy_c - vy_p) * a.rdz + (vz_c - vz_jm) * a.rdy;   // dVy/dz + dVz/dy
       float wzk = 1.f, fzh = 1.f;
       if (PML == PML_PROFILE) { wzk = a.wz[m]; fzh = a.whz[m]; }
-      if (PML == PML_SEP) wzk = a.wz[m];
-      if (PML == PML_W3D) w_c = ldn(a.w3 + c);
-      if (PML == PML_SEP) w_c = wxi * wyj * wzk;
-      const float* const* Si = a.Si;
+      if (PML == PML_SEP) { wzk = a.wz[m]; w_c = wxi * wyj * wzk; }
       if (interior) {
         // own cell, all six components (OK4.cu:307-329 / ANISO.cu:306-352)
-        float so0 = ldn(Si[0] + c), so1 = ldn(Si[1] + c), so2 = ldn(Si[2] + c);
-        float so3 = (im && jm) ? ldn(Si[3] + c) : 0.f;
-        float so4 = (im && km) ? ldn(Si[4] + c) : 0.f;
-        float so5 = (jm && km) ? ldn(Si[5] + c) : 0.f;
         if (MODE == MODE_ISO) {
-          const float m0 = ldn(a.C[0] + c), m1 = ldn(a.C[1] + c);
           s0 = fxh * (so0 + (m0 * ex + m1 * ey + m1 * ez));
           s1 = fyh * (so1 + (m1 * ex + m0 * ey + m1 * ez));
           s2 = fzh * (so2 + (m1 * ex + m1 * ey + m0 * ez));
@@ -166,15 +222,12 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_fused(const FusedArgs a) {
           s4 = (wyj * wzk * wxi) * (so4 + a.c66dt * dxz);
           s5 = (wxi * wyj * wzk) * (so5 + a.c66dt * dyz);
         } else {
-          const float C11 = ldn(a.C[0] + c), C22 = ldn(a.C[1] + c), C33 = ldn(a.C[2] + c);
-          const float C12 = ldn(a.C[3] + c), C13 = ldn(a.C[4] + c), C23 = ldn(a.C[5] + c);
-          const float C44 = ldn(a.C[6] + c), C55 = ldn(a.C[7] + c), C66 = ldn(a.C[8] + c);
-          s0 = so0 * w_c + (C11 * ex + C12 * ey + C13 * ez);
-          s1 = so1 * w_c + (C12 * ex + C22 * ey + C23 * ez);
-          s2 = so2 * w_c + (C13 * ex + C23 * ey + C33 * ez);
-          s3 = so3 * w_c + C66 * dxy;
-          s4 = so4 * w_c + C55 * dxz;
-          s5 = so5 * w_c + C44 * dyz;
+          s0 = so0 * w_c + (m0 * ex + m3 * ey + m4 * ez);
+          s1 = so1 * w_c + (m3 * ex + m1 * ey + m5 * ez);
+          s2 = so2 * w_c + (m4 * ex + m5 * ey + m2 * ez);
+          s3 = so3 * w_c + m8 * dxy;
+          s4 = so4 * w_c + m7 * dxz;
+          s5 = so5 * w_c + m6 * dyz;
         }
         if (!(im && jm)) s3 = 0.f;
         if (!(im && km)) s4 = 0.f;
@@ -195,25 +248,17 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_fused(const FusedArgs a) {
         }
       } else if (halo_lo) {
         // Syy of row j0-1, needed by the Vy update of row j0
-        const float so1 = ldn(Si[1] + c);
-        if (MODE == MODE_ISO) {
-          const float m0 = ldn(a.C[0] + c), m1 = ldn(a.C[1] + c);
-          s1 = fyh * (so1 + (m1 * ex + m0 * ey + m1 * ez));
-        } else {
-          const float C22 = ldn(a.C[1] + c), C12 = ldn(a.C[3] + c), C23 = ldn(a.C[5] + c);
-          s1 = so1 * w_c + (C12 * ex + C22 * ey + C23 * ez);
-        }
+        if (MODE == MODE_ISO) s1 = fyh * (so1 + (m1 * ex + m0 * ey + m1 * ez));
+        else s1 = so1 * w_c + (m3 * ex + m1 * ey + m5 * ez);
         if (have_src) s1 += src_at(a, e0, e1, amp, 1, i, j, m);
       } else {
         // Sxy and Syz of row j0+TY, needed by the Vx and Vz updates of row j0+TY-1
-        const float so3 = im ? ldn(Si[3] + c) : 0.f;
-        const float so5 = km ? ldn(Si[5] + c) : 0.f;
         if (MODE == MODE_ISO) {
           s3 = (wzk * wyj * wxi) * (so3 + a.c66dt * dxy);
           s5 = (wxi * wyj * wzk) * (so5 + a.c66dt * dyz);
         } else {
-          s3 = so3 * w_c + ldn(a.C[8] + c) * dxy;
-          s5 = so5 * w_c + ldn(a.C[6] + c) * dyz;
+          s3 = so3 * w_c + m8 * dxy;
+          s5 = so5 * w_c + m6 * dyz;
         }
         if (!im) s3 = 0.f;
         if (!km) s5 = 0.f;
@@ -224,40 +269,31 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_fused(const FusedArgs a) {
       }
       if (xl) {
         // Sxx of column i0-1 (Vx update of lane 0)
-        const int64_t cl = c - 1;
-        const float exl = (vx_c - Vx[cl]) * a.rdx;
-        const float eyl = (Vy[cl + sy] - Vy[cl]) * a.rdy;
-        const float ezl = (Vz[cl + sz] - Vz[cl]) * a.rdz;
-        const float so0 = ldn(a.Si[0] + cl);
+        const float exl = (vx_c - xa0) * a.rdx, eyl = (xa2 - xa1) * a.rdy, ezl = (xa4 - xa3) * a.rdz;
         if (MODE == MODE_ISO) {
-          const float m0 = ldn(a.C[0] + cl), m1 = ldn(a.C[1] + cl);
-          sxxL = a.whx[i - 1] * (so0 + (m0 * exl + m1 * eyl + m1 * ezl));
+          sxxL = a.whx[i - 1] * (xa5 + (xa6 * exl + xa7 * eyl + xa7 * ezl));
         } else {
           float wl = 1.f;
-          if (PML == PML_W3D) wl = ldn(a.w3 + cl);
+          if (PML == PML_W3D) wl = xlw;
           if (PML == PML_SEP) wl = a.wx[i - 1] * wyj * wzk;
-          sxxL = so0 * wl + (ldn(a.C[0] + cl) * exl + ldn(a.C[3] + cl) * eyl + ldn(a.C[4] + cl) * ezl);
+          sxxL = xa5 * wl + (xa6 * exl + xa7 * eyl + xw * ezl);
         }
         if (have_src) sxxL += src_at(a, e0, e1, amp, 0, i - 1, j, m);
       }
       if (xr) {
         // Sxy and Sxz of column i0+64 (Vy and Vz updates of lane 63)
-        const int64_t cr = c + 1;
-        const float vxr = vx_ip;  // Vx(i0+64, j, m)
-        const float dxyr = (Vy[cr] - vy_c) * a.rdx + (vxr - (jm ? Vx[cr - sy] : 0.f)) * a.rdy;
-        const float dxzr = (Vz[cr] - vz_c) * a.rdx + (vxr - (km ? Vx[cr - sz] : 0.f)) * a.rdz;
-        const float so3 = jm ? ldn(a.Si[3] + cr) : 0.f;
-        const float so4 = km ? ldn(a.Si[4] + cr) : 0.f;
+        const float dxyr = (xb0 - vy_c) * a.rdx + (vx_r - xb1) * a.rdy;
+        const float dxzr = (xb2 - vz_c) * a.rdx + (vx_r - xb3) * a.rdz;
         if (MODE == MODE_ISO) {
           const float wxr = a.wx[i + 1];
-          sxyR = (wzk * wyj * wxr) * (so3 + a.c66dt * dxyr);
-          sxzR = (wyj * wzk * wxr) * (so4 + a.c66dt * dxzr);
+          sxyR = (wzk * wyj * wxr) * (xb4 + a.c66dt * dxyr);
+          sxzR = (wyj * wzk * wxr) * (xb5 + a.c66dt * dxzr);
         } else {
           float wr = 1.f;
-          if (PML == PML_W3D) wr = ldn(a.w3 + cr);
+          if (PML == PML_W3D) wr = xbw;
           if (PML == PML_SEP) wr = a.wx[i + 1] * wyj * wzk;
-          sxyR = so3 * wr + ldn(a.C[8] + cr) * dxyr;
-          sxzR = so4 * wr + ldn(a.C[7] + cr) * dxzr;
+          sxyR = xb4 * wr + xb6 * dxyr;
+          sxzR = xb5 * wr + xb7 * dxzr;
         }
         if (!jm) sxyR = 0.f;
         if (!km) sxzR = 0.f;
@@ -271,29 +307,18 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_fused(const FusedArgs a) {
     LS[buf][1][w][lane] = s3;
     LS[buf][2][w][lane] = s5;
 
-    // ---- velocity of plane kv = m-1 (OK4.cu:351-378 / ANISO.cu:374-386) ----
-    const int kv = m - 1;
+    // ---- phase 4: V^{n+1} of plane m-1 (OK4.cu:351-378 / ANISO.cu:374-386) ----
     float sxx_im = __shfl_up(sp0, 1);
     float sxy_ip = __shfl_down(sp3, 1);
     float sxz_ip = __shfl_down(sp4, 1);
     if (lane == 0) sxx_im = sxxL_p;
     if (lane == 63) { sxy_ip = sxyR_p; sxz_ip = sxzR_p; }
-    if (interior && ok && kv >= k0 && kv < k1) {
+    if (vel) {
       const int pb = buf ^ 1;
       const float syy_jm = LS[pb][0][w - 1][lane];
       const float sxy_jp = LS[pb][1][w + 1][lane];
       const float syz_jp = LS[pb][2][w + 1][lane];
-      const int64_t cv = c - sz;
-      const bool ux = ux_ij && kv >= 1 && kv <= nz - 2;
-      const bool uy = uy_ij && kv >= 1 && kv <= nz - 2;
-      const bool uz = uz_ij && kv >= 1;
       float vx = vx_p, vy = vy_p, vz = vz_p;
-      float bx = a.bxdt, by = a.bydt, bz = a.bzdt;
-      if (MODE == MODE_ORTHO) {
-        bx = ux ? ldn(a.B[0] + cv) : 0.f;
-        by = uy ? ldn(a.B[1] + cv) : 0.f;
-        bz = uz ? ldn(a.B[2] + cv) : 0.f;
-      }
       float wzm = 1.f;
       if (PML == PML_PROFILE) wzm = kv >= 1 ? a.wz[kv - 1] : 1.f;
       if (ux) {

@@ -252,11 +252,23 @@ __global__ void k_advance(int32_t* it_base, int32_t g, int32_t set) {
 }
 
 // Plain vector copy, the memory-copy probe (Performance/PerfMC.m).
-__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src,
-                                              float4* __restrict__ dst, int64_t n4) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n4; t += stride)
-    dst[t] = src[t];
+// Each workgroup copies 4 x 256 16-byte vectors: all four loads are issued
+// before the first store, so every wave has 4 KiB in flight.
+typedef float v4f __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy(const v4f* __restrict__ src, v4f* __restrict__ dst,
+                                              int64_t n4) {
+  const int64_t base = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+  v4f a[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int64_t q = base + u * 256;
+    if (q < n4) a[u] = __builtin_nontemporal_load(src + q);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int64_t q = base + u * 256;
+    if (q < n4) __builtin_nontemporal_store(a[u], dst + q);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -332,9 +344,10 @@ hipError_t launch_advance(int32_t* it_base, int32_t g, int32_t set, hipStream_t 
 
 hipError_t launch_copy(const float* src, float* dst, int64_t n, hipStream_t s) {
   const int64_t n4 = n / 4;
-  const int blocks = 256 * 16;
-  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, s,
-                     reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst), n4);
+  const int64_t blocks = (n4 + 1023) / 1024;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, reinterpret_cast<const v4f*>(src),
+                     reinterpret_cast<v4f*>(dst), n4);
   return hipGetLastError();
 }
 
