@@ -28,6 +28,17 @@ __device__ __forceinline__ int xcd_remap_f(int b, int nb) {
 
 __device__ __forceinline__ float ldn(const float* p) { return __builtin_nontemporal_load(p); }
 
+// Workgroup barrier for data exchanged through LDS only.  The fences are
+// scoped to LDS ("local"), so the wave waits for its own LDS traffic
+// (lgkmcnt) but keeps its global loads in flight across the barrier; the
+// compiler still waits for each load before its first use.  A plain
+// __syncthreads() would also drain every outstanding global load.
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // sum of the injections of one stress component at one cell this step
 __device__ __forceinline__ float src_at(const FusedArgs& a, int e0, int e1, float amp, int comp,
                                         int i, int j, int k) {
@@ -190,7 +201,7 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_fused(const FusedArgs a) {
     LV[buf][0][w][lane] = vx_c;
     LV[buf][1][w][lane] = vy_c;
     LV[buf][2][w][lane] = vz_c;
-    __syncthreads();
+    lds_sync();
 
     float vx_ip = __shfl_down(vx_c, 1);
     float vy_im = __shfl_up(vy_c, 1);
