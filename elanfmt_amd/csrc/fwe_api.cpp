@@ -562,9 +562,9 @@ static fwe::FusedArgs make_fused(fwe_ctx* x, int from, int b) {
   return f;
 }
 
-static bool use_fused(const fwe_ctx* x) {
-  return x->kernel == FWE_KERNEL_FUSED || (x->kernel == FWE_KERNEL_AUTO && !x->cfg.free_surface);
-}
+// AUTO picks the schedule measured fastest on MI355X: the two-pass kernels
+// (the single-pass kernel moves fewer bytes but runs at lower bandwidth).
+static bool use_fused(const fwe_ctx* x) { return x->kernel == FWE_KERNEL_FUSED; }
 
 // one step from time level fbuf[from]; step index it (+ *it_base); `mid`
 // (optional) is recorded between the stress and velocity passes

@@ -28,6 +28,13 @@ __device__ __forceinline__ int xcd_remap_f(int b, int nb) {
 
 __device__ __forceinline__ float ldn(const float* p) { return __builtin_nontemporal_load(p); }
 
+// Unconditional streaming load of p[idx] that yields 0 when !use; lanes that
+// do not need it read p[0] instead (always in bounds), so no branch is taken.
+__device__ __forceinline__ float ldz(const float* p, bool use, int64_t idx) {
+  const float v = __builtin_nontemporal_load(p + (use ? idx : 0));
+  return use ? v : 0.f;
+}
+
 // Workgroup barrier for data exchanged through LDS only.  The fences are
 // scoped to LDS ("local"), so the wave waits for its own LDS traffic
 // (lgkmcnt) but keeps its global loads in flight across the barrier; the
@@ -128,73 +135,63 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_fused(const FusedArgs a) {
     const bool uz = vel && uz_ij && kv >= 1;
 
     // ---- phase 1: issue every global load of this plane ----
-    float vx_c = 0.f, vy_c = 0.f, vz_n = 0.f;
-    if (sm) {
-      if (lx) vx_c = Vx[c];
-      if (ly) vy_c = Vy[c];
-      if (ok) vz_n = Vz[c + sz];
-    }
-    float vx_r = 0.f, vy_l = 0.f, vz_l = 0.f;  // Vx(i0+64), Vy(i0-1), Vz(i0-1)
-    if (lane == 63 && sm && i + 1 <= nx && jin) vx_r = Vx[c + 1];
-    if (lane == 0 && sm && im && jin) { vy_l = Vy[c - 1]; vz_l = Vz[c - 1]; }
-    // old stress and moduli of the cells this thread updates (by role)
-    float so0 = 0.f, so1 = 0.f, so2 = 0.f, so3 = 0.f, so4 = 0.f, so5 = 0.f;
-    float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f, m6 = 0.f, m7 = 0.f, m8 = 0.f;
-    float w_c = 1.f;
+    // Branch-free: a load a lane does not need reads element 0 (one cached
+    // line) and is zeroed, so the loop body stays one basic block and the
+    // compiler can keep all of these loads in flight together.
     const bool st = sm && ok;
-    if (st) {
-      if (interior) {
-        so0 = ldn(a.Si[0] + c); so1 = ldn(a.Si[1] + c); so2 = ldn(a.Si[2] + c);
-        if (im && jm) so3 = ldn(a.Si[3] + c);
-        if (im && km) so4 = ldn(a.Si[4] + c);
-        if (jm && km) so5 = ldn(a.Si[5] + c);
-        m0 = ldn(a.C[0] + c); m1 = ldn(a.C[1] + c);
-        if (MODE == MODE_ORTHO) {
-          m2 = ldn(a.C[2] + c); m3 = ldn(a.C[3] + c); m4 = ldn(a.C[4] + c); m5 = ldn(a.C[5] + c);
-          m6 = ldn(a.C[6] + c); m7 = ldn(a.C[7] + c); m8 = ldn(a.C[8] + c);
-        }
-      } else if (halo_lo) {
-        so1 = ldn(a.Si[1] + c);
-        m1 = ldn(a.C[1] + c);  // c12 (iso) or C22 (ortho)
-        if (MODE == MODE_ISO) m0 = ldn(a.C[0] + c);
-        else { m3 = ldn(a.C[3] + c); m5 = ldn(a.C[5] + c); }
-      } else {
-        if (im) so3 = ldn(a.Si[3] + c);
-        if (km) so5 = ldn(a.Si[5] + c);
-        if (MODE == MODE_ORTHO) { m6 = ldn(a.C[6] + c); m8 = ldn(a.C[8] + c); }
-      }
-      if (PML == PML_W3D) w_c = ldn(a.w3 + c);
+    const float vx_c = ldz(Vx, sm && lx, c);
+    const float vy_c = ldz(Vy, sm && ly, c);
+    const float vz_n = ldz(Vz, st, c + sz);
+    const float vx_r = ldz(Vx, lane == 63 && sm && i + 1 <= nx && jin, c + 1);  // Vx(i0+64)
+    const float vy_l = ldz(Vy, lane == 0 && sm && im && jin, c - 1);            // Vy(i0-1)
+    const float vz_l = ldz(Vz, lane == 0 && sm && im && jin, c - 1);            // Vz(i0-1)
+    // old stress and moduli of the cells this thread updates (by role)
+    const bool ri = st && interior, rlo = st && halo_lo, rhi = st && halo_hi;
+    const float so0 = ldz(a.Si[0], ri, c);
+    const float so1 = ldz(a.Si[1], ri || rlo, c);
+    const float so2 = ldz(a.Si[2], ri, c);
+    const float so3 = ldz(a.Si[3], (ri && jm || rhi) && im, c);
+    const float so4 = ldz(a.Si[4], ri && im && km, c);
+    const float so5 = ldz(a.Si[5], (ri && jm || rhi) && km, c);
+    float m0, m1, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f, m6 = 0.f, m7 = 0.f, m8 = 0.f;
+    if (MODE == MODE_ISO) {
+      m0 = ldz(a.C[0], ri || rlo, c);  // c11
+      m1 = ldz(a.C[1], ri || rlo, c);  // c12
+    } else {
+      m0 = ldz(a.C[0], ri, c);         // C11
+      m1 = ldz(a.C[1], ri || rlo, c);  // C22
+      m2 = ldz(a.C[2], ri, c);         // C33
+      m3 = ldz(a.C[3], ri || rlo, c);  // C12
+      m4 = ldz(a.C[4], ri, c);         // C13
+      m5 = ldz(a.C[5], ri || rlo, c);  // C23
+      m6 = ldz(a.C[6], ri || rhi, c);  // C44
+      m7 = ldz(a.C[7], ri, c);         // C55
+      m8 = ldz(a.C[8], ri || rhi, c);  // C66
     }
+    float w_c = 1.f;
+    if (PML == PML_W3D) w_c = ldz(a.w3, st, c);
     // x-halo inputs: Sxx(i0-1) on lane 0, Sxy/Sxz(i0+64) on lane 63
-    float xa0 = 0.f, xa1 = 0.f, xa2 = 0.f, xa3 = 0.f, xa4 = 0.f, xa5 = 0.f, xa6 = 0.f, xa7 = 0.f, xw = 1.f;
-    if (st && xl) {
-      const int64_t cl = c - 1;
-      xa0 = Vx[cl]; xa1 = Vy[cl]; xa2 = Vy[cl + sy]; xa3 = Vz[cl]; xa4 = Vz[cl + sz];
-      xa5 = ldn(a.Si[0] + cl);
-      xa6 = ldn(a.C[0] + cl);  // c11 (iso) or C11 (ortho)
-      if (MODE == MODE_ISO) {
-        xa7 = ldn(a.C[1] + cl);  // c12
-      } else {
-        xa7 = ldn(a.C[3] + cl);  // C12
-        xw = ldn(a.C[4] + cl);   // C13
-      }
-    }
-    float xb0 = 0.f, xb1 = 0.f, xb2 = 0.f, xb3 = 0.f, xb4 = 0.f, xb5 = 0.f, xb6 = 0.f, xb7 = 0.f, xbw = 1.f;
-    if (st && xr) {
-      const int64_t cr = c + 1;
-      xb0 = Vy[cr]; xb1 = jm ? Vx[cr - sy] : 0.f; xb2 = Vz[cr]; xb3 = km ? Vx[cr - sz] : 0.f;
-      xb4 = jm ? ldn(a.Si[3] + cr) : 0.f; xb5 = km ? ldn(a.Si[4] + cr) : 0.f;
-      if (MODE == MODE_ORTHO) { xb6 = ldn(a.C[8] + cr); xb7 = ldn(a.C[7] + cr); }
-      if (PML == PML_W3D) xbw = ldn(a.w3 + cr);
-    }
-    float xlw = 1.f;
-    if (PML == PML_W3D && st && xl) xlw = ldn(a.w3 + c - 1);
+    const bool gl = st && xl, gr = st && xr;
+    const int64_t cl = c - 1, cr = c + 1;
+    const float xa0 = ldz(Vx, gl, cl), xa1 = ldz(Vy, gl, cl), xa2 = ldz(Vy, gl, cl + sy);
+    const float xa3 = ldz(Vz, gl, cl), xa4 = ldz(Vz, gl, cl + sz);
+    const float xa5 = ldz(a.Si[0], gl, cl);
+    const float xa6 = ldz(a.C[0], gl, cl);                                // c11 / C11
+    const float xa7 = ldz(a.C[MODE == MODE_ISO ? 1 : 3], gl, cl);         // c12 / C12
+    const float xw = MODE == MODE_ISO ? 0.f : ldz(a.C[4], gl, cl);        // C13
+    const float xlw = PML == PML_W3D ? ldz(a.w3, gl, cl) : 1.f;
+    const float xb0 = ldz(Vy, gr, cr), xb1 = ldz(Vx, gr && jm, cr - sy);
+    const float xb2 = ldz(Vz, gr, cr), xb3 = ldz(Vx, gr && km, cr - sz);
+    const float xb4 = ldz(a.Si[3], gr && jm, cr), xb5 = ldz(a.Si[4], gr && km, cr);
+    const float xb6 = MODE == MODE_ISO ? 0.f : ldz(a.C[8], gr, cr);       // C66
+    const float xb7 = MODE == MODE_ISO ? 0.f : ldz(a.C[7], gr, cr);       // C55
+    const float xbw = PML == PML_W3D ? ldz(a.w3, gr, cr) : 1.f;
     // buoyancy of the velocity plane m-1
     float bx = a.bxdt, by = a.bydt, bz = a.bzdt;
     if (MODE == MODE_ORTHO) {
-      bx = ux ? ldn(a.B[0] + cv) : 0.f;
-      by = uy ? ldn(a.B[1] + cv) : 0.f;
-      bz = uz ? ldn(a.B[2] + cv) : 0.f;
+      bx = ldz(a.B[0], ux, cv);
+      by = ldz(a.B[1], uy, cv);
+      bz = ldz(a.B[2], uz, cv);
     }
 
     // ---- phase 2: stage V^n of plane m ----

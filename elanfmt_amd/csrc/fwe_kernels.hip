@@ -30,6 +30,15 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 
 __device__ __forceinline__ float ld(const float* p) { return __builtin_nontemporal_load(p); }
 
+// Unconditional streaming load of p[idx] that yields 0 when !use; lanes that
+// do not need it read p[0] (always in bounds).  Keeping loads out of
+// branches lets the compiler keep a whole plane's loads in flight instead of
+// draining the memory counter at every control-flow join.
+__device__ __forceinline__ float ldz(const float* p, bool use, int64_t idx) {
+  const float v = __builtin_nontemporal_load(p + (use ? idx : 0));
+  return use ? v : 0.f;
+}
+
 // ---------------------------------------------------------------------------
 // Stress pass.  For cell (i,j,k) of the column tile:
 //   normal:  S_ii <- damp * S_ii + sum_j C_ij * dt * dV_j/dx_j
@@ -38,8 +47,10 @@ __device__ __forceinline__ float ld(const float* p) { return __builtin_nontempor
 // iso law (OK4.cu:296-329): damp multiplies (old + increment); ortho law
 // (ANISO.cu:306-352): damp multiplies the old value only.
 // Injection and BC1 are applied in the reference order (see StepArgs).
+// SINGLE (one wavefield): the z-neighbours Vx(k-1), Vy(k-1) and Vz(k) are
+// carried in registers from the previous plane.
 // ---------------------------------------------------------------------------
-template <int MODE, int PML, bool FS, int TY>
+template <int MODE, int PML, bool FS, int TY, bool SINGLE>
 __global__ __launch_bounds__(64 * TY) void k_stress(const StepArgs a) {
   const int lane = threadIdx.x, row = threadIdx.y;
   int lb = xcd_remap(blockIdx.x, gridDim.x);
@@ -56,10 +67,20 @@ __global__ __launch_bounds__(64 * TY) void k_stress(const StepArgs a) {
   const int e0 = a.inj_off[tile], e1 = a.inj_off[tile + 1];
   const bool im = i >= 1, jm = j >= 1;
   const float third = 1.0f / 3.0f;
+  const int nb = SINGLE ? 1 : a.nb;
 
   float fxh = 1.f, fyh = 1.f, wxi = 1.f, wyj = 1.f;
   if (PML == PML_PROFILE) { fxh = a.whx[i]; fyh = a.why[j]; }
   if (PML == PML_PROFILE || PML == PML_SEP) { wxi = a.wx[i]; wyj = a.wy[j]; }
+
+  // carried z-neighbours (single wavefield)
+  float cvx = 0.f, cvy = 0.f, cvz = 0.f;
+  if (SINGLE) {
+    const int64_t c0 = (int64_t)i + j * sy + (int64_t)k0 * sz;
+    cvx = ldz(a.V[0], k0 >= 1, c0 - sz);
+    cvy = ldz(a.V[1], k0 >= 1, c0 - sz);
+    cvz = ldz(a.V[2], true, c0);
+  }
 
   for (int k = k0; k < k1; ++k) {
     const int64_t c = (int64_t)i + j * sy + k * sz;
@@ -71,52 +92,54 @@ __global__ __launch_bounds__(64 * TY) void k_stress(const StepArgs a) {
     if (PML == PML_W3D) w = ld(a.w3 + c);
     if (PML == PML_SEP) w = wxi * wyj * wzk;
     // medium at this cell, shared by all wavefields of the batch
-    float m0, m1, m2, m3, m4, m5, m6, m7, m8;
+    float m0, m1, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f, m6 = 0.f, m7 = 0.f, m8 = 0.f;
+    m0 = ld(a.C[0] + c); m1 = ld(a.C[1] + c);
     if (MODE == MODE_ORTHO) {
-      m0 = ld(a.C[0] + c); m1 = ld(a.C[1] + c); m2 = ld(a.C[2] + c);
-      m3 = ld(a.C[3] + c); m4 = ld(a.C[4] + c); m5 = ld(a.C[5] + c);
+      m2 = ld(a.C[2] + c); m3 = ld(a.C[3] + c); m4 = ld(a.C[4] + c); m5 = ld(a.C[5] + c);
       m6 = ld(a.C[6] + c); m7 = ld(a.C[7] + c); m8 = ld(a.C[8] + c);
-    } else {
-      m0 = ld(a.C[0] + c); m1 = ld(a.C[1] + c);
-      m2 = m3 = m4 = m5 = m6 = m7 = m8 = 0.f;
     }
-    for (int b = 0; b < a.nb; ++b) {
+    for (int b = 0; b < nb; ++b) {
       const int64_t fo = (int64_t)b * a.fstride;
       const float* Vx = a.V[0] + fo;
       const float* Vy = a.V[1] + fo;
       const float* Vz = a.V[2] + fo;
-      const float vx = Vx[c], vy = Vy[c], vz = Vz[c];
-      const float exx = (Vx[c + 1] - vx), eyy = (Vy[c + sy] - vy), ezz = (Vz[c + sz] - vz);
-      const float vx_jm = jm ? Vx[c - sy] : 0.f;
-      const float vx_km = km ? Vx[c - sz] : 0.f;
-      const float vy_im = im ? Vy[c - 1] : 0.f;
-      const float vy_km = km ? Vy[c - sz] : 0.f;
-      const float vz_im = im ? Vz[c - 1] : 0.f;
-      const float vz_jm = jm ? Vz[c - sy] : 0.f;
       float* Sxx = a.S[0] + fo; float* Syy = a.S[1] + fo; float* Szz = a.S[2] + fo;
       float* Sxy = a.S[3] + fo; float* Sxz = a.S[4] + fo; float* Syz = a.S[5] + fo;
+      // all loads of this cell first, branch-free
+      const float vx = Vx[c], vy = Vy[c];
+      const float vx_ip = Vx[c + 1], vy_jp = Vy[c + sy], vz_kp = Vz[c + sz];
+      const float vz = SINGLE ? cvz : Vz[c];
+      const float vx_km = SINGLE ? cvx : ldz(Vx, km, c - sz);
+      const float vy_km = SINGLE ? cvy : ldz(Vy, km, c - sz);
+      const float vx_jm = ldz(Vx, jm, c - sy);
+      const float vy_im = ldz(Vy, im, c - 1);
+      const float vz_im = ldz(Vz, im, c - 1);
+      const float vz_jm = ldz(Vz, jm, c - sy);
       float sxx = ld(Sxx + c), syy = ld(Syy + c), szz = ld(Szz + c);
-      float sxy = (im && jm) ? ld(Sxy + c) : 0.f;
-      float sxz = (im && km) ? ld(Sxz + c) : 0.f;
-      float syz = (jm && km) ? ld(Syz + c) : 0.f;
+      float sxy = ldz(Sxy, im && jm, c);
+      float sxz = ldz(Sxz, im && km, c);
+      float syz = ldz(Syz, jm && km, c);
+      if (SINGLE) { cvx = vx; cvy = vy; cvz = vz_kp; }
+      const float ex = (vx_ip - vx) * a.rdx, ey = (vy_jp - vy) * a.rdy, ez = (vz_kp - vz) * a.rdz;
+      const float dxy = (vy - vy_im) * a.rdx + (vx - vx_jm) * a.rdy;
+      const float dxz = (vz - vz_im) * a.rdx + (vx - vx_km) * a.rdz;
+      const float dyz = (vy - vy_km) * a.rdz + (vz - vz_jm) * a.rdy;
       if (MODE == MODE_ISO) {
         // OK4.cu:307-309,323-329 with c11,c12 arrays holding c*dt
-        const float ex = exx * a.rdx, ey = eyy * a.rdy, ez = ezz * a.rdz;
         sxx = fxh * (sxx + (m0 * ex + m1 * ey + m1 * ez));
         syy = fyh * (syy + (m1 * ex + m0 * ey + m1 * ez));
         szz = fzh * (szz + (m1 * ex + m1 * ey + m0 * ez));
-        sxy = (wzk * wyj * wxi) * (sxy + a.c66dt * ((vy - vy_im) * a.rdx + (vx - vx_jm) * a.rdy));
-        sxz = (wyj * wzk * wxi) * (sxz + a.c66dt * ((vz - vz_im) * a.rdx + (vx - vx_km) * a.rdz));
-        syz = (wxi * wyj * wzk) * (syz + a.c66dt * ((vy - vy_km) * a.rdz + (vz - vz_jm) * a.rdy));
+        sxy = (wzk * wyj * wxi) * (sxy + a.c66dt * dxy);
+        sxz = (wyj * wzk * wxi) * (sxz + a.c66dt * dxz);
+        syz = (wxi * wyj * wzk) * (syz + a.c66dt * dyz);
       } else {
         // ANISO.cu:306-352 with Cij holding Cij*dt
-        const float ex = exx * a.rdx, ey = eyy * a.rdy, ez = ezz * a.rdz;
         sxx = sxx * w + (m0 * ex + m3 * ey + m4 * ez);
         syy = syy * w + (m3 * ex + m1 * ey + m5 * ez);
         szz = szz * w + (m4 * ex + m5 * ey + m2 * ez);
-        sxy = sxy * w + m8 * ((vy - vy_im) * a.rdx + (vx - vx_jm) * a.rdy);
-        sxz = sxz * w + m7 * ((vz - vz_im) * a.rdx + (vx - vx_km) * a.rdz);
-        syz = syz * w + m6 * ((vy - vy_km) * a.rdz + (vz - vz_jm) * a.rdy);
+        sxy = sxy * w + m8 * dxy;
+        sxz = sxz * w + m7 * dxz;
+        syz = syz * w + m6 * dyz;
       }
       // shear cells outside the reference update ranges stay zero
       if (!(im && jm)) sxy = 0.f;
@@ -167,8 +190,9 @@ __global__ __launch_bounds__(64 * TY) void k_stress(const StepArgs a) {
 //   Vx 1<=i<=nx-1, 1<=j<=ny-2, 1<=k<=nz-2
 //   Vy 1<=i<=nx-2, 1<=j<=ny-1, 1<=k<=nz-2
 //   Vz 1<=i<=nx-2, 1<=j<=ny-2, 1<=k<=nz-1
+// SINGLE: Szz(k-1), Sxz(k) and Syz(k) are carried from the previous plane.
 // ---------------------------------------------------------------------------
-template <int MODE, int PML, int TY>
+template <int MODE, int PML, int TY, bool SINGLE>
 __global__ __launch_bounds__(64 * TY) void k_velocity(const StepArgs a) {
   const int lane = threadIdx.x, row = threadIdx.y;
   int lb = xcd_remap(blockIdx.x, gridDim.x);
@@ -184,17 +208,27 @@ __global__ __launch_bounds__(64 * TY) void k_velocity(const StepArgs a) {
   const int64_t sy = a.sy, sz = a.sz;
   const int tile = tx + tyi * a.ntx;
   const int r0 = a.rec_off[tile], r1 = a.rec_off[tile + 1];
-  const bool xin = i >= 1 && j >= 1;  // all three need i>=1, j>=1, k>=1
+  const bool im = i >= 1, jm = j >= 1;
+  const bool xin = im && jm;  // all three need i>=1, j>=1, k>=1
   const bool ux_ij = xin && j <= ny - 2;
   const bool uy_ij = xin && i <= nx - 2;
   const bool uz_ij = xin && i <= nx - 2 && j <= ny - 2;
+  const int nb = SINGLE ? 1 : a.nb;
 
   float wxm = 1.f, wym = 1.f, txi = 1.f, tyj = 1.f;
   if (PML == PML_PROFILE) {
-    wxm = i >= 1 ? a.wx[i - 1] : 1.f;
-    wym = j >= 1 ? a.wy[j - 1] : 1.f;
+    wxm = im ? a.wx[i - 1] : 1.f;
+    wym = jm ? a.wy[j - 1] : 1.f;
   }
   if (PML == PML_SEP) { txi = a.wx[i]; tyj = a.wy[j]; }
+
+  float cszz = 0.f, csxz = 0.f, csyz = 0.f;  // Szz(k-1), Sxz(k), Syz(k)
+  if (SINGLE) {
+    const int64_t c0 = (int64_t)i + j * sy + (int64_t)k0 * sz;
+    cszz = ldz(a.S[2], k0 >= 1, c0 - sz);
+    csxz = ldz(a.S[4], true, c0);
+    csyz = ldz(a.S[5], true, c0);
+  }
 
   for (int k = k0; k < k1; ++k) {
     const int64_t c = (int64_t)i + j * sy + k * sz;
@@ -210,28 +244,31 @@ __global__ __launch_bounds__(64 * TY) void k_velocity(const StepArgs a) {
     }
     float wzm = 1.f;
     if (PML == PML_PROFILE) wzm = k >= 1 ? a.wz[k - 1] : 1.f;
-    for (int b = 0; b < a.nb; ++b) {
+    for (int b = 0; b < nb; ++b) {
       const int64_t fo = (int64_t)b * a.fstride;
       float* Vx = a.V[0] + fo; float* Vy = a.V[1] + fo; float* Vz = a.V[2] + fo;
       const float* Sxx = a.S[0] + fo; const float* Syy = a.S[1] + fo;
       const float* Szz = a.S[2] + fo; const float* Sxy = a.S[3] + fo;
       const float* Sxz = a.S[4] + fo; const float* Syz = a.S[5] + fo;
+      // all loads of this cell first, branch-free (c+1, c+sy, c+sz stay
+      // inside the padded extents; c-1, c-sy, c-sz only when valid)
       float vx = ld(Vx + c), vy = ld(Vy + c), vz = ld(Vz + c);
-      if (ux) {
-        const float dS = (Sxx[c] - Sxx[c - 1]) * a.rdx + (Sxy[c + sy] - Sxy[c]) * a.rdy +
-                         (Sxz[c + sz] - Sxz[c]) * a.rdz;
-        vx = (MODE == MODE_ISO) ? wxm * (vx + bx * dS) : vx * w + bx * dS;
-      }
-      if (uy) {
-        const float dS = (Syy[c] - Syy[c - sy]) * a.rdy + (Sxy[c + 1] - Sxy[c]) * a.rdx +
-                         (Syz[c + sz] - Syz[c]) * a.rdz;
-        vy = (MODE == MODE_ISO) ? wym * (vy + by * dS) : vy * w + by * dS;
-      }
-      if (uz) {
-        const float dS = (Szz[c] - Szz[c - sz]) * a.rdz + (Sxz[c + 1] - Sxz[c]) * a.rdx +
-                         (Syz[c + sy] - Syz[c]) * a.rdy;
-        vz = (MODE == MODE_ISO) ? wzm * (vz + bz * dS) : vz * w + bz * dS;
-      }
+      const float sxx = Sxx[c], sxx_im = ldz(Sxx, im, c - 1);
+      const float syy = Syy[c], syy_jm = ldz(Syy, jm, c - sy);
+      const float szz = Szz[c];
+      const float szz_km = SINGLE ? cszz : ldz(Szz, k >= 1, c - sz);
+      const float sxy = Sxy[c], sxy_jp = Sxy[c + sy], sxy_ip = Sxy[c + 1];
+      const float sxz = SINGLE ? csxz : Sxz[c];
+      const float sxz_kp = Sxz[c + sz], sxz_ip = Sxz[c + 1];
+      const float syz = SINGLE ? csyz : Syz[c];
+      const float syz_kp = Syz[c + sz], syz_jp = Syz[c + sy];
+      if (SINGLE) { cszz = szz; csxz = sxz_kp; csyz = syz_kp; }
+      const float dSx = (sxx - sxx_im) * a.rdx + (sxy_jp - sxy) * a.rdy + (sxz_kp - sxz) * a.rdz;
+      const float dSy = (syy - syy_jm) * a.rdy + (sxy_ip - sxy) * a.rdx + (syz_kp - syz) * a.rdz;
+      const float dSz = (szz - szz_km) * a.rdz + (sxz_ip - sxz) * a.rdx + (syz_jp - syz) * a.rdy;
+      if (ux) vx = (MODE == MODE_ISO) ? wxm * (vx + bx * dSx) : vx * w + bx * dSx;
+      if (uy) vy = (MODE == MODE_ISO) ? wym * (vy + by * dSy) : vy * w + by * dSy;
+      if (uz) vz = (MODE == MODE_ISO) ? wzm * (vz + bz * dSz) : vz * w + bz * dSz;
       if (ux) Vx[c] = vx;
       if (uy) Vy[c] = vy;
       if (uz) Vz[c] = vz;
@@ -274,15 +311,35 @@ __global__ __launch_bounds__(256) void k_copy(const v4f* __restrict__ src, v4f* 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <int MODE, int PML, bool FS>
-static hipError_t go_stress(const StepArgs& a, hipStream_t s) {
+template <int MODE, int PML, bool FS, bool SINGLE>
+static hipError_t go_stress1(const StepArgs& a, hipStream_t s) {
   const int nblk = a.ntx * a.nty * a.nkc;
   if (a.ty_rows == 4) {
-    hipLaunchKernelGGL((k_stress<MODE, PML, FS, 4>), dim3(nblk), dim3(64, 4), 0, s, a);
+    hipLaunchKernelGGL((k_stress<MODE, PML, FS, 4, SINGLE>), dim3(nblk), dim3(64, 4), 0, s, a);
   } else if (a.ty_rows == 8) {
-    hipLaunchKernelGGL((k_stress<MODE, PML, FS, 8>), dim3(nblk), dim3(64, 8), 0, s, a);
+    hipLaunchKernelGGL((k_stress<MODE, PML, FS, 8, SINGLE>), dim3(nblk), dim3(64, 8), 0, s, a);
   } else if (a.ty_rows == 2) {
-    hipLaunchKernelGGL((k_stress<MODE, PML, FS, 2>), dim3(nblk), dim3(64, 2), 0, s, a);
+    hipLaunchKernelGGL((k_stress<MODE, PML, FS, 2, SINGLE>), dim3(nblk), dim3(64, 2), 0, s, a);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int MODE, int PML, bool FS>
+static hipError_t go_stress(const StepArgs& a, hipStream_t s) {
+  return a.nb == 1 ? go_stress1<MODE, PML, FS, true>(a, s) : go_stress1<MODE, PML, FS, false>(a, s);
+}
+
+template <int MODE, int PML, bool SINGLE>
+static hipError_t go_vel1(const StepArgs& a, hipStream_t s) {
+  const int nblk = a.ntx * a.nty * a.nkc;
+  if (a.ty_rows == 4) {
+    hipLaunchKernelGGL((k_velocity<MODE, PML, 4, SINGLE>), dim3(nblk), dim3(64, 4), 0, s, a);
+  } else if (a.ty_rows == 8) {
+    hipLaunchKernelGGL((k_velocity<MODE, PML, 8, SINGLE>), dim3(nblk), dim3(64, 8), 0, s, a);
+  } else if (a.ty_rows == 2) {
+    hipLaunchKernelGGL((k_velocity<MODE, PML, 2, SINGLE>), dim3(nblk), dim3(64, 2), 0, s, a);
   } else {
     return hipErrorInvalidValue;
   }
@@ -291,17 +348,7 @@ static hipError_t go_stress(const StepArgs& a, hipStream_t s) {
 
 template <int MODE, int PML>
 static hipError_t go_vel(const StepArgs& a, hipStream_t s) {
-  const int nblk = a.ntx * a.nty * a.nkc;
-  if (a.ty_rows == 4) {
-    hipLaunchKernelGGL((k_velocity<MODE, PML, 4>), dim3(nblk), dim3(64, 4), 0, s, a);
-  } else if (a.ty_rows == 8) {
-    hipLaunchKernelGGL((k_velocity<MODE, PML, 8>), dim3(nblk), dim3(64, 8), 0, s, a);
-  } else if (a.ty_rows == 2) {
-    hipLaunchKernelGGL((k_velocity<MODE, PML, 2>), dim3(nblk), dim3(64, 2), 0, s, a);
-  } else {
-    return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
+  return a.nb == 1 ? go_vel1<MODE, PML, true>(a, s) : go_vel1<MODE, PML, false>(a, s);
 }
 
 hipError_t launch_stress(const StepArgs& a, int mode, int pml, int fs, hipStream_t s) {

@@ -87,8 +87,8 @@ enum { FWE_REC_VX = 0, FWE_REC_VY = 1, FWE_REC_VZ = 2 };
 enum { FWE_SRC_LAYOUT_OK4 = 0, FWE_SRC_LAYOUT_ANISO = 1 };
 /* Kernel schedule: TWO_PASS = a stress launch and a velocity launch per step
  * (in place, the reference's two-kernel structure); FUSED = one z-marching
- * launch per step over two time-level buffers (no free surface); AUTO =
- * FUSED unless the free surface is on. */
+ * launch per step over two time-level buffers (no free surface); AUTO = the
+ * schedule measured fastest on MI355X (currently TWO_PASS). */
 enum { FWE_KERNEL_AUTO = 0, FWE_KERNEL_TWO_PASS = 1, FWE_KERNEL_FUSED = 2 };
 
 typedef struct fwe_config {
