@@ -1034,3 +1034,161 @@ extern "C" fwe_status fwe_copy_probe(int32_t device, int64_t bytes, int32_t reps
   hipFree(a); hipFree(b);
   return FWE_OK;
 }
+
+// ---------------------------------------------------------------------------
+// groups: several independent single-wavefield models advanced together,
+// one stress launch and one velocity launch per step for all of them
+// ---------------------------------------------------------------------------
+struct fwe_group {
+  std::vector<fwe_ctx*> m;
+  int device = 0, mode = 0, pml = 0, fs = 0, ty = 4;
+  hipStream_t stream = nullptr;
+  StepArgs* tab = nullptr;
+  int32_t* start = nullptr;
+  int32_t* it_dev = nullptr;
+  int nblk = 0;
+};
+
+// rebuild the device table of the members' current step arguments
+static fwe_status group_table(fwe_group* g) {
+  std::vector<StepArgs> tab;
+  std::vector<int32_t> start(1, 0);
+  for (fwe_ctx* x : g->m) {
+    StepArgs a = make_args(x, x->cur);
+    a.it = 0;
+    a.it_base = g->it_dev;
+    tab.push_back(a);
+    start.push_back(start.back() + x->ntx * x->nty * x->nkc);
+  }
+  g->nblk = start.back();
+  HIPCHK(hipSetDevice(g->device));
+  HIPCHK(hipMemcpy(g->tab, tab.data(), sizeof(StepArgs) * tab.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g->start, start.data(), sizeof(int32_t) * start.size(), hipMemcpyHostToDevice));
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_group_create(int32_t n, fwe_ctx* const* members, fwe_group** out) {
+  if (!members || !out || n < 1) return fail(FWE_ERR_ARG, "bad group arguments");
+  *out = nullptr;
+  const fwe_config& c0 = members[0]->cfg;
+  for (int q = 0; q < n; ++q) {
+    fwe_ctx* x = members[q];
+    if (!x) return fail(FWE_ERR_ARG, "member %d is NULL", q);
+    const fwe_config& c = x->cfg;
+    if (c.device != c0.device || c.mode != c0.mode || c.pml != c0.pml || c.free_surface != c0.free_surface)
+      return fail(FWE_ERR_ARG, "member %d differs in device, law, damping or free surface", q);
+    if (c.nbatch != 1) return fail(FWE_ERR_UNSUPPORTED, "member %d has %d wavefields; groups take 1", q, c.nbatch);
+    if (x->it != members[0]->it) return fail(FWE_ERR_STATE, "member %d is at another step", q);
+    fwe_status st = ready(x);
+    if (st) return st;
+  }
+  fwe_group* g = new (std::nothrow) fwe_group();
+  if (!g) return fail(FWE_ERR_NOMEM, "host allocation");
+  g->m.assign(members, members + n);
+  g->device = c0.device; g->mode = c0.mode; g->pml = c0.pml; g->fs = c0.free_surface;
+  g->ty = members[0]->ty;
+  HIPCHK(hipSetDevice(g->device));
+  for (fwe_ctx* x : g->m) {
+    HIPCHK(hipStreamSynchronize(x->stream));
+    if (x->ty != g->ty) {
+      fwe_status st = apply_tiling(x, g->ty, std::min(x->kchunk, x->cfg.nz));
+      if (st) { delete g; return st; }
+    }
+  }
+  hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&g->tab, sizeof(StepArgs) * n);
+  if (e == hipSuccess) e = hipMalloc(&g->start, sizeof(int32_t) * (n + 1));
+  if (e == hipSuccess) e = hipMalloc(&g->it_dev, sizeof(int32_t));
+  if (e != hipSuccess) {
+    fwe_group_destroy(g);
+    return fail(FWE_ERR_HIP, "group create: %s", hipGetErrorString(e));
+  }
+  fwe_status st = group_table(g);
+  if (st) { fwe_group_destroy(g); return st; }
+  *out = g;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_group_destroy(fwe_group* g) {
+  if (!g) return FWE_OK;
+  hipSetDevice(g->device);
+  if (g->stream) hipStreamSynchronize(g->stream);
+  hipFree(g->tab); hipFree(g->start); hipFree(g->it_dev);
+  if (g->stream) hipStreamDestroy(g->stream);
+  delete g;
+  return FWE_OK;
+}
+
+static fwe_status group_run_impl(fwe_group* g, int32_t nsteps, double* ms) {
+  if (!g || nsteps < 0) return fail(FWE_ERR_ARG, "bad group run arguments");
+  HIPCHK(hipSetDevice(g->device));
+  for (fwe_ctx* x : g->m) {
+    if (x->it != g->m[0]->it) return fail(FWE_ERR_STATE, "members were advanced separately");
+    HIPCHK(hipStreamSynchronize(x->stream));
+  }
+  fwe_status st = group_table(g);
+  if (st) return st;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ms) {
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+  }
+  HIPCHK(fwe::launch_advance(g->it_dev, g->m[0]->it, 1, g->stream));
+  if (ms) HIPCHK(hipEventRecord(e0, g->stream));
+  for (int s = 0; s < nsteps; ++s) {
+    HIPCHK(fwe::launch_group(g->tab, g->start, (int)g->m.size(), g->nblk, g->ty, g->mode, g->pml, g->fs, false,
+                             g->stream));
+    HIPCHK(fwe::launch_group(g->tab, g->start, (int)g->m.size(), g->nblk, g->ty, g->mode, g->pml, g->fs, true,
+                             g->stream));
+    HIPCHK(fwe::launch_advance(g->it_dev, 1, 0, g->stream));
+  }
+  if (ms) HIPCHK(hipEventRecord(e1, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  for (fwe_ctx* x : g->m) x->it += nsteps;
+  if (ms) {
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    *ms = t;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+  }
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_group_run(fwe_group* g, int32_t nsteps) { return group_run_impl(g, nsteps, nullptr); }
+
+extern "C" fwe_status fwe_group_run_timed(fwe_group* g, int32_t nsteps, double* ms) {
+  return group_run_impl(g, nsteps, ms);
+}
+
+extern "C" fwe_status fwe_group_set_tiling(fwe_group* g, int32_t tile_rows, int32_t zchunk) {
+  if (!g) return fail(FWE_ERR_ARG, "NULL group");
+  for (fwe_ctx* x : g->m) {
+    fwe_status st = apply_tiling(x, tile_rows, std::min<int>(zchunk, x->cfg.nz));
+    if (st) return st;
+  }
+  g->ty = tile_rows;
+  return group_table(g);
+}
+
+extern "C" fwe_status fwe_group_autotune(fwe_group* g, int32_t trial_steps, double* best_ms_per_step) {
+  if (!g || trial_steps < 1) return fail(FWE_ERR_ARG, "bad autotune arguments");
+  static const int tys[3] = {2, 4, 8};
+  static const int kcs[6] = {4, 8, 16, 32, 64, 256};
+  double best = 1e30;
+  int bty = g->ty, bkc = 8;
+  fwe_status st;
+  for (int ty : tys)
+    for (int kc : kcs) {
+      if ((st = fwe_group_set_tiling(g, ty, kc))) return st;
+      double ms = 0;
+      if ((st = group_run_impl(g, 1, &ms))) return st;
+      if ((st = group_run_impl(g, trial_steps, &ms))) return st;
+      if (ms / trial_steps < best) { best = ms / trial_steps; bty = ty; bkc = kc; }
+    }
+  if ((st = fwe_group_set_tiling(g, bty, bkc))) return st;
+  for (fwe_ctx* x : g->m)
+    if ((st = fwe_reset(x))) return st;
+  if (best_ms_per_step) *best_ms_per_step = best;
+  return FWE_OK;
+}

@@ -51,9 +51,8 @@ __device__ __forceinline__ float ldz(const float* p, bool use, int64_t idx) {
 // carried in registers from the previous plane.
 // ---------------------------------------------------------------------------
 template <int MODE, int PML, bool FS, int TY, bool SINGLE>
-__global__ __launch_bounds__(64 * TY) void k_stress(const StepArgs a) {
+__device__ __forceinline__ void stress_tile(const StepArgs& a, int lb) {
   const int lane = threadIdx.x, row = threadIdx.y;
-  int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = lb % a.ntx;
   lb /= a.ntx;
   const int tyi = lb % a.nty;
@@ -193,9 +192,8 @@ __global__ __launch_bounds__(64 * TY) void k_stress(const StepArgs a) {
 // SINGLE: Szz(k-1), Sxz(k) and Syz(k) are carried from the previous plane.
 // ---------------------------------------------------------------------------
 template <int MODE, int PML, int TY, bool SINGLE>
-__global__ __launch_bounds__(64 * TY) void k_velocity(const StepArgs a) {
+__device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb) {
   const int lane = threadIdx.x, row = threadIdx.y;
-  int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = lb % a.ntx;
   lb /= a.ntx;
   const int tyi = lb % a.nty;
@@ -280,6 +278,39 @@ __global__ __launch_bounds__(64 * TY) void k_velocity(const StepArgs a) {
       }
     }
   }
+}
+
+// One model per launch: workgroup -> tile of that model.
+template <int MODE, int PML, bool FS, int TY, bool SINGLE>
+__global__ __launch_bounds__(64 * TY) void k_stress(const StepArgs a) {
+  stress_tile<MODE, PML, FS, TY, SINGLE>(a, xcd_remap(blockIdx.x, gridDim.x));
+}
+template <int MODE, int PML, int TY, bool SINGLE>
+__global__ __launch_bounds__(64 * TY) void k_velocity(const StepArgs a) {
+  velocity_tile<MODE, PML, TY, SINGLE>(a, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// Many models per launch (an uneven batch of independent grids): the
+// workgroups of all models form one grid; start[m] is the first workgroup
+// of model m and tab[m] its step arguments (read from device memory).
+__device__ __forceinline__ int group_model(const int32_t* __restrict__ start, int nm, int lb) {
+  int m = 0;
+  while (m + 1 < nm && lb >= start[m + 1]) ++m;
+  return m;
+}
+template <int MODE, int PML, bool FS, int TY>
+__global__ __launch_bounds__(64 * TY) void k_stress_group(const StepArgs* __restrict__ tab,
+                                                         const int32_t* __restrict__ start, int32_t nm) {
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int m = group_model(start, nm, lb);
+  stress_tile<MODE, PML, FS, TY, true>(tab[m], lb - start[m]);
+}
+template <int MODE, int PML, int TY>
+__global__ __launch_bounds__(64 * TY) void k_velocity_group(const StepArgs* __restrict__ tab,
+                                                           const int32_t* __restrict__ start, int32_t nm) {
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int m = group_model(start, nm, lb);
+  velocity_tile<MODE, PML, TY, true>(tab[m], lb - start[m]);
 }
 
 // Step counter of a captured time-loop graph: one lane advances it after the
@@ -380,6 +411,44 @@ hipError_t launch_velocity(const StepArgs& a, int mode, int pml, hipStream_t s) 
     case PML_NONE: return go_vel<MODE_ORTHO, PML_NONE>(a, s);
     case PML_W3D: return go_vel<MODE_ORTHO, PML_W3D>(a, s);
     case PML_SEP: return go_vel<MODE_ORTHO, PML_SEP>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int MODE, int PML, bool FS>
+static hipError_t go_group(const StepArgs* tab, const int32_t* start, int nm, int nblk, int ty, bool vel,
+                           hipStream_t s) {
+  if (ty == 2) {
+    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, 2>), dim3(nblk), dim3(64, 2), 0, s, tab, start, nm);
+    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, 2>), dim3(nblk), dim3(64, 2), 0, s, tab, start, nm);
+  } else if (ty == 4) {
+    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, 4>), dim3(nblk), dim3(64, 4), 0, s, tab, start, nm);
+    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, 4>), dim3(nblk), dim3(64, 4), 0, s, tab, start, nm);
+  } else if (ty == 8) {
+    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, 8>), dim3(nblk), dim3(64, 8), 0, s, tab, start, nm);
+    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, 8>), dim3(nblk), dim3(64, 8), 0, s, tab, start, nm);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_group(const StepArgs* tab, const int32_t* start, int nm, int nblk, int ty, int mode, int pml,
+                        int fs, bool vel, hipStream_t s) {
+  if (mode == MODE_ISO) {
+    if (pml == PML_PROFILE) return fs ? go_group<MODE_ISO, PML_PROFILE, true>(tab, start, nm, nblk, ty, vel, s)
+                                      : go_group<MODE_ISO, PML_PROFILE, false>(tab, start, nm, nblk, ty, vel, s);
+    if (pml == PML_NONE) return fs ? go_group<MODE_ISO, PML_NONE, true>(tab, start, nm, nblk, ty, vel, s)
+                                   : go_group<MODE_ISO, PML_NONE, false>(tab, start, nm, nblk, ty, vel, s);
+    return hipErrorInvalidValue;
+  }
+  switch (pml) {
+    case PML_NONE: return fs ? go_group<MODE_ORTHO, PML_NONE, true>(tab, start, nm, nblk, ty, vel, s)
+                             : go_group<MODE_ORTHO, PML_NONE, false>(tab, start, nm, nblk, ty, vel, s);
+    case PML_W3D: return fs ? go_group<MODE_ORTHO, PML_W3D, true>(tab, start, nm, nblk, ty, vel, s)
+                            : go_group<MODE_ORTHO, PML_W3D, false>(tab, start, nm, nblk, ty, vel, s);
+    case PML_SEP: return fs ? go_group<MODE_ORTHO, PML_SEP, true>(tab, start, nm, nblk, ty, vel, s)
+                            : go_group<MODE_ORTHO, PML_SEP, false>(tab, start, nm, nblk, ty, vel, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -105,6 +105,12 @@ SIGNATURES = {
     "fwe_load_legacy_inputs": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(LegacyParams)]),
     "fwe_write_legacy_outputs": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_char_p), C.c_int32]),
     "fwe_copy_probe": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, _DP]),
+    "fwe_group_create": (C.c_int, [C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "fwe_group_destroy": (C.c_int, [C.c_void_p]),
+    "fwe_group_run": (C.c_int, [C.c_void_p, C.c_int32]),
+    "fwe_group_run_timed": (C.c_int, [C.c_void_p, C.c_int32, _DP]),
+    "fwe_group_set_tiling": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "fwe_group_autotune": (C.c_int, [C.c_void_p, C.c_int32, _DP]),
     # generators (fwe_gen.h)
     "fwe_rng_seed": (None, [C.POINTER(RNG), C.c_uint64]),
     "fwe_rng_next": (C.c_uint64, [C.POINTER(RNG)]),
@@ -331,3 +337,41 @@ def copy_probe(nbytes, reps=20, device=0):
     ms = C.c_double()
     _check(lib().fwe_copy_probe(device, nbytes, reps, C.byref(ms)))
     return ms.value
+
+
+class Group:
+    """fwe_group: several single-wavefield Solvers advanced by one launch per
+    half-step.  The solvers must stay alive while the group is used."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        arr = (C.c_void_p * len(self.solvers))(*[s._h.value for s in self.solvers])
+        self._h = C.c_void_p()
+        _check(lib().fwe_group_create(len(self.solvers), arr, C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().fwe_group_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, nsteps):
+        _check(lib().fwe_group_run(self._h, nsteps))
+
+    def run_timed(self, nsteps):
+        ms = C.c_double()
+        _check(lib().fwe_group_run_timed(self._h, nsteps, C.byref(ms)))
+        return ms.value
+
+    def set_tiling(self, tile_rows, zchunk):
+        _check(lib().fwe_group_set_tiling(self._h, tile_rows, zchunk))
+
+    def autotune(self, trial_steps=3):
+        ms = C.c_double()
+        _check(lib().fwe_group_autotune(self._h, trial_steps, C.byref(ms)))
+        return ms.value

@@ -263,6 +263,8 @@ def config(idx, **kw):
         kw.setdefault("nx", 512), kw.setdefault("ny", 512), kw.setdefault("nz", 256)
         kw.setdefault("nt", 4000), kw.setdefault("nrec_side", 16)
         return ortho_layered(elementary=True, **kw)
+    if idx == 3:
+        return uneven_batch(**kw)
     raise ValueError(idx)
 
 
@@ -282,3 +284,47 @@ def alg_bytes(mode, pml, nbatch=1, fused=False):
         return 4 * (nbatch * 15 + 2), 4 * (nbatch * 12)
     # stress: R V3 S6 C9 [w], W S6 ; velocity: R S6 V3 b3 [w], W V3
     return 4 * (nbatch * 15 + 9 + w), 4 * (nbatch * 12 + 3 + w)
+
+
+def random_field(nx, ny, nz, corr, seed):
+    out = _fa(nx * ny * nz)
+    fwe.lib().fwe_gen_random_field(nx, ny, nz, corr, seed, fwe.fptr(out))
+    return out.reshape((nx, ny, nz), order="F")
+
+
+def iso_random(n, seed, vp0=5500.0, vs0=3175.0, rho0=2650.0, pert=0.08, corr=10.0, dx=50.0, L=16, nt=1500,
+               f0=None, nrec_side=4):
+    """Heterogeneous isotropic cube: background vp/vs/rho times (1 + pert*g),
+    g a unit-variance Gaussian random field of correlation length `corr`
+    cells; run through the ortho law (C11=C22=C33, C12=C13=C23, harmonic
+    edge shear moduli, staggered buoyancy) as the reference's iso-het
+    station variants are.  Own lune MT, Ricker f0 ~ U[2,5] Hz, separable
+    damping on all faces."""
+    rng = np.random.default_rng(seed)
+    g = random_field(n, n, n, corr, seed * 7 + 3).astype(np.float64)
+    s = 1.0 + pert * g
+    vp, vs, rho = vp0 * s, vs0 * s, rho0 * s
+    c11 = (rho * vp * vp).astype(np.float32)
+    c12 = (rho * (vp * vp - 2 * vs * vs)).astype(np.float32)
+    c44 = (rho * vs * vs).astype(np.float32)
+    cij, buoy = ortho_from_cells([c11, c11, c11, c12, c12, c12, c44, c44, c44], (1.0 / rho).astype(np.float32))
+    dt = cfl_dt(float(vp.max()), dx, dx, dx, 0.5)
+    f0 = float(rng.uniform(2.0, 5.0)) if f0 is None else f0
+    prof = tuple(cell_profile(pml_profile(n, L, dt)) for _ in range(3))
+    stf = ricker(nt, dt, f0, 1.2 / f0) * np.float32(dt / dx ** 3)
+    m, gam, dl = mt_lune_random(seed * 131 + 7, 1e15)
+    c = n // 2 - 1
+    inj = fwe.mt_injections(fwe.SRC_LAYOUT_OK4, c, c, c, m)
+    xs = np.linspace(L + 4, n - L - 4, nrec_side).round().astype(int)
+    rec = _rec_grid(xs, xs, L + 2)
+    return Workload(f"isorand{n}", n, n, n, dx, dx, dx, dt, nt, fwe.MODE_ORTHO, fwe.PML_SEPARABLE,
+                    cij=cij, buoy=buoy, prof=prof, stf=[stf], injections=inj, receivers=rec,
+                    info=dict(f0=f0, mt=m.tolist(), seed=seed, pml_cells=L))
+
+
+def uneven_batch(nmodels=32, seed=11, edges=(96, 128, 160, 192, 256), nt=1500, **kw):
+    """An uneven batch of independent heterogeneous isotropic cubes with
+    seeded edge lengths drawn from `edges`."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice(np.asarray(edges), size=nmodels)
+    return [iso_random(int(n), seed * 1000 + q, nt=nt, **kw) for q, n in enumerate(sizes)]

@@ -252,6 +252,25 @@ fwe_status fwe_load_legacy_inputs(fwe_ctx* ctx, const char* dir,
 fwe_status fwe_write_legacy_outputs(fwe_ctx* ctx, const char* dir,
                                     const char* const* rec_names, int32_t n_names);
 
+/* ---- groups of independent models ---------------------------------------- */
+/* The reference runs one model per process; a group advances n fully set-up
+ * single-wavefield contexts (same device, law, damping kind and free-surface
+ * flag; grids, media, sources and receivers free) with ONE stress launch and
+ * ONE velocity launch per step covering every model's tiles, so an uneven
+ * batch of small grids fills the GPU.  Members keep their own fields and
+ * traces (read them with fwe_get_traces / fwe_get_field); their step
+ * counters advance together.  The group uses the two-pass kernels and one
+ * tile shape for all members.  No counterpart in the reference. */
+typedef struct fwe_group fwe_group;
+fwe_status fwe_group_create(int32_t n, fwe_ctx* const* members, fwe_group** out);
+fwe_status fwe_group_destroy(fwe_group* g);
+fwe_status fwe_group_run(fwe_group* g, int32_t nsteps);
+/* as fwe_group_run; *ms = GPU milliseconds of the nsteps steps */
+fwe_status fwe_group_run_timed(fwe_group* g, int32_t nsteps, double* ms);
+fwe_status fwe_group_set_tiling(fwe_group* g, int32_t tile_rows, int32_t zchunk);
+/* time each tile shape, keep the fastest, reset every member */
+fwe_status fwe_group_autotune(fwe_group* g, int32_t trial_steps, double* best_ms_per_step);
+
 /* ---- measurement -------------------------------------------------------- */
 /* Stream-copy probe: copies `bytes` bytes device->device `reps` times with a
  * plain vector copy kernel and returns the mean ms per copy (the memory-copy

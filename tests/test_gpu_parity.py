@@ -337,3 +337,34 @@ def test_fused_batch_and_graph(gpu):
         for f in range(9):
             assert rel_l2(fb[b][f], s2.get_field(f, b)) < 2e-6, (b, f)
     s2.close()
+
+
+def test_group_equals_separate_runs(gpu):
+    """An uneven group of models advanced by one launch per half-step gives
+    every member exactly the traces of its separate run; autotune over the
+    group keeps that."""
+    from elanfmt_amd import fwe, workloads as W
+
+    wls = [W.iso_random(n, seed=q, L=5, nt=40, f0=15.0, nrec_side=2) for q, n in enumerate((36, 44, 52))]
+    ref = []
+    for wl in wls:
+        s = W.make_solver(wl)
+        s.run(wl.nt)
+        ref.append(s.traces())
+        s.close()
+    sol = [W.make_solver(wl) for wl in wls]
+    g = fwe.Group(sol)
+    g.run(15)
+    g.run(25)
+    for s, r in zip(sol, ref):
+        assert s.step() == 40
+        assert np.abs(r).max() > 0
+        assert np.array_equal(s.traces(), r)
+    ms = g.autotune(2)
+    assert ms > 0 and all(s.step() == 0 for s in sol)
+    g.run(40)
+    for s, r in zip(sol, ref):
+        assert np.array_equal(s.traces(), r)
+    g.close()
+    for s in sol:
+        s.close()
