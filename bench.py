@@ -1,13 +1,24 @@
 """Benchmark of the FDTD time loop on one MI355X.
 
-Workload (default): the layered orthorhombic 384x384x192 FP32 box with
-24-cell separable damping on all faces, a lune-uniform moment tensor,
-Ricker 3 Hz and 64 three-component receivers; one step = one full
+Default workload (--config 1): the layered orthorhombic 384x384x192 FP32
+box with 24-cell separable damping on all faces, a lune-uniform moment
+tensor, Ricker 3 Hz and 64 three-component receivers; one step = one full
 velocity-stress update of all 28.3M cells.  Inputs are resident in HBM
 before the timed region.  Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps 3000] [--warmup 50] [--config 1]
+  python bench.py [--gpus 1] [--steps K] [--warmup 50] [--config 1]
                   [--kernel auto|two_pass|fused] [--cpu-steps 300] [--no-cpu]
+
+Other configurations:
+  --config 0  isotropic homogeneous 128^3, 600 steps (a parity case; fits
+              in the last-level cache, so not a bandwidth figure)
+  --config 2  six elementary-tensor wavefields over one 512x512x256 medium
+              in one batched launch per half-step, against six plain runs
+              of the same medium; then 1000 lune tensors synthesised by
+              linear combination and 10 of them checked against direct runs
+  --config 3  32 isotropic random-field cubes (edges 96..256) advanced as
+              one group (one launch per half-step over all models), against
+              the same models run one after the other
 """
 import argparse
 import json
@@ -15,10 +26,14 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+METRIC = "Gcell-updates/s + % of 8 TB/s HBM; batched-6-MT vs 6 plain runs at equal error"
+DEFAULT_STEPS = {0: 600, 1: 3000, 2: 1000, 3: 1500}
 
 
 def _traffic(kernel_name, workload):
@@ -34,15 +49,14 @@ def _traffic(kernel_name, workload):
         return None
 
 
-def cpu_baseline(wl_builder, steps, threads):
+def cpu_baseline(wl, steps, threads):
     """The FP64 oracle (CPU restatement of the reference step, OpenMP over z
-    planes) on a bounded number of steps of the same workload."""
+    planes) on a bounded number of steps of one wavefield of the workload."""
     from elanfmt_amd import workloads as W
     from oracle import Oracle
 
-    wl = wl_builder()
-    o = Oracle(wl.nx, wl.ny, wl.nz, wl.dx, wl.dy, wl.dz, wl.dt, wl.nt, mode=wl.mode, pml=wl.pml,
-               free_surface=wl.free_surface)
+    o = Oracle(wl.nx, wl.ny, wl.nz, wl.dx, wl.dy, wl.dz, wl.dt, max(wl.nt, steps + 2), mode=wl.mode,
+               pml=wl.pml, free_surface=wl.free_surface)
     W.apply(o, wl)
     o.run(2, threads)  # first touch
     t0 = time.perf_counter()
@@ -52,12 +66,227 @@ def cpu_baseline(wl_builder, steps, threads):
     return wl.cells * steps / el / 1e9, el
 
 
+def _cpu_entry(args, wl_single, name):
+    if args.no_cpu or args.cpu_steps <= 0:
+        return None
+    threads = min(16, os.cpu_count() or 1)
+    v, el = cpu_baseline(wl_single, args.cpu_steps, threads)
+    return {"value": round(v, 5), "unit": "Gcell/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle, {args.cpu_steps} steps of {name} ({el:.1f} s), OpenMP over z planes"}
+
+
+def _timed_run(solver, steps, warmup, autotune):
+    """autotune (library autotuner: every tile shape timed, fastest kept,
+    state reset) -> warmup -> reset -> timed run.  Returns wall seconds and
+    the per-kernel event times (ms)."""
+    tune = solver.autotune(3) if autotune else None
+    if warmup:
+        solver.run(warmup)
+    solver.reset()
+    solver.synchronize()
+    t0 = time.perf_counter()
+    ms_a, ms_b, _ = solver.run_timed(steps)
+    solver.synchronize()
+    return time.perf_counter() - t0, ms_a, ms_b, tune
+
+
+def _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells):
+    from elanfmt_amd import workloads as W
+
+    ba, bb = W.alg_bytes(wl.mode, wl.pml, wl.nbatch, fused=fused)
+    fs = "true" if wl.free_surface else "false"
+    if fused:
+        kernels = {"fused": {"ms_per_step": round(ms_a / steps, 5), "alg_bytes_per_cell": ba,
+                             "GBs": round(ba * cells / (ms_a / steps * 1e-3) / 1e9, 1)}}
+        return kernels, ("fused", ms_a, ba, f"k_fused<{wl.mode}, {wl.pml}, {tiling[0]}>"), ba + bb
+    kernels = {
+        "stress": {"ms_per_launch": round(ms_a / steps, 5), "alg_bytes_per_cell": ba,
+                   "GBs": round(ba * cells / (ms_a / steps * 1e-3) / 1e9, 1)},
+        "velocity": {"ms_per_launch": round(ms_b / steps, 5), "alg_bytes_per_cell": bb,
+                     "GBs": round(bb * cells / (ms_b / steps * 1e-3) / 1e9, 1)},
+    }
+    if ms_a >= ms_b:
+        dom = ("stress", ms_a, ba, f"k_stress<{wl.mode}, {wl.pml}, {fs}, {tiling[0]}>")
+    else:
+        dom = ("velocity", ms_b, bb, f"k_velocity<{wl.mode}, {wl.pml}, {tiling[0]}>")
+    return kernels, dom, ba + bb
+
+
+def _roofline(dom, steps, cells, wname):
+    name, ms, b, kname = dom
+    ach = b * cells / (ms / steps * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": _traffic(kname, wname)}
+
+
+def _line(value, steps, warmup, el, config, **extra):
+    out = {"metric": METRIC, "value": round(value, 3), "unit": "Gcell/s", "n_gpus": 1, "steps": steps,
+           "warmup": warmup, "ms_per_step": round(el / steps * 1e3, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": config}
+    out.update({k: v for k, v in extra.items() if v is not None})
+    return out
+
+
+def bench_plain(args, steps):
+    """One wavefield, the plain two-pass (or single-pass) kernel."""
+    from elanfmt_amd import fwe, workloads as W
+
+    wl = W.config(args.config, nt=max(steps, 1))
+    kind = {"auto": fwe.KERNEL_AUTO, "two_pass": fwe.KERNEL_TWO_PASS, "fused": fwe.KERNEL_FUSED}[args.kernel]
+    copy_ms = fwe.copy_probe(1 << 30, reps=10)
+    copy_gbs = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
+    s = W.make_solver(wl)
+    s.set_kernel(kind)
+    fused = s.kernel() == fwe.KERNEL_FUSED
+    el, ms_a, ms_b, tune = _timed_run(s, steps, args.warmup, not args.no_autotune)
+    finite = bool(np.isfinite(s.traces()).all())
+    tiling = s.tiling()
+    s.close()
+    cells = wl.cells
+    kernels, dom, bstep = _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells)
+    desc = (f"{wl.name}: layered orthorhombic FP32, {wl.info.get('pml_cells')}-cell separable damping, "
+            f"lune MT, Ricker {wl.info.get('f0')} Hz, {len(wl.receivers)} receiver traces"
+            if wl.mode == fwe.MODE_ORTHO else
+            f"{wl.name}: isotropic homogeneous FP32, {wl.info.get('pml_cells')}-cell profile damping, "
+            f"double couple, Ricker {wl.info.get('f0')} Hz, {len(wl.receivers)} receiver traces")
+    config = {"workload": desc, "grid": [wl.nx, wl.ny, wl.nz], "wavefields": wl.nbatch, "nt": wl.nt,
+              "kernel": "fused single-pass" if fused else "two-pass", "tiling": list(tiling),
+              "autotune_ms_per_step": None if tune is None else round(tune, 4), "parallelism": "single GPU"}
+    return _line(cells * wl.nbatch * steps / el / 1e9, steps, args.warmup, el, config,
+                 hbm_pct_step=round(100 * bstep * cells * steps / el / 1e9 / PEAK_HBM_GBS, 2),
+                 kernels=kernels, roofline=_roofline(dom, steps, cells, wl.name),
+                 copy_probe_GBs=round(copy_gbs, 1), traces_finite=finite,
+                 cpu_baseline=_cpu_entry(args, W.config(args.config, nt=args.cpu_steps + 2), wl.name))
+
+
+def bench_batched(args, steps):
+    """Six elementary-tensor wavefields sharing the 512x512x256 medium in one
+    batched launch per half-step, against six plain single-wavefield runs
+    over the same medium (one per elementary tensor, run one after the
+    other); then 1000 seeded lune tensors by linear combination, 10 checked
+    against direct single-tensor runs."""
+    from elanfmt_amd import fwe, workloads as W
+
+    m0 = 1e15
+    wl = W.config(2, nt=max(steps, 1), m0=m0)
+    cells = wl.cells
+    print(f"# batched: {wl.name} x{wl.nbatch}, {steps} steps", file=sys.stderr, flush=True)
+    s = W.make_solver(wl)
+    el_b, ms_a, ms_b, tune = _timed_run(s, steps, args.warmup, not args.no_autotune)
+    tiling = s.tiling()
+    kernels, dom, bstep = _kernel_entries(wl, False, tiling, ms_a, ms_b, steps, cells)
+    # 1000 lune tensors by linear combination of the six elementary runs
+    mts = np.array([W.mt_lune_random(50000 + q, m0)[0] for q in range(1000)])
+    t0 = time.perf_counter()
+    synth = s.combine_traces(mts / m0)
+    el_syn = time.perf_counter() - t0
+    s.close()
+
+    # six plain runs, one per elementary tensor
+    nr = len(wl.receivers) // 6
+    el_p, plain_tiling = 0.0, None
+    xs, ys, zs = wl.nx // 2 - 1, wl.ny // 2 - 1, wl.nz // 2 - 1
+    base = W.config(2, nt=max(steps, 1), m0=m0)
+    base.nbatch, base.stf = 1, base.stf[:1]
+    base.receivers = [t for t in base.receivers if t[4] == 0]
+    tuned = None
+    for e in range(6):
+        me = np.zeros(6)
+        me[e] = m0
+        base.injections = [t for t in fwe.mt_injections(fwe.SRC_LAYOUT_OK4, xs, ys, zs, me) if t[5] != 0]
+        p = W.make_solver(base)
+        if tuned is None:
+            p.autotune(3)
+            tuned = p.tiling()
+        else:
+            p.set_tiling(*tuned)
+        el, _, _, _ = _timed_run(p, steps, args.warmup if e == 0 else 0, False)
+        el_p += el
+        plain_tiling = p.tiling()
+        p.close()
+    print(f"# batched {el_b:.2f} s, six plain {el_p:.2f} s", file=sys.stderr, flush=True)
+
+    # 10 direct single-tensor runs against the synthesised traces
+    errs = []
+    for q in range(10):
+        base.injections = fwe.mt_injections(fwe.SRC_LAYOUT_OK4, xs, ys, zs, mts[q])
+        d = W.make_solver(base)
+        d.set_tiling(*tuned)
+        d.run(steps)
+        tr = d.traces().astype(np.float64)
+        d.close()
+        sy = synth[q].astype(np.float64)
+        errs.append(float(np.linalg.norm(sy - tr) / max(np.linalg.norm(tr), 1e-300)))
+    config = {"workload": f"{wl.name}: layered orthorhombic FP32, 24-cell separable damping, six elementary "
+                          f"moment tensors (Mxx..Myz) as one batch of wavefields, {nr} receiver traces each",
+              "grid": [wl.nx, wl.ny, wl.nz], "wavefields": 6, "nt": wl.nt, "kernel": "two-pass batched",
+              "tiling": list(tiling), "autotune_ms_per_step": None if tune is None else round(tune, 4),
+              "parallelism": "single GPU"}
+    return _line(cells * 6 * steps / el_b / 1e9, steps, args.warmup, el_b, config,
+                 hbm_pct_step=round(100 * bstep * cells * steps / el_b / 1e9 / PEAK_HBM_GBS, 2),
+                 kernels=kernels, roofline=_roofline(dom, steps, cells, wl.name + "x6"),
+                 batched_vs_plain={"batched_s": round(el_b, 3), "six_plain_runs_s": round(el_p, 3),
+                                   "speedup": round(el_p / el_b, 3), "plain_tiling": list(plain_tiling)},
+                 synthesis={"tensors": len(mts), "seconds": round(el_syn, 3),
+                            "checked_direct_runs": len(errs), "max_rel_l2": float(f"{max(errs):.3e}")})
+
+
+def bench_uneven(args, steps):
+    """32 random-field cubes as one group (one stress and one velocity launch
+    per step over all models' tiles) against the same models run one after
+    the other, each with its own autotuned tiling."""
+    from elanfmt_amd import fwe, workloads as W
+
+    wls = W.config(3, nt=max(steps, 1))
+    cells = sum(w.cells for w in wls)
+    print(f"# uneven: {len(wls)} models, {cells / 1e6:.1f} M cells, {steps} steps", file=sys.stderr, flush=True)
+    sol = [W.make_solver(w) for w in wls]
+    g = fwe.Group(sol)
+    tune = g.autotune(3) if not args.no_autotune else None
+    g.run(args.warmup)
+    for s in sol:
+        s.reset()
+        s.synchronize()
+    t0 = time.perf_counter()
+    g.run_timed(steps)
+    for s in sol:
+        s.synchronize()
+    el_g = time.perf_counter() - t0
+    ms_grp = [0.0, 0.0]
+    tr_g = [s.traces() for s in sol]
+    g.close()
+    tiling = sol[0].tiling()
+    el_seq = 0.0
+    for s, w in zip(sol, wls):
+        el, a, b, _ = _timed_run(s, steps, 0, not args.no_autotune)
+        el_seq += el
+        ms_grp[0] += a
+        ms_grp[1] += b
+    same = all(np.array_equal(s.traces(), t) for s, t in zip(sol, tr_g))
+    for s in sol:
+        s.close()
+    from elanfmt_amd import workloads as Wm
+
+    ba, bb = Wm.alg_bytes(fwe.MODE_ORTHO, fwe.PML_SEPARABLE)
+    config = {"workload": f"uneven batch: {len(wls)} isotropic random-field cubes (edges "
+                          f"{sorted(set(w.nx for w in wls))}), ortho law, 16-cell separable damping, own lune MT "
+                          f"and Ricker f0 each", "models": len(wls), "cells": cells, "nt": steps,
+              "kernel": "two-pass group", "tiling": list(tiling),
+              "autotune_ms_per_step": None if tune is None else round(tune, 4), "parallelism": "single GPU"}
+    return _line(cells * steps / el_g / 1e9, steps, args.warmup, el_g, config,
+                 hbm_pct_step=round(100 * (ba + bb) * cells * steps / el_g / 1e9 / PEAK_HBM_GBS, 2),
+                 group_vs_sequential={"group_s": round(el_g, 3), "sequential_s": round(el_seq, 3),
+                                      "speedup": round(el_seq / el_g, 3), "traces_identical": same,
+                                      "sequential_stress_ms": round(ms_grp[0], 2),
+                                      "sequential_velocity_ms": round(ms_grp[1], 2)})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 3])
     ap.add_argument("--kernel", default="auto", choices=["auto", "two_pass", "fused"])
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
@@ -65,100 +294,22 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the z-slab decomposition it would use is described in DESIGN.md); run with --gpus 1",
-              file=sys.stderr)
+              "(models and wavefields would be placed on GPUs whole, a single large model split in z-slabs); "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    from elanfmt_amd import fwe, workloads as W
+    from elanfmt_amd import fwe
 
     if fwe.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
-    steps = args.steps
-    wl = W.config(args.config, nt=max(steps, 1))
-    kind = {"auto": fwe.KERNEL_AUTO, "two_pass": fwe.KERNEL_TWO_PASS, "fused": fwe.KERNEL_FUSED}[args.kernel]
-    # copy probe: achievable stream bandwidth on this device (1 GiB copy)
-    copy_ms = fwe.copy_probe(1 << 30, reps=10)
-    copy_gbs = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
-
-    s = W.make_solver(wl)
-    s.set_kernel(kind)
-    fused = s.kernel() == fwe.KERNEL_FUSED
-    tune_ms = None
-    if not args.no_autotune:
-        # library autotuner: times each tile shape, keeps the fastest, resets
-        tune_ms = s.autotune(3)
-    s.run(args.warmup)
-    s.synchronize()
-    s.reset()
-    s.synchronize()
-    t0 = time.perf_counter()
-    ms_a, ms_b, ms_w = s.run_timed(steps)
-    s.synchronize()
-    el = time.perf_counter() - t0
-    tr = s.traces()
-    finite = bool(__import__("numpy").isfinite(tr).all())
-    tiling = s.tiling()
-    s.close()
-
-    cells = wl.cells
-    ba, bb = W.alg_bytes(wl.mode, wl.pml, wl.nbatch, fused=fused)
-    gcells = cells * wl.nbatch * steps / el / 1e9
-    if fused:
-        kernels = {"fused": {"ms_per_step": round(ms_a / steps, 5), "alg_bytes_per_cell": ba,
-                             "GBs": round(ba * cells / (ms_a / steps * 1e-3) / 1e9, 1)}}
-        dom, ms_dom, b_dom = "fused", ms_a, ba
-        kname = f"k_fused<{wl.mode}, {wl.pml}, {tiling[0]}>"
+    steps = args.steps if args.steps is not None else DEFAULT_STEPS[args.config]
+    if args.config in (0, 1):
+        out = bench_plain(args, steps)
+    elif args.config == 2:
+        out = bench_batched(args, steps)
     else:
-        kernels = {
-            "stress": {"ms_per_launch": round(ms_a / steps, 5), "alg_bytes_per_cell": ba,
-                       "GBs": round(ba * cells / (ms_a / steps * 1e-3) / 1e9, 1)},
-            "velocity": {"ms_per_launch": round(ms_b / steps, 5), "alg_bytes_per_cell": bb,
-                         "GBs": round(bb * cells / (ms_b / steps * 1e-3) / 1e9, 1)},
-        }
-        dom, ms_dom, b_dom = ("stress", ms_a, ba) if ms_a >= ms_b else ("velocity", ms_b, bb)
-        kname = (f"k_stress<{wl.mode}, {wl.pml}, {'true' if wl.free_surface else 'false'}, {tiling[0]}>"
-                 if dom == "stress" else f"k_velocity<{wl.mode}, {wl.pml}, {tiling[0]}>")
-    per_launch_s = ms_dom / steps * 1e-3
-    ach = b_dom * cells / per_launch_s / 1e9
-    step_gbs = (ba + bb) * cells * steps / el / 1e9
-    wname = wl.name
-    out = {
-        "metric": "Gcell-updates/s + % of 8 TB/s HBM; batched-6-MT vs 6 plain runs at equal error",
-        "value": round(gcells, 3),
-        "unit": "Gcell/s",
-        "n_gpus": 1,
-        "steps": steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(el / steps * 1e3, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": (f"{wname}: layered orthorhombic FP32, {wl.info.get('pml_cells')}-cell separable damping, "
-                         f"lune MT, Ricker {wl.info.get('f0')} Hz, {len(wl.receivers)} receiver traces")
-            if wl.mode == fwe.MODE_ORTHO else f"{wname}: isotropic homogeneous FP32",
-            "grid": [wl.nx, wl.ny, wl.nz], "wavefields": wl.nbatch, "nt": wl.nt,
-            "kernel": "fused single-pass" if fused else "two-pass", "tiling": list(tiling),
-            "autotune_ms_per_step": None if tune_ms is None else round(tune_ms, 4),
-            "parallelism": "single GPU",
-        },
-        "hbm_pct_step": round(100 * step_gbs / PEAK_HBM_GBS, 2),
-        "kernels": kernels,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 4),
-                     "traffic": _traffic(kname, wname)},
-        "copy_probe_GBs": round(copy_gbs, 1),
-        "traces_finite": finite,
-    }
-    if not args.no_cpu and args.cpu_steps > 0:
-        threads = min(16, os.cpu_count() or 1)
-        v, el_cpu = cpu_baseline(lambda: W.config(args.config, nt=args.cpu_steps + 2), args.cpu_steps, threads)
-        out["cpu_baseline"] = {"value": round(v, 5), "unit": "Gcell/s", "cores": threads, "kind": "port",
-                               "sample": f"FP64 oracle, {args.cpu_steps} steps of {wname} "
-                                         f"({el_cpu:.1f} s), OpenMP over z planes"}
+        out = bench_uneven(args, steps)
     print(json.dumps(out), flush=True)
 
 

@@ -813,6 +813,44 @@ extern "C" fwe_status fwe_get_traces(fwe_ctx* x, float* out) {
   return FWE_OK;
 }
 
+extern "C" fwe_status fwe_combine_traces(fwe_ctx* x, int32_t n, const float* coef, float* out) {
+  if (!x || n < 0 || (n > 0 && (!coef || !out))) return fail(FWE_ERR_ARG, "bad combine arguments");
+  const int nb = x->cfg.nbatch, nt = x->cfg.nt;
+  if (x->nrec == 0 || x->nrec % nb) return fail(FWE_ERR_STATE, "receivers must be %d equal per-wavefield sets", nb);
+  const int nr = x->nrec / nb;
+  for (int b = 0; b < nb; ++b)
+    for (int r = 0; r < nr; ++r) {
+      const PointEntry& p = x->rec_host[(size_t)b * nr + r];
+      const PointEntry& p0 = x->rec_host[r];
+      if (p.batch != b || p.comp != p0.comp || p.i != p0.i || p.j != p0.j || p.k != p0.k)
+        return fail(FWE_ERR_STATE, "receiver %d of wavefield %d does not repeat receiver %d of wavefield 0",
+                    b * nr + r, b, r);
+    }
+  if (n == 0 || nt == 0) return FWE_OK;
+  HIPCHK(hipSetDevice(x->cfg.device));
+  const int64_t per = (int64_t)nr * nt;
+  // output staged through a bounded device buffer (<= 256 MiB or one tensor)
+  const int mc = (int)std::max<int64_t>(1, std::min<int64_t>(n, (int64_t(256) << 20) / (4 * per)));
+  float *dc = nullptr, *dout = nullptr;
+  fwe_status st = FWE_OK;
+  hipError_t e = hipMalloc(&dc, sizeof(float) * (size_t)mc * nb);
+  if (e == hipSuccess) e = hipMalloc(&dout, sizeof(float) * (size_t)mc * per);
+  for (int m0 = 0; m0 < n && e == hipSuccess; m0 += mc) {
+    const int m = std::min(mc, n - m0);
+    e = hipMemcpyAsync(dc, coef + (size_t)m0 * nb, sizeof(float) * (size_t)m * nb, hipMemcpyHostToDevice,
+                       x->stream);
+    if (e == hipSuccess) e = fwe::launch_combine(x->traces, nb, per, dc, m, dout, x->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(out + (size_t)m0 * per, dout, sizeof(float) * (size_t)m * per, hipMemcpyDeviceToHost,
+                         x->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
+  }
+  if (e != hipSuccess) st = fail(FWE_ERR_HIP, "combine: %s", hipGetErrorString(e));
+  hipFree(dc);
+  hipFree(dout);
+  return st;
+}
+
 extern "C" fwe_status fwe_get_field(fwe_ctx* x, int32_t batch, int32_t field, float* out) {
   if (!x || !out) return fail(FWE_ERR_ARG, "NULL argument");
   if (batch < 0 || batch >= x->cfg.nbatch || field < 0 || field > 8) return fail(FWE_ERR_ARG, "bad batch/field");
@@ -1124,6 +1162,7 @@ static fwe_status group_run_impl(fwe_group* g, int32_t nsteps, double* ms) {
   HIPCHK(hipSetDevice(g->device));
   for (fwe_ctx* x : g->m) {
     if (x->it != g->m[0]->it) return fail(FWE_ERR_STATE, "members were advanced separately");
+    if (x->ty != g->ty) return fail(FWE_ERR_STATE, "a member was re-tiled outside the group");
     HIPCHK(hipStreamSynchronize(x->stream));
   }
   fwe_status st = group_table(g);
@@ -1163,6 +1202,9 @@ extern "C" fwe_status fwe_group_run_timed(fwe_group* g, int32_t nsteps, double* 
 
 extern "C" fwe_status fwe_group_set_tiling(fwe_group* g, int32_t tile_rows, int32_t zchunk) {
   if (!g) return fail(FWE_ERR_ARG, "NULL group");
+  if (tile_rows != 2 && tile_rows != 4 && tile_rows != 8) return fail(FWE_ERR_ARG, "tile rows must be 2, 4 or 8");
+  for (fwe_ctx* x : g->m)
+    if (std::min<int>(zchunk, x->cfg.nz) < 3) return fail(FWE_ERR_ARG, "z chunk must be at least 3");
   for (fwe_ctx* x : g->m) {
     fwe_status st = apply_tiling(x, tile_rows, std::min<int>(zchunk, x->cfg.nz));
     if (st) return st;

@@ -17,6 +17,7 @@
 // are read once and written once (18 streams) and the medium is read once
 // (9 moduli + 3 buoyancies): 30 FP32 streams against 39 for two passes.
 #include "fwe_internal.h"
+#include "fwe_stencil.h"
 
 namespace fwe {
 
@@ -52,7 +53,7 @@ __device__ __forceinline__ float src_at(const FusedArgs& a, int e0, int e1, floa
   float s = 0.f;
   for (int e = e0; e < e1; ++e) {
     const PointEntry p = a.inj[e];
-    if (p.comp == comp && p.i == i && p.j == j && p.k == k && p.batch == a.batch) s += p.coef * amp;
+    if (p.comp == comp && p.i == i && p.j == j && p.k == k && p.batch == a.batch) s = fmaf(p.coef, amp, s);
   }
   return s;
 }
@@ -150,9 +151,9 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_fused(const FusedArgs a) {
     const float so0 = ldz(a.Si[0], ri, c);
     const float so1 = ldz(a.Si[1], ri || rlo, c);
     const float so2 = ldz(a.Si[2], ri, c);
-    const float so3 = ldz(a.Si[3], (ri && jm || rhi) && im, c);
+    const float so3 = ldz(a.Si[3], ((ri && jm) || rhi) && im, c);
     const float so4 = ldz(a.Si[4], ri && im && km, c);
-    const float so5 = ldz(a.Si[5], (ri && jm || rhi) && km, c);
+    const float so5 = ldz(a.Si[5], ((ri && jm) || rhi) && km, c);
     float m0, m1, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f, m6 = 0.f, m7 = 0.f, m8 = 0.f;
     if (MODE == MODE_ISO) {
       m0 = ldz(a.C[0], ri || rlo, c);  // c11
@@ -214,28 +215,28 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_fused(const FusedArgs a) {
     float sxxL = 0.f, sxyR = 0.f, sxzR = 0.f;
     if (st) {
       const float ex = (vx_ip - vx_c) * a.rdx, ey = (vy_jp - vy_c) * a.rdy, ez = (vz_n - vz_c) * a.rdz;
-      const float dxy = (vy_c - vy_im) * a.rdx + (vx_c - vx_jm) * a.rdy;  // dVy/dx + dVx/dy
-      const float dxz = (vz_c - vz_im) * a.rdx + (vx_c - vx_p) * a.rdz;   // dVz/dx + dVx/dz
-      const float dyz = (vy_c - vy_p) * a.rdz + (vz_c - vz_jm) * a.rdy;   // dVy/dz + dVz/dy
+      const float dxy = rate2(vy_c - vy_im, a.rdx, vx_c - vx_jm, a.rdy);  // dVy/dx + dVx/dy
+      const float dxz = rate2(vz_c - vz_im, a.rdx, vx_c - vx_p, a.rdz);   // dVz/dx + dVx/dz
+      const float dyz = rate2(vy_c - vy_p, a.rdz, vz_c - vz_jm, a.rdy);   // dVy/dz + dVz/dy
       float wzk = 1.f, fzh = 1.f;
       if (PML == PML_PROFILE) { wzk = a.wz[m]; fzh = a.whz[m]; }
       if (PML == PML_SEP) { wzk = a.wz[m]; w_c = wxi * wyj * wzk; }
       if (interior) {
         // own cell, all six components (OK4.cu:307-329 / ANISO.cu:306-352)
         if (MODE == MODE_ISO) {
-          s0 = fxh * (so0 + (m0 * ex + m1 * ey + m1 * ez));
-          s1 = fyh * (so1 + (m1 * ex + m0 * ey + m1 * ez));
-          s2 = fzh * (so2 + (m1 * ex + m1 * ey + m0 * ez));
-          s3 = (wzk * wyj * wxi) * (so3 + a.c66dt * dxy);
-          s4 = (wyj * wzk * wxi) * (so4 + a.c66dt * dxz);
-          s5 = (wxi * wyj * wzk) * (so5 + a.c66dt * dyz);
+          s0 = iso_normal(fxh, so0, m0, ex, m1, ey, m1, ez);
+          s1 = iso_normal(fyh, so1, m1, ex, m0, ey, m1, ez);
+          s2 = iso_normal(fzh, so2, m1, ex, m1, ey, m0, ez);
+          s3 = iso_shear(wzk * wyj * wxi, so3, a.c66dt, dxy);
+          s4 = iso_shear(wyj * wzk * wxi, so4, a.c66dt, dxz);
+          s5 = iso_shear(wxi * wyj * wzk, so5, a.c66dt, dyz);
         } else {
-          s0 = so0 * w_c + (m0 * ex + m3 * ey + m4 * ez);
-          s1 = so1 * w_c + (m3 * ex + m1 * ey + m5 * ez);
-          s2 = so2 * w_c + (m4 * ex + m5 * ey + m2 * ez);
-          s3 = so3 * w_c + m8 * dxy;
-          s4 = so4 * w_c + m7 * dxz;
-          s5 = so5 * w_c + m6 * dyz;
+          s0 = ortho_normal(w_c, so0, m0, ex, m3, ey, m4, ez);
+          s1 = ortho_normal(w_c, so1, m3, ex, m1, ey, m5, ez);
+          s2 = ortho_normal(w_c, so2, m4, ex, m5, ey, m2, ez);
+          s3 = ortho_shear(w_c, so3, m8, dxy);
+          s4 = ortho_shear(w_c, so4, m7, dxz);
+          s5 = ortho_shear(w_c, so5, m6, dyz);
         }
         if (!(im && jm)) s3 = 0.f;
         if (!(im && km)) s4 = 0.f;
@@ -256,17 +257,17 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_fused(const FusedArgs a) {
         }
       } else if (halo_lo) {
         // Syy of row j0-1, needed by the Vy update of row j0
-        if (MODE == MODE_ISO) s1 = fyh * (so1 + (m1 * ex + m0 * ey + m1 * ez));
-        else s1 = so1 * w_c + (m3 * ex + m1 * ey + m5 * ez);
+        if (MODE == MODE_ISO) s1 = iso_normal(fyh, so1, m1, ex, m0, ey, m1, ez);
+        else s1 = ortho_normal(w_c, so1, m3, ex, m1, ey, m5, ez);
         if (have_src) s1 += src_at(a, e0, e1, amp, 1, i, j, m);
       } else {
         // Sxy and Syz of row j0+TY, needed by the Vx and Vz updates of row j0+TY-1
         if (MODE == MODE_ISO) {
-          s3 = (wzk * wyj * wxi) * (so3 + a.c66dt * dxy);
-          s5 = (wxi * wyj * wzk) * (so5 + a.c66dt * dyz);
+          s3 = iso_shear(wzk * wyj * wxi, so3, a.c66dt, dxy);
+          s5 = iso_shear(wxi * wyj * wzk, so5, a.c66dt, dyz);
         } else {
-          s3 = so3 * w_c + m8 * dxy;
-          s5 = so5 * w_c + m6 * dyz;
+          s3 = ortho_shear(w_c, so3, m8, dxy);
+          s5 = ortho_shear(w_c, so5, m6, dyz);
         }
         if (!im) s3 = 0.f;
         if (!km) s5 = 0.f;
@@ -279,29 +280,29 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_fused(const FusedArgs a) {
         // Sxx of column i0-1 (Vx update of lane 0)
         const float exl = (vx_c - xa0) * a.rdx, eyl = (xa2 - xa1) * a.rdy, ezl = (xa4 - xa3) * a.rdz;
         if (MODE == MODE_ISO) {
-          sxxL = a.whx[i - 1] * (xa5 + (xa6 * exl + xa7 * eyl + xa7 * ezl));
+          sxxL = iso_normal(a.whx[i - 1], xa5, xa6, exl, xa7, eyl, xa7, ezl);
         } else {
           float wl = 1.f;
           if (PML == PML_W3D) wl = xlw;
           if (PML == PML_SEP) wl = a.wx[i - 1] * wyj * wzk;
-          sxxL = xa5 * wl + (xa6 * exl + xa7 * eyl + xw * ezl);
+          sxxL = ortho_normal(wl, xa5, xa6, exl, xa7, eyl, xw, ezl);
         }
         if (have_src) sxxL += src_at(a, e0, e1, amp, 0, i - 1, j, m);
       }
       if (xr) {
         // Sxy and Sxz of column i0+64 (Vy and Vz updates of lane 63)
-        const float dxyr = (xb0 - vy_c) * a.rdx + (vx_r - xb1) * a.rdy;
-        const float dxzr = (xb2 - vz_c) * a.rdx + (vx_r - xb3) * a.rdz;
+        const float dxyr = rate2(xb0 - vy_c, a.rdx, vx_r - xb1, a.rdy);
+        const float dxzr = rate2(xb2 - vz_c, a.rdx, vx_r - xb3, a.rdz);
         if (MODE == MODE_ISO) {
           const float wxr = a.wx[i + 1];
-          sxyR = (wzk * wyj * wxr) * (xb4 + a.c66dt * dxyr);
-          sxzR = (wyj * wzk * wxr) * (xb5 + a.c66dt * dxzr);
+          sxyR = iso_shear(wzk * wyj * wxr, xb4, a.c66dt, dxyr);
+          sxzR = iso_shear(wyj * wzk * wxr, xb5, a.c66dt, dxzr);
         } else {
           float wr = 1.f;
           if (PML == PML_W3D) wr = xbw;
           if (PML == PML_SEP) wr = a.wx[i + 1] * wyj * wzk;
-          sxyR = xb4 * wr + xb6 * dxyr;
-          sxzR = xb5 * wr + xb7 * dxzr;
+          sxyR = ortho_shear(wr, xb4, xb6, dxyr);
+          sxzR = ortho_shear(wr, xb5, xb7, dxzr);
         }
         if (!jm) sxyR = 0.f;
         if (!km) sxzR = 0.f;
@@ -330,18 +331,18 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_fused(const FusedArgs a) {
       float wzm = 1.f;
       if (PML == PML_PROFILE) wzm = kv >= 1 ? a.wz[kv - 1] : 1.f;
       if (ux) {
-        const float dS = (sp0 - sxx_im) * a.rdx + (sxy_jp - sp3) * a.rdy + (s4 - sp4) * a.rdz;
-        vx = (MODE == MODE_ISO) ? wxm * (vx + bx * dS) : vx * w_p + bx * dS;
+        const float dS = rate3(sp0 - sxx_im, a.rdx, sxy_jp - sp3, a.rdy, s4 - sp4, a.rdz);
+        vx = (MODE == MODE_ISO) ? iso_vel(wxm, vx, bx, dS) : ortho_vel(w_p, vx, bx, dS);
         a.Vo[0][cv] = vx;
       }
       if (uy) {
-        const float dS = (sp1 - syy_jm) * a.rdy + (sxy_ip - sp3) * a.rdx + (s5 - sp5) * a.rdz;
-        vy = (MODE == MODE_ISO) ? wym * (vy + by * dS) : vy * w_p + by * dS;
+        const float dS = rate3(sp1 - syy_jm, a.rdy, sxy_ip - sp3, a.rdx, s5 - sp5, a.rdz);
+        vy = (MODE == MODE_ISO) ? iso_vel(wym, vy, by, dS) : ortho_vel(w_p, vy, by, dS);
         a.Vo[1][cv] = vy;
       }
       if (uz) {
-        const float dS = (sp2 - szz_pp) * a.rdz + (sxz_ip - sp4) * a.rdx + (syz_jp - sp5) * a.rdy;
-        vz = (MODE == MODE_ISO) ? wzm * (vz + bz * dS) : vz * w_p + bz * dS;
+        const float dS = rate3(sp2 - szz_pp, a.rdz, sxz_ip - sp4, a.rdx, syz_jp - sp5, a.rdy);
+        vz = (MODE == MODE_ISO) ? iso_vel(wzm, vz, bz, dS) : ortho_vel(w_p, vz, bz, dS);
         a.Vo[2][cv] = vz;
       }
       for (int q = q0; q < q1; ++q) {

@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/fwe.h"
+
 namespace fwe {
 
 // Tile of one workgroup in the xy plane: 64 lanes along x (one wavefront
@@ -108,5 +110,8 @@ hipError_t launch_group(const StepArgs* tab, const int32_t* start, int nm, int n
 // *it_base = g (set) or *it_base += g
 hipError_t launch_advance(int32_t* it_base, int32_t g, int32_t set, hipStream_t s);
 hipError_t launch_copy(const float* src, float* dst, int64_t n, hipStream_t s);
+// out[m][q] = sum_e coef[m][e] * tr[e][q] for m < mc, q < per; all device
+hipError_t launch_combine(const float* tr, int nb, int64_t per, const float* coef, int mc, float* out,
+                          hipStream_t s);
 
 }  // namespace fwe

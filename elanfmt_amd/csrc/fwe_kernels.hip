@@ -17,6 +17,7 @@
 //    two stencil kernels (per-tile injection and receiver lists).
 // The stencil is HBM-bound (~0.8 flop/B), so there is no MFMA here.
 #include "fwe_internal.h"
+#include "fwe_stencil.h"
 
 namespace fwe {
 
@@ -120,25 +121,25 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb) {
       float syz = ldz(Syz, jm && km, c);
       if (SINGLE) { cvx = vx; cvy = vy; cvz = vz_kp; }
       const float ex = (vx_ip - vx) * a.rdx, ey = (vy_jp - vy) * a.rdy, ez = (vz_kp - vz) * a.rdz;
-      const float dxy = (vy - vy_im) * a.rdx + (vx - vx_jm) * a.rdy;
-      const float dxz = (vz - vz_im) * a.rdx + (vx - vx_km) * a.rdz;
-      const float dyz = (vy - vy_km) * a.rdz + (vz - vz_jm) * a.rdy;
+      const float dxy = rate2(vy - vy_im, a.rdx, vx - vx_jm, a.rdy);
+      const float dxz = rate2(vz - vz_im, a.rdx, vx - vx_km, a.rdz);
+      const float dyz = rate2(vy - vy_km, a.rdz, vz - vz_jm, a.rdy);
       if (MODE == MODE_ISO) {
         // OK4.cu:307-309,323-329 with c11,c12 arrays holding c*dt
-        sxx = fxh * (sxx + (m0 * ex + m1 * ey + m1 * ez));
-        syy = fyh * (syy + (m1 * ex + m0 * ey + m1 * ez));
-        szz = fzh * (szz + (m1 * ex + m1 * ey + m0 * ez));
-        sxy = (wzk * wyj * wxi) * (sxy + a.c66dt * dxy);
-        sxz = (wyj * wzk * wxi) * (sxz + a.c66dt * dxz);
-        syz = (wxi * wyj * wzk) * (syz + a.c66dt * dyz);
+        sxx = iso_normal(fxh, sxx, m0, ex, m1, ey, m1, ez);
+        syy = iso_normal(fyh, syy, m1, ex, m0, ey, m1, ez);
+        szz = iso_normal(fzh, szz, m1, ex, m1, ey, m0, ez);
+        sxy = iso_shear(wzk * wyj * wxi, sxy, a.c66dt, dxy);
+        sxz = iso_shear(wyj * wzk * wxi, sxz, a.c66dt, dxz);
+        syz = iso_shear(wxi * wyj * wzk, syz, a.c66dt, dyz);
       } else {
         // ANISO.cu:306-352 with Cij holding Cij*dt
-        sxx = sxx * w + (m0 * ex + m3 * ey + m4 * ez);
-        syy = syy * w + (m3 * ex + m1 * ey + m5 * ez);
-        szz = szz * w + (m4 * ex + m5 * ey + m2 * ez);
-        sxy = sxy * w + m8 * dxy;
-        sxz = sxz * w + m7 * dxz;
-        syz = syz * w + m6 * dyz;
+        sxx = ortho_normal(w, sxx, m0, ex, m3, ey, m4, ez);
+        syy = ortho_normal(w, syy, m3, ex, m1, ey, m5, ez);
+        szz = ortho_normal(w, szz, m4, ex, m5, ey, m2, ez);
+        sxy = ortho_shear(w, sxy, m8, dxy);
+        sxz = ortho_shear(w, sxz, m7, dxz);
+        syz = ortho_shear(w, syz, m6, dyz);
       }
       // shear cells outside the reference update ranges stay zero
       if (!(im && jm)) sxy = 0.f;
@@ -153,10 +154,10 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb) {
         for (int e = e0; e < e1; ++e) {
           const PointEntry p = a.inj[e];
           if (p.i == i && p.j == j && p.batch == b) {
-            if (p.k == k) src[p.comp] += p.coef * amp;
+            if (p.k == k) src[p.comp] = fmaf(p.coef, amp, src[p.comp]);
             if (FS && k == 2 && p.k == 1) {
-              if (p.comp == 4) src_xz1 += p.coef * amp;
-              if (p.comp == 5) src_yz1 += p.coef * amp;
+              if (p.comp == 4) src_xz1 = fmaf(p.coef, amp, src_xz1);
+              if (p.comp == 5) src_yz1 = fmaf(p.coef, amp, src_yz1);
             }
           }
         }
@@ -169,8 +170,8 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb) {
         // BC1 (OK4.cu:279-282): Szz(k=0)=0, Sxz/Syz(k=1) = Sxz/Syz(k=2)/3
         if (k == 0) szz = 0.f;
         if (k == 2) {
-          if (im) Sxz[c - sz] = third * sxz + (a.src_after_bc ? src_xz1 : 0.f);
-          if (jm) Syz[c - sz] = third * syz + (a.src_after_bc ? src_yz1 : 0.f);
+          if (im) Sxz[c - sz] = fmaf(third, sxz, a.src_after_bc ? src_xz1 : 0.f);
+          if (jm) Syz[c - sz] = fmaf(third, syz, a.src_after_bc ? src_yz1 : 0.f);
         }
       }
       if (a.src_after_bc) {
@@ -261,12 +262,12 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb) {
       const float syz = SINGLE ? csyz : Syz[c];
       const float syz_kp = Syz[c + sz], syz_jp = Syz[c + sy];
       if (SINGLE) { cszz = szz; csxz = sxz_kp; csyz = syz_kp; }
-      const float dSx = (sxx - sxx_im) * a.rdx + (sxy_jp - sxy) * a.rdy + (sxz_kp - sxz) * a.rdz;
-      const float dSy = (syy - syy_jm) * a.rdy + (sxy_ip - sxy) * a.rdx + (syz_kp - syz) * a.rdz;
-      const float dSz = (szz - szz_km) * a.rdz + (sxz_ip - sxz) * a.rdx + (syz_jp - syz) * a.rdy;
-      if (ux) vx = (MODE == MODE_ISO) ? wxm * (vx + bx * dSx) : vx * w + bx * dSx;
-      if (uy) vy = (MODE == MODE_ISO) ? wym * (vy + by * dSy) : vy * w + by * dSy;
-      if (uz) vz = (MODE == MODE_ISO) ? wzm * (vz + bz * dSz) : vz * w + bz * dSz;
+      const float dSx = rate3(sxx - sxx_im, a.rdx, sxy_jp - sxy, a.rdy, sxz_kp - sxz, a.rdz);
+      const float dSy = rate3(syy - syy_jm, a.rdy, sxy_ip - sxy, a.rdx, syz_kp - syz, a.rdz);
+      const float dSz = rate3(szz - szz_km, a.rdz, sxz_ip - sxz, a.rdx, syz_jp - syz, a.rdy);
+      if (ux) vx = (MODE == MODE_ISO) ? iso_vel(wxm, vx, bx, dSx) : ortho_vel(w, vx, bx, dSx);
+      if (uy) vy = (MODE == MODE_ISO) ? iso_vel(wym, vy, by, dSy) : ortho_vel(w, vy, by, dSy);
+      if (uz) vz = (MODE == MODE_ISO) ? iso_vel(wzm, vz, bz, dSz) : ortho_vel(w, vz, bz, dSz);
       if (ux) Vx[c] = vx;
       if (uy) Vy[c] = vy;
       if (uz) Vz[c] = vz;
@@ -336,6 +337,28 @@ __global__ __launch_bounds__(256) void k_copy(const v4f* __restrict__ src, v4f* 
   for (int u = 0; u < 4; ++u) {
     const int64_t q = base + u * 256;
     if (q < n4) __builtin_nontemporal_store(a[u], dst + q);
+  }
+}
+
+// Seismograms of arbitrary moment tensors from the traces of the elementary
+// ones, by linearity of the wave equation in the source:
+//   out[m][q] = sum_e coef[m][e] * tr[e][q],  q < per (receivers x samples).
+// Each thread reads its nb elementary samples once and writes mc outputs,
+// so the kernel is bound by the output stream.
+__global__ __launch_bounds__(256) void k_combine(const float* __restrict__ tr, int32_t nb, int64_t per,
+                                                 const float* __restrict__ coef, int32_t mc,
+                                                 float* __restrict__ out) {
+  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= per) return;
+  float e[FWE_MAX_BATCH];
+#pragma unroll
+  for (int b = 0; b < FWE_MAX_BATCH; ++b) e[b] = b < nb ? tr[(int64_t)b * per + q] : 0.f;
+  for (int m = 0; m < mc; ++m) {
+    float s = 0.f;
+#pragma unroll
+    for (int b = 0; b < FWE_MAX_BATCH; ++b)
+      if (b < nb) s = fmaf(coef[m * nb + b], e[b], s);
+    __builtin_nontemporal_store(s, out + (int64_t)m * per + q);
   }
 }
 
@@ -464,6 +487,15 @@ hipError_t launch_copy(const float* src, float* dst, int64_t n, hipStream_t s) {
   if (blocks > 0x7fffffff) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, reinterpret_cast<const v4f*>(src),
                      reinterpret_cast<v4f*>(dst), n4);
+  return hipGetLastError();
+}
+
+hipError_t launch_combine(const float* tr, int nb, int64_t per, const float* coef, int mc, float* out,
+                          hipStream_t s) {
+  const int64_t blocks = (per + 255) / 256;
+  if (nb < 1 || nb > FWE_MAX_BATCH || blocks > 0x7fffffff) return hipErrorInvalidValue;
+  if (blocks == 0 || mc == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_combine, dim3((unsigned)blocks), dim3(256), 0, s, tr, nb, per, coef, mc, out);
   return hipGetLastError();
 }
 

@@ -99,6 +99,7 @@ SIGNATURES = {
     "fwe_get_step": (C.c_int, [C.c_void_p, _IP]),
     "fwe_set_step": (C.c_int, [C.c_void_p, C.c_int32]),
     "fwe_get_traces": (C.c_int, [C.c_void_p, _FP]),
+    "fwe_combine_traces": (C.c_int, [C.c_void_p, C.c_int32, _FP, _FP]),
     "fwe_get_field": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _FP]),
     "fwe_set_field": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _FP]),
     "fwe_read_legacy_params": (C.c_int, [C.c_char_p, C.POINTER(LegacyParams)]),
@@ -314,6 +315,19 @@ class Solver:
         if st not in (OK, ERR_NONFINITE):
             _check(st)
         return out[: self.nrec * self.cfg.nt].reshape(self.nrec, self.cfg.nt)
+
+    def combine_traces(self, coef):
+        """Traces of len(coef) sources, each a linear combination (coef row,
+        one weight per wavefield) of the nbatch elementary-source wavefields;
+        shape (n, receivers per wavefield, nt)."""
+        cf = np.ascontiguousarray(np.atleast_2d(coef), dtype=np.float32)  # row m = source m
+        n, nb = cf.shape
+        if nb != self.cfg.nbatch:
+            raise ValueError(f"coef needs {self.cfg.nbatch} columns")
+        nr = self.nrec // self.cfg.nbatch
+        out = np.empty(max(1, n * nr * self.cfg.nt), np.float32)
+        _check(lib().fwe_combine_traces(self._h, n, fptr(cf), fptr(out)))
+        return out[: n * nr * self.cfg.nt].reshape(n, nr, self.cfg.nt)
 
     def get_field(self, field, batch=0):
         shp = field_shape(self.cfg.nx, self.cfg.ny, self.cfg.nz, field)

@@ -230,6 +230,14 @@ fwe_status fwe_set_step(fwe_ctx* ctx, int32_t it);
 /* Receiver traces, out[r*nt + it] (Rec..Rec_y2 of OK4.cu:454-459,525-529).
  * Returns FWE_ERR_NONFINITE (after copying) if any sample is NaN/Inf. */
 fwe_status fwe_get_traces(fwe_ctx* ctx, float* out);
+/* Seismograms of n moment tensors by linear combination of the traces of the
+ * nbatch elementary-source wavefields: out[(m*nr + r)*nt + it] =
+ * sum_b coef[m*nbatch + b] * trace[(b*nr + r)*nt + it], where the receiver
+ * list is nbatch repeats of the same nr receivers, repeat b on wavefield b.
+ * No counterpart in the reference, which runs one source per process
+ * (OK4.cu:336-342 injects the single MTS.dat tensor); this is the batched
+ * elementary-tensor form of that injection. */
+fwe_status fwe_combine_traces(fwe_ctx* ctx, int32_t n, const float* coef, float* out);
 /* Copy one field of wavefield `batch` to/from a host array of its reference
  * shape (gather/SaveArray, OK4.cu:75,164,530-532; read_data, OK4.cu:166-175). */
 fwe_status fwe_get_field(fwe_ctx* ctx, int32_t batch, int32_t field, float* out);

@@ -345,7 +345,7 @@ def test_group_equals_separate_runs(gpu):
     group keeps that."""
     from elanfmt_amd import fwe, workloads as W
 
-    wls = [W.iso_random(n, seed=q, L=5, nt=40, f0=15.0, nrec_side=2) for q, n in enumerate((36, 44, 52))]
+    wls = [W.iso_random(n, seed=q, L=5, nt=120, f0=15.0, nrec_side=2) for q, n in enumerate((36, 44, 52))]
     ref = []
     for wl in wls:
         s = W.make_solver(wl)
@@ -354,17 +354,46 @@ def test_group_equals_separate_runs(gpu):
         s.close()
     sol = [W.make_solver(wl) for wl in wls]
     g = fwe.Group(sol)
-    g.run(15)
-    g.run(25)
+    g.run(45)
+    g.run(75)
     for s, r in zip(sol, ref):
-        assert s.step() == 40
+        assert s.step() == 120
         assert np.abs(r).max() > 0
         assert np.array_equal(s.traces(), r)
     ms = g.autotune(2)
     assert ms > 0 and all(s.step() == 0 for s in sol)
-    g.run(40)
+    g.run(120)
     for s, r in zip(sol, ref):
         assert np.array_equal(s.traces(), r)
     g.close()
     for s in sol:
         s.close()
+
+
+def test_combine_elementary_traces_equals_direct_runs(gpu):
+    """Six elementary-tensor wavefields in one batched run, combined with the
+    weights of a general tensor, reproduce the direct single-tensor run
+    (linearity in the source) to FP32 round-off."""
+    from conftest import rel_l2
+    from elanfmt_amd import fwe, workloads as W
+
+    m0 = 1e15
+    wl = W.ortho_layered(nx=66, ny=40, nz=36, L=5, nt=60, f0=12.0, nrec_side=3, elementary=True, m0=m0)
+    s = W.make_solver(wl)
+    s.run(wl.nt)
+    mts = [W.mt_lune_random(100 + q, m0)[0] for q in range(3)]
+    comb = s.combine_traces(np.array(mts) / m0)
+    s.close()
+    nr = len(wl.receivers) // 6
+    xs, ys, zs = wl.nx // 2 - 1, wl.ny // 2 - 1, wl.nz // 2 - 1
+    for q, m in enumerate(mts):
+        wl1 = W.ortho_layered(nx=66, ny=40, nz=36, L=5, nt=60, f0=12.0, nrec_side=3)
+        wl1.injections = fwe.mt_injections(fwe.SRC_LAYOUT_OK4, xs, ys, zs, m)
+        d = W.make_solver(wl1)
+        d.run(wl1.nt)
+        direct = d.traces()
+        d.close()
+        assert direct.shape == comb[q].shape == (nr, wl.nt)
+        err = rel_l2(comb[q], direct)
+        print(f"combine MT {q}: rel-L2 {err:.2e}", flush=True)
+        assert np.abs(direct).max() > 0 and err < 1e-5
