@@ -702,7 +702,10 @@ extern "C" fwe_status fwe_set_graph_steps(fwe_ctx* x, int32_t g) {
 static fwe_status apply_tiling(fwe_ctx* x, int ty, int kchunk) {
   const fwe_config& c = x->cfg;
   if (ty != 2 && ty != 4 && ty != 8) return fail(FWE_ERR_ARG, "tile rows must be 2, 4 or 8");
-  if (kchunk < 3 || kchunk > c.nz) return fail(FWE_ERR_ARG, "z chunk must be in 3..nz");
+  // the free surface (BC1 of plane 1 written after plane 2) and the
+  // single-pass kernel's halo planes need planes 0..2 in one chunk
+  const int kmin = (c.free_surface || use_fused(x)) ? 3 : 1;
+  if (kchunk < kmin || kchunk > c.nz) return fail(FWE_ERR_ARG, "z chunk must be in %d..nz", kmin);
   x->ty = ty;
   x->ntx = (c.nx + fwe::TILE_X - 1) / fwe::TILE_X;
   x->nty = (c.ny + ty - 1) / ty;
@@ -722,6 +725,7 @@ extern "C" fwe_status fwe_set_kernel(fwe_ctx* x, int32_t kind) {
     return fail(FWE_ERR_UNSUPPORTED, "the single-pass kernel has no free surface");
   drop_graph(x);
   x->kernel = kind;
+  if (use_fused(x) && x->kchunk < 3) return apply_tiling(x, x->ty, std::min(3, x->cfg.nz));
   return FWE_OK;
 }
 
@@ -751,10 +755,11 @@ extern "C" fwe_status fwe_autotune(fwe_ctx* x, int32_t trial_steps, double* best
   int best_ty = x->ty, best_kc = x->kchunk;
   double best = 1e30;
   static const int tys[3] = {2, 4, 8};
-  static const int kcs[8] = {4, 8, 12, 16, 24, 32, 48, 64};
+  static const int kcs[10] = {1, 2, 4, 8, 12, 16, 24, 32, 48, 64};
+  const int kmin = (x->cfg.free_surface || use_fused(x)) ? 3 : 1;
   for (int ty : tys) {
     for (int kc : kcs) {
-      if (kc > nz) continue;
+      if (kc > nz || kc < kmin) continue;
       if ((st = apply_tiling(x, ty, kc))) return st;
       if ((st = run_impl(x, 1, nullptr, nullptr, nullptr))) return st;  // warm-up
       double a = 0, b = 0, w = 0;
@@ -1204,7 +1209,8 @@ extern "C" fwe_status fwe_group_set_tiling(fwe_group* g, int32_t tile_rows, int3
   if (!g) return fail(FWE_ERR_ARG, "NULL group");
   if (tile_rows != 2 && tile_rows != 4 && tile_rows != 8) return fail(FWE_ERR_ARG, "tile rows must be 2, 4 or 8");
   for (fwe_ctx* x : g->m)
-    if (std::min<int>(zchunk, x->cfg.nz) < 3) return fail(FWE_ERR_ARG, "z chunk must be at least 3");
+    if (std::min<int>(zchunk, x->cfg.nz) < (x->cfg.free_surface ? 3 : 1))
+      return fail(FWE_ERR_ARG, "z chunk too small for member with free surface");
   for (fwe_ctx* x : g->m) {
     fwe_status st = apply_tiling(x, tile_rows, std::min<int>(zchunk, x->cfg.nz));
     if (st) return st;
@@ -1216,12 +1222,16 @@ extern "C" fwe_status fwe_group_set_tiling(fwe_group* g, int32_t tile_rows, int3
 extern "C" fwe_status fwe_group_autotune(fwe_group* g, int32_t trial_steps, double* best_ms_per_step) {
   if (!g || trial_steps < 1) return fail(FWE_ERR_ARG, "bad autotune arguments");
   static const int tys[3] = {2, 4, 8};
-  static const int kcs[6] = {4, 8, 16, 32, 64, 256};
+  static const int kcs[8] = {1, 2, 4, 8, 16, 32, 64, 256};
   double best = 1e30;
   int bty = g->ty, bkc = 8;
   fwe_status st;
+  int kmin = g->fs ? 3 : 1;
+  for (fwe_ctx* x : g->m)
+    if (use_fused(x)) kmin = 3;
   for (int ty : tys)
     for (int kc : kcs) {
+      if (kc < kmin) continue;
       if ((st = fwe_group_set_tiling(g, ty, kc))) return st;
       double ms = 0;
       if ((st = group_run_impl(g, 1, &ms))) return st;

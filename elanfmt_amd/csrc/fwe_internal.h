@@ -101,6 +101,15 @@ struct FusedArgs {
 
 hipError_t launch_fused(const FusedArgs& a, int mode, int pml, hipStream_t s);
 
+// per (law, damping form) launchers, instantiated in fwe_step.hip
+template <int MODE, int PML>
+hipError_t stress_mp(const StepArgs& a, int fs, hipStream_t s);
+template <int MODE, int PML>
+hipError_t velocity_mp(const StepArgs& a, hipStream_t s);
+template <int MODE, int PML>
+hipError_t group_mp(const StepArgs* tab, const int32_t* start, int nm, int nblk, int ty, int fs, bool vel,
+                    hipStream_t s);
+
 hipError_t launch_stress(const StepArgs& a, int mode, int pml, int fs, hipStream_t s);
 hipError_t launch_velocity(const StepArgs& a, int mode, int pml, hipStream_t s);
 // one stress (vel=false) or velocity (vel=true) launch over the tiles of nm

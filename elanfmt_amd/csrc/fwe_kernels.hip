@@ -1,318 +1,9 @@
-// fwe_kernels.hip -- gfx950 kernels of the staggered-grid velocity-stress
-// FDTD step (stress + damping + BC1 + injection, velocity + damping +
-// receivers).  Semantics follow the reference kernels
-//   compute_StressPrf / update_Qxft   OK4.cu:285-388   (iso law)
-//   compute_StressOrtho / update_VelocityFromStress / BC1 / source / record
-//                                      ANISO.cu:228-389 (ortho law)
-// Layout and schedule are MI355X-first:
-//  * every 3-D array (fields, medium, 3-D weights) shares one padded layout,
-//    x fastest, row pitch a multiple of 32 floats (128 B lines), so one index
-//    serves all staggered arrays and a 64-lane wavefront reads 256 B
-//    contiguous per array;
-//  * a workgroup owns a 64 x TY column tile and marches a chunk of z planes,
-//    carrying the z-neighbours it needs in registers;
-//  * workgroup ids are remapped so that neighbouring tiles run on the same
-//    XCD and share its L2 for the x/y halo re-reads;
-//  * the reference's separate source/BC1/record launches are fused into the
-//    two stencil kernels (per-tile injection and receiver lists).
-// The stencil is HBM-bound (~0.8 flop/B), so there is no MFMA here.
+// fwe_kernels.hip -- launch dispatch of the step kernels (instantiated per
+// law and damping form in fwe_step.hip) and the small kernels of libfwe.so:
+// the graph step counter, the memory-copy probe and the trace combination.
 #include "fwe_internal.h"
-#include "fwe_stencil.h"
 
 namespace fwe {
-
-// Workgroup w is dispatched to XCD (w mod 8).  Give each XCD a contiguous
-// range of logical tiles so x/y neighbours share an L2.
-__device__ __forceinline__ int xcd_remap(int b, int nb) {
-  const int q = nb >> 3, r = nb & 7;
-  const int x = b & 7, w = b >> 3;
-  return x * q + (x < r ? x : r) + w;
-}
-
-__device__ __forceinline__ float ld(const float* p) { return __builtin_nontemporal_load(p); }
-
-// Unconditional streaming load of p[idx] that yields 0 when !use; lanes that
-// do not need it read p[0] (always in bounds).  Keeping loads out of
-// branches lets the compiler keep a whole plane's loads in flight instead of
-// draining the memory counter at every control-flow join.
-__device__ __forceinline__ float ldz(const float* p, bool use, int64_t idx) {
-  const float v = __builtin_nontemporal_load(p + (use ? idx : 0));
-  return use ? v : 0.f;
-}
-
-// ---------------------------------------------------------------------------
-// Stress pass.  For cell (i,j,k) of the column tile:
-//   normal:  S_ii <- damp * S_ii + sum_j C_ij * dt * dV_j/dx_j
-//   shear:   S_xy <- damp * S_xy + C66 * dt * (dVy/dx + dVx/dy)   (i,j >= 1)
-//            S_xz ... (i,k >= 1),  S_yz ... (j,k >= 1)
-// iso law (OK4.cu:296-329): damp multiplies (old + increment); ortho law
-// (ANISO.cu:306-352): damp multiplies the old value only.
-// Injection and BC1 are applied in the reference order (see StepArgs).
-// SINGLE (one wavefield): the z-neighbours Vx(k-1), Vy(k-1) and Vz(k) are
-// carried in registers from the previous plane.
-// ---------------------------------------------------------------------------
-template <int MODE, int PML, bool FS, int TY, bool SINGLE>
-__device__ __forceinline__ void stress_tile(const StepArgs& a, int lb) {
-  const int lane = threadIdx.x, row = threadIdx.y;
-  const int tx = lb % a.ntx;
-  lb /= a.ntx;
-  const int tyi = lb % a.nty;
-  const int kc = lb / a.nty;
-  const int i = tx * TILE_X + lane, j = tyi * TY + row;
-  if (i >= a.nx || j >= a.ny) return;
-  const int k0 = kc * a.kchunk;
-  const int k1 = min(k0 + a.kchunk, a.nz);
-  const int64_t sy = a.sy, sz = a.sz;
-  const int tile = tx + tyi * a.ntx;
-  const int e0 = a.inj_off[tile], e1 = a.inj_off[tile + 1];
-  const bool im = i >= 1, jm = j >= 1;
-  const float third = 1.0f / 3.0f;
-  const int nb = SINGLE ? 1 : a.nb;
-
-  float fxh = 1.f, fyh = 1.f, wxi = 1.f, wyj = 1.f;
-  if (PML == PML_PROFILE) { fxh = a.whx[i]; fyh = a.why[j]; }
-  if (PML == PML_PROFILE || PML == PML_SEP) { wxi = a.wx[i]; wyj = a.wy[j]; }
-
-  // carried z-neighbours (single wavefield)
-  float cvx = 0.f, cvy = 0.f, cvz = 0.f;
-  if (SINGLE) {
-    const int64_t c0 = (int64_t)i + j * sy + (int64_t)k0 * sz;
-    cvx = ldz(a.V[0], k0 >= 1, c0 - sz);
-    cvy = ldz(a.V[1], k0 >= 1, c0 - sz);
-    cvz = ldz(a.V[2], true, c0);
-  }
-
-  for (int k = k0; k < k1; ++k) {
-    const int64_t c = (int64_t)i + j * sy + k * sz;
-    const bool km = k >= 1;
-    float wzk = 1.f, fzh = 1.f;
-    if (PML == PML_PROFILE) { wzk = a.wz[k]; fzh = a.whz[k]; }
-    if (PML == PML_SEP) wzk = a.wz[k];
-    float w = 1.f;
-    if (PML == PML_W3D) w = ld(a.w3 + c);
-    if (PML == PML_SEP) w = wxi * wyj * wzk;
-    // medium at this cell, shared by all wavefields of the batch
-    float m0, m1, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f, m6 = 0.f, m7 = 0.f, m8 = 0.f;
-    m0 = ld(a.C[0] + c); m1 = ld(a.C[1] + c);
-    if (MODE == MODE_ORTHO) {
-      m2 = ld(a.C[2] + c); m3 = ld(a.C[3] + c); m4 = ld(a.C[4] + c); m5 = ld(a.C[5] + c);
-      m6 = ld(a.C[6] + c); m7 = ld(a.C[7] + c); m8 = ld(a.C[8] + c);
-    }
-    for (int b = 0; b < nb; ++b) {
-      const int64_t fo = (int64_t)b * a.fstride;
-      const float* Vx = a.V[0] + fo;
-      const float* Vy = a.V[1] + fo;
-      const float* Vz = a.V[2] + fo;
-      float* Sxx = a.S[0] + fo; float* Syy = a.S[1] + fo; float* Szz = a.S[2] + fo;
-      float* Sxy = a.S[3] + fo; float* Sxz = a.S[4] + fo; float* Syz = a.S[5] + fo;
-      // all loads of this cell first, branch-free
-      const float vx = Vx[c], vy = Vy[c];
-      const float vx_ip = Vx[c + 1], vy_jp = Vy[c + sy], vz_kp = Vz[c + sz];
-      const float vz = SINGLE ? cvz : Vz[c];
-      const float vx_km = SINGLE ? cvx : ldz(Vx, km, c - sz);
-      const float vy_km = SINGLE ? cvy : ldz(Vy, km, c - sz);
-      const float vx_jm = ldz(Vx, jm, c - sy);
-      const float vy_im = ldz(Vy, im, c - 1);
-      const float vz_im = ldz(Vz, im, c - 1);
-      const float vz_jm = ldz(Vz, jm, c - sy);
-      float sxx = ld(Sxx + c), syy = ld(Syy + c), szz = ld(Szz + c);
-      float sxy = ldz(Sxy, im && jm, c);
-      float sxz = ldz(Sxz, im && km, c);
-      float syz = ldz(Syz, jm && km, c);
-      if (SINGLE) { cvx = vx; cvy = vy; cvz = vz_kp; }
-      const float ex = (vx_ip - vx) * a.rdx, ey = (vy_jp - vy) * a.rdy, ez = (vz_kp - vz) * a.rdz;
-      const float dxy = rate2(vy - vy_im, a.rdx, vx - vx_jm, a.rdy);
-      const float dxz = rate2(vz - vz_im, a.rdx, vx - vx_km, a.rdz);
-      const float dyz = rate2(vy - vy_km, a.rdz, vz - vz_jm, a.rdy);
-      if (MODE == MODE_ISO) {
-        // OK4.cu:307-309,323-329 with c11,c12 arrays holding c*dt
-        sxx = iso_normal(fxh, sxx, m0, ex, m1, ey, m1, ez);
-        syy = iso_normal(fyh, syy, m1, ex, m0, ey, m1, ez);
-        szz = iso_normal(fzh, szz, m1, ex, m1, ey, m0, ez);
-        sxy = iso_shear(wzk * wyj * wxi, sxy, a.c66dt, dxy);
-        sxz = iso_shear(wyj * wzk * wxi, sxz, a.c66dt, dxz);
-        syz = iso_shear(wxi * wyj * wzk, syz, a.c66dt, dyz);
-      } else {
-        // ANISO.cu:306-352 with Cij holding Cij*dt
-        sxx = ortho_normal(w, sxx, m0, ex, m3, ey, m4, ez);
-        syy = ortho_normal(w, syy, m3, ex, m1, ey, m5, ez);
-        szz = ortho_normal(w, szz, m4, ex, m5, ey, m2, ez);
-        sxy = ortho_shear(w, sxy, m8, dxy);
-        sxz = ortho_shear(w, sxz, m7, dxz);
-        syz = ortho_shear(w, syz, m6, dyz);
-      }
-      // shear cells outside the reference update ranges stay zero
-      if (!(im && jm)) sxy = 0.f;
-      if (!(im && km)) sxz = 0.f;
-      if (!(jm && km)) syz = 0.f;
-      // point injections of this cell (OK4.cu:336-342 / ANISO.cu:228-252)
-      float src[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      float src_xz1 = 0.f, src_yz1 = 0.f;  // injections at (i,j,1), for BC1
-      if (e1 > e0) {
-        const int it = a.it + (a.it_base ? *a.it_base : 0);
-        const float amp = (it < a.nstf) ? a.stf[(int64_t)b * a.nstf + it] : 0.f;
-        for (int e = e0; e < e1; ++e) {
-          const PointEntry p = a.inj[e];
-          if (p.i == i && p.j == j && p.batch == b) {
-            if (p.k == k) src[p.comp] = fmaf(p.coef, amp, src[p.comp]);
-            if (FS && k == 2 && p.k == 1) {
-              if (p.comp == 4) src_xz1 = fmaf(p.coef, amp, src_xz1);
-              if (p.comp == 5) src_yz1 = fmaf(p.coef, amp, src_yz1);
-            }
-          }
-        }
-      }
-      if (!a.src_after_bc) {
-        sxx += src[0]; syy += src[1]; szz += src[2];
-        sxy += src[3]; sxz += src[4]; syz += src[5];
-      }
-      if (FS) {
-        // BC1 (OK4.cu:279-282): Szz(k=0)=0, Sxz/Syz(k=1) = Sxz/Syz(k=2)/3
-        if (k == 0) szz = 0.f;
-        if (k == 2) {
-          if (im) Sxz[c - sz] = fmaf(third, sxz, a.src_after_bc ? src_xz1 : 0.f);
-          if (jm) Syz[c - sz] = fmaf(third, syz, a.src_after_bc ? src_yz1 : 0.f);
-        }
-      }
-      if (a.src_after_bc) {
-        sxx += src[0]; syy += src[1]; szz += src[2];
-        sxy += src[3]; sxz += src[4]; syz += src[5];
-      }
-      Sxx[c] = sxx; Syy[c] = syy; Szz[c] = szz;
-      Sxy[c] = sxy; Sxz[c] = sxz; Syz[c] = syz;
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Velocity pass (OK4.cu:351-378 iso, ANISO.cu:374-386 ortho) and receiver
-// sampling after the update (OK4.cu:383-386).  Ranges:
-//   Vx 1<=i<=nx-1, 1<=j<=ny-2, 1<=k<=nz-2
-//   Vy 1<=i<=nx-2, 1<=j<=ny-1, 1<=k<=nz-2
-//   Vz 1<=i<=nx-2, 1<=j<=ny-2, 1<=k<=nz-1
-// SINGLE: Szz(k-1), Sxz(k) and Syz(k) are carried from the previous plane.
-// ---------------------------------------------------------------------------
-template <int MODE, int PML, int TY, bool SINGLE>
-__device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb) {
-  const int lane = threadIdx.x, row = threadIdx.y;
-  const int tx = lb % a.ntx;
-  lb /= a.ntx;
-  const int tyi = lb % a.nty;
-  const int kc = lb / a.nty;
-  const int i = tx * TILE_X + lane, j = tyi * TY + row;
-  if (i >= a.nx || j >= a.ny) return;
-  const int k0 = kc * a.kchunk;
-  const int k1 = min(k0 + a.kchunk, a.nz);
-  const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const int64_t sy = a.sy, sz = a.sz;
-  const int tile = tx + tyi * a.ntx;
-  const int r0 = a.rec_off[tile], r1 = a.rec_off[tile + 1];
-  const bool im = i >= 1, jm = j >= 1;
-  const bool xin = im && jm;  // all three need i>=1, j>=1, k>=1
-  const bool ux_ij = xin && j <= ny - 2;
-  const bool uy_ij = xin && i <= nx - 2;
-  const bool uz_ij = xin && i <= nx - 2 && j <= ny - 2;
-  const int nb = SINGLE ? 1 : a.nb;
-
-  float wxm = 1.f, wym = 1.f, txi = 1.f, tyj = 1.f;
-  if (PML == PML_PROFILE) {
-    wxm = im ? a.wx[i - 1] : 1.f;
-    wym = jm ? a.wy[j - 1] : 1.f;
-  }
-  if (PML == PML_SEP) { txi = a.wx[i]; tyj = a.wy[j]; }
-
-  float cszz = 0.f, csxz = 0.f, csyz = 0.f;  // Szz(k-1), Sxz(k), Syz(k)
-  if (SINGLE) {
-    const int64_t c0 = (int64_t)i + j * sy + (int64_t)k0 * sz;
-    cszz = ldz(a.S[2], k0 >= 1, c0 - sz);
-    csxz = ldz(a.S[4], true, c0);
-    csyz = ldz(a.S[5], true, c0);
-  }
-
-  for (int k = k0; k < k1; ++k) {
-    const int64_t c = (int64_t)i + j * sy + k * sz;
-    const bool ux = ux_ij && k >= 1 && k <= nz - 2;
-    const bool uy = uy_ij && k >= 1 && k <= nz - 2;
-    const bool uz = uz_ij && k >= 1;
-    float w = 1.f;
-    if (PML == PML_W3D) w = ld(a.w3 + c);
-    if (PML == PML_SEP) w = txi * tyj * a.wz[k];
-    float bx = a.bxdt, by = a.bydt, bz = a.bzdt;
-    if (MODE == MODE_ORTHO) {
-      bx = ld(a.B[0] + c); by = ld(a.B[1] + c); bz = ld(a.B[2] + c);
-    }
-    float wzm = 1.f;
-    if (PML == PML_PROFILE) wzm = k >= 1 ? a.wz[k - 1] : 1.f;
-    for (int b = 0; b < nb; ++b) {
-      const int64_t fo = (int64_t)b * a.fstride;
-      float* Vx = a.V[0] + fo; float* Vy = a.V[1] + fo; float* Vz = a.V[2] + fo;
-      const float* Sxx = a.S[0] + fo; const float* Syy = a.S[1] + fo;
-      const float* Szz = a.S[2] + fo; const float* Sxy = a.S[3] + fo;
-      const float* Sxz = a.S[4] + fo; const float* Syz = a.S[5] + fo;
-      // all loads of this cell first, branch-free (c+1, c+sy, c+sz stay
-      // inside the padded extents; c-1, c-sy, c-sz only when valid)
-      float vx = ld(Vx + c), vy = ld(Vy + c), vz = ld(Vz + c);
-      const float sxx = Sxx[c], sxx_im = ldz(Sxx, im, c - 1);
-      const float syy = Syy[c], syy_jm = ldz(Syy, jm, c - sy);
-      const float szz = Szz[c];
-      const float szz_km = SINGLE ? cszz : ldz(Szz, k >= 1, c - sz);
-      const float sxy = Sxy[c], sxy_jp = Sxy[c + sy], sxy_ip = Sxy[c + 1];
-      const float sxz = SINGLE ? csxz : Sxz[c];
-      const float sxz_kp = Sxz[c + sz], sxz_ip = Sxz[c + 1];
-      const float syz = SINGLE ? csyz : Syz[c];
-      const float syz_kp = Syz[c + sz], syz_jp = Syz[c + sy];
-      if (SINGLE) { cszz = szz; csxz = sxz_kp; csyz = syz_kp; }
-      const float dSx = rate3(sxx - sxx_im, a.rdx, sxy_jp - sxy, a.rdy, sxz_kp - sxz, a.rdz);
-      const float dSy = rate3(syy - syy_jm, a.rdy, sxy_ip - sxy, a.rdx, syz_kp - syz, a.rdz);
-      const float dSz = rate3(szz - szz_km, a.rdz, sxz_ip - sxz, a.rdx, syz_jp - syz, a.rdy);
-      if (ux) vx = (MODE == MODE_ISO) ? iso_vel(wxm, vx, bx, dSx) : ortho_vel(w, vx, bx, dSx);
-      if (uy) vy = (MODE == MODE_ISO) ? iso_vel(wym, vy, by, dSy) : ortho_vel(w, vy, by, dSy);
-      if (uz) vz = (MODE == MODE_ISO) ? iso_vel(wzm, vz, bz, dSz) : ortho_vel(w, vz, bz, dSz);
-      if (ux) Vx[c] = vx;
-      if (uy) Vy[c] = vy;
-      if (uz) Vz[c] = vz;
-      for (int r = r0; r < r1; ++r) {
-        const PointEntry p = a.rec[r];
-        const int it = a.it + (a.it_base ? *a.it_base : 0);
-        if (p.i == i && p.j == j && p.k == k && p.batch == b && it < a.nt)
-          a.traces[(int64_t)p.slot * a.nt + it] = p.comp == 0 ? vx : (p.comp == 1 ? vy : vz);
-      }
-    }
-  }
-}
-
-// One model per launch: workgroup -> tile of that model.
-template <int MODE, int PML, bool FS, int TY, bool SINGLE>
-__global__ __launch_bounds__(64 * TY) void k_stress(const StepArgs a) {
-  stress_tile<MODE, PML, FS, TY, SINGLE>(a, xcd_remap(blockIdx.x, gridDim.x));
-}
-template <int MODE, int PML, int TY, bool SINGLE>
-__global__ __launch_bounds__(64 * TY) void k_velocity(const StepArgs a) {
-  velocity_tile<MODE, PML, TY, SINGLE>(a, xcd_remap(blockIdx.x, gridDim.x));
-}
-
-// Many models per launch (an uneven batch of independent grids): the
-// workgroups of all models form one grid; start[m] is the first workgroup
-// of model m and tab[m] its step arguments (read from device memory).
-__device__ __forceinline__ int group_model(const int32_t* __restrict__ start, int nm, int lb) {
-  int m = 0;
-  while (m + 1 < nm && lb >= start[m + 1]) ++m;
-  return m;
-}
-template <int MODE, int PML, bool FS, int TY>
-__global__ __launch_bounds__(64 * TY) void k_stress_group(const StepArgs* __restrict__ tab,
-                                                         const int32_t* __restrict__ start, int32_t nm) {
-  const int lb = xcd_remap(blockIdx.x, gridDim.x);
-  const int m = group_model(start, nm, lb);
-  stress_tile<MODE, PML, FS, TY, true>(tab[m], lb - start[m]);
-}
-template <int MODE, int PML, int TY>
-__global__ __launch_bounds__(64 * TY) void k_velocity_group(const StepArgs* __restrict__ tab,
-                                                           const int32_t* __restrict__ start, int32_t nm) {
-  const int lb = xcd_remap(blockIdx.x, gridDim.x);
-  const int m = group_model(start, nm, lb);
-  velocity_tile<MODE, PML, TY, true>(tab[m], lb - start[m]);
-}
 
 // Step counter of a captured time-loop graph: one lane advances it after the
 // G steps of one replay (graph nodes run in order on one stream).
@@ -363,115 +54,47 @@ __global__ __launch_bounds__(256) void k_combine(const float* __restrict__ tr, i
 }
 
 // ---------------------------------------------------------------------------
-// launchers
+// dispatch: law x damping form -> instantiation unit
 // ---------------------------------------------------------------------------
-template <int MODE, int PML, bool FS, bool SINGLE>
-static hipError_t go_stress1(const StepArgs& a, hipStream_t s) {
-  const int nblk = a.ntx * a.nty * a.nkc;
-  if (a.ty_rows == 4) {
-    hipLaunchKernelGGL((k_stress<MODE, PML, FS, 4, SINGLE>), dim3(nblk), dim3(64, 4), 0, s, a);
-  } else if (a.ty_rows == 8) {
-    hipLaunchKernelGGL((k_stress<MODE, PML, FS, 8, SINGLE>), dim3(nblk), dim3(64, 8), 0, s, a);
-  } else if (a.ty_rows == 2) {
-    hipLaunchKernelGGL((k_stress<MODE, PML, FS, 2, SINGLE>), dim3(nblk), dim3(64, 2), 0, s, a);
-  } else {
-    return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
-template <int MODE, int PML, bool FS>
-static hipError_t go_stress(const StepArgs& a, hipStream_t s) {
-  return a.nb == 1 ? go_stress1<MODE, PML, FS, true>(a, s) : go_stress1<MODE, PML, FS, false>(a, s);
-}
-
-template <int MODE, int PML, bool SINGLE>
-static hipError_t go_vel1(const StepArgs& a, hipStream_t s) {
-  const int nblk = a.ntx * a.nty * a.nkc;
-  if (a.ty_rows == 4) {
-    hipLaunchKernelGGL((k_velocity<MODE, PML, 4, SINGLE>), dim3(nblk), dim3(64, 4), 0, s, a);
-  } else if (a.ty_rows == 8) {
-    hipLaunchKernelGGL((k_velocity<MODE, PML, 8, SINGLE>), dim3(nblk), dim3(64, 8), 0, s, a);
-  } else if (a.ty_rows == 2) {
-    hipLaunchKernelGGL((k_velocity<MODE, PML, 2, SINGLE>), dim3(nblk), dim3(64, 2), 0, s, a);
-  } else {
-    return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
-template <int MODE, int PML>
-static hipError_t go_vel(const StepArgs& a, hipStream_t s) {
-  return a.nb == 1 ? go_vel1<MODE, PML, true>(a, s) : go_vel1<MODE, PML, false>(a, s);
-}
-
 hipError_t launch_stress(const StepArgs& a, int mode, int pml, int fs, hipStream_t s) {
   if (mode == MODE_ISO) {
-    if (pml == PML_PROFILE) return fs ? go_stress<MODE_ISO, PML_PROFILE, true>(a, s)
-                                      : go_stress<MODE_ISO, PML_PROFILE, false>(a, s);
-    if (pml == PML_NONE) return fs ? go_stress<MODE_ISO, PML_NONE, true>(a, s)
-                                   : go_stress<MODE_ISO, PML_NONE, false>(a, s);
+    if (pml == PML_PROFILE) return stress_mp<MODE_ISO, PML_PROFILE>(a, fs, s);
+    if (pml == PML_NONE) return stress_mp<MODE_ISO, PML_NONE>(a, fs, s);
     return hipErrorInvalidValue;
   }
   switch (pml) {
-    case PML_NONE: return fs ? go_stress<MODE_ORTHO, PML_NONE, true>(a, s)
-                             : go_stress<MODE_ORTHO, PML_NONE, false>(a, s);
-    case PML_W3D: return fs ? go_stress<MODE_ORTHO, PML_W3D, true>(a, s)
-                            : go_stress<MODE_ORTHO, PML_W3D, false>(a, s);
-    case PML_SEP: return fs ? go_stress<MODE_ORTHO, PML_SEP, true>(a, s)
-                            : go_stress<MODE_ORTHO, PML_SEP, false>(a, s);
+    case PML_NONE: return stress_mp<MODE_ORTHO, PML_NONE>(a, fs, s);
+    case PML_W3D: return stress_mp<MODE_ORTHO, PML_W3D>(a, fs, s);
+    case PML_SEP: return stress_mp<MODE_ORTHO, PML_SEP>(a, fs, s);
     default: return hipErrorInvalidValue;
   }
 }
 
 hipError_t launch_velocity(const StepArgs& a, int mode, int pml, hipStream_t s) {
   if (mode == MODE_ISO) {
-    if (pml == PML_PROFILE) return go_vel<MODE_ISO, PML_PROFILE>(a, s);
-    if (pml == PML_NONE) return go_vel<MODE_ISO, PML_NONE>(a, s);
+    if (pml == PML_PROFILE) return velocity_mp<MODE_ISO, PML_PROFILE>(a, s);
+    if (pml == PML_NONE) return velocity_mp<MODE_ISO, PML_NONE>(a, s);
     return hipErrorInvalidValue;
   }
   switch (pml) {
-    case PML_NONE: return go_vel<MODE_ORTHO, PML_NONE>(a, s);
-    case PML_W3D: return go_vel<MODE_ORTHO, PML_W3D>(a, s);
-    case PML_SEP: return go_vel<MODE_ORTHO, PML_SEP>(a, s);
+    case PML_NONE: return velocity_mp<MODE_ORTHO, PML_NONE>(a, s);
+    case PML_W3D: return velocity_mp<MODE_ORTHO, PML_W3D>(a, s);
+    case PML_SEP: return velocity_mp<MODE_ORTHO, PML_SEP>(a, s);
     default: return hipErrorInvalidValue;
   }
-}
-
-template <int MODE, int PML, bool FS>
-static hipError_t go_group(const StepArgs* tab, const int32_t* start, int nm, int nblk, int ty, bool vel,
-                           hipStream_t s) {
-  if (ty == 2) {
-    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, 2>), dim3(nblk), dim3(64, 2), 0, s, tab, start, nm);
-    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, 2>), dim3(nblk), dim3(64, 2), 0, s, tab, start, nm);
-  } else if (ty == 4) {
-    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, 4>), dim3(nblk), dim3(64, 4), 0, s, tab, start, nm);
-    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, 4>), dim3(nblk), dim3(64, 4), 0, s, tab, start, nm);
-  } else if (ty == 8) {
-    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, 8>), dim3(nblk), dim3(64, 8), 0, s, tab, start, nm);
-    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, 8>), dim3(nblk), dim3(64, 8), 0, s, tab, start, nm);
-  } else {
-    return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
 }
 
 hipError_t launch_group(const StepArgs* tab, const int32_t* start, int nm, int nblk, int ty, int mode, int pml,
                         int fs, bool vel, hipStream_t s) {
   if (mode == MODE_ISO) {
-    if (pml == PML_PROFILE) return fs ? go_group<MODE_ISO, PML_PROFILE, true>(tab, start, nm, nblk, ty, vel, s)
-                                      : go_group<MODE_ISO, PML_PROFILE, false>(tab, start, nm, nblk, ty, vel, s);
-    if (pml == PML_NONE) return fs ? go_group<MODE_ISO, PML_NONE, true>(tab, start, nm, nblk, ty, vel, s)
-                                   : go_group<MODE_ISO, PML_NONE, false>(tab, start, nm, nblk, ty, vel, s);
+    if (pml == PML_PROFILE) return group_mp<MODE_ISO, PML_PROFILE>(tab, start, nm, nblk, ty, fs, vel, s);
+    if (pml == PML_NONE) return group_mp<MODE_ISO, PML_NONE>(tab, start, nm, nblk, ty, fs, vel, s);
     return hipErrorInvalidValue;
   }
   switch (pml) {
-    case PML_NONE: return fs ? go_group<MODE_ORTHO, PML_NONE, true>(tab, start, nm, nblk, ty, vel, s)
-                             : go_group<MODE_ORTHO, PML_NONE, false>(tab, start, nm, nblk, ty, vel, s);
-    case PML_W3D: return fs ? go_group<MODE_ORTHO, PML_W3D, true>(tab, start, nm, nblk, ty, vel, s)
-                            : go_group<MODE_ORTHO, PML_W3D, false>(tab, start, nm, nblk, ty, vel, s);
-    case PML_SEP: return fs ? go_group<MODE_ORTHO, PML_SEP, true>(tab, start, nm, nblk, ty, vel, s)
-                            : go_group<MODE_ORTHO, PML_SEP, false>(tab, start, nm, nblk, ty, vel, s);
+    case PML_NONE: return group_mp<MODE_ORTHO, PML_NONE>(tab, start, nm, nblk, ty, fs, vel, s);
+    case PML_W3D: return group_mp<MODE_ORTHO, PML_W3D>(tab, start, nm, nblk, ty, fs, vel, s);
+    case PML_SEP: return group_mp<MODE_ORTHO, PML_SEP>(tab, start, nm, nblk, ty, fs, vel, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -236,11 +236,11 @@ def test_graph_replay_equals_direct_launches(gpu):
 def test_autotune_and_tilings_agree(gpu):
     """Every tiling gives the same result bit for bit; autotune picks one of
     them and leaves the context reset."""
-    from elanfmt_amd import workloads as W
+    from elanfmt_amd import fwe, workloads as W
 
     wl = W.ortho_layered(nx=70, ny=30, nz=26, L=4, nt=30, nrec_side=2)
     ref = None
-    for ty, kc in ((4, 8), (2, 3), (8, 26), (4, 5)):
+    for ty, kc in ((4, 8), (2, 3), (8, 26), (4, 5), (4, 1), (8, 2)):
         s = W.make_solver(wl)
         s.set_tiling(ty, kc)
         assert s.tiling() == (ty, kc)
@@ -255,6 +255,13 @@ def test_autotune_and_tilings_agree(gpu):
     assert ms > 0 and s.step() == 0
     s.run(30)
     assert np.array_equal(s.traces(), ref)
+    s.close()
+    # chunks below three planes are refused with a free surface
+    wl.free_surface = 1
+    s = W.make_solver(wl)
+    with pytest.raises(fwe.FweError):
+        s.set_tiling(4, 2)
+    s.set_tiling(4, 3)
     s.close()
 
 
