@@ -39,14 +39,25 @@ DEFAULT_STEPS = {0: 600, 1: 3000, 2: 1000, 3: 1500}
 def _traffic(kernel_name, workload):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE, each scaled by the same counter on the copy probe), kept in
-    profiles/traffic.json."""
+    profiles/traffic.json.  Falls back to the same kernel at another tile
+    height when the autotuner picked one that was not profiled.  Returns
+    (bytes, profiled kernel name) or (None, None)."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p) as f:
-            d = json.load(f)
-        return d[workload][kernel_name]["bytes_per_launch"]
+            d = json.load(f)[workload]
     except Exception:
-        return None
+        return None, None
+    if kernel_name in d:
+        return d[kernel_name]["bytes_per_launch"], kernel_name
+    stem, args = kernel_name.split("<")[0], kernel_name.split("<")[1].rstrip(">").split(", ")
+    for k, v in d.items():
+        if k.startswith(stem + "<"):
+            a = k.split("<")[1].rstrip(">").split(", ")
+            # same law, damping, free surface and wavefield count
+            if a[:-2] == args[:-2] and a[-1] == args[-1]:
+                return v["bytes_per_launch"], k
+    return None, None
 
 
 def cpu_baseline(wl, steps, threads):
@@ -106,17 +117,22 @@ def _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells):
                      "GBs": round(bb * cells / (ms_b / steps * 1e-3) / 1e9, 1)},
     }
     if ms_a >= ms_b:
-        dom = ("stress", ms_a, ba, f"k_stress<{wl.mode}, {wl.pml}, {fs}, {tiling[0]}>")
+        dom = ("stress", ms_a, ba, f"k_stress<{wl.mode}, {wl.pml}, {fs}, {tiling[0]}, {wl.nbatch}>")
     else:
-        dom = ("velocity", ms_b, bb, f"k_velocity<{wl.mode}, {wl.pml}, {tiling[0]}>")
+        dom = ("velocity", ms_b, bb, f"k_velocity<{wl.mode}, {wl.pml}, {tiling[0]}, {wl.nbatch}>")
     return kernels, dom, ba + bb
 
 
 def _roofline(dom, steps, cells, wname):
     name, ms, b, kname = dom
     ach = b * cells / (ms / steps * 1e-3) / 1e9
-    return {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": _traffic(kname, wname)}
+    traffic, prof_k = _traffic(kname, wname)
+    out = {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+           "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel_name": kname,
+           "alg_bytes_per_launch": b * cells}
+    if traffic is not None and prof_k != kname:
+        out["traffic_profiled_as"] = prof_k
+    return out
 
 
 def _line(value, steps, warmup, el, config, **extra):
