@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/fwe.h"
+#include "../../include/fwe_gen.h"
 #include "fwe_internal.h"
 
 using fwe::PointEntry;
@@ -97,6 +98,12 @@ struct fwe_ctx {
   hipGraphExec_t gexec = nullptr;
   int graph_steps = 0;
   int graph_from = 0;  // time-level buffer the captured graph starts from
+  // learned boundary attached with fwe_set_learned_boundary
+  fwe_lb* lb = nullptr;
+  float lb_zinv = 1.f;
+  fwe::LbCell* lb_cells = nullptr;  // layer cells grouped by face
+  int64_t lb_face_off[7] = {0, 0, 0, 0, 0, 0, 0};
+  float* lb_D = nullptr;            // corrections [cells][9]
 };
 
 static void drop_graph(fwe_ctx* x) {
@@ -248,6 +255,7 @@ extern "C" fwe_status fwe_destroy(fwe_ctx* x) {
   hipFree(x->it_dev);
   for (auto ev : x->events) hipEventDestroy(ev);
   hipFree(x->fbuf[0]); hipFree(x->fbuf[1]); hipFree(x->medium); hipFree(x->w3); hipFree(x->prof);
+  hipFree(x->lb_cells); hipFree(x->lb_D);
   hipFree(x->stf); hipFree(x->traces); hipFree(x->inj_off); hipFree(x->inj);
   hipFree(x->injx_off); hipFree(x->injx);
   hipFree(x->rec_off); hipFree(x->rec);
@@ -568,9 +576,27 @@ static bool use_fused(const fwe_ctx* x) { return x->kernel == FWE_KERNEL_FUSED; 
 
 // one step from time level fbuf[from]; step index it (+ *it_base); `mid`
 // (optional) is recorded between the stress and velocity passes
+static hipError_t lb_before(fwe_ctx* x, int from);
+static hipError_t lb_after(fwe_ctx* x, int from);
+
 static hipError_t step_launch(fwe_ctx* x, int from, int it, const int32_t* it_base, hipEvent_t mid) {
   const fwe_config& c = x->cfg;
   hipError_t e = hipSuccess;
+  if (x->lb) {
+    // learned boundary: corrections from the fields before the step, added
+    // after it (two-pass kernels only, checked when attached)
+    e = lb_before(x, from);
+    if (e == hipSuccess) {
+      StepArgs a = make_args(x, from);
+      a.it = it;
+      a.it_base = it_base;
+      e = fwe::launch_stress(a, c.mode, c.pml, c.free_surface, x->stream);
+      if (e == hipSuccess && mid) e = hipEventRecord(mid, x->stream);
+      if (e == hipSuccess) e = fwe::launch_velocity(a, c.mode, c.pml, x->stream);
+    }
+    if (e == hipSuccess) e = lb_after(x, from);
+    return e;
+  }
   if (use_fused(x)) {
     for (int b = 0; b < c.nbatch && e == hipSuccess; ++b) {
       fwe::FusedArgs f = make_fused(x, from, b);
@@ -723,6 +749,8 @@ extern "C" fwe_status fwe_set_kernel(fwe_ctx* x, int32_t kind) {
     return fail(FWE_ERR_ARG, "bad kernel kind %d", kind);
   if (kind == FWE_KERNEL_FUSED && x->cfg.free_surface)
     return fail(FWE_ERR_UNSUPPORTED, "the single-pass kernel has no free surface");
+  if (kind == FWE_KERNEL_FUSED && x->lb)
+    return fail(FWE_ERR_UNSUPPORTED, "the single-pass kernel has no learned boundary");
   drop_graph(x);
   x->kernel = kind;
   if (use_fused(x) && x->kchunk < 3) return apply_tiling(x, x->ty, std::min(3, x->cfg.nz));
@@ -1242,5 +1270,388 @@ extern "C" fwe_status fwe_group_autotune(fwe_group* g, int32_t trial_steps, doub
   for (fwe_ctx* x : g->m)
     if ((st = fwe_reset(x))) return st;
   if (best_ms_per_step) *best_ms_per_step = best;
+  return FWE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// learned thin boundary (kernels in fwe_lb.hip)
+// ---------------------------------------------------------------------------
+using fwe::LbCell;
+using fwe::LbGeom;
+
+struct fwe_lb {
+  int device = 0, L = 6, np = 0;
+  hipStream_t stream = nullptr;
+  float* P = nullptr;     // 6 faces x np, FP32 masters
+  float* Am = nullptr;    // Adam first moments
+  float* Av = nullptr;    // Adam second moments
+  float* grad = nullptr;  // np (one face at a time)
+  int adam_t[6] = {0, 0, 0, 0, 0, 0};
+  // cell lists of the last thin geometry trained on
+  int gx = -1, gy = -1, gz = -1;
+  LbCell* cells = nullptr;
+  int64_t face_off[7] = {0, 0, 0, 0, 0, 0, 0};
+  // per-sample scratch, capacity cap
+  int64_t cap = 0;
+  float *X = nullptr, *R = nullptr, *T = nullptr, *M = nullptr, *Y = nullptr;
+  float *H1 = nullptr, *H2 = nullptr, *D1 = nullptr, *D2 = nullptr, *G = nullptr;
+  int32_t* depth = nullptr;
+  float* part = nullptr;
+  float* loss = nullptr;
+};
+
+static const int LB_MLP_BLOCKS = 1024;  // resident blocks of the MLP kernels
+static const int LB_CHUNK = 512;        // samples per gradient partial sum
+
+static int lb_mlp_blocks(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(LB_MLP_BLOCKS, (n + 3) / 4)); }
+
+// Cells of an nx x ny x nz grid closer than L cells to a face, grouped by
+// face (the first of x-, x+, y-, y+, z-, z+ at the smallest distance),
+// x fastest within a face.
+static void lb_cells_host(int nx, int ny, int nz, int L, std::vector<LbCell>& out, int64_t off[7]) {
+  std::vector<LbCell> per[6];
+  auto visit = [&](int i, int j, int k) {
+    const int dd[6] = {i, nx - 1 - i, j, ny - 1 - j, k, nz - 1 - k};
+    int f = 0;
+    for (int q = 1; q < 6; ++q)
+      if (dd[q] < dd[f]) f = q;
+    if (dd[f] < L) per[f].push_back(LbCell{i, j, k, (f << 8) | dd[f]});
+  };
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j) {
+      const bool inner_row = j >= L && j < ny - L && k >= L && k < nz - L;
+      if (inner_row) {
+        for (int i = 0; i < std::min(L, nx); ++i) visit(i, j, k);
+        for (int i = std::max(L, nx - L); i < nx; ++i) visit(i, j, k);
+      } else {
+        for (int i = 0; i < nx; ++i) visit(i, j, k);
+      }
+    }
+  out.clear();
+  off[0] = 0;
+  for (int f = 0; f < 6; ++f) {
+    out.insert(out.end(), per[f].begin(), per[f].end());
+    off[f + 1] = (int64_t)out.size();
+  }
+}
+
+static LbGeom lb_geom(const fwe_ctx* x, int from, float zinv) {
+  LbGeom g;
+  for (int f = 0; f < 9; ++f) g.F[f] = x->fbuf[from] + (int64_t)f * x->nelem;
+  g.sy = x->sy;
+  g.sz = x->sz;
+  g.nx = x->cfg.nx;
+  g.ny = x->cfg.ny;
+  g.nz = x->cfg.nz;
+  g.zinv = zinv;
+  return g;
+}
+
+static hipError_t lb_before(fwe_ctx* x, int from) {
+  const LbGeom g = lb_geom(x, from, x->lb_zinv);
+  hipError_t e = hipSuccess;
+  for (int f = 0; f < 6 && e == hipSuccess; ++f) {
+    const int64_t o = x->lb_face_off[f], n = x->lb_face_off[f + 1] - o;
+    e = fwe::lb_launch_predict(x->lb->P + (int64_t)f * x->lb->np, x->lb->L, g, x->lb_cells + o, (int)n,
+                               x->lb_D + o * fwe::LB_OUT, lb_mlp_blocks(n), x->stream);
+  }
+  return e;
+}
+
+static hipError_t lb_after(fwe_ctx* x, int from) {
+  return fwe::lb_launch_add(lb_geom(x, from, x->lb_zinv), x->lb_cells, (int)x->lb_face_off[6], x->lb_D, x->stream);
+}
+
+extern "C" fwe_status fwe_lb_param_count(int32_t layer, int32_t* n) {
+  if (!n || layer < 1 || layer > fwe::LB_MAXL) return fail(FWE_ERR_ARG, "layer must be in 1..%d", fwe::LB_MAXL);
+  *n = fwe::lb_nparams(layer);
+  return FWE_OK;
+}
+
+static void lb_free_scratch(fwe_lb* lb) {
+  float* fs[] = {lb->X, lb->R, lb->T, lb->M, lb->Y, lb->H1, lb->H2, lb->D1, lb->D2, lb->G, lb->part};
+  for (float* p : fs) hipFree(p);
+  hipFree(lb->depth);
+  lb->X = lb->R = lb->T = lb->M = lb->Y = lb->H1 = lb->H2 = lb->D1 = lb->D2 = lb->G = lb->part = nullptr;
+  lb->depth = nullptr;
+  lb->cap = 0;
+}
+
+static fwe_status lb_reserve(fwe_lb* lb, int64_t n) {
+  if (n <= lb->cap) return FWE_OK;
+  lb_free_scratch(lb);
+  const int64_t nchunk = (n + LB_CHUNK - 1) / LB_CHUNK;
+  hipError_t e = hipSuccess;
+  auto al = [&](float** p, int64_t count) {
+    if (e == hipSuccess) e = hipMalloc(p, sizeof(float) * (size_t)std::max<int64_t>(1, count));
+  };
+  al(&lb->X, n * fwe::LB_IN);
+  al(&lb->R, n);
+  al(&lb->T, n * fwe::LB_OUT);
+  al(&lb->M, n * fwe::LB_OUT);
+  al(&lb->Y, n * fwe::LB_OUT);
+  al(&lb->H1, n * fwe::LB_H);
+  al(&lb->H2, n * fwe::LB_H);
+  al(&lb->D1, n * fwe::LB_H);
+  al(&lb->D2, n * fwe::LB_H);
+  al(&lb->G, n * fwe::LB_OUT);
+  al(&lb->part, nchunk * fwe::LB_IN * fwe::LB_H);
+  if (e == hipSuccess) e = hipMalloc(&lb->depth, sizeof(int32_t) * (size_t)std::max<int64_t>(1, n));
+  if (e != hipSuccess) {
+    lb_free_scratch(lb);
+    return fail(FWE_ERR_NOMEM, "learned boundary scratch for %lld samples: %s", (long long)n, hipGetErrorString(e));
+  }
+  lb->cap = n;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_lb_init(fwe_lb* lb, uint64_t seed) {
+  if (!lb) return fail(FWE_ERR_ARG, "NULL learned boundary");
+  const int np = lb->np, L = lb->L;
+  std::vector<float> h((size_t)np * 6, 0.f);
+  fwe_rng r;
+  fwe_rng_seed(&r, seed);
+  for (int f = 0; f < 6; ++f) {
+    float* p = h.data() + (size_t)f * np;
+    // He-normal hidden layers; zero output layer, so an initialised network
+    // adds no correction and the layer starts as plain damping
+    const double s1 = sqrt(2.0 / fwe::LB_IN), s2 = sqrt(2.0 / fwe::LB_H);
+    for (int q = 0; q < fwe::LB_IN * fwe::LB_H; ++q) p[fwe::lb_off_w1() + q] = (float)(s1 * fwe_rng_normal(&r));
+    for (int q = 0; q < fwe::LB_H * fwe::LB_H; ++q) p[fwe::lb_off_w2(L) + q] = (float)(s2 * fwe_rng_normal(&r));
+  }
+  HIPCHK(hipSetDevice(lb->device));
+  HIPCHK(hipMemcpy(lb->P, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(lb->Am, 0, sizeof(float) * h.size()));
+  HIPCHK(hipMemset(lb->Av, 0, sizeof(float) * h.size()));
+  for (int f = 0; f < 6; ++f) lb->adam_t[f] = 0;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_lb_destroy(fwe_lb* lb) {
+  if (!lb) return FWE_OK;
+  hipSetDevice(lb->device);
+  if (lb->stream) hipStreamSynchronize(lb->stream);
+  lb_free_scratch(lb);
+  hipFree(lb->P); hipFree(lb->Am); hipFree(lb->Av); hipFree(lb->grad); hipFree(lb->cells); hipFree(lb->loss);
+  if (lb->stream) hipStreamDestroy(lb->stream);
+  delete lb;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_lb_create(int32_t device, int32_t layer, fwe_lb** out) {
+  if (!out) return fail(FWE_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (layer < 1 || layer > fwe::LB_MAXL) return fail(FWE_ERR_ARG, "layer must be in 1..%d", fwe::LB_MAXL);
+  int nd = 0;
+  if (hipGetDeviceCount(&nd) != hipSuccess || device < 0 || device >= nd)
+    return fail(FWE_ERR_HIP, "no HIP device %d", device);
+  fwe_lb* lb = new (std::nothrow) fwe_lb();
+  if (!lb) return fail(FWE_ERR_NOMEM, "host allocation");
+  lb->device = device;
+  lb->L = layer;
+  lb->np = fwe::lb_nparams(layer);
+  HIPCHK(hipSetDevice(device));
+  const size_t pb = sizeof(float) * (size_t)lb->np * 6;
+  hipError_t e = hipStreamCreateWithFlags(&lb->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&lb->P, pb);
+  if (e == hipSuccess) e = hipMalloc(&lb->Am, pb);
+  if (e == hipSuccess) e = hipMalloc(&lb->Av, pb);
+  if (e == hipSuccess) e = hipMalloc(&lb->grad, sizeof(float) * lb->np);
+  if (e == hipSuccess) e = hipMalloc(&lb->loss, sizeof(float) * LB_MLP_BLOCKS);
+  if (e != hipSuccess) {
+    fwe_lb_destroy(lb);
+    return fail(FWE_ERR_HIP, "learned boundary create: %s", hipGetErrorString(e));
+  }
+  fwe_status st = fwe_lb_init(lb, 1);
+  if (st) { fwe_lb_destroy(lb); return st; }
+  *out = lb;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_lb_set_params(fwe_lb* lb, int32_t face, const float* p) {
+  if (!lb || !p || face < 0 || face > 5) return fail(FWE_ERR_ARG, "bad learned-boundary parameter arguments");
+  HIPCHK(hipSetDevice(lb->device));
+  HIPCHK(hipStreamSynchronize(lb->stream));
+  HIPCHK(hipMemcpy(lb->P + (int64_t)face * lb->np, p, sizeof(float) * lb->np, hipMemcpyHostToDevice));
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_lb_get_params(fwe_lb* lb, int32_t face, float* p) {
+  if (!lb || !p || face < 0 || face > 5) return fail(FWE_ERR_ARG, "bad learned-boundary parameter arguments");
+  HIPCHK(hipSetDevice(lb->device));
+  HIPCHK(hipStreamSynchronize(lb->stream));
+  HIPCHK(hipMemcpy(p, lb->P + (int64_t)face * lb->np, sizeof(float) * lb->np, hipMemcpyDeviceToHost));
+  return FWE_OK;
+}
+
+// gradient of the loss w.r.t. one face's parameters from the activations of
+// samples [o, o+n) into lb->grad
+static hipError_t lb_grad(fwe_lb* lb, int64_t o, int n) {
+  const int L = lb->L;
+  float* g = lb->grad;
+  hipStream_t s = lb->stream;
+  const float* X = lb->X + o * fwe::LB_IN;
+  const float* H1 = lb->H1 + o * fwe::LB_H;
+  const float* H2 = lb->H2 + o * fwe::LB_H;
+  const float* D1 = lb->D1 + o * fwe::LB_H;
+  const float* D2 = lb->D2 + o * fwe::LB_H;
+  const float* G = lb->G + o * fwe::LB_OUT;
+  const int32_t* dep = lb->depth + o;
+  hipError_t e = fwe::lb_launch_atb(X, fwe::LB_IN, nullptr, fwe::LB_IN, D1, fwe::LB_H, fwe::LB_H, n, LB_CHUNK,
+                                    lb->part, g + fwe::lb_off_w1(), s);
+  if (e == hipSuccess)
+    e = fwe::lb_launch_atb(nullptr, 0, dep, L, D1, fwe::LB_H, fwe::LB_H, n, LB_CHUNK, lb->part, g + fwe::lb_off_b1(), s);
+  if (e == hipSuccess)
+    e = fwe::lb_launch_atb(H1, fwe::LB_H, nullptr, fwe::LB_H, D2, fwe::LB_H, fwe::LB_H, n, LB_CHUNK, lb->part,
+                           g + fwe::lb_off_w2(L), s);
+  if (e == hipSuccess)
+    e = fwe::lb_launch_atb(nullptr, 0, nullptr, 1, D2, fwe::LB_H, fwe::LB_H, n, LB_CHUNK, lb->part, g + fwe::lb_off_b2(L), s);
+  if (e == hipSuccess)
+    e = fwe::lb_launch_atb(H2, fwe::LB_H, nullptr, fwe::LB_H, G, fwe::LB_OUT, fwe::LB_OUT, n, LB_CHUNK, lb->part,
+                           g + fwe::lb_off_w3(L), s);
+  if (e == hipSuccess)
+    e = fwe::lb_launch_atb(nullptr, 0, nullptr, 1, G, fwe::LB_OUT, fwe::LB_OUT, n, LB_CHUNK, lb->part,
+                           g + fwe::lb_off_b3(L), s);
+  return e;
+}
+
+// forward + backward of samples [o, o+n) of face f (targets in lb->T, M);
+// adds the summed squared error to *lsum
+static fwe_status lb_fb(fwe_lb* lb, int f, int64_t o, int n, bool want_y, double* lsum) {
+  const int nblk = lb_mlp_blocks(n);
+  HIPCHK(fwe::lb_launch_mlp(lb->P + (int64_t)f * lb->np, lb->L, n, lb->X + o * fwe::LB_IN, lb->depth + o,
+                            lb->T + o * fwe::LB_OUT, lb->M + o * fwe::LB_OUT, 1.f / (9.f * (float)n),
+                            want_y ? lb->Y + o * fwe::LB_OUT : nullptr, lb->H1 + o * fwe::LB_H,
+                            lb->H2 + o * fwe::LB_H, lb->D1 + o * fwe::LB_H, lb->D2 + o * fwe::LB_H,
+                            lb->G + o * fwe::LB_OUT, lb->loss, nblk, lb->stream));
+  HIPCHK(lb_grad(lb, o, n));
+  std::vector<float> lp(nblk);
+  HIPCHK(hipMemcpyAsync(lp.data(), lb->loss, sizeof(float) * nblk, hipMemcpyDeviceToHost, lb->stream));
+  HIPCHK(hipStreamSynchronize(lb->stream));
+  double t = 0;
+  for (float v : lp) t += v;
+  *lsum += t;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_lb_eval(fwe_lb* lb, int32_t face, int32_t n, const float* x, const int32_t* depth,
+                                  const float* target, const float* mask, float* y, float* grad, double* loss) {
+  if (!lb || face < 0 || face > 5 || n < 1 || !x || !depth || (grad && (!target || !mask)))
+    return fail(FWE_ERR_ARG, "bad learned-boundary eval arguments");
+  for (int s = 0; s < n; ++s)
+    if (depth[s] < 0 || depth[s] >= lb->L) return fail(FWE_ERR_ARG, "sample %d: depth %d outside 0..%d", s, depth[s], lb->L - 1);
+  HIPCHK(hipSetDevice(lb->device));
+  fwe_status st = lb_reserve(lb, n);
+  if (st) return st;
+  hipStream_t s = lb->stream;
+  HIPCHK(hipMemcpyAsync(lb->X, x, sizeof(float) * (size_t)n * fwe::LB_IN, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(lb->depth, depth, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  const bool train = target && mask;
+  if (train) {
+    HIPCHK(hipMemcpyAsync(lb->T, target, sizeof(float) * (size_t)n * fwe::LB_OUT, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(lb->M, mask, sizeof(float) * (size_t)n * fwe::LB_OUT, hipMemcpyHostToDevice, s));
+    double l = 0;
+    if ((st = lb_fb(lb, face, 0, n, true, &l))) return st;
+    if (loss) *loss = l / (9.0 * n);
+    if (grad) HIPCHK(hipMemcpyAsync(grad, lb->grad, sizeof(float) * lb->np, hipMemcpyDeviceToHost, s));
+  } else {
+    HIPCHK(fwe::lb_launch_mlp(lb->P + (int64_t)face * lb->np, lb->L, n, lb->X, lb->depth, nullptr, nullptr, 0.f,
+                              lb->Y, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, lb_mlp_blocks(n), s));
+  }
+  if (y) HIPCHK(hipMemcpyAsync(y, lb->Y, sizeof(float) * (size_t)n * fwe::LB_OUT, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return FWE_OK;
+}
+
+static fwe_status lb_plain_ctx(const fwe_ctx* x, const char* what) {
+  if (x->cfg.nbatch != 1) return fail(FWE_ERR_UNSUPPORTED, "%s: one wavefield only", what);
+  if (x->cfg.free_surface) return fail(FWE_ERR_UNSUPPORTED, "%s: the learned boundary replaces damping on all six faces; no free surface", what);
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_lb_train_step(fwe_lb* lb, fwe_ctx* thick, fwe_ctx* thin, int32_t offset, float impedance,
+                                        float lr, double* loss) {
+  if (!lb || !thick || !thin || offset < 0 || !(impedance > 0.f) || !(lr >= 0.f))
+    return fail(FWE_ERR_ARG, "bad learned-boundary training arguments");
+  fwe_status st;
+  if ((st = lb_plain_ctx(thick, "thick run")) || (st = lb_plain_ctx(thin, "thin run"))) return st;
+  if (thin->lb) return fail(FWE_ERR_STATE, "the thin run must not have a learned boundary attached while training");
+  if (thick->cfg.device != lb->device || thin->cfg.device != lb->device)
+    return fail(FWE_ERR_ARG, "runs and learned boundary on different devices");
+  const fwe_config &a = thick->cfg, &b = thin->cfg;
+  if (a.nx != b.nx + 2 * offset || a.ny != b.ny + 2 * offset || a.nz != b.nz + 2 * offset)
+    return fail(FWE_ERR_ARG, "thick grid must be the thin grid plus %d cells on every face", offset);
+  if ((st = ready(thick)) || (st = ready(thin))) return st;
+  HIPCHK(hipSetDevice(lb->device));
+  if (lb->gx != b.nx || lb->gy != b.ny || lb->gz != b.nz) {
+    std::vector<LbCell> h;
+    lb_cells_host(b.nx, b.ny, b.nz, lb->L, h, lb->face_off);
+    hipFree(lb->cells);
+    lb->cells = nullptr;
+    lb->gx = -1;
+    HIPCHK(hipMalloc(&lb->cells, sizeof(LbCell) * std::max<size_t>(1, h.size())));
+    HIPCHK(hipMemcpy(lb->cells, h.data(), sizeof(LbCell) * h.size(), hipMemcpyHostToDevice));
+    lb->gx = b.nx; lb->gy = b.ny; lb->gz = b.nz;
+  }
+  const int64_t ncell = lb->face_off[6];
+  if ((st = lb_reserve(lb, ncell))) return st;
+  const float zinv = 1.f / impedance;
+  HIPCHK(hipStreamSynchronize(thick->stream));
+  HIPCHK(hipStreamSynchronize(thin->stream));
+  // thin window of the thick run at step n -> thin fields; features
+  HIPCHK(fwe::lb_launch_crop(lb_geom(thick, thick->cur, zinv), lb_geom(thin, thin->cur, zinv), offset, lb->stream));
+  HIPCHK(fwe::lb_launch_gather(lb_geom(thin, thin->cur, zinv), lb->cells, (int)ncell, lb->X, lb->R, lb->depth,
+                               lb->stream));
+  HIPCHK(hipStreamSynchronize(lb->stream));
+  // one plain thin step and one thick step from the same state
+  thin->it = thick->it;
+  if ((st = run_impl(thin, 1, nullptr, nullptr, nullptr))) return st;
+  if ((st = run_impl(thick, 1, nullptr, nullptr, nullptr))) return st;
+  HIPCHK(hipStreamSynchronize(thin->stream));
+  HIPCHK(hipStreamSynchronize(thick->stream));
+  HIPCHK(fwe::lb_launch_targets(lb_geom(thin, thin->cur, zinv), lb_geom(thick, thick->cur, zinv), offset, lb->cells,
+                                (int)ncell, lb->R, lb->T, lb->M, lb->stream));
+  double lsum = 0;
+  for (int f = 0; f < 6; ++f) {
+    const int64_t o = lb->face_off[f], n = lb->face_off[f + 1] - o;
+    if (n == 0) continue;
+    if ((st = lb_fb(lb, f, o, (int)n, false, &lsum))) return st;
+    if (lr > 0.f) {
+      const int t = ++lb->adam_t[f];
+      const float c1 = (float)(1.0 / (1.0 - pow(0.9, t))), c2 = (float)(1.0 / (1.0 - pow(0.999, t)));
+      const int64_t po = (int64_t)f * lb->np;
+      HIPCHK(fwe::lb_launch_adam(lb->P + po, lb->Am + po, lb->Av + po, lb->grad, lb->np, lr, 0.9f, 0.999f, 1e-8f, c1,
+                                 c2, lb->stream));
+    }
+  }
+  HIPCHK(hipStreamSynchronize(lb->stream));
+  if (loss) *loss = lsum / (9.0 * (double)std::max<int64_t>(1, ncell));
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_set_learned_boundary(fwe_ctx* x, fwe_lb* lb, float impedance) {
+  if (!x) return fail(FWE_ERR_ARG, "NULL ctx");
+  HIPCHK(hipSetDevice(x->cfg.device));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  drop_graph(x);
+  hipFree(x->lb_cells);
+  hipFree(x->lb_D);
+  x->lb_cells = nullptr;
+  x->lb_D = nullptr;
+  x->lb = nullptr;
+  if (!lb) return FWE_OK;
+  fwe_status st = lb_plain_ctx(x, "learned boundary");
+  if (st) return st;
+  if (!(impedance > 0.f)) return fail(FWE_ERR_ARG, "impedance must be positive");
+  if (lb->device != x->cfg.device) return fail(FWE_ERR_ARG, "learned boundary on another device");
+  if (use_fused(x)) return fail(FWE_ERR_UNSUPPORTED, "the single-pass kernel has no learned boundary");
+  std::vector<LbCell> h;
+  lb_cells_host(x->cfg.nx, x->cfg.ny, x->cfg.nz, lb->L, h, x->lb_face_off);
+  HIPCHK(hipMalloc(&x->lb_cells, sizeof(LbCell) * std::max<size_t>(1, h.size())));
+  HIPCHK(hipMemcpy(x->lb_cells, h.data(), sizeof(LbCell) * h.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&x->lb_D, sizeof(float) * std::max<size_t>(1, h.size() * fwe::LB_OUT)));
+  HIPCHK(hipMemset(x->lb_D, 0, sizeof(float) * std::max<size_t>(1, h.size() * fwe::LB_OUT)));
+  x->lb = lb;
+  x->lb_zinv = 1.f / impedance;
   return FWE_OK;
 }

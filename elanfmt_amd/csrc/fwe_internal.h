@@ -123,4 +123,49 @@ hipError_t launch_copy(const float* src, float* dst, int64_t n, hipStream_t s);
 hipError_t launch_combine(const float* tr, int nb, int64_t per, const float* coef, int mc, float* out,
                           hipStream_t s);
 
+// ---------------------------------------------------------------------------
+// learned thin boundary (fwe_lb.hip)
+// ---------------------------------------------------------------------------
+constexpr int LB_IN = 81, LB_H = 64, LB_OUT = 9, LB_MAXL = 16;
+// parameters of one face, FP32: W1 [81][64], b1 [L][64], W2 [64][64], b2 [64], W3 [64][9], b3 [9]
+__host__ __device__ constexpr int lb_off_w1() { return 0; }
+__host__ __device__ constexpr int lb_off_b1() { return LB_IN * LB_H; }
+__host__ __device__ constexpr int lb_off_w2(int L) { return LB_IN * LB_H + L * LB_H; }
+__host__ __device__ constexpr int lb_off_b2(int L) { return lb_off_w2(L) + LB_H * LB_H; }
+__host__ __device__ constexpr int lb_off_w3(int L) { return lb_off_b2(L) + LB_H; }
+__host__ __device__ constexpr int lb_off_b3(int L) { return lb_off_w3(L) + LB_H * LB_OUT; }
+__host__ __device__ constexpr int lb_nparams(int L) { return lb_off_b3(L) + LB_OUT; }
+
+// a cell of the layer: fd = face << 8 | depth (0 = outermost); faces
+// 0..5 = x-, x+, y-, y+, z-, z+
+struct LbCell {
+  int32_t i, j, k, fd;
+};
+
+// one wavefield's field set in the padded layout
+struct LbGeom {
+  const float* F[9];
+  int64_t sy, sz;
+  int32_t nx, ny, nz;
+  float zinv;
+};
+
+hipError_t lb_launch_mlp(const float* P, int L, int n, const float* X, const int32_t* depth, const float* T,
+                         const float* M, float inv_count, float* Y, float* H1, float* H2, float* D1, float* D2,
+                         float* G, float* loss_part, int nblk, hipStream_t s);
+// out[p][q] = sum_s a(s,p) B[s][q]; a = A[s][p], onehot(depth[s]) or 1;
+// part holds ceil(n/chunk) * P * Q floats
+hipError_t lb_launch_atb(const float* A, int lda, const int32_t* onehot, int P, const float* B, int ldb, int Q,
+                         int n, int chunk, float* part, float* out, hipStream_t s);
+hipError_t lb_launch_adam(float* P, float* m, float* v, const float* g, int n, float lr, float b1, float b2,
+                          float eps, float c1, float c2, hipStream_t s);
+hipError_t lb_launch_gather(const LbGeom& g, const LbCell* cells, int n, float* X, float* R, int32_t* depth,
+                            hipStream_t s);
+hipError_t lb_launch_targets(const LbGeom& thin, const LbGeom& thick, int off, const LbCell* cells, int n,
+                             const float* R, float* T, float* M, hipStream_t s);
+hipError_t lb_launch_predict(const float* P, int L, const LbGeom& g, const LbCell* cells, int n, float* D,
+                             int nblk, hipStream_t s);
+hipError_t lb_launch_add(const LbGeom& g, const LbCell* cells, int n, const float* D, hipStream_t s);
+hipError_t lb_launch_crop(const LbGeom& thick, const LbGeom& thin, int off, hipStream_t s);
+
 }  // namespace fwe

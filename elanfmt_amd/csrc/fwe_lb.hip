@@ -1,0 +1,416 @@
+// fwe_lb.hip -- kernels of the learned thin boundary: a per-face MLP that
+// corrects the nine fields of the cells of an L-cell absorbing layer after
+// every step, trained on the GPU against runs with a thick damping layer.
+//
+// Network of one face (81 -> 64 -> 64 -> 9, SiLU):
+//   x  = stencil of the cell, 9 fields at 9 cells (the cell, its 6 face
+//        neighbours, and two cells out along the face normal), stresses
+//        divided by the impedance Z, the whole vector divided by its RMS r
+//   a1 = W1 x + b1[depth]      h1 = silu(a1)
+//   a2 = W2 h1 + b2            h2 = silu(a2)
+//   y  = W3 h2 + b3            correction: dV = r y[0:3], dS = Z r y[3:9]
+// W1, W2, W3 are used rounded to bfloat16 (kept as FP32 master copies for
+// training); all sums accumulate in FP32.
+//
+// One wavefront evaluates one sample: lane h owns hidden unit h.  Weights
+// are staged in LDS as bfloat16 for the whole block.
+#include <algorithm>
+
+#include "fwe_internal.h"
+
+namespace fwe {
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf / nan
+  u += 0x7fffu + ((u >> 16) & 1u);                                   // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ __forceinline__ float sigm(float a) { return 1.f / (1.f + __expf(-a)); }
+
+constexpr int LB_WAVES = 4;  // samples in flight per block
+
+struct LbShared {
+  uint16_t w1[LB_IN * LB_H];   // [k][h]
+  uint16_t w2[LB_H * LB_H];    // [m][h]
+  uint16_t w2t[LB_H * LB_H];   // [h][m]
+  uint16_t w3[LB_H * LB_OUT];  // [h][o]
+  float b1[LB_MAXL * LB_H];
+  float b2[LB_H];
+  float b3[LB_OUT];
+  float xs[LB_WAVES][LB_IN + 3];
+  float h1s[LB_WAVES][LB_H];
+  float h2s[LB_WAVES][LB_H];
+  float gs[LB_WAVES][12];
+  float d2s[LB_WAVES][LB_H];
+  float red[256];
+};
+
+__device__ void lb_stage(LbShared& sh, const float* __restrict__ P, int L, bool transpose) {
+  const int t = threadIdx.x;
+  for (int q = t; q < LB_IN * LB_H; q += 256) sh.w1[q] = to_bf16(P[lb_off_w1() + q]);
+  for (int q = t; q < LB_H * LB_H; q += 256) {
+    const uint16_t v = to_bf16(P[lb_off_w2(L) + q]);
+    sh.w2[q] = v;
+    if (transpose) sh.w2t[(q % LB_H) * LB_H + q / LB_H] = v;
+  }
+  for (int q = t; q < LB_H * LB_OUT; q += 256) sh.w3[q] = to_bf16(P[lb_off_w3(L) + q]);
+  for (int q = t; q < L * LB_H; q += 256) sh.b1[q] = P[lb_off_b1() + q];
+  for (int q = t; q < LB_H; q += 256) sh.b2[q] = P[lb_off_b2(L) + q];
+  for (int q = t; q < LB_OUT; q += 256) sh.b3[q] = P[lb_off_b3(L) + q];
+}
+
+// hidden layers of the sample in sh.xs[w]; returns a1, a2 of lane h and
+// leaves h2 in sh.h2s[w].  Every thread of the block calls it (barriers).
+__device__ __forceinline__ void lb_hidden(LbShared& sh, int w, int h, int d, bool valid, float* h1out,
+                                          float* h2out, float& a1, float& a2) {
+  a1 = 0.f;
+  if (valid) {
+    float s = sh.b1[d * LB_H + h];
+    for (int k = 0; k < LB_IN; ++k) s = fmaf(bf(sh.w1[k * LB_H + h]), sh.xs[w][k], s);
+    a1 = s;
+    const float h1 = a1 * sigm(a1);
+    sh.h1s[w][h] = h1;
+    if (h1out) h1out[h] = h1;
+  }
+  __syncthreads();
+  a2 = 0.f;
+  if (valid) {
+    float s = sh.b2[h];
+    for (int m = 0; m < LB_H; ++m) s = fmaf(bf(sh.w2[m * LB_H + h]), sh.h1s[w][m], s);
+    a2 = s;
+    const float h2 = a2 * sigm(a2);
+    sh.h2s[w][h] = h2;
+    if (h2out) h2out[h] = h2;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ float lb_output(const LbShared& sh, int w, int o) {
+  float s = sh.b3[o];
+  for (int h = 0; h < LB_H; ++h) s = fmaf(bf(sh.w3[h * LB_OUT + o]), sh.h2s[w][h], s);
+  return s;
+}
+
+__device__ __forceinline__ float dsilu(float a) {
+  const float s = sigm(a);
+  return s * (1.f + a * (1.f - s));
+}
+
+// Forward (and, with T != nullptr, the backward pass to the activation
+// gradients) of n samples of one face.  Loss = sum M (y - T)^2 * inv_count.
+// Outputs per sample: Y [9], H1, H2, D1 = dL/da1, D2 = dL/da2 [64], G =
+// dL/dy [9]; per block: the partial loss.
+__global__ __launch_bounds__(256) void k_lb_mlp(const float* __restrict__ P, int32_t L, int32_t n,
+                                               const float* __restrict__ X, const int32_t* __restrict__ depth,
+                                               const float* __restrict__ T, const float* __restrict__ M,
+                                               float inv_count, float* __restrict__ Y, float* __restrict__ H1,
+                                               float* __restrict__ H2, float* __restrict__ D1,
+                                               float* __restrict__ D2, float* __restrict__ G,
+                                               float* __restrict__ loss_part) {
+  __shared__ LbShared sh;
+  const bool train = T != nullptr;
+  lb_stage(sh, P, L, train);
+  __syncthreads();
+  const int w = threadIdx.x >> 6, h = threadIdx.x & 63;
+  float lsum = 0.f;
+  for (int64_t base = (int64_t)blockIdx.x * LB_WAVES; base < n; base += (int64_t)gridDim.x * LB_WAVES) {
+    const int64_t s = base + w;
+    const bool valid = s < n;
+    int d = 0;
+    if (valid) {
+      sh.xs[w][h] = X[s * LB_IN + h];
+      if (h + 64 < LB_IN) sh.xs[w][h + 64] = X[s * LB_IN + h + 64];
+      d = depth[s];
+    }
+    __syncthreads();
+    float a1, a2;
+    lb_hidden(sh, w, h, d, valid, H1 ? H1 + s * LB_H : nullptr, H2 ? H2 + s * LB_H : nullptr, a1, a2);
+    if (valid && h < LB_OUT) {
+      const float y = lb_output(sh, w, h);
+      if (Y) Y[s * LB_OUT + h] = y;
+      if (train) {
+        const float mk = M[s * LB_OUT + h];
+        const float e = y - T[s * LB_OUT + h];
+        lsum = fmaf(mk * e, e, lsum);
+        const float g = 2.f * mk * e * inv_count;
+        sh.gs[w][h] = g;
+        G[s * LB_OUT + h] = g;
+      }
+    }
+    __syncthreads();
+    if (!train) continue;
+    if (valid) {
+      float dh2 = 0.f;
+      for (int o = 0; o < LB_OUT; ++o) dh2 = fmaf(bf(sh.w3[h * LB_OUT + o]), sh.gs[w][o], dh2);
+      const float da2 = dh2 * dsilu(a2);
+      sh.d2s[w][h] = da2;
+      D2[s * LB_H + h] = da2;
+    }
+    __syncthreads();
+    if (valid) {
+      float dh1 = 0.f;
+      for (int q = 0; q < LB_H; ++q) dh1 = fmaf(bf(sh.w2t[q * LB_H + h]), sh.d2s[w][q], dh1);
+      D1[s * LB_H + h] = dh1 * dsilu(a1);
+    }
+  }
+  if (loss_part) {
+    sh.red[threadIdx.x] = lsum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float t = 0.f;
+      for (int q = 0; q < 256; ++q) t += sh.red[q];
+      loss_part[blockIdx.x] = t;
+    }
+  }
+}
+
+// Partial sums over chunks of samples:  part[c][p][q] = sum_{s in chunk c}
+// a(s,p) * B[s][q], with a(s,p) = A[s][p] (A != nullptr), the one-hot of
+// depth[s] (onehot), or 1.  Reduced in a fixed order by k_lb_reduce.
+__global__ __launch_bounds__(256) void k_lb_atb(const float* __restrict__ A, int32_t lda,
+                                               const int32_t* __restrict__ onehot, int32_t P,
+                                               const float* __restrict__ B, int32_t ldb, int32_t Q, int32_t n,
+                                               int32_t chunk, float* __restrict__ part) {
+  const int q = threadIdx.x, p = blockIdx.y * 4 + threadIdx.y;
+  const int c = blockIdx.x;
+  if (q >= Q || p >= P) return;
+  const int64_t s0 = (int64_t)c * chunk, s1 = min((int64_t)n, s0 + chunk);
+  float acc = 0.f;
+  for (int64_t s = s0; s < s1; ++s) {
+    const float a = A ? A[s * lda + p] : (onehot ? (onehot[s] == p ? 1.f : 0.f) : 1.f);
+    acc = fmaf(a, B[s * ldb + q], acc);
+  }
+  part[((int64_t)c * P + p) * Q + q] = acc;
+}
+
+__global__ void k_lb_reduce(const float* __restrict__ part, int32_t nchunk, int32_t count, float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float s = 0.f;
+  for (int c = 0; c < nchunk; ++c) s += part[(int64_t)c * count + i];
+  out[i] = s;
+}
+
+// Adam on one face's parameters (bias corrections c1 = 1/(1-b1^t), c2 = 1/(1-b2^t)).
+__global__ void k_lb_adam(float* __restrict__ P, float* __restrict__ m, float* __restrict__ v,
+                          const float* __restrict__ g, int32_t n, float lr, float b1, float b2, float eps, float c1,
+                          float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  P[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// stencil gather from a field set (one wavefield, padded layout)
+// ---------------------------------------------------------------------------
+
+// neighbour q of a cell whose layer face has normal axis ax
+__device__ __forceinline__ void lb_nbr(int q, int ax, int& di, int& dj, int& dk) {
+  di = dj = dk = 0;
+  switch (q) {
+    case 1: di = -1; break;
+    case 2: di = 1; break;
+    case 3: dj = -1; break;
+    case 4: dj = 1; break;
+    case 5: dk = -1; break;
+    case 6: dk = 1; break;
+    case 7: (ax == 0 ? di : ax == 1 ? dj : dk) = -2; break;
+    case 8: (ax == 0 ? di : ax == 1 ? dj : dk) = 2; break;
+    default: break;
+  }
+}
+
+// input e (0..80) of the cell: field e % 9 at neighbour e / 9
+__device__ __forceinline__ float lb_input(const LbGeom& g, const LbCell& c, int e) {
+  const int q = e / 9, f = e - q * 9;
+  int di, dj, dk;
+  lb_nbr(q, (c.fd >> 8) >> 1, di, dj, dk);
+  const int i = c.i + di, j = c.j + dj, k = c.k + dk;
+  if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) return 0.f;
+  const float v = g.F[f][i + j * g.sy + (int64_t)k * g.sz];
+  return f >= 3 ? v * g.zinv : v;
+}
+
+// Training features of n cells: X [n][81] normalised, R [n] = RMS, depth.
+__global__ __launch_bounds__(256) void k_lb_gather(const LbGeom g, const LbCell* __restrict__ cells, int32_t n,
+                                                  float* __restrict__ X, float* __restrict__ R,
+                                                  int32_t* __restrict__ depth) {
+  const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (s >= n) return;
+  const LbCell c = cells[s];
+  float ss = 0.f;
+  for (int e = 0; e < LB_IN; ++e) {
+    const float v = lb_input(g, c, e);
+    X[s * LB_IN + e] = v;
+    ss = fmaf(v, v, ss);
+  }
+  const float r = sqrtf(ss * (1.f / LB_IN)) + 1e-30f;
+  const float ri = 1.f / r;
+  for (int e = 0; e < LB_IN; ++e) X[s * LB_IN + e] *= ri;
+  R[s] = r;
+  depth[s] = c.fd & 0xff;
+}
+
+// Shear components that the stress pass keeps at zero for this cell are
+// masked (a correction there would be overwritten).
+__device__ __forceinline__ bool lb_comp_live(const LbCell& c, int f) {
+  if (f == 6) return c.i > 0 && c.j > 0;
+  if (f == 7) return c.i > 0 && c.k > 0;
+  if (f == 8) return c.j > 0 && c.k > 0;
+  return true;
+}
+
+// Targets: T = (thick(n+1) at cell+off - thin step) / (r * unit), unit = 1
+// for velocities and Z for stresses; M = live components.
+__global__ __launch_bounds__(256) void k_lb_targets(const LbGeom thin, const LbGeom thick, int32_t off,
+                                                   const LbCell* __restrict__ cells, int32_t n,
+                                                   const float* __restrict__ R, float* __restrict__ T,
+                                                   float* __restrict__ M) {
+  const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (s >= n) return;
+  const LbCell c = cells[s];
+  const float ri = 1.f / R[s];
+  const int64_t a = c.i + c.j * thin.sy + (int64_t)c.k * thin.sz;
+  const int64_t b = (c.i + off) + (c.j + off) * thick.sy + (int64_t)(c.k + off) * thick.sz;
+  for (int f = 0; f < LB_OUT; ++f) {
+    const bool live = lb_comp_live(c, f);
+    const float d = thick.F[f][b] - thin.F[f][a];
+    T[s * LB_OUT + f] = live ? d * ri * (f >= 3 ? thin.zinv : 1.f) : 0.f;
+    M[s * LB_OUT + f] = live ? 1.f : 0.f;
+  }
+}
+
+// Inference, before the step: corrections of n cells of one face into D [n][9].
+__global__ __launch_bounds__(256) void k_lb_predict(const float* __restrict__ P, int32_t L, const LbGeom g,
+                                                   const LbCell* __restrict__ cells, int32_t n,
+                                                   float* __restrict__ D) {
+  __shared__ LbShared sh;
+  lb_stage(sh, P, L, false);
+  __syncthreads();
+  const int w = threadIdx.x >> 6, h = threadIdx.x & 63;
+  const float z = 1.f / g.zinv;
+  for (int64_t base = (int64_t)blockIdx.x * LB_WAVES; base < n; base += (int64_t)gridDim.x * LB_WAVES) {
+    const int64_t s = base + w;
+    const bool valid = s < n;
+    LbCell c{0, 0, 0, 0};
+    float r = 1.f;
+    if (valid) {
+      c = cells[s];
+      const float v0 = lb_input(g, c, h);
+      const float v1 = h + 64 < LB_IN ? lb_input(g, c, h + 64) : 0.f;
+      float ss = fmaf(v0, v0, v1 * v1);
+      for (int o = 32; o >= 1; o >>= 1) ss += __shfl_xor(ss, o);
+      r = sqrtf(ss * (1.f / LB_IN)) + 1e-30f;
+      const float ri = 1.f / r;
+      sh.xs[w][h] = v0 * ri;
+      if (h + 64 < LB_IN) sh.xs[w][h + 64] = v1 * ri;
+    }
+    __syncthreads();
+    float a1, a2;
+    lb_hidden(sh, w, h, c.fd & 0xff, valid, nullptr, nullptr, a1, a2);
+    if (valid && h < LB_OUT) {
+      const float y = lb_output(sh, w, h);
+      D[s * LB_OUT + h] = lb_comp_live(c, h) ? y * r * (h >= 3 ? z : 1.f) : 0.f;
+    }
+    __syncthreads();
+  }
+}
+
+// Inference, after the step: fields += D.
+__global__ __launch_bounds__(256) void k_lb_add(const LbGeom g, const LbCell* __restrict__ cells, int32_t n,
+                                               const float* __restrict__ D) {
+  const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (s >= n) return;
+  const LbCell c = cells[s];
+  const int64_t a = c.i + c.j * g.sy + (int64_t)c.k * g.sz;
+  for (int f = 0; f < LB_OUT; ++f) {
+    float* p = const_cast<float*>(g.F[f]) + a;
+    *p += D[s * LB_OUT + f];
+  }
+}
+
+// Copy the thin window [off, off+n] of a thick field set (every array,
+// including the staggered +1 planes) into a thin field set.
+__global__ __launch_bounds__(256) void k_lb_crop(const LbGeom thick, const LbGeom thin, int32_t off) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int k = blockIdx.z;
+  if (i > thin.nx || j > thin.ny || k > thin.nz) return;
+  const int64_t a = i + j * thin.sy + (int64_t)k * thin.sz;
+  const int64_t b = (i + off) + (j + off) * thick.sy + (int64_t)(k + off) * thick.sz;
+  for (int f = 0; f < 9; ++f) const_cast<float*>(thin.F[f])[a] = thick.F[f][b];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static unsigned lb_blocks(int64_t n, int per) { return (unsigned)std::max<int64_t>(1, (n + per - 1) / per); }
+
+hipError_t lb_launch_mlp(const float* P, int L, int n, const float* X, const int32_t* depth, const float* T,
+                         const float* M, float inv_count, float* Y, float* H1, float* H2, float* D1, float* D2,
+                         float* G, float* loss_part, int nblk, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lb_mlp, dim3(nblk), dim3(256), 0, s, P, L, n, X, depth, T, M, inv_count, Y, H1, H2, D1,
+                     D2, G, loss_part);
+  return hipGetLastError();
+}
+
+hipError_t lb_launch_atb(const float* A, int lda, const int32_t* onehot, int P, const float* B, int ldb, int Q,
+                         int n, int chunk, float* part, float* out, hipStream_t s) {
+  const int nchunk = std::max(1, (n + chunk - 1) / chunk);
+  if (Q > 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_lb_atb, dim3(nchunk, (P + 3) / 4), dim3(64, 4), 0, s, A, lda, onehot, P, B, ldb, Q, n,
+                     chunk, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_lb_reduce, dim3(lb_blocks((int64_t)P * Q, 256)), dim3(256), 0, s, part, nchunk, P * Q, out);
+  return hipGetLastError();
+}
+
+hipError_t lb_launch_adam(float* P, float* m, float* v, const float* g, int n, float lr, float b1, float b2,
+                          float eps, float c1, float c2, hipStream_t s) {
+  hipLaunchKernelGGL(k_lb_adam, dim3(lb_blocks(n, 256)), dim3(256), 0, s, P, m, v, g, n, lr, b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+
+hipError_t lb_launch_gather(const LbGeom& g, const LbCell* cells, int n, float* X, float* R, int32_t* depth,
+                            hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lb_gather, dim3(lb_blocks(n, 256)), dim3(256), 0, s, g, cells, n, X, R, depth);
+  return hipGetLastError();
+}
+
+hipError_t lb_launch_targets(const LbGeom& thin, const LbGeom& thick, int off, const LbCell* cells, int n,
+                             const float* R, float* T, float* M, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lb_targets, dim3(lb_blocks(n, 256)), dim3(256), 0, s, thin, thick, off, cells, n, R, T, M);
+  return hipGetLastError();
+}
+
+hipError_t lb_launch_predict(const float* P, int L, const LbGeom& g, const LbCell* cells, int n, float* D,
+                             int nblk, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lb_predict, dim3(nblk), dim3(256), 0, s, P, L, g, cells, n, D);
+  return hipGetLastError();
+}
+
+hipError_t lb_launch_add(const LbGeom& g, const LbCell* cells, int n, const float* D, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lb_add, dim3(lb_blocks(n, 256)), dim3(256), 0, s, g, cells, n, D);
+  return hipGetLastError();
+}
+
+hipError_t lb_launch_crop(const LbGeom& thick, const LbGeom& thin, int off, hipStream_t s) {
+  hipLaunchKernelGGL(k_lb_crop, dim3((thin.nx + 64) / 64, (thin.ny + 4) / 4, thin.nz + 1), dim3(256), 0, s, thick,
+                     thin, off);
+  return hipGetLastError();
+}
+
+}  // namespace fwe

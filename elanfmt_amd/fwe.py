@@ -112,6 +112,15 @@ SIGNATURES = {
     "fwe_group_run_timed": (C.c_int, [C.c_void_p, C.c_int32, _DP]),
     "fwe_group_set_tiling": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "fwe_group_autotune": (C.c_int, [C.c_void_p, C.c_int32, _DP]),
+    "fwe_lb_param_count": (C.c_int, [C.c_int32, _IP]),
+    "fwe_lb_create": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "fwe_lb_destroy": (C.c_int, [C.c_void_p]),
+    "fwe_lb_init": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "fwe_lb_set_params": (C.c_int, [C.c_void_p, C.c_int32, _FP]),
+    "fwe_lb_get_params": (C.c_int, [C.c_void_p, C.c_int32, _FP]),
+    "fwe_lb_eval": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _FP, _IP, _FP, _FP, _FP, _FP, _DP]),
+    "fwe_lb_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_float, _DP]),
+    "fwe_set_learned_boundary": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float]),
     # generators (fwe_gen.h)
     "fwe_rng_seed": (None, [C.POINTER(RNG), C.c_uint64]),
     "fwe_rng_next": (C.c_uint64, [C.POINTER(RNG)]),
@@ -389,3 +398,72 @@ class Group:
         ms = C.c_double()
         _check(lib().fwe_group_autotune(self._h, trial_steps, C.byref(ms)))
         return ms.value
+
+
+class LearnedBoundary:
+    """fwe_lb: per-face networks of a thin absorbing layer (see
+    include/fwe.h).  Faces 0..5 = x-, x+, y-, y+, z-, z+."""
+
+    def __init__(self, layer=6, device=0):
+        self.layer = layer
+        n = C.c_int32()
+        _check(lib().fwe_lb_param_count(layer, C.byref(n)))
+        self.nparams = n.value
+        self._h = C.c_void_p()
+        _check(lib().fwe_lb_create(device, layer, C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().fwe_lb_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self, seed):
+        _check(lib().fwe_lb_init(self._h, seed))
+
+    def get_params(self, face):
+        out = np.zeros(self.nparams, np.float32)
+        _check(lib().fwe_lb_get_params(self._h, face, fptr(out)))
+        return out
+
+    def set_params(self, face, p):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        if a.size != self.nparams:
+            raise ValueError(f"need {self.nparams} parameters")
+        _check(lib().fwe_lb_set_params(self._h, face, fptr(a)))
+
+    def eval(self, face, x, depth, target=None, mask=None):
+        """(y [n,9], loss, grad [nparams]) of one face's network on explicit
+        samples; loss and grad are None without target/mask."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        n = x.shape[0]
+        d = np.ascontiguousarray(depth, dtype=np.int32)
+        y = np.zeros((n, 9), np.float32)
+        if target is None:
+            _check(lib().fwe_lb_eval(self._h, face, n, fptr(x), d.ctypes.data_as(_IP), None, None, fptr(y), None,
+                                     None))
+            return y, None, None
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        m = np.ascontiguousarray(mask, dtype=np.float32)
+        g = np.zeros(self.nparams, np.float32)
+        loss = C.c_double()
+        _check(lib().fwe_lb_eval(self._h, face, n, fptr(x), d.ctypes.data_as(_IP), fptr(t), fptr(m), fptr(y),
+                                 fptr(g), C.byref(loss)))
+        return y, loss.value, g
+
+    def train_step(self, thick, thin, offset, impedance, lr):
+        loss = C.c_double()
+        _check(lib().fwe_lb_train_step(self._h, thick._h, thin._h, offset, impedance, lr, C.byref(loss)))
+        return loss.value
+
+    def attach(self, solver, impedance):
+        _check(lib().fwe_set_learned_boundary(solver._h, self._h, impedance))
+
+    @staticmethod
+    def detach(solver):
+        _check(lib().fwe_set_learned_boundary(solver._h, None, 1.0))

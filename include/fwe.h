@@ -279,6 +279,45 @@ fwe_status fwe_group_set_tiling(fwe_group* g, int32_t tile_rows, int32_t zchunk)
 /* time each tile shape, keep the fastest, reset every member */
 fwe_status fwe_group_autotune(fwe_group* g, int32_t trial_steps, double* best_ms_per_step);
 
+/* ---- learned thin boundary ---------------------------------------------- */
+/* A thin absorbing layer of `layer` cells on all six faces whose cells are
+ * corrected after every step by a small per-face network: the nine fields
+ * at a 9-cell stencil (the cell, its six neighbours, two more cells along
+ * the face normal; stresses divided by an impedance Z; the vector divided
+ * by its RMS r) -> 64 -> 64 -> 9 (SiLU), adding r*y to the velocities and
+ * Z*r*y to the stresses.  W1 [81][64], b1 [layer][64] (one first-layer bias
+ * per depth), W2 [64][64], b2 [64], W3 [64][9], b3 [9], stored FP32 and used
+ * rounded to bfloat16, FP32 accumulation.  Faces 0..5 = x-, x+, y-, y+, z-,
+ * z+.  Replaces the reference's PML weights (OK4.m:297-341, ANIS.m:466-512)
+ * over most of their thickness; there is no network in the reference. */
+typedef struct fwe_lb fwe_lb;
+fwe_status fwe_lb_param_count(int32_t layer, int32_t* n_per_face);
+fwe_status fwe_lb_create(int32_t device, int32_t layer, fwe_lb** out);
+fwe_status fwe_lb_destroy(fwe_lb* lb);
+/* He-normal hidden layers, zero output layer (no correction); resets Adam */
+fwe_status fwe_lb_init(fwe_lb* lb, uint64_t seed);
+fwe_status fwe_lb_set_params(fwe_lb* lb, int32_t face, const float* params);
+fwe_status fwe_lb_get_params(fwe_lb* lb, int32_t face, float* params);
+/* Network of one face on n explicit samples (host arrays): x [n][81],
+ * depth [n].  y [n][9] (optional).  With target and mask [n][9]: loss =
+ * sum mask*(y-target)^2 / (9n) and its gradient [n_per_face] w.r.t. the
+ * parameters (the used bfloat16 values). */
+fwe_status fwe_lb_eval(fwe_lb* lb, int32_t face, int32_t n, const float* x, const int32_t* depth,
+                       const float* target, const float* mask, float* y, float* grad, double* loss);
+/* One teacher-forced training step.  `thick` is a run whose grid is the
+ * thin grid plus `offset` cells on every face (thick damping); its fields at
+ * step n are copied into `thin` (same medium, `layer`-cell plain damping),
+ * both advance one step, and each face's network is fitted (one Adam step,
+ * rate lr) to the difference between the thick step and the thin step in
+ * the layer cells.  `thick` ends one step further; *loss is the mean
+ * squared error before the update. */
+fwe_status fwe_lb_train_step(fwe_lb* lb, fwe_ctx* thick, fwe_ctx* thin, int32_t offset, float impedance, float lr,
+                             double* loss);
+/* Attach (lb != NULL) or detach the learned boundary: every later step
+ * adds the corrections to the layer cells.  Two-pass kernels, one
+ * wavefield, no free surface. */
+fwe_status fwe_set_learned_boundary(fwe_ctx* ctx, fwe_lb* lb, float impedance);
+
 /* ---- measurement -------------------------------------------------------- */
 /* Stream-copy probe: copies `bytes` bytes device->device `reps` times with a
  * plain vector copy kernel and returns the mean ms per copy (the memory-copy

@@ -1,0 +1,93 @@
+"""Learned thin boundary on the GPU: the network kernels against the FP64
+CPU reference (oracle/lb_ref.py), the attached boundary against the oracle
+step plus CPU corrections, and training."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(rng, L, scale=0.15, out_scale=0.05):
+    from oracle import lb_ref
+
+    w = lb_ref.unpack(rng.normal(0, scale, lb_ref.nparams(L)), L)
+    w["w3"] *= out_scale / scale
+    w["b3"] *= out_scale / scale
+    return lb_ref.pack(w).astype(np.float32)
+
+
+def test_network_forward_and_gradient_match_cpu(gpu):
+    from elanfmt_amd import fwe
+    from oracle import lb_ref
+
+    rng = np.random.default_rng(11)
+    L, n = 6, 1500
+    lb = fwe.LearnedBoundary(L)
+    for face in (0, 5):
+        p = _params(rng, L, out_scale=0.3)
+        lb.set_params(face, p)
+        assert np.array_equal(lb.get_params(face), p)
+        x = rng.normal(0, 1, (n, 81)).astype(np.float32)
+        d = rng.integers(0, L, n).astype(np.int32)
+        t = rng.normal(0, 0.3, (n, 9)).astype(np.float32)
+        m = (rng.random((n, 9)) > 0.1).astype(np.float32)
+        y, loss, g = lb.eval(face, x, d, t, m)
+        yr, lr_, gr = lb_ref.loss_grad(p.astype(np.float64), L, x, d, t, m)
+        print(f"face {face}: y {rel_l2(y, yr):.2e} loss {abs(loss - lr_) / lr_:.2e} grad {rel_l2(g, gr):.2e}",
+              flush=True)
+        assert rel_l2(y, yr) < 1e-5
+        assert abs(loss - lr_) <= 1e-5 * lr_
+        assert rel_l2(g, gr) < 1e-4
+        y2, _, _ = lb.eval(face, x, d)
+        assert np.array_equal(y2, y)
+    lb.close()
+
+
+def test_attached_boundary_matches_oracle_plus_cpu_corrections(gpu):
+    """Every step: corrections from the fields before the step, computed by
+    the CPU network on the CPU stencil features, added after the oracle
+    step; the HIP path must follow within FP32 round-off."""
+    from elanfmt_amd import fwe, workloads as W
+    from oracle import Oracle, lb_ref
+
+    rng = np.random.default_rng(12)
+    L = 3
+    wl = W.ortho_layered(nx=22, ny=20, nz=18, L=L, nt=30, f0=25.0, nrec_side=2)
+    c = (10, 9, 8)
+    wl.injections = fwe.mt_injections(fwe.SRC_LAYOUT_OK4, *c, [1e15, -5e14, 2e14, 3e14, -1e14, 2e14])
+    z = 1.5e7
+    params = [_params(rng, L) for _ in range(6)]
+    lb = fwe.LearnedBoundary(L)
+    for f in range(6):
+        lb.set_params(f, params[f])
+    s = W.make_solver(wl)
+    o = Oracle(wl.nx, wl.ny, wl.nz, wl.dx, wl.dy, wl.dz, wl.dt, wl.nt, mode=wl.mode, pml=wl.pml)
+    W.apply(o, wl)
+    # start both from a state with energy in the layer
+    s.run(12)
+    o.run(12)
+    for f in range(9):
+        o.set_field(f, s.get_field(f).astype(np.float64))
+    lb.attach(s, z)
+    p64 = [p.astype(np.float64) for p in params]
+    for step in range(4):
+        fields = [o.get_field(f) for f in range(9)]
+        corr = lb_ref.corrections(p64, L, fields, wl.nx, wl.ny, wl.nz, z)
+        o.run(1)
+        new = [o.get_field(f) for f in range(9)]
+        for i, j, k, comp, v in corr:
+            new[comp][i, j, k] += v
+        for f in range(9):
+            o.set_field(f, new[f])
+        s.run(1)
+        errs = [rel_l2(s.get_field(f), new[f]) for f in range(9)]
+        print(f"step {step}: max field rel-L2 {max(errs):.2e}", flush=True)
+        assert max(errs) < 1e-4, errs
+    # corrections were not negligible
+    assert max(abs(v) for *_, v in corr) > 0
+    lb.detach(s)
+    s.close()
+    o.close()
+    lb.close()
