@@ -40,7 +40,7 @@ def _traffic(kernel_name, workload):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE, each scaled by the same counter on the copy probe), kept in
     profiles/traffic.json.  Falls back to the same kernel at another tile
-    height when the autotuner picked one that was not profiled.  Returns
+    shape when the autotuner picked one that was not profiled.  Returns
     (bytes, profiled kernel name) or (None, None)."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -54,8 +54,8 @@ def _traffic(kernel_name, workload):
     for k, v in d.items():
         if k.startswith(stem + "<"):
             a = k.split("<")[1].rstrip(">").split(", ")
-            # same law, damping, free surface and wavefield count
-            if a[:-2] == args[:-2] and a[-1] == args[-1]:
+            # same law, damping, free surface (stress) and wavefield count
+            if a[:2] == args[:2] and a[-1] == args[-1] and (stem != "k_stress" or a[2] == args[2]):
                 return v["bytes_per_launch"], k
     return None, None
 
@@ -109,7 +109,7 @@ def _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells):
     if fused:
         kernels = {"fused": {"ms_per_step": round(ms_a / steps, 5), "alg_bytes_per_cell": ba,
                              "GBs": round(ba * cells / (ms_a / steps * 1e-3) / 1e9, 1)}}
-        return kernels, ("fused", ms_a, ba, f"k_fused<{wl.mode}, {wl.pml}, {tiling[0]}>"), ba + bb
+        return kernels, ("fused", ms_a, ba, f"k_fused<{wl.mode}, {wl.pml}, {tiling[1]}>"), ba + bb
     kernels = {
         "stress": {"ms_per_launch": round(ms_a / steps, 5), "alg_bytes_per_cell": ba,
                    "GBs": round(ba * cells / (ms_a / steps * 1e-3) / 1e9, 1)},
@@ -117,9 +117,9 @@ def _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells):
                      "GBs": round(bb * cells / (ms_b / steps * 1e-3) / 1e9, 1)},
     }
     if ms_a >= ms_b:
-        dom = ("stress", ms_a, ba, f"k_stress<{wl.mode}, {wl.pml}, {fs}, {tiling[0]}, {wl.nbatch}>")
+        dom = ("stress", ms_a, ba, f"k_stress<{wl.mode}, {wl.pml}, {fs}, {tiling[0]}, {tiling[1]}, {wl.nbatch}>")
     else:
-        dom = ("velocity", ms_b, bb, f"k_velocity<{wl.mode}, {wl.pml}, {tiling[0]}, {wl.nbatch}>")
+        dom = ("velocity", ms_b, bb, f"k_velocity<{wl.mode}, {wl.pml}, {tiling[0]}, {tiling[1]}, {wl.nbatch}>")
     return kernels, dom, ba + bb
 
 
@@ -156,7 +156,7 @@ def bench_plain(args, steps):
     fused = s.kernel() == fwe.KERNEL_FUSED
     el, ms_a, ms_b, tune = _timed_run(s, steps, args.warmup, not args.no_autotune)
     finite = bool(np.isfinite(s.traces()).all())
-    tiling = s.tiling()
+    tiling = (s.tile_cols(),) + tuple(s.tiling())
     s.close()
     cells = wl.cells
     kernels, dom, bstep = _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells)
@@ -189,7 +189,7 @@ def bench_batched(args, steps):
     print(f"# batched: {wl.name} x{wl.nbatch}, {steps} steps", file=sys.stderr, flush=True)
     s = W.make_solver(wl)
     el_b, ms_a, ms_b, tune = _timed_run(s, steps, args.warmup, not args.no_autotune)
-    tiling = s.tiling()
+    tiling = (s.tile_cols(),) + tuple(s.tiling())
     kernels, dom, bstep = _kernel_entries(wl, False, tiling, ms_a, ms_b, steps, cells)
     # 1000 lune tensors by linear combination of the six elementary runs
     mts = np.array([W.mt_lune_random(50000 + q, m0)[0] for q in range(1000)])
@@ -213,12 +213,13 @@ def bench_batched(args, steps):
         p = W.make_solver(base)
         if tuned is None:
             p.autotune(3)
-            tuned = p.tiling()
+            tuned = (p.tile_cols(),) + tuple(p.tiling())
         else:
-            p.set_tiling(*tuned)
+            p.set_tiling(*tuned[1:])
+            p.set_tile_cols(tuned[0])
         el, _, _, _ = _timed_run(p, steps, args.warmup if e == 0 else 0, False)
         el_p += el
-        plain_tiling = p.tiling()
+        plain_tiling = (p.tile_cols(),) + tuple(p.tiling())
         p.close()
     print(f"# batched {el_b:.2f} s, six plain {el_p:.2f} s", file=sys.stderr, flush=True)
 
@@ -227,7 +228,8 @@ def bench_batched(args, steps):
     for q in range(10):
         base.injections = fwe.mt_injections(fwe.SRC_LAYOUT_OK4, xs, ys, zs, mts[q])
         d = W.make_solver(base)
-        d.set_tiling(*tuned)
+        d.set_tiling(*tuned[1:])
+        d.set_tile_cols(tuned[0])
         d.run(steps)
         tr = d.traces().astype(np.float64)
         d.close()

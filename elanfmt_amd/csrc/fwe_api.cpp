@@ -87,7 +87,7 @@ struct fwe_ctx {
   std::vector<float> stf_host;  // batch 0, for the legacy Src output
   bool have_medium = false, have_damp = false;
   float c66dt = 0, bxdt = 0, bydt = 0, bzdt = 0;
-  int ty = 4, kchunk = 8, ntx = 0, nty = 0, nkc = 0;
+  int tx = 64, ty = 4, kchunk = 8, ntx = 0, nty = 0, nkc = 0;
   int it = 0;
   hipStream_t stream = nullptr;
   std::vector<hipEvent_t> events;
@@ -158,7 +158,9 @@ static void choose_tiling(fwe_ctx* x) {
   int ty = ety ? atoi(ety) : 4;
   if (ty != 2 && ty != 4 && ty != 8) ty = 4;
   x->ty = ty;
-  x->ntx = (c.nx + fwe::TILE_X - 1) / fwe::TILE_X;
+  const char* etx = getenv("FWE_TILE_X");
+  x->tx = (etx && atoi(etx) == 128) ? 128 : 64;
+  x->ntx = (c.nx + x->tx - 1) / x->tx;
   x->nty = (c.ny + ty - 1) / ty;
   int kc;
   if (ekc) {
@@ -425,7 +427,7 @@ static fwe_status upload_tiled(fwe_ctx* x, std::vector<std::pair<int, PointEntry
 // each entry in the list of the tile that owns its cell
 static fwe_status upload_points(fwe_ctx* x, const std::vector<PointEntry>& v, int32_t** off, PointEntry** dev) {
   std::vector<std::pair<int, PointEntry>> tv;
-  for (auto& p : v) tv.push_back({p.i / fwe::TILE_X + (p.j / x->ty) * x->ntx, p});
+  for (auto& p : v) tv.push_back({p.i / x->tx + (p.j / x->ty) * x->ntx, p});
   return upload_tiled(x, tv, off, dev);
 }
 
@@ -435,7 +437,7 @@ static fwe_status upload_grown(fwe_ctx* x) {
   std::vector<std::pair<int, PointEntry>> tv;
   for (auto& p : x->inj_host) {
     for (int tx = 0; tx < x->ntx; ++tx) {
-      const int lo = tx * fwe::TILE_X - 1, hi = tx * fwe::TILE_X + fwe::TILE_X;
+      const int lo = tx * x->tx - 1, hi = tx * x->tx + x->tx;
       if (p.i < lo || p.i > hi) continue;
       for (int ty = 0; ty < x->nty; ++ty) {
         const int jl = ty * x->ty - 1, jh = ty * x->ty + x->ty;
@@ -518,6 +520,7 @@ static StepArgs make_args(fwe_ctx* x, int from) {
   memset(&a, 0, sizeof(a));
   a.nx = c.nx; a.ny = c.ny; a.nz = c.nz; a.nb = c.nbatch;
   a.sy = x->sy; a.sz = x->sz; a.fstride = 9 * x->nelem;
+  a.tile_x = x->tx;
   for (int q = 0; q < 3; ++q) a.V[q] = x->fbuf[from] + (int64_t)q * x->nelem;
   for (int q = 0; q < 6; ++q) a.S[q] = x->fbuf[from] + (int64_t)(3 + q) * x->nelem;
   if (c.mode == FWE_MODE_ORTHO) {
@@ -725,15 +728,18 @@ extern "C" fwe_status fwe_set_graph_steps(fwe_ctx* x, int32_t g) {
   return FWE_OK;
 }
 
-static fwe_status apply_tiling(fwe_ctx* x, int ty, int kchunk) {
+static fwe_status apply_tiling(fwe_ctx* x, int tx, int ty, int kchunk) {
   const fwe_config& c = x->cfg;
   if (ty != 2 && ty != 4 && ty != 8) return fail(FWE_ERR_ARG, "tile rows must be 2, 4 or 8");
+  if (tx != 64 && tx != 128) return fail(FWE_ERR_ARG, "tile columns must be 64 or 128");
+  if (tx != 64 && use_fused(x)) return fail(FWE_ERR_ARG, "the single-pass kernel takes 64-column tiles");
   // the free surface (BC1 of plane 1 written after plane 2) and the
   // single-pass kernel's halo planes need planes 0..2 in one chunk
   const int kmin = (c.free_surface || use_fused(x)) ? 3 : 1;
   if (kchunk < kmin || kchunk > c.nz) return fail(FWE_ERR_ARG, "z chunk must be in %d..nz", kmin);
+  x->tx = tx;
   x->ty = ty;
-  x->ntx = (c.nx + fwe::TILE_X - 1) / fwe::TILE_X;
+  x->ntx = (c.nx + tx - 1) / tx;
   x->nty = (c.ny + ty - 1) / ty;
   x->kchunk = kchunk;
   x->nkc = (c.nz + kchunk - 1) / kchunk;
@@ -753,7 +759,8 @@ extern "C" fwe_status fwe_set_kernel(fwe_ctx* x, int32_t kind) {
     return fail(FWE_ERR_UNSUPPORTED, "the single-pass kernel has no learned boundary");
   drop_graph(x);
   x->kernel = kind;
-  if (use_fused(x) && x->kchunk < 3) return apply_tiling(x, x->ty, std::min(3, x->cfg.nz));
+  if (use_fused(x) && (x->kchunk < 3 || x->tx != 64))
+    return apply_tiling(x, 64, x->ty, std::max(x->kchunk, std::min(3, x->cfg.nz)));
   return FWE_OK;
 }
 
@@ -765,7 +772,18 @@ extern "C" fwe_status fwe_get_kernel(const fwe_ctx* x, int32_t* kind) {
 
 extern "C" fwe_status fwe_set_tiling(fwe_ctx* x, int32_t tile_rows, int32_t zchunk) {
   if (!x) return fail(FWE_ERR_ARG, "NULL ctx");
-  return apply_tiling(x, tile_rows, zchunk);
+  return apply_tiling(x, x->tx, tile_rows, zchunk);
+}
+
+extern "C" fwe_status fwe_set_tile_cols(fwe_ctx* x, int32_t cols) {
+  if (!x) return fail(FWE_ERR_ARG, "NULL ctx");
+  return apply_tiling(x, cols, x->ty, x->kchunk);
+}
+
+extern "C" fwe_status fwe_get_tile_cols(const fwe_ctx* x, int32_t* cols) {
+  if (!x || !cols) return fail(FWE_ERR_ARG, "NULL argument");
+  *cols = x->tx;
+  return FWE_OK;
 }
 
 extern "C" fwe_status fwe_get_tiling(const fwe_ctx* x, int32_t* tile_rows, int32_t* zchunk) {
@@ -780,23 +798,27 @@ extern "C" fwe_status fwe_autotune(fwe_ctx* x, int32_t trial_steps, double* best
   fwe_status st = ready(x);
   if (st) return st;
   const int nz = x->cfg.nz;
-  int best_ty = x->ty, best_kc = x->kchunk;
+  int best_tx = x->tx, best_ty = x->ty, best_kc = x->kchunk;
   double best = 1e30;
+  static const int txs[2] = {64, 128};
   static const int tys[3] = {2, 4, 8};
   static const int kcs[10] = {1, 2, 4, 8, 12, 16, 24, 32, 48, 64};
   const int kmin = (x->cfg.free_surface || use_fused(x)) ? 3 : 1;
-  for (int ty : tys) {
-    for (int kc : kcs) {
-      if (kc > nz || kc < kmin) continue;
-      if ((st = apply_tiling(x, ty, kc))) return st;
-      if ((st = run_impl(x, 1, nullptr, nullptr, nullptr))) return st;  // warm-up
-      double a = 0, b = 0, w = 0;
-      if ((st = run_impl(x, trial_steps, &a, &b, &w))) return st;
-      const double per = (a + b) / trial_steps;
-      if (per < best) { best = per; best_ty = ty; best_kc = kc; }
+  for (int tx : txs) {
+    if (tx != 64 && use_fused(x)) continue;
+    for (int ty : tys) {
+      for (int kc : kcs) {
+        if (kc > nz || kc < kmin) continue;
+        if ((st = apply_tiling(x, tx, ty, kc))) return st;
+        if ((st = run_impl(x, 1, nullptr, nullptr, nullptr))) return st;  // warm-up
+        double a = 0, b = 0, w = 0;
+        if ((st = run_impl(x, trial_steps, &a, &b, &w))) return st;
+        const double per = (a + b) / trial_steps;
+        if (per < best) { best = per; best_tx = tx; best_ty = ty; best_kc = kc; }
+      }
     }
   }
-  if ((st = apply_tiling(x, best_ty, best_kc))) return st;
+  if ((st = apply_tiling(x, best_tx, best_ty, best_kc))) return st;
   if (best_ms_per_step) *best_ms_per_step = best;
   return fwe_reset(x);
 }
@@ -1161,8 +1183,8 @@ extern "C" fwe_status fwe_group_create(int32_t n, fwe_ctx* const* members, fwe_g
   HIPCHK(hipSetDevice(g->device));
   for (fwe_ctx* x : g->m) {
     HIPCHK(hipStreamSynchronize(x->stream));
-    if (x->ty != g->ty) {
-      fwe_status st = apply_tiling(x, g->ty, std::min(x->kchunk, x->cfg.nz));
+    if (x->ty != g->ty || x->tx != 64) {
+      fwe_status st = apply_tiling(x, 64, g->ty, std::min(x->kchunk, x->cfg.nz));
       if (st) { delete g; return st; }
     }
   }
@@ -1195,7 +1217,7 @@ static fwe_status group_run_impl(fwe_group* g, int32_t nsteps, double* ms) {
   HIPCHK(hipSetDevice(g->device));
   for (fwe_ctx* x : g->m) {
     if (x->it != g->m[0]->it) return fail(FWE_ERR_STATE, "members were advanced separately");
-    if (x->ty != g->ty) return fail(FWE_ERR_STATE, "a member was re-tiled outside the group");
+    if (x->ty != g->ty || x->tx != 64) return fail(FWE_ERR_STATE, "a member was re-tiled outside the group");
     HIPCHK(hipStreamSynchronize(x->stream));
   }
   fwe_status st = group_table(g);
@@ -1240,7 +1262,7 @@ extern "C" fwe_status fwe_group_set_tiling(fwe_group* g, int32_t tile_rows, int3
     if (std::min<int>(zchunk, x->cfg.nz) < (x->cfg.free_surface ? 3 : 1))
       return fail(FWE_ERR_ARG, "z chunk too small for member with free surface");
   for (fwe_ctx* x : g->m) {
-    fwe_status st = apply_tiling(x, tile_rows, std::min<int>(zchunk, x->cfg.nz));
+    fwe_status st = apply_tiling(x, 64, tile_rows, std::min<int>(zchunk, x->cfg.nz));
     if (st) return st;
   }
   g->ty = tile_rows;
@@ -1347,19 +1369,34 @@ static LbGeom lb_geom(const fwe_ctx* x, int from, float zinv) {
   return g;
 }
 
+// field set of wavefield b
+static LbGeom lb_geom_b(const fwe_ctx* x, int from, int b, float zinv) {
+  LbGeom g = lb_geom(x, from, zinv);
+  for (int f = 0; f < 9; ++f) g.F[f] += (int64_t)b * 9 * x->nelem;
+  return g;
+}
+
 static hipError_t lb_before(fwe_ctx* x, int from) {
-  const LbGeom g = lb_geom(x, from, x->lb_zinv);
   hipError_t e = hipSuccess;
-  for (int f = 0; f < 6 && e == hipSuccess; ++f) {
-    const int64_t o = x->lb_face_off[f], n = x->lb_face_off[f + 1] - o;
-    e = fwe::lb_launch_predict(x->lb->P + (int64_t)f * x->lb->np, x->lb->L, g, x->lb_cells + o, (int)n,
-                               x->lb_D + o * fwe::LB_OUT, lb_mlp_blocks(n), x->stream);
+  const int64_t nc = x->lb_face_off[6];
+  for (int b = 0; b < x->cfg.nbatch && e == hipSuccess; ++b) {
+    const LbGeom g = lb_geom_b(x, from, b, x->lb_zinv);
+    for (int f = 0; f < 6 && e == hipSuccess; ++f) {
+      const int64_t o = x->lb_face_off[f], n = x->lb_face_off[f + 1] - o;
+      e = fwe::lb_launch_predict(x->lb->P + (int64_t)f * x->lb->np, x->lb->L, g, x->lb_cells + o, (int)n,
+                                 x->lb_D + (b * nc + o) * fwe::LB_OUT, lb_mlp_blocks(n), x->stream);
+    }
   }
   return e;
 }
 
 static hipError_t lb_after(fwe_ctx* x, int from) {
-  return fwe::lb_launch_add(lb_geom(x, from, x->lb_zinv), x->lb_cells, (int)x->lb_face_off[6], x->lb_D, x->stream);
+  hipError_t e = hipSuccess;
+  const int64_t nc = x->lb_face_off[6];
+  for (int b = 0; b < x->cfg.nbatch && e == hipSuccess; ++b)
+    e = fwe::lb_launch_add(lb_geom_b(x, from, b, x->lb_zinv), x->lb_cells, (int)nc, x->lb_D + b * nc * fwe::LB_OUT,
+                           x->stream);
+  return e;
 }
 
 extern "C" fwe_status fwe_lb_param_count(int32_t layer, int32_t* n) {
@@ -1570,7 +1607,7 @@ static fwe_status lb_plain_ctx(const fwe_ctx* x, const char* what) {
 }
 
 extern "C" fwe_status fwe_lb_train_step(fwe_lb* lb, fwe_ctx* thick, fwe_ctx* thin, int32_t offset, float impedance,
-                                        float lr, double* loss) {
+                                        float lr, int32_t resync, double* loss) {
   if (!lb || !thick || !thin || offset < 0 || !(impedance > 0.f) || !(lr >= 0.f))
     return fail(FWE_ERR_ARG, "bad learned-boundary training arguments");
   fwe_status st;
@@ -1598,24 +1635,37 @@ extern "C" fwe_status fwe_lb_train_step(fwe_lb* lb, fwe_ctx* thick, fwe_ctx* thi
   const float zinv = 1.f / impedance;
   HIPCHK(hipStreamSynchronize(thick->stream));
   HIPCHK(hipStreamSynchronize(thin->stream));
-  // thin window of the thick run at step n -> thin fields; features
-  HIPCHK(fwe::lb_launch_crop(lb_geom(thick, thick->cur, zinv), lb_geom(thin, thin->cur, zinv), offset, lb->stream));
+  // thin window of the thick run at step n -> thin fields (resync), or the
+  // thin run's own state; features
+  if (resync || thin->it != thick->it)
+    HIPCHK(fwe::lb_launch_crop(lb_geom(thick, thick->cur, zinv), lb_geom(thin, thin->cur, zinv), offset, lb->stream));
   HIPCHK(fwe::lb_launch_gather(lb_geom(thin, thin->cur, zinv), lb->cells, (int)ncell, lb->X, lb->R, lb->depth,
                                lb->stream));
+  // RMS level of the batch (weights of the loss)
+  std::vector<float> rh((size_t)ncell);
+  HIPCHK(hipMemcpyAsync(rh.data(), lb->R, sizeof(float) * rh.size(), hipMemcpyDeviceToHost, lb->stream));
   HIPCHK(hipStreamSynchronize(lb->stream));
+  double r2 = 0;
+  for (float v : rh) r2 += (double)v * v;
+  r2 /= (double)std::max<int64_t>(1, ncell);
   // one plain thin step and one thick step from the same state
   thin->it = thick->it;
   if ((st = run_impl(thin, 1, nullptr, nullptr, nullptr))) return st;
   if ((st = run_impl(thick, 1, nullptr, nullptr, nullptr))) return st;
   HIPCHK(hipStreamSynchronize(thin->stream));
   HIPCHK(hipStreamSynchronize(thick->stream));
+  if (!(r2 > 1e-60)) {  // no wave in the layer yet: nothing to fit
+    if (loss) *loss = 0.0;
+    return FWE_OK;
+  }
   HIPCHK(fwe::lb_launch_targets(lb_geom(thin, thin->cur, zinv), lb_geom(thick, thick->cur, zinv), offset, lb->cells,
-                                (int)ncell, lb->R, lb->T, lb->M, lb->stream));
+                                (int)ncell, lb->R, (float)(1e-4 * sqrt(r2)), (float)(1.0 / r2), lb->T, lb->M,
+                                lb->stream));
   double lsum = 0;
   for (int f = 0; f < 6; ++f) {
     const int64_t o = lb->face_off[f], n = lb->face_off[f + 1] - o;
     if (n == 0) continue;
-    if ((st = lb_fb(lb, f, o, (int)n, false, &lsum))) return st;
+    if ((st = lb_fb(lb, f, o, (int)n, true, &lsum))) return st;
     if (lr > 0.f) {
       const int t = ++lb->adam_t[f];
       const float c1 = (float)(1.0 / (1.0 - pow(0.9, t))), c2 = (float)(1.0 / (1.0 - pow(0.999, t)));
@@ -1624,6 +1674,8 @@ extern "C" fwe_status fwe_lb_train_step(fwe_lb* lb, fwe_ctx* thick, fwe_ctx* thi
                                  c2, lb->stream));
     }
   }
+  // the thin run continues as it would with the boundary attached
+  HIPCHK(fwe::lb_launch_apply_y(lb_geom(thin, thin->cur, zinv), lb->cells, (int)ncell, lb->Y, lb->R, lb->stream));
   HIPCHK(hipStreamSynchronize(lb->stream));
   if (loss) *loss = lsum / (9.0 * (double)std::max<int64_t>(1, ncell));
   return FWE_OK;
@@ -1640,8 +1692,8 @@ extern "C" fwe_status fwe_set_learned_boundary(fwe_ctx* x, fwe_lb* lb, float imp
   x->lb_D = nullptr;
   x->lb = nullptr;
   if (!lb) return FWE_OK;
-  fwe_status st = lb_plain_ctx(x, "learned boundary");
-  if (st) return st;
+  if (x->cfg.free_surface)
+    return fail(FWE_ERR_UNSUPPORTED, "learned boundary: it replaces damping on all six faces; no free surface");
   if (!(impedance > 0.f)) return fail(FWE_ERR_ARG, "impedance must be positive");
   if (lb->device != x->cfg.device) return fail(FWE_ERR_ARG, "learned boundary on another device");
   if (use_fused(x)) return fail(FWE_ERR_UNSUPPORTED, "the single-pass kernel has no learned boundary");
@@ -1649,8 +1701,9 @@ extern "C" fwe_status fwe_set_learned_boundary(fwe_ctx* x, fwe_lb* lb, float imp
   lb_cells_host(x->cfg.nx, x->cfg.ny, x->cfg.nz, lb->L, h, x->lb_face_off);
   HIPCHK(hipMalloc(&x->lb_cells, sizeof(LbCell) * std::max<size_t>(1, h.size())));
   HIPCHK(hipMemcpy(x->lb_cells, h.data(), sizeof(LbCell) * h.size(), hipMemcpyHostToDevice));
-  HIPCHK(hipMalloc(&x->lb_D, sizeof(float) * std::max<size_t>(1, h.size() * fwe::LB_OUT)));
-  HIPCHK(hipMemset(x->lb_D, 0, sizeof(float) * std::max<size_t>(1, h.size() * fwe::LB_OUT)));
+  const size_t nd = std::max<size_t>(1, h.size() * fwe::LB_OUT * x->cfg.nbatch);
+  HIPCHK(hipMalloc(&x->lb_D, sizeof(float) * nd));
+  HIPCHK(hipMemset(x->lb_D, 0, sizeof(float) * nd));
   x->lb = lb;
   x->lb_zinv = 1.f / impedance;
   return FWE_OK;

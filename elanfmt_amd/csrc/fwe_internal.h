@@ -9,9 +9,10 @@
 
 namespace fwe {
 
-// Tile of one workgroup in the xy plane: 64 lanes along x (one wavefront
-// row, coalesced x-fastest access), TILE_Y rows along y.  Each workgroup
-// marches a chunk of z planes.
+// Tile of one workgroup in the xy plane: TILE_X lanes along x (one
+// wavefront row, coalesced x-fastest access) for the single-pass and group
+// kernels, 64 or 128 for the two-pass kernels; TILE_Y rows along y.  Each
+// workgroup marches a chunk of z planes.
 constexpr int TILE_X = 64;
 
 enum Mode { MODE_ISO = 0, MODE_ORTHO = 1 };
@@ -49,6 +50,7 @@ struct StepArgs {
   float rdx, rdy, rdz;   // 1/dx ...
   float dtdx, dtdy, dtdz; // dt/dx ... (iso stress)
   // tiling
+  int32_t tile_x;       // columns per tile (64 or 128)
   int32_t ty_rows;      // TILE_Y
   int32_t ntx, nty, nkc, kchunk;
   // sources
@@ -162,10 +164,12 @@ hipError_t lb_launch_adam(float* P, float* m, float* v, const float* g, int n, f
 hipError_t lb_launch_gather(const LbGeom& g, const LbCell* cells, int n, float* X, float* R, int32_t* depth,
                             hipStream_t s);
 hipError_t lb_launch_targets(const LbGeom& thin, const LbGeom& thick, int off, const LbCell* cells, int n,
-                             const float* R, float* T, float* M, hipStream_t s);
+                             const float* R, float rmin, float wscale, float* T, float* M, hipStream_t s);
 hipError_t lb_launch_predict(const float* P, int L, const LbGeom& g, const LbCell* cells, int n, float* D,
                              int nblk, hipStream_t s);
 hipError_t lb_launch_add(const LbGeom& g, const LbCell* cells, int n, const float* D, hipStream_t s);
+hipError_t lb_launch_apply_y(const LbGeom& g, const LbCell* cells, int n, const float* Y, const float* R,
+                             hipStream_t s);
 hipError_t lb_launch_crop(const LbGeom& thick, const LbGeom& thin, int off, hipStream_t s);
 
 }  // namespace fwe

@@ -9,6 +9,7 @@
 //   a1 = W1 x + b1[depth]      h1 = silu(a1)
 //   a2 = W2 h1 + b2            h2 = silu(a2)
 //   y  = W3 h2 + b3            correction: dV = r y[0:3], dS = Z r y[3:9]
+// applied after the step in its dissipative part only (lb_shrink).
 // W1, W2, W3 are used rounded to bfloat16 (kept as FP32 master copies for
 // training); all sums accumulate in FP32.
 //
@@ -268,22 +269,27 @@ __device__ __forceinline__ bool lb_comp_live(const LbCell& c, int f) {
 }
 
 // Targets: T = (thick(n+1) at cell+off - thin step) / (r * unit), unit = 1
-// for velocities and Z for stresses; M = live components.
+// for velocities and Z for stresses.  Weights M = (r / r_ref)^2 on live
+// components of cells with r >= rmin (wscale = 1 / r_ref^2), so the loss is
+// the squared error in physical units relative to the batch's RMS level.
 __global__ __launch_bounds__(256) void k_lb_targets(const LbGeom thin, const LbGeom thick, int32_t off,
                                                    const LbCell* __restrict__ cells, int32_t n,
-                                                   const float* __restrict__ R, float* __restrict__ T,
-                                                   float* __restrict__ M) {
+                                                   const float* __restrict__ R, float rmin, float wscale,
+                                                   float* __restrict__ T, float* __restrict__ M) {
   const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (s >= n) return;
   const LbCell c = cells[s];
-  const float ri = 1.f / R[s];
+  const float r = R[s];
+  const bool big = r >= rmin;
+  const float ri = big ? 1.f / r : 0.f;
+  const float wt = big ? r * r * wscale : 0.f;
   const int64_t a = c.i + c.j * thin.sy + (int64_t)c.k * thin.sz;
   const int64_t b = (c.i + off) + (c.j + off) * thick.sy + (int64_t)(c.k + off) * thick.sz;
   for (int f = 0; f < LB_OUT; ++f) {
-    const bool live = lb_comp_live(c, f);
+    const bool live = big && lb_comp_live(c, f);
     const float d = thick.F[f][b] - thin.F[f][a];
     T[s * LB_OUT + f] = live ? d * ri * (f >= 3 ? thin.zinv : 1.f) : 0.f;
-    M[s * LB_OUT + f] = live ? 1.f : 0.f;
+    M[s * LB_OUT + f] = live ? wt : 0.f;
   }
 }
 
@@ -323,7 +329,14 @@ __global__ __launch_bounds__(256) void k_lb_predict(const float* __restrict__ P,
   }
 }
 
-// Inference, after the step: fields += D.
+// The dissipative part of a correction: the stepped value v may move toward
+// zero by at most its own magnitude and never grow, so the layer acts as a
+// cell- and wave-dependent sponge (stable wherever plain damping is).
+__device__ __forceinline__ float lb_shrink(float v, float d) {
+  return v > 0.f ? v + fminf(fmaxf(d, -v), 0.f) : (v < 0.f ? v + fmaxf(fminf(d, -v), 0.f) : v);
+}
+
+// Inference, after the step: fields <- shrink(fields, D).
 __global__ __launch_bounds__(256) void k_lb_add(const LbGeom g, const LbCell* __restrict__ cells, int32_t n,
                                                const float* __restrict__ D) {
   const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -332,8 +345,24 @@ __global__ __launch_bounds__(256) void k_lb_add(const LbGeom g, const LbCell* __
   const int64_t a = c.i + c.j * g.sy + (int64_t)c.k * g.sz;
   for (int f = 0; f < LB_OUT; ++f) {
     float* p = const_cast<float*>(g.F[f]) + a;
-    *p += D[s * LB_OUT + f];
+    *p = lb_shrink(*p, D[s * LB_OUT + f]);
   }
+}
+
+// Training rollout: the corrections predicted before the update, from Y
+// [n][9] and the stencil RMS R, applied as k_lb_predict + k_lb_add do.
+__global__ __launch_bounds__(256) void k_lb_apply_y(const LbGeom g, const LbCell* __restrict__ cells, int32_t n,
+                                                   const float* __restrict__ Y, const float* __restrict__ R) {
+  const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (s >= n) return;
+  const LbCell c = cells[s];
+  const int64_t a = c.i + c.j * g.sy + (int64_t)c.k * g.sz;
+  const float r = R[s], z = 1.f / g.zinv;
+  for (int f = 0; f < LB_OUT; ++f)
+    if (lb_comp_live(c, f)) {
+      float* p = const_cast<float*>(g.F[f]) + a;
+      *p = lb_shrink(*p, Y[s * LB_OUT + f] * r * (f >= 3 ? z : 1.f));
+    }
 }
 
 // Copy the thin window [off, off+n] of a thick field set (every array,
@@ -388,9 +417,10 @@ hipError_t lb_launch_gather(const LbGeom& g, const LbCell* cells, int n, float* 
 }
 
 hipError_t lb_launch_targets(const LbGeom& thin, const LbGeom& thick, int off, const LbCell* cells, int n,
-                             const float* R, float* T, float* M, hipStream_t s) {
+                             const float* R, float rmin, float wscale, float* T, float* M, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_lb_targets, dim3(lb_blocks(n, 256)), dim3(256), 0, s, thin, thick, off, cells, n, R, T, M);
+  hipLaunchKernelGGL(k_lb_targets, dim3(lb_blocks(n, 256)), dim3(256), 0, s, thin, thick, off, cells, n, R, rmin,
+                     wscale, T, M);
   return hipGetLastError();
 }
 
@@ -404,6 +434,13 @@ hipError_t lb_launch_predict(const float* P, int L, const LbGeom& g, const LbCel
 hipError_t lb_launch_add(const LbGeom& g, const LbCell* cells, int n, const float* D, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_lb_add, dim3(lb_blocks(n, 256)), dim3(256), 0, s, g, cells, n, D);
+  return hipGetLastError();
+}
+
+hipError_t lb_launch_apply_y(const LbGeom& g, const LbCell* cells, int n, const float* Y, const float* R,
+                             hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lb_apply_y, dim3(lb_blocks(n, 256)), dim3(256), 0, s, g, cells, n, Y, R);
   return hipGetLastError();
 }
 

@@ -11,7 +11,8 @@
 //    x fastest, row pitch a multiple of 32 floats (128 B lines), so one index
 //    serves all staggered arrays and a 64-lane wavefront reads 256 B
 //    contiguous per array;
-//  * a workgroup owns a 64 x TY column tile and marches a chunk of z planes,
+//  * a workgroup owns a TX x TY column tile (TX = 64 or 128: one or two
+//    wavefronts per row) and marches a chunk of z planes,
 //    carrying the z-neighbours it needs in registers;
 //  * workgroup ids are remapped so that neighbouring tiles run on the same
 //    XCD and share its L2 for the x/y halo re-reads;
@@ -55,14 +56,14 @@ __device__ __forceinline__ float ldz(const float* p, bool use, int64_t idx) {
 // Vx(k-1), Vy(k-1) and Vz(k) are carried in registers from the previous
 // plane, so every field value is read from memory once per plane.
 // ---------------------------------------------------------------------------
-template <int MODE, int PML, bool FS, int TY, int NB>
+template <int MODE, int PML, bool FS, int TX, int TY, int NB>
 __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb) {
   const int lane = threadIdx.x, row = threadIdx.y;
   const int tx = lb % a.ntx;
   lb /= a.ntx;
   const int tyi = lb % a.nty;
   const int kc = lb / a.nty;
-  const int i = tx * TILE_X + lane, j = tyi * TY + row;
+  const int i = tx * TX + lane, j = tyi * TY + row;
   if (i >= a.nx || j >= a.ny) return;
   const int k0 = kc * a.kchunk;
   const int k1 = min(k0 + a.kchunk, a.nz);
@@ -196,14 +197,14 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb) {
 // Per wavefield, Szz(k-1), Sxz(k) and Syz(k) are carried from the previous
 // plane.
 // ---------------------------------------------------------------------------
-template <int MODE, int PML, int TY, int NB>
+template <int MODE, int PML, int TX, int TY, int NB>
 __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb) {
   const int lane = threadIdx.x, row = threadIdx.y;
   const int tx = lb % a.ntx;
   lb /= a.ntx;
   const int tyi = lb % a.nty;
   const int kc = lb / a.nty;
-  const int i = tx * TILE_X + lane, j = tyi * TY + row;
+  const int i = tx * TX + lane, j = tyi * TY + row;
   if (i >= a.nx || j >= a.ny) return;
   const int k0 = kc * a.kchunk;
   const int k1 = min(k0 + a.kchunk, a.nz);
@@ -287,13 +288,13 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb) {
 }
 
 // One model per launch: workgroup -> tile of that model.
-template <int MODE, int PML, bool FS, int TY, int NB>
-__global__ __launch_bounds__(64 * TY) void k_stress(const StepArgs a) {
-  stress_tile<MODE, PML, FS, TY, NB>(a, xcd_remap(blockIdx.x, gridDim.x));
+template <int MODE, int PML, bool FS, int TX, int TY, int NB>
+__global__ __launch_bounds__(TX * TY) void k_stress(const StepArgs a) {
+  stress_tile<MODE, PML, FS, TX, TY, NB>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
-template <int MODE, int PML, int TY, int NB>
-__global__ __launch_bounds__(64 * TY) void k_velocity(const StepArgs a) {
-  velocity_tile<MODE, PML, TY, NB>(a, xcd_remap(blockIdx.x, gridDim.x));
+template <int MODE, int PML, int TX, int TY, int NB>
+__global__ __launch_bounds__(TX * TY) void k_velocity(const StepArgs a) {
+  velocity_tile<MODE, PML, TX, TY, NB>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // Many models per launch (an uneven batch of independent grids): the
@@ -309,32 +310,39 @@ __global__ __launch_bounds__(64 * TY) void k_stress_group(const StepArgs* __rest
                                                          const int32_t* __restrict__ start, int32_t nm) {
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int m = group_model(start, nm, lb);
-  stress_tile<MODE, PML, FS, TY, 1>(tab[m], lb - start[m]);
+  stress_tile<MODE, PML, FS, 64, TY, 1>(tab[m], lb - start[m]);
 }
 template <int MODE, int PML, int TY>
 __global__ __launch_bounds__(64 * TY) void k_velocity_group(const StepArgs* __restrict__ tab,
                                                            const int32_t* __restrict__ start, int32_t nm) {
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int m = group_model(start, nm, lb);
-  velocity_tile<MODE, PML, TY, 1>(tab[m], lb - start[m]);
+  velocity_tile<MODE, PML, 64, TY, 1>(tab[m], lb - start[m]);
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <int MODE, int PML, bool FS, int NB>
-static hipError_t go_stress1(const StepArgs& a, hipStream_t s) {
+template <int MODE, int PML, bool FS, int TX, int NB>
+static hipError_t go_stress2(const StepArgs& a, hipStream_t s) {
   const int nblk = a.ntx * a.nty * a.nkc;
   if (a.ty_rows == 4) {
-    hipLaunchKernelGGL((k_stress<MODE, PML, FS, 4, NB>), dim3(nblk), dim3(64, 4), 0, s, a);
+    hipLaunchKernelGGL((k_stress<MODE, PML, FS, TX, 4, NB>), dim3(nblk), dim3(TX, 4), 0, s, a);
   } else if (a.ty_rows == 8) {
-    hipLaunchKernelGGL((k_stress<MODE, PML, FS, 8, NB>), dim3(nblk), dim3(64, 8), 0, s, a);
+    hipLaunchKernelGGL((k_stress<MODE, PML, FS, TX, 8, NB>), dim3(nblk), dim3(TX, 8), 0, s, a);
   } else if (a.ty_rows == 2) {
-    hipLaunchKernelGGL((k_stress<MODE, PML, FS, 2, NB>), dim3(nblk), dim3(64, 2), 0, s, a);
+    hipLaunchKernelGGL((k_stress<MODE, PML, FS, TX, 2, NB>), dim3(nblk), dim3(TX, 2), 0, s, a);
   } else {
     return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <int MODE, int PML, bool FS, int NB>
+static hipError_t go_stress1(const StepArgs& a, hipStream_t s) {
+  if (a.tile_x == 64) return go_stress2<MODE, PML, FS, 64, NB>(a, s);
+  if (a.tile_x == 128) return go_stress2<MODE, PML, FS, 128, NB>(a, s);
+  return hipErrorInvalidValue;
 }
 
 template <int MODE, int PML, bool FS>
@@ -352,19 +360,26 @@ static hipError_t go_stress(const StepArgs& a, hipStream_t s) {
   }
 }
 
-template <int MODE, int PML, int NB>
-static hipError_t go_vel1(const StepArgs& a, hipStream_t s) {
+template <int MODE, int PML, int TX, int NB>
+static hipError_t go_vel2(const StepArgs& a, hipStream_t s) {
   const int nblk = a.ntx * a.nty * a.nkc;
   if (a.ty_rows == 4) {
-    hipLaunchKernelGGL((k_velocity<MODE, PML, 4, NB>), dim3(nblk), dim3(64, 4), 0, s, a);
+    hipLaunchKernelGGL((k_velocity<MODE, PML, TX, 4, NB>), dim3(nblk), dim3(TX, 4), 0, s, a);
   } else if (a.ty_rows == 8) {
-    hipLaunchKernelGGL((k_velocity<MODE, PML, 8, NB>), dim3(nblk), dim3(64, 8), 0, s, a);
+    hipLaunchKernelGGL((k_velocity<MODE, PML, TX, 8, NB>), dim3(nblk), dim3(TX, 8), 0, s, a);
   } else if (a.ty_rows == 2) {
-    hipLaunchKernelGGL((k_velocity<MODE, PML, 2, NB>), dim3(nblk), dim3(64, 2), 0, s, a);
+    hipLaunchKernelGGL((k_velocity<MODE, PML, TX, 2, NB>), dim3(nblk), dim3(TX, 2), 0, s, a);
   } else {
     return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <int MODE, int PML, int NB>
+static hipError_t go_vel1(const StepArgs& a, hipStream_t s) {
+  if (a.tile_x == 64) return go_vel2<MODE, PML, 64, NB>(a, s);
+  if (a.tile_x == 128) return go_vel2<MODE, PML, 128, NB>(a, s);
+  return hipErrorInvalidValue;
 }
 
 template <int MODE, int PML>

@@ -112,6 +112,8 @@ SIGNATURES = {
     "fwe_group_run_timed": (C.c_int, [C.c_void_p, C.c_int32, _DP]),
     "fwe_group_set_tiling": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "fwe_group_autotune": (C.c_int, [C.c_void_p, C.c_int32, _DP]),
+    "fwe_set_tile_cols": (C.c_int, [C.c_void_p, C.c_int32]),
+    "fwe_get_tile_cols": (C.c_int, [C.c_void_p, _IP]),
     "fwe_lb_param_count": (C.c_int, [C.c_int32, _IP]),
     "fwe_lb_create": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "fwe_lb_destroy": (C.c_int, [C.c_void_p]),
@@ -119,7 +121,8 @@ SIGNATURES = {
     "fwe_lb_set_params": (C.c_int, [C.c_void_p, C.c_int32, _FP]),
     "fwe_lb_get_params": (C.c_int, [C.c_void_p, C.c_int32, _FP]),
     "fwe_lb_eval": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _FP, _IP, _FP, _FP, _FP, _FP, _DP]),
-    "fwe_lb_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_float, _DP]),
+    "fwe_lb_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_float, C.c_int32,
+                                    _DP]),
     "fwe_set_learned_boundary": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float]),
     # generators (fwe_gen.h)
     "fwe_rng_seed": (None, [C.POINTER(RNG), C.c_uint64]),
@@ -304,6 +307,14 @@ class Solver:
         _check(lib().fwe_get_tiling(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def set_tile_cols(self, cols):
+        _check(lib().fwe_set_tile_cols(self._h, cols))
+
+    def tile_cols(self):
+        c = C.c_int32()
+        _check(lib().fwe_get_tile_cols(self._h, C.byref(c)))
+        return c.value
+
     def autotune(self, trial_steps=5):
         ms = C.c_double()
         _check(lib().fwe_autotune(self._h, trial_steps, C.byref(ms)))
@@ -456,9 +467,10 @@ class LearnedBoundary:
                                  fptr(g), C.byref(loss)))
         return y, loss.value, g
 
-    def train_step(self, thick, thin, offset, impedance, lr):
+    def train_step(self, thick, thin, offset, impedance, lr, resync=True):
         loss = C.c_double()
-        _check(lib().fwe_lb_train_step(self._h, thick._h, thin._h, offset, impedance, lr, C.byref(loss)))
+        _check(lib().fwe_lb_train_step(self._h, thick._h, thin._h, offset, impedance, lr, int(resync),
+                                       C.byref(loss)))
         return loss.value
 
     def attach(self, solver, impedance):

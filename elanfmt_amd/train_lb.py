@@ -1,0 +1,53 @@
+"""Train the learned thin boundary on the GPU and save its parameters.
+
+  python -m elanfmt_amd.train_lb [--runs 200] [--out elanfmt_amd/data/learned_boundary_L6.npz]
+
+Each run: a seeded random layered orthorhombic medium and lune moment
+tensor on a 40^3 interior, a teacher run on the interior plus 32 cells per
+face (the thin layer's damping over the thin window, a damping ramp
+beyond), and a thin run with a 6-cell layer; see elanfmt_amd/learned.py.
+After training, the layer is evaluated on held-out media against a padded
+reference, and the numbers are stored with the parameters.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+
+def main():
+    from elanfmt_amd import fwe, learned
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--runs", type=int, default=200)
+    ap.add_argument("--layer", type=int, default=6)
+    ap.add_argument("--lr", type=float, default=5e-4)
+    ap.add_argument("--resync", type=int, default=8)
+    ap.add_argument("--r-thin", type=float, default=1e-3)
+    ap.add_argument("--seed0", type=int, default=1000)
+    ap.add_argument("--out", default=os.path.join(ROOT, "elanfmt_amd", "data", f"learned_boundary_L6.npz"))
+    a = ap.parse_args()
+    lb = fwe.LearnedBoundary(a.layer)
+    t0 = time.time()
+    log = learned.train(lb, runs=a.runs, lr=a.lr, resync_every=a.resync, R_thin=a.r_thin, seed0=a.seed0,
+                        log_every=200, progress_every=10)
+    t_train = time.time() - t0
+    print(f"trained {log.runs} runs, {log.steps} steps in {t_train:.1f} s; loss {log.first_loss:.3e} -> "
+          f"{log.last_loss:.3e}", flush=True)
+    evals = {}
+    for sd in (7, 8, 9):
+        evals[sd] = learned.evaluate(lb, seed=sd, R_thin=a.r_thin)
+        print(f"held-out medium {sd}: {evals[sd]}", flush=True)
+    learned.save(lb, a.out, runs=a.runs, layer=a.layer, lr=a.lr, resync_every=a.resync, R_thin=a.r_thin,
+                 seed0=a.seed0, train_seconds=round(t_train, 1), steps=log.steps,
+                 loss_curve=[(s, float(f"{l:.4e}")) for s, l in log.losses], evaluation=evals)
+    print(json.dumps({"out": a.out, "evaluation": evals}), flush=True)
+    lb.close()
+
+
+if __name__ == "__main__":
+    main()

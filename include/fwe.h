@@ -220,6 +220,11 @@ fwe_status fwe_get_tiling(const fwe_ctx* ctx, int32_t* tile_rows, int32_t* zchun
 /* Time trial_steps steps for each tiling candidate, keep the fastest, and
  * reset the context (fields, traces, step counter). */
 fwe_status fwe_autotune(fwe_ctx* ctx, int32_t trial_steps, double* best_ms_per_step);
+/* Columns per tile of the two-pass kernels: 64 (one wavefront per tile row)
+ * or 128 (two; halves the x-halo lines per tile).  The single-pass kernel
+ * and groups use 64.  fwe_autotune tries both. */
+fwe_status fwe_set_tile_cols(fwe_ctx* ctx, int32_t cols);
+fwe_status fwe_get_tile_cols(const fwe_ctx* ctx, int32_t* cols);
 /* Steps advanced since create/reset. */
 fwe_status fwe_get_step(const fwe_ctx* ctx, int32_t* it);
 /* Set the step counter (restart from saved fields: the reference's unused
@@ -284,8 +289,10 @@ fwe_status fwe_group_autotune(fwe_group* g, int32_t trial_steps, double* best_ms
  * corrected after every step by a small per-face network: the nine fields
  * at a 9-cell stencil (the cell, its six neighbours, two more cells along
  * the face normal; stresses divided by an impedance Z; the vector divided
- * by its RMS r) -> 64 -> 64 -> 9 (SiLU), adding r*y to the velocities and
- * Z*r*y to the stresses.  W1 [81][64], b1 [layer][64] (one first-layer bias
+ * by its RMS r) -> 64 -> 64 -> 9 (SiLU), corrections r*y for the velocities
+ * and Z*r*y for the stresses, applied after the step in their dissipative
+ * part only (a value may move toward zero by at most its magnitude and
+ * never grow), so the layer is an adaptive sponge.  W1 [81][64], b1 [layer][64] (one first-layer bias
  * per depth), W2 [64][64], b2 [64], W3 [64][9], b3 [9], stored FP32 and used
  * rounded to bfloat16, FP32 accumulation.  Faces 0..5 = x-, x+, y-, y+, z-,
  * z+.  Replaces the reference's PML weights (OK4.m:297-341, ANIS.m:466-512)
@@ -304,18 +311,23 @@ fwe_status fwe_lb_get_params(fwe_lb* lb, int32_t face, float* params);
  * parameters (the used bfloat16 values). */
 fwe_status fwe_lb_eval(fwe_lb* lb, int32_t face, int32_t n, const float* x, const int32_t* depth,
                        const float* target, const float* mask, float* y, float* grad, double* loss);
-/* One teacher-forced training step.  `thick` is a run whose grid is the
- * thin grid plus `offset` cells on every face (thick damping); its fields at
- * step n are copied into `thin` (same medium, `layer`-cell plain damping),
- * both advance one step, and each face's network is fitted (one Adam step,
- * rate lr) to the difference between the thick step and the thin step in
- * the layer cells.  `thick` ends one step further; *loss is the mean
- * squared error before the update. */
+/* One training step.  `thick` is a run whose grid is the thin grid plus
+ * `offset` cells on every face (thick damping).  With resync (or when the
+ * step counters differ) its fields at step n are first copied into `thin`
+ * (same medium, `layer`-cell plain damping); otherwise `thin` continues from
+ * its own state.  Both advance one step; each face's network is fitted (one
+ * Adam step, rate lr) to the difference between the thick step and the
+ * thin plain step in the layer cells, and `thin` then receives the
+ * corrections predicted before the update, as with the boundary attached.  Each cell's error is weighted by (r / r_ref)^2, r its
+ * stencil RMS and r_ref the RMS over the layer, so the loss is the squared
+ * error in physical units; cells with r < 1e-4 r_ref are left out.
+ * `thick` ends one step further; *loss is the weighted mean squared error
+ * before the update (0 while the layer holds no wave). */
 fwe_status fwe_lb_train_step(fwe_lb* lb, fwe_ctx* thick, fwe_ctx* thin, int32_t offset, float impedance, float lr,
-                             double* loss);
+                             int32_t resync, double* loss);
 /* Attach (lb != NULL) or detach the learned boundary: every later step
- * adds the corrections to the layer cells.  Two-pass kernels, one
- * wavefield, no free surface. */
+ * applies the corrections to the layer cells of every wavefield.  Two-pass
+ * kernels, no free surface. */
 fwe_status fwe_set_learned_boundary(fwe_ctx* ctx, fwe_lb* lb, float impedance);
 
 /* ---- measurement -------------------------------------------------------- */

@@ -2,7 +2,8 @@
 
 FP64 numpy forward/backward of the per-face network of include/fwe.h
 (81 -> 64 -> 64 -> 9, SiLU, W1/W2/W3 used rounded to bfloat16), the 9-cell
-stencil features of a layer cell, and the correction step: the checker for
+stencil features of a layer cell, and the correction step (the dissipative
+part of each correction, `shrink`): the checker for
 libfwe.so's fwe_lb_* kernels.  The backward pass is pinned by central
 finite differences (tests/test_lb_ref.py); there is no counterpart in the
 reference, whose boundary is plain damping (OK4.m:297-341, ANIS.m:466-512).
@@ -139,6 +140,16 @@ def live(i, j, k, comp):
     if comp == 8:
         return j > 0 and k > 0
     return True
+
+
+def shrink(v, d):
+    """Dissipative part of correction d applied to the stepped value v: v
+    moves toward zero by at most |v| and never grows."""
+    if v > 0:
+        return v + min(max(d, -v), 0.0)
+    if v < 0:
+        return v + max(min(d, -v), 0.0)
+    return v
 
 
 def corrections(params, L, fields, nx, ny, nz, z):

@@ -112,6 +112,7 @@ def test_batch_members_equal_single_runs(gpu):
     wl.receivers = [t[:4] + (b,) for b in range(len(ms)) for t in rec0]
     wl.stf = wl.stf * len(ms)
     s = W.make_solver(wl)
+    s.set_tile_cols(128)  # the batch on 128-column tiles, the singles on 64
     s.run(wl.nt)
     tb = s.traces()
     s.close()
@@ -240,10 +241,12 @@ def test_autotune_and_tilings_agree(gpu):
 
     wl = W.ortho_layered(nx=70, ny=30, nz=26, L=4, nt=30, nrec_side=2)
     ref = None
-    for ty, kc in ((4, 8), (2, 3), (8, 26), (4, 5), (4, 1), (8, 2)):
+    for tx, ty, kc in ((64, 4, 8), (64, 2, 3), (64, 8, 26), (64, 4, 5), (64, 4, 1), (64, 8, 2), (128, 4, 8),
+                       (128, 8, 1), (128, 2, 26)):
         s = W.make_solver(wl)
         s.set_tiling(ty, kc)
-        assert s.tiling() == (ty, kc)
+        s.set_tile_cols(tx)
+        assert s.tiling() == (ty, kc) and s.tile_cols() == tx
         s.run(30)
         t = s.traces()
         s.close()
