@@ -19,6 +19,10 @@ Other configurations:
   --config 3  32 isotropic random-field cubes (edges 96..256) advanced as
               one group (one launch per half-step over all models), against
               the same models run one after the other
+  --config 4  the learned 6-cell boundary on a 512x512x256 grid: receiver
+              errors against a 2x-padded reference (with the plain 6-cell
+              and 24-cell layers beside it), and six elementary tensors with
+              the learned layer against six plain 24-cell runs
 """
 import argparse
 import json
@@ -33,7 +37,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 METRIC = "Gcell-updates/s + % of 8 TB/s HBM; batched-6-MT vs 6 plain runs at equal error"
-DEFAULT_STEPS = {0: 600, 1: 3000, 2: 1000, 3: 1500}
+DEFAULT_STEPS = {0: 600, 1: 3000, 2: 1000, 3: 1500, 4: 200}
 
 
 def _traffic(kernel_name, workload):
@@ -299,12 +303,98 @@ def bench_uneven(args, steps):
                                       "sequential_velocity_ms": round(ms_grp[1], 2)})
 
 
+def bench_learned(args, steps):
+    """Learned thin boundary at full size: a 512x512x256 grid whose outer 6
+    cells are the learned layer (interior 500x500x244), against the same grid
+    with its plain 6-cell layer, the 24-cell plain layer over the same
+    interior (548x548x292), and a 2x-padded reference (1024x1024x512, 24-cell
+    layer); receiver errors over `eval_steps`.  Throughput: six elementary
+    tensors as one batch on the learned grid against six plain 24-cell runs,
+    as interior cell-updates per second."""
+    from elanfmt_amd import learned as Lr
+
+    L, fb = 6, 24
+    thin, inner = (512, 512, 256), (500, 500, 244)
+    fall = tuple(q + 2 * fb for q in inner)
+    ref = (1024, 1024, 512)
+    lb, meta = Lr.load(os.path.join(ROOT, "elanfmt_amd", "data", "learned_boundary_L6.npz"))
+    seed, f0, nt = 4242, 3.0, args.eval_steps
+    print(f"# learned: medium {ref} on the host", file=sys.stderr, flush=True)
+    med = Lr.layered_medium(ref, seed)
+    from elanfmt_amd import workloads as W
+
+    dt = W.cfl_dt(med["vmax"], Lr.DX, Lr.DX, Lr.DX, 0.5)
+    z = Lr.impedance([c[256:768, 256:768, 128:384] for c in med["cells"][:1]], med["bcell"][256:768, 256:768, 128:384])
+    m, _, _ = W.mt_lune_random(seed, 1e15)
+    # 128 receivers (8 x 16) on the plane 10 cells inside the thin layer, x/y/z velocity
+    xs = np.linspace(L + 10, thin[0] - L - 11, 8).round().astype(int).tolist()
+    ys = np.linspace(L + 10, thin[1] - L - 11, 16).round().astype(int).tolist()
+    rec_thin = [(c, x, y, L + 10, 0) for y in ys for x in xs for c in range(3)]
+    src_thin = (thin[0] // 2 - 1, thin[1] // 2 - 1, thin[2] // 2 - 1)
+
+    def grid_run(g, damp, R=1e-8, with_lb=False, nsteps=nt, nbatch=1, timed=False):
+        o = tuple((r - q) // 2 for r, q in zip(ref, g))
+        sh = tuple((q - t) // 2 for q, t in zip(g, thin))
+        rec = [(c, i + sh[0], j + sh[1], k + sh[2], b) for (c, i, j, k, b) in rec_thin]
+        src = tuple(a + b for a, b in zip(src_thin, sh))
+        mm = m if nbatch == 1 else [np.eye(6)[e] * 1e15 for e in range(nbatch)]
+        wl = Lr.workload(f"g{g}", g, damp, Lr.crop(med, o, g), dt, src, mm, f0, nsteps, rec, R=R, nbatch=nbatch)
+        sv = W.make_solver(wl)
+        if with_lb:
+            lb.attach(sv, z)
+        if timed:
+            sv.run(10)
+            sv.reset()
+            sv.synchronize()
+            t0 = time.perf_counter()
+            sv.run(nsteps)
+            sv.synchronize()
+            el = time.perf_counter() - t0
+            sv.close()
+            return el
+        sv.run(nsteps)
+        tr = sv.traces().astype(np.float64)
+        sv.close()
+        return tr
+
+    print(f"# learned: reference run {ref} x {nt} steps", file=sys.stderr, flush=True)
+    tref = grid_run(ref, fb)
+    errs = {}
+    for name, g, damp, R, use in (("thin_plain_6", thin, L, meta["R_thin"], False),
+                                  ("thin_learned_6", thin, L, meta["R_thin"], True), ("plain_24", fall, fb, 1e-8, False)):
+        print(f"# learned: {name} {g}", file=sys.stderr, flush=True)
+        t = grid_run(g, damp, R, use)
+        errs[name] = float(np.linalg.norm(t - tref) / np.linalg.norm(tref))
+    print(f"# learned: timing {steps} steps", file=sys.stderr, flush=True)
+    t_b6 = grid_run(thin, L, meta["R_thin"], True, nsteps=steps, nbatch=6, timed=True)
+    t_p1 = grid_run(fall, fb, 1e-8, False, nsteps=steps, timed=True)
+    t_l1 = grid_run(thin, L, meta["R_thin"], True, nsteps=steps, timed=True)
+    t_t1 = grid_run(thin, L, meta["R_thin"], False, nsteps=steps, timed=True)
+    lb.close()
+    cells_in = inner[0] * inner[1] * inner[2]
+    value = cells_in * 6 * steps / t_b6 / 1e9
+    config = {"workload": f"learned 6-cell layer on 512x512x256 (interior 500x500x244), layered orthorhombic FP32, "
+                          f"lune MT, Ricker {f0} Hz, 128 receivers x 3 components; networks trained by "
+                          f"elanfmt_amd/train_lb.py ({meta['runs']} runs)",
+              "grid": list(thin), "interior": list(inner), "wavefields": 6, "eval_steps": nt,
+              "reference_grid": list(ref), "parallelism": "single GPU"}
+    return _line(value, steps, 0, t_b6, config, unit_note="interior cell-updates/s (6 wavefields)",
+                 trace_error_vs_padded_reference=errs,
+                 throughput={"batched6_learned_s": round(t_b6, 3), "six_plain24_runs_s": round(6 * t_p1, 3),
+                             "speedup": round(6 * t_p1 / t_b6, 3),
+                             "note": "six plain runs timed as 6 x one measured 24-cell run",
+                             "learned_thin_ms_per_step": round(t_l1 / steps * 1e3, 3),
+                             "plain_thin_ms_per_step": round(t_t1 / steps * 1e3, 3),
+                             "plain24_ms_per_step": round(t_p1 / steps * 1e3, 3)})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 3])
+    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 3, 4])
+    ap.add_argument("--eval-steps", type=int, default=1500)
     ap.add_argument("--kernel", default="auto", choices=["auto", "two_pass", "fused"])
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
@@ -326,8 +416,10 @@ def main():
         out = bench_plain(args, steps)
     elif args.config == 2:
         out = bench_batched(args, steps)
-    else:
+    elif args.config == 3:
         out = bench_uneven(args, steps)
+    else:
+        out = bench_learned(args, steps)
     print(json.dumps(out), flush=True)
 
 

@@ -34,13 +34,16 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 }
 
 __device__ __forceinline__ float ld(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void st(float* p, float v) { __builtin_nontemporal_store(v, p); }
 
-// Unconditional streaming load of p[idx] that yields 0 when !use; lanes that
-// do not need it read p[0] (always in bounds).  Keeping loads out of
-// branches lets the compiler keep a whole plane's loads in flight instead of
-// draining the memory counter at every control-flow join.
+// Unconditional load of p[idx] that yields 0 when !use; lanes that do not
+// need it read p[0] (always in bounds).  Keeping loads out of branches lets
+// the compiler keep a whole plane's loads in flight instead of draining the
+// memory counter at every control-flow join.  These are the neighbour loads
+// (c-1, c-sy, c-sz), re-read by adjacent tiles, so they go through the
+// caches; the streamed centre values use ld().
 __device__ __forceinline__ float ldz(const float* p, bool use, int64_t idx) {
-  const float v = __builtin_nontemporal_load(p + (use ? idx : 0));
+  const float v = p[use ? idx : 0];
   return use ? v : 0.f;
 }
 
@@ -182,8 +185,9 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb) {
         sxx += src[0]; syy += src[1]; szz += src[2];
         sxy += src[3]; sxz += src[4]; syz += src[5];
       }
-      Sxx[c] = sxx; Syy[c] = syy; Szz[c] = szz;
-      Sxy[c] = sxy; Sxz[c] = sxz; Syz[c] = syz;
+      // streaming stores: the outputs are not read again in this pass
+      st(Sxx + c, sxx); st(Syy + c, syy); st(Szz + c, szz);
+      st(Sxy + c, sxy); st(Sxz + c, sxz); st(Syz + c, syz);
     }
   }
 }
@@ -244,7 +248,8 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb) {
     if (PML == PML_SEP) w = txi * tyj * a.wz[k];
     float bx = a.bxdt, by = a.bydt, bz = a.bzdt;
     if (MODE == MODE_ORTHO) {
-      bx = ld(a.B[0] + c); by = ld(a.B[1] + c); bz = ld(a.B[2] + c);
+      // buoyancy through the caches (measured faster than streaming here)
+      bx = a.B[0][c]; by = a.B[1][c]; bz = a.B[2][c];
     }
     float wzm = 1.f;
     if (PML == PML_PROFILE) wzm = k >= 1 ? a.wz[k - 1] : 1.f;
@@ -274,9 +279,11 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb) {
       if (ux) vx = (MODE == MODE_ISO) ? iso_vel(wxm, vx, bx, dSx) : ortho_vel(w, vx, bx, dSx);
       if (uy) vy = (MODE == MODE_ISO) ? iso_vel(wym, vy, by, dSy) : ortho_vel(w, vy, by, dSy);
       if (uz) vz = (MODE == MODE_ISO) ? iso_vel(wzm, vz, bz, dSz) : ortho_vel(w, vz, bz, dSz);
-      if (ux) Vx[c] = vx;
-      if (uy) Vy[c] = vy;
-      if (uz) Vz[c] = vz;
+      // every cell is written (unchanged outside the update ranges) with
+      // streaming stores, so whole lines leave the CU
+      st(Vx + c, vx);
+      st(Vy + c, vy);
+      st(Vz + c, vz);
       for (int r = r0; r < r1; ++r) {
         const PointEntry p = a.rec[r];
         const int it = a.it + (a.it_base ? *a.it_base : 0);

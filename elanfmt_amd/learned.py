@@ -31,10 +31,15 @@ def impedance(cells, bcell):
     return float(np.sqrt(np.mean(cells[0], dtype=np.float64) * np.mean(1.0 / bcell, dtype=np.float64)))
 
 
+def _dims(n):
+    return (n, n, n) if np.isscalar(n) else tuple(int(v) for v in n)
+
+
 def layered_medium(n, seed, nl=5):
-    """Layered orthorhombic medium on an n^3 grid (as the 384x384x192
-    workload): cell moduli, cell buoyancy, staggered reference inputs."""
-    cells, bcell, _, vmax = W.layered_ortho(n, n, n, nl, seed)
+    """Layered orthorhombic medium on an n^3 (or nx x ny x nz) grid, as the
+    384x384x192 workload: cell moduli, cell buoyancy, staggered inputs."""
+    nx, ny, nz = _dims(n)
+    cells, bcell, _, vmax = W.layered_ortho(nx, ny, nz, nl, seed)
     cij, buoy = W.ortho_from_cells(cells, bcell)
     return dict(cells=cells, bcell=bcell, cij=cij, buoy=buoy, vmax=vmax)
 
@@ -44,9 +49,12 @@ _B_EXTRA = [(1, 0, 0), (0, 1, 0), (0, 0, 1)]
 
 
 def crop(med, o, n):
-    """Window [o, o+n)^3 of a medium (staggered arrays keep their +1)."""
+    """Window [o, o+n) per axis of a medium (staggered arrays keep their +1);
+    o and n are scalars or per-axis triples."""
+    o3, n3 = _dims(o), _dims(n)
+
     def sl(a, ex):
-        return a[o:o + n + ex[0], o:o + n + ex[1], o:o + n + ex[2]]
+        return a[o3[0]:o3[0] + n3[0] + ex[0], o3[1]:o3[1] + n3[1] + ex[1], o3[2]:o3[2] + n3[2] + ex[2]]
 
     return dict(cells=[sl(c, (0, 0, 0)) for c in med["cells"]], bcell=sl(med["bcell"], (0, 0, 0)),
                 cij=[sl(c, e) for c, e in zip(med["cij"], _CIJ_EXTRA)],
@@ -75,16 +83,23 @@ def teacher_taper(n_thin, L, off, dt, vmax, R_thin, R_ext=1e-6):
     return out
 
 
-def workload(name, n, damp, med, dt, src, m, f0, nt, receivers=(), R=1e-8, taper=None):
-    """Single-wavefield ortho workload on an n^3 grid with `damp` cells of
-    separable damping (reflection target R) on every face, or the given
-    1-D taper on all three axes."""
-    prof = tuple((np.asarray(taper, np.float32) if taper is not None else W.cell_profile(W.pml_profile(n, damp, dt, R=R)))
-                 for _ in range(3))
+def workload(name, n, damp, med, dt, src, m, f0, nt, receivers=(), R=1e-8, taper=None, nbatch=1):
+    """Ortho workload on an n^3 (or nx x ny x nz) grid with `damp` cells of
+    separable damping (reflection target R) on every face, or the given 1-D
+    taper on all three axes (cubes).  With nbatch > 1, `m` is a list of
+    nbatch tensors, one per wavefield, and receivers repeat per wavefield."""
+    nx, ny, nz = _dims(n)
+    if taper is not None:
+        prof = tuple(np.asarray(taper, np.float32) for _ in range(3))
+    else:
+        prof = tuple(W.cell_profile(W.pml_profile(q, damp, dt, R=R)) for q in (nx, ny, nz))
     stf = W.ricker(nt, dt, f0, 1.2 / f0) * np.float32(dt / DX ** 3)
-    inj = fwe.mt_injections(fwe.SRC_LAYOUT_OK4, *src, m)
-    return W.Workload(name, n, n, n, DX, DX, DX, dt, nt, fwe.MODE_ORTHO, fwe.PML_SEPARABLE, cij=med["cij"],
-                      buoy=med["buoy"], prof=prof, stf=[stf], injections=inj, receivers=list(receivers))
+    ms = [m] if nbatch == 1 else list(m)
+    inj = sum((fwe.mt_injections(fwe.SRC_LAYOUT_OK4, *src, mb, batch=b) for b, mb in enumerate(ms)), [])
+    rec = [t[:4] + (b,) for b in range(nbatch) for t in receivers]
+    return W.Workload(name, nx, ny, nz, DX, DX, DX, dt, nt, fwe.MODE_ORTHO, fwe.PML_SEPARABLE, nbatch=nbatch,
+                      cij=med["cij"], buoy=med["buoy"], prof=prof, stf=[stf] * nbatch, injections=inj,
+                      receivers=rec)
 
 
 @dataclass
