@@ -173,7 +173,8 @@ static void choose_tiling(fwe_ctx* x) {
     kc = (int)((c.nz + nkc - 1) / nkc);
     if (kc < 8) kc = 8;
   }
-  if (kc < 3) kc = 3;  // BC1 needs planes 1 and 2 in one chunk
+  const int kmin = c.free_surface ? 3 : 1;  // BC1 needs planes 1 and 2 in one chunk
+  if (kc < kmin) kc = kmin;
   if (kc > c.nz) kc = c.nz;
   x->kchunk = kc;
   x->nkc = (c.nz + kc - 1) / kc;
