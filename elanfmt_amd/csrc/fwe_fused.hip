@@ -27,12 +27,12 @@ __device__ __forceinline__ int xcd_remap_f(int b, int nb) {
   return x * q + (x < r ? x : r) + w;
 }
 
-__device__ __forceinline__ float ldn(const float* p) { return __builtin_nontemporal_load(p); }
-
-// Unconditional streaming load of p[idx] that yields 0 when !use; lanes that
-// do not need it read p[0] instead (always in bounds), so no branch is taken.
+// Unconditional load of p[idx] that yields 0 when !use; lanes that do not
+// need it read p[0] instead (always in bounds), so no branch is taken.  The
+// halo rows and columns are re-read by neighbouring tiles, so these loads go
+// through the caches; the outputs leave with streaming stores.
 __device__ __forceinline__ float ldz(const float* p, bool use, int64_t idx) {
-  const float v = __builtin_nontemporal_load(p + (use ? idx : 0));
+  const float v = p[use ? idx : 0];
   return use ? v : 0.f;
 }
 
@@ -250,10 +250,10 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_fused(const FusedArgs a) {
           s5 += src_at(a, e0, e1, amp, 5, i, j, m);
         }
         if (m >= k0 && m < k1) {
-          a.So[0][c] = s0; a.So[1][c] = s1; a.So[2][c] = s2;
-          if (im && jm) a.So[3][c] = s3;
-          if (im && km) a.So[4][c] = s4;
-          if (jm && km) a.So[5][c] = s5;
+          __builtin_nontemporal_store(s0, a.So[0] + c); __builtin_nontemporal_store(s1, a.So[1] + c); __builtin_nontemporal_store(s2, a.So[2] + c);
+          if (im && jm) __builtin_nontemporal_store(s3, a.So[3] + c);
+          if (im && km) __builtin_nontemporal_store(s4, a.So[4] + c);
+          if (jm && km) __builtin_nontemporal_store(s5, a.So[5] + c);
         }
       } else if (halo_lo) {
         // Syy of row j0-1, needed by the Vy update of row j0
@@ -333,17 +333,17 @@ __global__ __launch_bounds__(64 * (TY + 2)) void k_fused(const FusedArgs a) {
       if (ux) {
         const float dS = rate3(sp0 - sxx_im, a.rdx, sxy_jp - sp3, a.rdy, s4 - sp4, a.rdz);
         vx = (MODE == MODE_ISO) ? iso_vel(wxm, vx, bx, dS) : ortho_vel(w_p, vx, bx, dS);
-        a.Vo[0][cv] = vx;
+        __builtin_nontemporal_store(vx, a.Vo[0] + cv);
       }
       if (uy) {
         const float dS = rate3(sp1 - syy_jm, a.rdy, sxy_ip - sp3, a.rdx, s5 - sp5, a.rdz);
         vy = (MODE == MODE_ISO) ? iso_vel(wym, vy, by, dS) : ortho_vel(w_p, vy, by, dS);
-        a.Vo[1][cv] = vy;
+        __builtin_nontemporal_store(vy, a.Vo[1] + cv);
       }
       if (uz) {
         const float dS = rate3(sp2 - szz_pp, a.rdz, sxz_ip - sp4, a.rdx, syz_jp - sp5, a.rdy);
         vz = (MODE == MODE_ISO) ? iso_vel(wzm, vz, bz, dS) : ortho_vel(w_p, vz, bz, dS);
-        a.Vo[2][cv] = vz;
+        __builtin_nontemporal_store(vz, a.Vo[2] + cv);
       }
       for (int q = q0; q < q1; ++q) {
         const PointEntry p = a.rec[q];
