@@ -13,7 +13,11 @@
  * Further keys: nx= ny= nz= (instead of the NB and OVER keys), nt= (default 3000 iso,
  * 4500 ortho as OK4.cu:43 / ANISO.cu:43), mode=iso|ortho, pml=profile|
  * weights|taper|none, fs=0|1 (BC1), xsrc= ysrc= zsrc=, rec=C:i:j:k:name[,..]
- * (C in x|y|z), dir=<path>, device=<n>, nout=<n> (progress every n steps).
+ * (C in x|y|z), dir=<path>, device=<n>, nout=<n> (progress every n steps),
+ * snap=<n> (every n steps, all nine fields as <s>_0_<field>.res, s = 1, 2,
+ * ..., and the step in <s>_0_step.inf), restart=<s> (start from snapshot s:
+ * the read_data convention "%d_%d_%s.%s" of OK4.cu:166-175, for all nine
+ * fields; traces before the restart step stay zero).
  * Outputs: 0_infos.inf and 0_0_{Src,Rec,Rec_x,Rec_y,Rec_y2,Vx,Vy,Vz}.res.
  */
 #include <math.h>
@@ -30,7 +34,8 @@ static int usage(void) {
   fprintf(stderr,
           "usage: fwe_run [NBX=.. NBY=.. NBZ=.. OVERX=.. OVERY=.. OVERZ=..] [nx= ny= nz=] [nt=]\n"
           "               [mode=iso|ortho] [pml=profile|weights|taper|none] [fs=0|1]\n"
-          "               [xsrc= ysrc= zsrc=] [rec=x:i:j:k:name,...] [dir=.] [device=0] [nout=200]\n");
+          "               [xsrc= ysrc= zsrc=] [rec=x:i:j:k:name,...] [dir=.] [device=0] [nout=200]\n"
+          "               [snap=<steps>] [restart=<snapshot>]\n");
   return 2;
 }
 
@@ -49,9 +54,75 @@ static double now_s(void) {
     }                                                                                   \
   } while (0)
 
+static const char* const FIELD_NAMES[9] = {"Vx", "Vy", "Vz", "Sxx", "Syy", "Szz", "Sxy", "Sxz", "Syz"};
+
+/* elements of field f in its reference (staggered) shape */
+static size_t field_elems(const fwe_config* c, int f) {
+  const size_t nx = c->nx, ny = c->ny, nz = c->nz;
+  switch (f) {
+    case 0: return (nx + 1) * ny * nz;
+    case 1: return nx * (ny + 1) * nz;
+    case 2: return nx * ny * (nz + 1);
+    case 6: return (nx + 1) * (ny + 1) * nz;
+    case 7: return (nx + 1) * ny * (nz + 1);
+    case 8: return nx * (ny + 1) * (nz + 1);
+    default: return nx * ny * nz;
+  }
+}
+
+static char* snap_path(const char* dir, int s, const char* name, const char* ext) {
+  static char buf[4096];
+  snprintf(buf, sizeof(buf), "%s/%d_0_%s.%s", dir, s, name, ext);
+  return buf;
+}
+
+/* all nine fields of wavefield 0 as <s>_0_<field>.res, and the step */
+static int write_snapshot(fwe_ctx* ctx, const char* dir, int s, const fwe_config* c, int step) {
+  for (int f = 0; f < 9; ++f) {
+    const size_t n = field_elems(c, f);
+    float* h = (float*)malloc(sizeof(float) * n);
+    if (!h) { fprintf(stderr, "fwe_run: out of host memory\n"); return 1; }
+    if (fwe_get_field(ctx, 0, f, h) != FWE_OK) { fprintf(stderr, "fwe_run: %s\n", fwe_last_error()); free(h); return 1; }
+    FILE* fp = fopen(snap_path(dir, s, FIELD_NAMES[f], "res"), "wb");
+    const int ok = fp && fwrite(h, sizeof(float), n, fp) == n;
+    if (fp) fclose(fp);
+    free(h);
+    if (!ok) { fprintf(stderr, "fwe_run: cannot write snapshot %d\n", s); return 1; }
+  }
+  FILE* fp = fopen(snap_path(dir, s, "step", "inf"), "w");
+  if (!fp) { fprintf(stderr, "fwe_run: cannot write snapshot %d\n", s); return 1; }
+  fprintf(fp, "%d\n", step);
+  fclose(fp);
+  return 0;
+}
+
+static int read_snapshot(fwe_ctx* ctx, const char* dir, int s, const fwe_config* c, int* step) {
+  FILE* fp = fopen(snap_path(dir, s, "step", "inf"), "r");
+  if (!fp || fscanf(fp, "%d", step) != 1 || *step < 0) {
+    fprintf(stderr, "fwe_run: snapshot %d: missing or bad %s\n", s, snap_path(dir, s, "step", "inf"));
+    if (fp) fclose(fp);
+    return 1;
+  }
+  fclose(fp);
+  for (int f = 0; f < 9; ++f) {
+    const size_t n = field_elems(c, f);
+    float* h = (float*)malloc(sizeof(float) * n);
+    if (!h) { fprintf(stderr, "fwe_run: out of host memory\n"); return 1; }
+    fp = fopen(snap_path(dir, s, FIELD_NAMES[f], "res"), "rb");
+    const int ok = fp && fread(h, sizeof(float), n, fp) == n;
+    if (fp) fclose(fp);
+    if (!ok) { fprintf(stderr, "fwe_run: snapshot %d: missing or short %s\n", s, snap_path(dir, s, FIELD_NAMES[f], "res")); free(h); return 1; }
+    const fwe_status st = fwe_set_field(ctx, 0, f, h);
+    free(h);
+    if (st != FWE_OK) { fprintf(stderr, "fwe_run: %s\n", fwe_last_error()); return 1; }
+  }
+  if (fwe_set_step(ctx, *step) != FWE_OK) { fprintf(stderr, "fwe_run: %s\n", fwe_last_error()); return 1; }
+  return 0;
+}
+
 int main(int argc, char** argv) {
   int nbx = -1, nby = -1, nbz = -1, ovx = 0, ovy = 0, ovz = 0, nx = -1, ny = -1, nz = -1;
-  int nt = -1, mode = FWE_MODE_ISO, pml = -1, fs = 1, device = 0, nout = 200;
+  int nt = -1, mode = FWE_MODE_ISO, pml = -1, fs = 1, device = 0, nout = 200, snap = 0, restart = -1;
   int xsrc = -1, ysrc = -1, zsrc = -1;
   int npars[3] = {-1, -1, -1};
   const char* dir = ".";
@@ -88,6 +159,8 @@ int main(int argc, char** argv) {
     else if (KEY("dir")) dir = v;
     else if (KEY("device")) device = atoi(v);
     else if (KEY("nout")) nout = atoi(v);
+    else if (KEY("snap")) snap = atoi(v);
+    else if (KEY("restart")) restart = atoi(v);
     else { fprintf(stderr, "fwe_run: unknown key in '%s'\n", argv[a]); return usage(); }
 #undef KEY
   }
@@ -179,13 +252,24 @@ int main(int argc, char** argv) {
 
   printf("fwe_run: %s law, %dx%dx%d cells, %d steps, dt=%.6g, free surface %s (%s)\n",
          mode == FWE_MODE_ISO ? "iso" : "ortho", nx, ny, nz, nt, cfg.dt, fs ? "on" : "off", fwe_version());
+  int done = 0, isnap = 0;
+  if (restart >= 0) {
+    if (read_snapshot(ctx, dir, restart, &cfg, &done)) { fwe_destroy(ctx); return 1; }
+    printf("fwe_run: restart from snapshot %d at step %d\n", restart, done);
+    isnap = restart;
+  }
   const double t0 = now_s();
-  int done = 0;
+  const int first = done;
   while (done < nt) {
-    const int chunk = (nout > 0 && nt - done > nout) ? nout : nt - done;
+    int chunk = nt - done;
+    if (nout > 0 && chunk > nout - done % nout) chunk = nout - done % nout;
+    if (snap > 0 && chunk > snap - done % snap) chunk = snap - done % snap;
     CHECK(fwe_run(ctx, chunk));
     done += chunk;
-    if (nout > 0 && done < nt) {
+    if (snap > 0 && done % snap == 0 && done < nt) {
+      if (write_snapshot(ctx, dir, ++isnap, &cfg, done)) { fwe_destroy(ctx); return 1; }
+    }
+    if (nout > 0 && done % nout == 0 && done < nt) {
       CHECK(fwe_synchronize(ctx));
       printf("it=%05d > \n", done);
       fflush(stdout);
@@ -194,8 +278,8 @@ int main(int argc, char** argv) {
   CHECK(fwe_synchronize(ctx));
   const double el = now_s() - t0;
   const double cells = (double)nx * ny * nz;
-  printf("time %.3f s after %d iterations, %.2f it/s, %.3f Gcell-updates/s\n", el, nt, nt / el,
-         cells * nt / el * 1e-9);
+  printf("time %.3f s after %d iterations, %.2f it/s, %.3f Gcell-updates/s\n", el, nt - first,
+         (nt - first) / el, cells * (nt - first) / el * 1e-9);
   fwe_status st = fwe_write_legacy_outputs(ctx, dir, np, nrec);
   if (st == FWE_ERR_NONFINITE) fprintf(stderr, "fwe_run: warning: %s\n", fwe_last_error());
   else if (st != FWE_OK) { fprintf(stderr, "fwe_run: %s\n", fwe_last_error()); fwe_destroy(ctx); return 1; }

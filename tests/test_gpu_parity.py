@@ -211,6 +211,47 @@ def test_legacy_drop_in_binary(gpu, tmp_path):
     assert rel_l2(vx, o.get_field(0)) < TOL
 
 
+def test_legacy_binary_snapshots_and_restart(gpu, tmp_path):
+    """fwe_run snap=30 writes all nine fields every 30 steps; a second run
+    with restart=1 in a directory holding the inputs and snapshot 1 ends
+    with the same fields, and the same traces from the restart step on."""
+    import shutil
+    import subprocess
+
+    from elanfmt_amd import fwe, legacy, workloads as W
+
+    wl = W.iso_homogeneous(n=36, L=6, nt=70, free_surface=1)
+    m = np.array(wl.info["mt"], np.float32)
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    legacy.write_inputs(str(a), wl, mts=m)
+    legacy.write_inputs(str(b), wl, mts=m)
+    args = [fwe.RUN_PATH, "nx=36", "ny=36", "nz=36", "nt=70", "mode=iso", "fs=1", "xsrc=17", "ysrc=17", "zsrc=17",
+            "rec=z:22:20:1:Rec,x:22:20:1:Rec_x,y:22:23:1:Rec_y,y:22:22:1:Rec_y2", "nout=0"]
+    r = subprocess.run(args + ["snap=30"], cwd=str(a), capture_output=True, text=True, timeout=120)
+    print(r.stdout[-300:], r.stderr[-300:], flush=True)
+    assert r.returncode == 0
+    assert (a / "1_0_step.inf").read_text().split() == ["30"] and (a / "2_0_step.inf").read_text().split() == ["60"]
+    assert not (a / "3_0_step.inf").exists()
+    for name in ["Vx", "Vy", "Vz", "Sxx", "Syy", "Szz", "Sxy", "Sxz", "Syz", "step"]:
+        ext = "inf" if name == "step" else "res"
+        shutil.copy(a / f"1_0_{name}.{ext}", b / f"1_0_{name}.{ext}")
+    r = subprocess.run(args + ["restart=1"], cwd=str(b), capture_output=True, text=True, timeout=120)
+    print(r.stdout[-300:], r.stderr[-300:], flush=True)
+    assert r.returncode == 0
+    shapes = {"Vx": (37, 36, 36), "Vy": (36, 37, 36), "Vz": (36, 36, 37)}
+    for name, shp in shapes.items():
+        assert np.array_equal(legacy.read_res(str(a), name, shp), legacy.read_res(str(b), name, shp)), name
+    for name in ["Rec", "Rec_x", "Rec_y", "Rec_y2"]:
+        ta, tb = legacy.read_res(str(a), name), legacy.read_res(str(b), name)
+        assert np.array_equal(ta[30:], tb[30:]) and np.all(tb[:30] == 0), name
+    assert np.abs(legacy.read_res(str(a), "Rec")).max() > 0
+    # a missing snapshot is an error, not a crash
+    r = subprocess.run(args + ["restart=7"], cwd=str(b), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "snapshot 7" in r.stderr
+
+
 def test_graph_replay_equals_direct_launches(gpu):
     """The captured time loop (kernels reading the step index from device
     memory) gives the same traces and fields bit for bit, including a
@@ -407,3 +448,25 @@ def test_combine_elementary_traces_equals_direct_runs(gpu):
         err = rel_l2(comb[q], direct)
         print(f"combine MT {q}: rel-L2 {err:.2e}", flush=True)
         assert np.abs(direct).max() > 0 and err < 1e-5
+
+
+def test_generated_inputs_drive_the_binary(gpu, tmp_path):
+    """fwe_gen_inputs writes an ortho input set that fwe_run runs as the
+    reference's a.exe would (no MATLAB step)."""
+    import os
+    import subprocess
+
+    from elanfmt_amd import fwe, legacy
+
+    tool = os.path.join(os.path.dirname(fwe.RUN_PATH), "fwe_gen_inputs")
+    r = subprocess.run([tool, f"dir={tmp_path}", "nx=64", "ny=48", "nz=40", "nt=80", "mode=ortho", "L=8", "seed=3",
+                        "lune=4", "f0=12"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([fwe.RUN_PATH, "nx=64", "ny=48", "nz=40", "nt=80", "mode=ortho", "pml=taper", "fs=0",
+                        "xsrc=31", "ysrc=23", "zsrc=19", "rec=x:40:23:12:Rec,y:40:23:12:Rec_x,z:40:23:12:Rec_y",
+                        "nout=0"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    print(r.stdout[-300:], r.stderr[-300:], flush=True)
+    assert r.returncode == 0
+    assert legacy.read_infos(str(tmp_path)) == (4, 64, 48, 40, 1, 1)
+    tr = legacy.read_res(str(tmp_path), "Rec")
+    assert tr.shape == (80,) and np.all(np.isfinite(tr)) and np.abs(tr).max() > 0
