@@ -1384,8 +1384,8 @@ static hipError_t lb_before(fwe_ctx* x, int from) {
     const LbGeom g = lb_geom_b(x, from, b, x->lb_zinv);
     for (int f = 0; f < 6 && e == hipSuccess; ++f) {
       const int64_t o = x->lb_face_off[f], n = x->lb_face_off[f + 1] - o;
-      e = fwe::lb_launch_predict(x->lb->P + (int64_t)f * x->lb->np, x->lb->L, g, x->lb_cells + o, (int)n,
-                                 x->lb_D + (b * nc + o) * fwe::LB_OUT, lb_mlp_blocks(n), x->stream);
+      e = fwe::lb_launch_predict_mfma(x->lb->P + (int64_t)f * x->lb->np, x->lb->L, g, x->lb_cells + o, (int)n,
+                                      x->lb_D + (b * nc + o) * fwe::LB_OUT, x->stream);
     }
   }
   return e;

@@ -336,6 +336,150 @@ __device__ __forceinline__ float lb_shrink(float v, float d) {
   return v > 0.f ? v + fminf(fmaxf(d, -v), 0.f) : (v < 0.f ? v + fmaxf(fminf(d, -v), 0.f) : v);
 }
 
+// Inference on the matrix cores, before the step: corrections of n cells of
+// one face into D [n][9].  A wavefront evaluates 16 cells at a time with
+// v_mfma_f32_16x16x16_bf16: layer 1 is X[16 x 96] W1[96 x 64] (inputs padded
+// 81 -> 96), layer 2 H1[16 x 64] W2[64 x 64], layer 3 H2[16 x 64] W3[64 x 16]
+// (outputs padded 9 -> 16).  Operand layout of that instruction: A[i][k] in
+// lane i%16 + 16*(k/4), element k%4; B[k][n] in lane n + 16*(k/4); the
+// result D[m][n] in lane n + 16*(m/4), element m%4.  Inputs and hidden
+// activations are rounded to bfloat16, products accumulate in FP32; the
+// weights (B operands) stay in registers for the whole launch, and the
+// activations change layout through a per-wave LDS tile.
+typedef short bf4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float silu(float a) { return a * sigm(a); }
+
+__global__ __launch_bounds__(256) void k_lb_predict_mfma(const float* __restrict__ P, int32_t L, const LbGeom g,
+                                                        const LbCell* __restrict__ cells, int32_t n,
+                                                        float* __restrict__ D) {
+  __shared__ uint16_t hs[LB_WAVES][16][LB_H + 4];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int row = l & 15, grp = l >> 4;
+  // weights as B operands, rounded to bfloat16 once
+  bf4 w1[6][4], w2[4][4], w3[4];
+#pragma unroll
+  for (int s = 0; s < 6; ++s)
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int e = s * 16 + 4 * grp + j;
+        w1[s][t][j] = (short)(e < LB_IN ? to_bf16(P[lb_off_w1() + e * LB_H + t * 16 + row]) : 0);
+      }
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        w2[s][t][j] = (short)to_bf16(P[lb_off_w2(L) + (s * 16 + 4 * grp + j) * LB_H + t * 16 + row]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      w3[s][j] = (short)(row < LB_OUT ? to_bf16(P[lb_off_w3(L) + (s * 16 + 4 * grp + j) * LB_OUT + row]) : 0);
+  }
+  float b2v[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) b2v[t] = P[lb_off_b2(L) + t * 16 + row];
+  const float b3v = row < LB_OUT ? P[lb_off_b3(L) + row] : 0.f;
+  const float* __restrict__ b1 = P + lb_off_b1();
+  const float z = 1.f / g.zinv;
+
+  for (int64_t base = ((int64_t)blockIdx.x * LB_WAVES + w) * 16; base < n;
+       base += (int64_t)gridDim.x * LB_WAVES * 16) {
+    // the A row of this lane: cell base + row
+    const int64_t sa = base + row;
+    const bool va = sa < n;
+    const LbCell c = va ? cells[sa] : LbCell{0, 0, 0, 0};
+    float v[6][4];
+    float ss = 0.f;
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int e = s * 16 + 4 * grp + j;
+        v[s][j] = (va && e < LB_IN) ? lb_input(g, c, e) : 0.f;
+        ss = fmaf(v[s][j], v[s][j], ss);
+      }
+    ss += __shfl_xor(ss, 16);
+    ss += __shfl_xor(ss, 32);
+    const float r = sqrtf(ss * (1.f / LB_IN)) + 1e-30f;
+    const float ri = 1.f / r;
+    bf4 a[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) a[s][j] = (short)to_bf16(v[s][j] * ri);
+    // per result row m = 4*grp + j: depth, RMS and shear flags of cell base + m
+    const int flags = (c.fd & 0xff) | ((c.i > 0) << 8) | ((c.j > 0) << 9) | ((c.k > 0) << 10);
+    int fm[4];
+    float rm[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      fm[j] = __shfl(flags, 4 * grp + j);
+      rm[j] = __shfl(r, 4 * grp + j);
+    }
+    // layer 1
+    f4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < 6; ++s) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[s], w1[s][t], acc[t], 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        hs[w][4 * grp + j][t * 16 + row] = to_bf16(silu(acc[t][j] + b1[(fm[j] & 0xff) * LB_H + t * 16 + row]));
+    wave_sync();
+    bf4 a2[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) a2[s][j] = (short)hs[w][row][s * 16 + 4 * grp + j];
+    wave_sync();
+    // layer 2
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < 4; ++s) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[s], w2[s][t], acc[t], 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) hs[w][4 * grp + j][t * 16 + row] = to_bf16(silu(acc[t][j] + b2v[t]));
+    wave_sync();
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) a2[s][j] = (short)hs[w][row][s * 16 + 4 * grp + j];
+    wave_sync();
+    // layer 3: lane holds y[m = 4*grp + j][o = row]
+    f4 y = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[s], w3[s], y, 0, 0, 0);
+    if (row < LB_OUT) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t sm = base + 4 * grp + j;
+        if (sm < n) {
+          const LbCell cm{(fm[j] >> 8) & 1, (fm[j] >> 9) & 1, (fm[j] >> 10) & 1, 0};
+          D[sm * LB_OUT + row] = lb_comp_live(cm, row) ? (y[j] + b3v) * rm[j] * (row >= 3 ? z : 1.f) : 0.f;
+        }
+      }
+    }
+  }
+}
+
 // Inference, after the step: fields <- shrink(fields, D).
 __global__ __launch_bounds__(256) void k_lb_add(const LbGeom g, const LbCell* __restrict__ cells, int32_t n,
                                                const float* __restrict__ D) {
@@ -428,6 +572,14 @@ hipError_t lb_launch_predict(const float* P, int L, const LbGeom& g, const LbCel
                              int nblk, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_lb_predict, dim3(nblk), dim3(256), 0, s, P, L, g, cells, n, D);
+  return hipGetLastError();
+}
+
+hipError_t lb_launch_predict_mfma(const float* P, int L, const LbGeom& g, const LbCell* cells, int n, float* D,
+                                  hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int nblk = (int)std::min<int64_t>(2048, (n + 16 * LB_WAVES - 1) / (16 * LB_WAVES));
+  hipLaunchKernelGGL(k_lb_predict_mfma, dim3(nblk), dim3(256), 0, s, P, L, g, cells, n, D);
   return hipGetLastError();
 }
 

@@ -47,16 +47,20 @@ def _sig(a):
     return 1.0 / (1.0 + np.exp(-a))
 
 
-def forward(p, L, x, depth, rounded=True):
-    """y [n,9] and the cache for backward."""
+def forward(p, L, x, depth, rounded=True, act_bf16=False):
+    """y [n,9] and the cache for backward.  rounded: weights used as
+    bfloat16 (training and inference); act_bf16: inputs and hidden
+    activations rounded to bfloat16 before each product (the matrix-core
+    inference kernel)."""
     w = unpack(p, L)
     W1, W2, W3 = (bf16(w[k]) if rounded else w[k] for k in ("w1", "w2", "w3"))
+    q = bf16 if act_bf16 else (lambda a: a)
     x = np.asarray(x, np.float64)
-    a1 = x @ W1 + w["b1"][np.asarray(depth)]
+    a1 = q(x) @ W1 + w["b1"][np.asarray(depth)]
     h1 = a1 * _sig(a1)
-    a2 = h1 @ W2 + w["b2"]
+    a2 = q(h1) @ W2 + w["b2"]
     h2 = a2 * _sig(a2)
-    y = h2 @ W3 + w["b3"]
+    y = q(h2) @ W3 + w["b3"]
     return y, (x, np.asarray(depth), a1, h1, a2, h2, W2, W3)
 
 
@@ -154,7 +158,8 @@ def shrink(v, d):
 
 def corrections(params, L, fields, nx, ny, nz, z):
     """Per-cell corrections [(i, j, k, comp, value)] of the layer, from the
-    fields before a step; params = list of 6 packed parameter vectors."""
+    fields before a step, as the inference kernel computes them (bfloat16
+    inputs and activations); params = list of 6 packed parameter vectors."""
     cells = layer_cells(nx, ny, nz, L)
     X, R, dep = features(fields, cells, nx, ny, nz, 1.0 / z)
     out = []
@@ -163,7 +168,7 @@ def corrections(params, L, fields, nx, ny, nz, z):
         sel = np.nonzero(faces == f)[0]
         if sel.size == 0:
             continue
-        y, _ = forward(params[f], L, X[sel], dep[sel])
+        y, _ = forward(params[f], L, X[sel], dep[sel], act_bf16=True)
         for row, s in enumerate(sel):
             i, j, k = cells[s][:3]
             for comp in range(9):
