@@ -166,3 +166,29 @@ def test_trained_boundary_beats_the_plain_thin_layer(gpu):
         ratios.append(r["thin_learned"] / r["thin_plain"])
     assert np.mean(ratios) < 0.95
     lb.close()
+
+
+def test_learned_boundary_in_a_captured_time_loop(gpu):
+    """With the learned boundary attached, replaying a captured graph of
+    steps gives the same fields as direct launches."""
+    from elanfmt_amd import fwe, workloads as W
+
+    rng = np.random.default_rng(14)
+    L = 4
+    lb = fwe.LearnedBoundary(L)
+    for f in range(6):
+        lb.set_params(f, _params(rng, L))
+    wl = W.ortho_layered(nx=44, ny=40, nz=36, L=L, nt=60, f0=20.0, nrec_side=2)
+    out = []
+    for g in (0, 8):
+        s = W.make_solver(wl)
+        lb.attach(s, 1.5e7)
+        s.set_graph_steps(g)
+        s.run(37)
+        s.run(23)
+        out.append(([s.get_field(f) for f in range(9)], s.traces()))
+        s.close()
+    for f in range(9):
+        assert np.array_equal(out[0][0][f], out[1][0][f]), f
+    assert np.array_equal(out[0][1], out[1][1]) and np.abs(out[0][1]).max() > 0
+    lb.close()

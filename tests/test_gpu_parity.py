@@ -470,3 +470,62 @@ def test_generated_inputs_drive_the_binary(gpu, tmp_path):
     assert legacy.read_infos(str(tmp_path)) == (4, 64, 48, 40, 1, 1)
     tr = legacy.read_res(str(tmp_path), "Rec")
     assert tr.shape == (80,) and np.all(np.isfinite(tr)) and np.abs(tr).max() > 0
+
+
+@pytest.mark.parametrize("dims,tile", [((67, 45, 23), (128, 8, 5)), ((50, 19, 17), (64, 2, 1)), ((130, 9, 12), (128, 4, 3))])
+def test_ragged_grids_match_the_oracle(gpu, dims, tile):
+    """Sizes that are not multiples of the tile (and narrower than one
+    tile) on both laws, with free surface, against the FP64 oracle."""
+    from elanfmt_amd import fwe, workloads as W
+
+    nx, ny, nz = dims
+    wl = W.ortho_layered(nx=nx, ny=ny, nz=nz, L=3, nt=40, f0=20.0, nrec_side=2)
+    wl.free_surface = 1
+    wl.injections = fwe.mt_injections(fwe.SRC_LAYOUT_ANISO, nx // 2, ny // 2, nz // 2, [1e15, -3e14, 2e14, 4e14, 1e14, -2e14])
+    wl.receivers = [(q, i, j, k, 0) for q in range(3) for (i, j, k) in ((nx - 2, ny - 2, 1), (1, 1, nz - 2), (nx // 3, ny // 2, 2))]
+    from oracle import Oracle
+
+    s = W.make_solver(wl)
+    s.set_tile_cols(tile[0])
+    s.set_tiling(tile[1], max(3, tile[2]))
+    s.run(wl.nt)
+    g = s.traces().astype(np.float64)
+    gf = [s.get_field(f) for f in range(9)]
+    s.close()
+    o = Oracle(nx, ny, nz, wl.dx, wl.dy, wl.dz, wl.dt, wl.nt, mode=wl.mode, pml=wl.pml, free_surface=1)
+    W.apply(o, wl)
+    o.run(wl.nt)
+    assert np.abs(o.traces()).max() > 0
+    assert rel_l2(g, o.traces()) < TOL
+    for f in range(9):
+        assert rel_l2(gf[f], o.get_field(f)) < TOL, f
+
+
+def test_eight_wavefield_iso_batch_equals_single_runs(gpu):
+    """The largest batch (8 wavefields) on the iso law: each member equals
+    its single-wavefield run bit for bit."""
+    from elanfmt_amd import fwe, workloads as W
+
+    wl = W.iso_homogeneous(n=40, L=5, nt=45, rec_depth=3, nrec_side=2)
+    rng = np.random.default_rng(21)
+    ms = [rng.normal(0, 1e15, 6) for _ in range(8)]
+    rec0 = list(wl.receivers)
+    singles = []
+    for m in ms:
+        wl.nbatch, wl.stf, wl.receivers = 1, wl.stf[:1], rec0
+        wl.injections = fwe.mt_injections(fwe.SRC_LAYOUT_OK4, 19, 19, 19, m)
+        s = W.make_solver(wl)
+        s.run(wl.nt)
+        singles.append(s.traces())
+        s.close()
+    wl.nbatch = 8
+    wl.injections = sum((fwe.mt_injections(fwe.SRC_LAYOUT_OK4, 19, 19, 19, m, batch=b) for b, m in enumerate(ms)), [])
+    wl.receivers = [t[:4] + (b,) for b in range(8) for t in rec0]
+    wl.stf = wl.stf * 8
+    s = W.make_solver(wl)
+    s.run(wl.nt)
+    tb = s.traces()
+    s.close()
+    n0 = len(rec0)
+    for b in range(8):
+        assert np.array_equal(tb[b * n0:(b + 1) * n0], singles[b]), b
