@@ -111,34 +111,40 @@ class TrainLog:
     losses: list = None
 
 
-def train(lb: fwe.LearnedBoundary, runs=200, n_int=40, warm=60, steps=200, lr=5e-4, seed0=1000, f0=(3.0, 8.0),
+def train(lb: fwe.LearnedBoundary, runs=200, n_int=40, warm=None, steps=None, lr=5e-4, seed0=1000, f0=(3.0, 8.0),
           R_thin=1e-3, resync_every=8, log_every=0, progress_every=0):
     """Training on `runs` thick-damping runs over seeded random layered
-    media and lune moment tensors.  Each run: `warm` plain steps, then
-    `steps` training steps (one Adam step per face each); the thin run is
-    re-synchronised to the thick run every `resync_every` steps and rolls
-    out with its own corrections in between."""
+    media and lune moment tensors.  n_int is the interior edge, or a
+    sequence of edges drawn per run.  Each run: `warm` plain steps (default
+    1.5 n_int, until the first wavefront reaches the layer), then `steps`
+    training steps (default 5 n_int; one Adam step per face each); the thin
+    run is re-synchronised to the thick run every `resync_every` steps and
+    rolls out with its own corrections in between."""
     L = lb.layer
-    n_thin, n_thick = n_int + 2 * L, n_int + 2 * THICK
     off = THICK - L
+    sizes = [int(n_int)] if np.isscalar(n_int) else [int(v) for v in n_int]
     log = TrainLog(losses=[])
     for r in range(runs):
         seed = seed0 + r
         rng = np.random.default_rng(seed)
+        ni = sizes[int(rng.integers(len(sizes)))]
+        n_thin, n_thick = ni + 2 * L, ni + 2 * THICK
+        wu = warm if warm is not None else max(60, int(1.5 * ni))
+        nsteps = steps if steps is not None else 5 * ni
         med = layered_medium(n_thick, seed)
         dt = W.cfl_dt(med["vmax"], DX, DX, DX, 0.5)
         z = impedance(med["cells"], med["bcell"])
         ff = float(rng.uniform(*f0))
         m, _, _ = W.mt_lune_random(seed * 17 + 5, 1e15)
-        c = n_int // 2 + rng.integers(-n_int // 6, n_int // 6 + 1, 3)
-        nt = warm + steps + 2
+        c = ni // 2 + rng.integers(-ni // 6, ni // 6 + 1, 3)
+        nt = wu + nsteps + 2
         taper = teacher_taper(n_thin, L, off, dt, med["vmax"], R_thin)
         thick = W.make_solver(workload("thick", n_thick, THICK, med, dt, tuple(int(v) + THICK for v in c), m, ff, nt,
                                        taper=taper))
         thin = W.make_solver(workload("thin", n_thin, L, crop(med, off, n_thin), dt, tuple(int(v) + L for v in c),
                                       m, ff, nt, R=R_thin))
-        thick.run(warm)
-        for t in range(steps):
+        thick.run(wu)
+        for t in range(nsteps):
             loss = lb.train_step(thick, thin, off, z, lr, resync=(t % resync_every == 0))
             if log.steps == 0:
                 log.first_loss = loss

@@ -3,8 +3,8 @@
   python -m elanfmt_amd.train_lb [--runs 200] [--out elanfmt_amd/data/learned_boundary_L6.npz]
 
 Each run: a seeded random layered orthorhombic medium and lune moment
-tensor on a 40^3 interior, a teacher run on the interior plus 32 cells per
-face (the thin layer's damping over the thin window, a damping ramp
+tensor on an interior of one of the --sizes edges, a teacher run on the
+interior plus 32 cells per face (the thin layer's damping over the thin window, a damping ramp
 beyond), and a thin run with a 6-cell layer; see elanfmt_amd/learned.py.
 After training, the layer is evaluated on held-out media against a padded
 reference, and the numbers are stored with the parameters.
@@ -29,12 +29,14 @@ def main():
     ap.add_argument("--resync", type=int, default=8)
     ap.add_argument("--r-thin", type=float, default=1e-3)
     ap.add_argument("--seed0", type=int, default=1000)
+    ap.add_argument("--sizes", default="40", help="interior edges drawn per run, comma separated")
     ap.add_argument("--out", default=os.path.join(ROOT, "elanfmt_amd", "data", f"learned_boundary_L6.npz"))
     a = ap.parse_args()
     lb = fwe.LearnedBoundary(a.layer)
     t0 = time.time()
-    log = learned.train(lb, runs=a.runs, lr=a.lr, resync_every=a.resync, R_thin=a.r_thin, seed0=a.seed0,
-                        log_every=200, progress_every=10)
+    sizes = [int(v) for v in a.sizes.split(",")]
+    log = learned.train(lb, runs=a.runs, n_int=sizes, lr=a.lr, resync_every=a.resync, R_thin=a.r_thin,
+                        seed0=a.seed0, log_every=200, progress_every=10)
     t_train = time.time() - t0
     print(f"trained {log.runs} runs, {log.steps} steps in {t_train:.1f} s; loss {log.first_loss:.3e} -> "
           f"{log.last_loss:.3e}", flush=True)
@@ -42,8 +44,11 @@ def main():
     for sd in (7, 8, 9):
         evals[sd] = learned.evaluate(lb, seed=sd, R_thin=a.r_thin)
         print(f"held-out medium {sd}: {evals[sd]}", flush=True)
+    for n_int, nt, f0 in ((80, 500, 5.0), (120, 700, 4.0)):
+        evals[f"interior{n_int}"] = learned.evaluate(lb, seed=7, n_int=n_int, nt=nt, f0=f0, R_thin=a.r_thin)
+        print(f"held-out medium 7, interior {n_int}: {evals[f'interior{n_int}']}", flush=True)
     learned.save(lb, a.out, runs=a.runs, layer=a.layer, lr=a.lr, resync_every=a.resync, R_thin=a.r_thin,
-                 seed0=a.seed0, train_seconds=round(t_train, 1), steps=log.steps,
+                 seed0=a.seed0, sizes=sizes, train_seconds=round(t_train, 1), steps=log.steps,
                  loss_curve=[(s, float(f"{l:.4e}")) for s, l in log.losses], evaluation=evals)
     print(json.dumps({"out": a.out, "evaluation": evals}), flush=True)
     lb.close()
