@@ -20,6 +20,7 @@
 //    two stencil kernels (per-tile injection and receiver lists).
 // The stencil is HBM-bound (~0.8 flop/B), so there is no MFMA here.
 #pragma once
+#include <type_traits>
 #include "fwe_internal.h"
 #include "fwe_stencil.h"
 
@@ -230,15 +231,13 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb) {
   if (PML == PML_SEP) { txi = a.wx[i]; tyj = a.wy[j]; }
 
   float cszz[NB], csxz[NB], csyz[NB];  // Szz(k-1), Sxz(k), Syz(k)
-#pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    const int64_t c0 = (int64_t)i + j * sy + (int64_t)k0 * sz + (int64_t)b * a.fstride;
-    cszz[b] = ldz(a.S[2], k0 >= 1, c0 - sz);
-    csxz[b] = ldz(a.S[4], true, c0);
-    csyz[b] = ldz(a.S[5], true, c0);
-  }
 
-  for (int k = k0; k < k1; ++k) {
+  // The first plane of the chunk reads its z-neighbours from memory in the
+  // same batch as its other loads (a prologue load before the loop would
+  // make the loop head wait for it, and with one-plane chunks that doubles
+  // the memory round trips of the workgroup); later planes use the carries.
+  auto plane = [&](const int k, auto first_tag) __attribute__((always_inline)) {
+    constexpr bool FIRST = decltype(first_tag)::value;
     const int64_t c = (int64_t)i + j * sy + k * sz;
     const bool ux = ux_ij && k >= 1 && k <= nz - 2;
     const bool uy = uy_ij && k >= 1 && k <= nz - 2;
@@ -266,11 +265,11 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb) {
       const float sxx = Sxx[c], sxx_im = ldz(Sxx, im, c - 1);
       const float syy = Syy[c], syy_jm = ldz(Syy, jm, c - sy);
       const float szz = Szz[c];
-      const float szz_km = cszz[b];
+      const float szz_km = FIRST ? ldz(Szz, k >= 1, c - sz) : cszz[b];
       const float sxy = Sxy[c], sxy_jp = Sxy[c + sy], sxy_ip = Sxy[c + 1];
-      const float sxz = csxz[b];
+      const float sxz = FIRST ? Sxz[c] : csxz[b];
       const float sxz_kp = Sxz[c + sz], sxz_ip = Sxz[c + 1];
-      const float syz = csyz[b];
+      const float syz = FIRST ? Syz[c] : csyz[b];
       const float syz_kp = Syz[c + sz], syz_jp = Syz[c + sy];
       cszz[b] = szz; csxz[b] = sxz_kp; csyz[b] = syz_kp;
       const float dSx = rate3(sxx - sxx_im, a.rdx, sxy_jp - sxy, a.rdy, sxz_kp - sxz, a.rdz);
@@ -291,7 +290,9 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb) {
           a.traces[(int64_t)p.slot * a.nt + it] = p.comp == 0 ? vx : (p.comp == 1 ? vy : vz);
       }
     }
-  }
+  };
+  plane(k0, std::true_type{});
+  for (int k = k0 + 1; k < k1; ++k) plane(k, std::false_type{});
 }
 
 // One model per launch: workgroup -> tile of that model.
