@@ -173,7 +173,7 @@ static void choose_tiling(fwe_ctx* x) {
     kc = (int)((c.nz + nkc - 1) / nkc);
     if (kc < 8) kc = 8;
   }
-  const int kmin = c.free_surface ? 3 : 1;  // BC1 needs planes 1 and 2 in one chunk
+  const int kmin = c.free_surface == FWE_FS_BC1 ? 3 : 1;  // BC1 needs planes 1 and 2 in one chunk
   if (kc < kmin) kc = kmin;
   if (kc > c.nz) kc = c.nz;
   x->kchunk = kc;
@@ -186,7 +186,7 @@ extern "C" fwe_status fwe_config_default(fwe_config* cfg, int32_t mode) {
   memset(cfg, 0, sizeof(*cfg));
   cfg->mode = mode;
   cfg->pml = mode == FWE_MODE_ISO ? FWE_PML_PROFILE : FWE_PML_WEIGHTS3D;
-  cfg->free_surface = 1;
+  cfg->free_surface = FWE_FS_BC1;
   cfg->src_after_bc = mode == FWE_MODE_ISO ? 0 : 1;
   cfg->nbatch = 1;
   return FWE_OK;
@@ -204,6 +204,8 @@ static fwe_status check_cfg(const fwe_config* c) {
   if (c->mode == FWE_MODE_ORTHO && c->pml == FWE_PML_PROFILE)
     return fail(FWE_ERR_UNSUPPORTED, "ortho law takes WEIGHTS3D, SEPARABLE or NONE damping");
   if (c->pml < 0 || c->pml > 3) return fail(FWE_ERR_ARG, "bad pml");
+  if (c->free_surface < FWE_FS_NONE || c->free_surface > FWE_FS_VACUUM)
+    return fail(FWE_ERR_ARG, "free_surface must be 0 (none), 1 (BC1) or 2 (vacuum layer)");
   return FWE_OK;
 }
 
@@ -532,6 +534,8 @@ static StepArgs make_args(fwe_ctx* x, int from) {
     a.C[1] = x->medium + x->nelem;
   }
   a.c66dt = x->c66dt; a.bxdt = x->bxdt; a.bydt = x->bydt; a.bzdt = x->bzdt;
+  a.vac_k = c.free_surface == FWE_FS_VACUUM ? 1 : -1;
+  a.vac_cdt = (float)(FWE_VACUUM_SHEAR_MODULUS * c.dt);
   const size_t ox = 0, oy = c.nx + 1, oz = oy + c.ny + 1, oh = oz + c.nz + 1;
   a.wx = x->prof + ox; a.wy = x->prof + oy; a.wz = x->prof + oz;
   a.whx = x->prof + oh + ox; a.why = x->prof + oh + oy; a.whz = x->prof + oh + oz;
@@ -594,7 +598,7 @@ static hipError_t step_launch(fwe_ctx* x, int from, int it, const int32_t* it_ba
       StepArgs a = make_args(x, from);
       a.it = it;
       a.it_base = it_base;
-      e = fwe::launch_stress(a, c.mode, c.pml, c.free_surface, x->stream);
+      e = fwe::launch_stress(a, c.mode, c.pml, c.free_surface == FWE_FS_BC1, x->stream);
       if (e == hipSuccess && mid) e = hipEventRecord(mid, x->stream);
       if (e == hipSuccess) e = fwe::launch_velocity(a, c.mode, c.pml, x->stream);
     }
@@ -613,7 +617,7 @@ static hipError_t step_launch(fwe_ctx* x, int from, int it, const int32_t* it_ba
     StepArgs a = make_args(x, from);
     a.it = it;
     a.it_base = it_base;
-    e = fwe::launch_stress(a, c.mode, c.pml, c.free_surface, x->stream);
+    e = fwe::launch_stress(a, c.mode, c.pml, c.free_surface == FWE_FS_BC1, x->stream);
     if (e == hipSuccess && mid) e = hipEventRecord(mid, x->stream);
     if (e == hipSuccess) e = fwe::launch_velocity(a, c.mode, c.pml, x->stream);
   }
@@ -736,7 +740,7 @@ static fwe_status apply_tiling(fwe_ctx* x, int tx, int ty, int kchunk) {
   if (tx != 64 && use_fused(x)) return fail(FWE_ERR_ARG, "the single-pass kernel takes 64-column tiles");
   // the free surface (BC1 of plane 1 written after plane 2) and the
   // single-pass kernel's halo planes need planes 0..2 in one chunk
-  const int kmin = (c.free_surface || use_fused(x)) ? 3 : 1;
+  const int kmin = (c.free_surface == FWE_FS_BC1 || use_fused(x)) ? 3 : 1;
   if (kchunk < kmin || kchunk > c.nz) return fail(FWE_ERR_ARG, "z chunk must be in %d..nz", kmin);
   x->tx = tx;
   x->ty = ty;
@@ -804,7 +808,7 @@ extern "C" fwe_status fwe_autotune(fwe_ctx* x, int32_t trial_steps, double* best
   static const int txs[2] = {64, 128};
   static const int tys[3] = {2, 4, 8};
   static const int kcs[10] = {1, 2, 4, 8, 12, 16, 24, 32, 48, 64};
-  const int kmin = (x->cfg.free_surface || use_fused(x)) ? 3 : 1;
+  const int kmin = (x->cfg.free_surface == FWE_FS_BC1 || use_fused(x)) ? 3 : 1;
   for (int tx : txs) {
     if (tx != 64 && use_fused(x)) continue;
     for (int ty : tys) {
@@ -1179,7 +1183,7 @@ extern "C" fwe_status fwe_group_create(int32_t n, fwe_ctx* const* members, fwe_g
   fwe_group* g = new (std::nothrow) fwe_group();
   if (!g) return fail(FWE_ERR_NOMEM, "host allocation");
   g->m.assign(members, members + n);
-  g->device = c0.device; g->mode = c0.mode; g->pml = c0.pml; g->fs = c0.free_surface;
+  g->device = c0.device; g->mode = c0.mode; g->pml = c0.pml; g->fs = c0.free_surface == FWE_FS_BC1;
   g->ty = members[0]->ty;
   HIPCHK(hipSetDevice(g->device));
   for (fwe_ctx* x : g->m) {
@@ -1260,7 +1264,7 @@ extern "C" fwe_status fwe_group_set_tiling(fwe_group* g, int32_t tile_rows, int3
   if (!g) return fail(FWE_ERR_ARG, "NULL group");
   if (tile_rows != 2 && tile_rows != 4 && tile_rows != 8) return fail(FWE_ERR_ARG, "tile rows must be 2, 4 or 8");
   for (fwe_ctx* x : g->m)
-    if (std::min<int>(zchunk, x->cfg.nz) < (x->cfg.free_surface ? 3 : 1))
+    if (std::min<int>(zchunk, x->cfg.nz) < (x->cfg.free_surface == FWE_FS_BC1 ? 3 : 1))
       return fail(FWE_ERR_ARG, "z chunk too small for member with free surface");
   for (fwe_ctx* x : g->m) {
     fwe_status st = apply_tiling(x, 64, tile_rows, std::min<int>(zchunk, x->cfg.nz));

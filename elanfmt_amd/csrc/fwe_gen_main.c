@@ -11,7 +11,10 @@
  *
  * Raw float32, x fastest, in the reference's staggered shapes.  Keys:
  *   dir= nx= ny= nz= dx=(50) nt= cfl=(0.5) mode=iso|ortho
- *   iso:   vp=(6000) vs=(3464) rho=(2700) L=(20) R=(1e-8) fs=(0|1: top undamped)
+ *   iso:   vp=(6000) vs=(3464) rho=(2700) L=(20) R=(1e-8)
+ *   fs=(0|1|2): 1 and 2 leave the top undamped; 2 also empties the top cell
+ *          layer (zero moduli at k = 0) for the vacuum-layer free surface, as
+ *          the BC2 driver does (vizme_FastBiotED_GPU3D_OK_2024.m:238,243)
  *   ortho: seed=(1) layers=(5) L=(24) R=(1e-8) pml=taper|weights
  *   source: sdr=strike:dip:rake (30:60:90) or lune=<seed>; m0=(1e15); f0=(4)
  * The source time function is the Ricker of OK4.cu:210-218 with t0 = 1.2/f0,
@@ -27,8 +30,9 @@
 static int usage(void) {
   fprintf(stderr,
           "usage: fwe_gen_inputs nx= ny= nz= nt= [dir=.] [dx=50] [cfl=0.5] [mode=iso|ortho]\n"
-          "       iso:   [vp=6000 vs=3464 rho=2700 L=20 R=1e-8 fs=0]\n"
+          "       iso:   [vp=6000 vs=3464 rho=2700 L=20 R=1e-8]\n"
           "       ortho: [seed=1 layers=5 L=24 R=1e-8 pml=taper|weights]\n"
+          "       [fs=0|1|2]\n"
           "       [sdr=30:60:90 | lune=<seed>] [m0=1e15] [f0=4]\n");
   return 2;
 }
@@ -56,10 +60,10 @@ static float* alloc(size_t n) {
 }
 
 /* the ANISO damping of one axis from the OK4 profile: cell-centred factors */
-static float* cell_taper(int n, int L, double R, double dt) {
+static float* cell_taper(int n, int L, double R, double dt, int top_undamped) {
   float* w = alloc((size_t)n + 1);
   float* t = alloc((size_t)n);
-  fwe_gen_pml_profile(n, L, R, dt, 1, 1, 0, w);
+  fwe_gen_pml_profile(n, L, R, dt, 1, 1, top_undamped, w);
   fwe_gen_cell_profile(n, w, t);
   free(w);
   return t;
@@ -105,7 +109,7 @@ int main(int argc, char** argv) {
     }
 #undef KEY
   }
-  if (nx < 3 || ny < 3 || nz < 3 || nt < 1) return usage();
+  if (nx < 3 || ny < 3 || nz < 3 || nt < 1 || fs < 0 || fs > 2) return usage();
   if (L < 0) L = ortho ? 24 : 20;
   if (2 * L > nx || 2 * L > ny || 2 * L > nz) {
     fprintf(stderr, "fwe_gen_inputs: damping of %d cells does not fit the grid\n", L);
@@ -126,6 +130,8 @@ int main(int argc, char** argv) {
       return 1;
     }
     free(tops);
+    if (fs == 2)  // vacuum cells at k = 0 (the edge moduli follow by harmonic averaging)
+      for (int q = 0; q < 9; ++q) memset(cij[q], 0, sizeof(float) * (size_t)nx * ny);
   }
   const double dt = fwe_gen_cfl_dt(vmax, dx, dy, dz, cfl);
 
@@ -160,9 +166,10 @@ int main(int argc, char** argv) {
 
   if (!ortho) {
     float* a = alloc(nc);
-    for (size_t q = 0; q < nc; ++q) a[q] = (float)(rho * vp * vp);
+    const size_t top = fs == 2 ? (size_t)nx * ny : 0;  // vacuum cells at k = 0
+    for (size_t q = 0; q < nc; ++q) a[q] = q < top ? 0.f : (float)(rho * vp * vp);
     bad = bad || put(dir, "c11u_het.dat", a, nc);
-    for (size_t q = 0; q < nc; ++q) a[q] = (float)(rho * (vp * vp - 2 * vs * vs));
+    for (size_t q = 0; q < nc; ++q) a[q] = q < top ? 0.f : (float)(rho * (vp * vp - 2 * vs * vs));
     bad = bad || put(dir, "c12u_het.dat", a, nc);
     free(a);
     const int ns[3] = {nx, ny, nz};
@@ -189,7 +196,7 @@ int main(int argc, char** argv) {
     bad = bad || put(dir, "vrho_x_11_hc.dat", bx, nbx) || put(dir, "vrho_y_11_hc.dat", by, nby) ||
           put(dir, "vrho_z_11_hc.dat", bz, nbz);
     free(bx); free(by); free(bz);
-    float *tx = cell_taper(nx, L, R, dt), *ty = cell_taper(ny, L, R, dt), *tz = cell_taper(nz, L, R, dt);
+    float *tx = cell_taper(nx, L, R, dt, 0), *ty = cell_taper(ny, L, R, dt, 0), *tz = cell_taper(nz, L, R, dt, fs != 0);
     if (weights3d) {
       float* w = alloc(nc);
       for (int k = 0; k < nz; ++k)

@@ -42,6 +42,10 @@ struct StepArgs {
   const float* B[3];
   // iso medium: c11*dt, c12*dt arrays, scalars c66*dt and b*dt
   float c66dt, bxdt, bydt, bzdt;
+  // vacuum-layer top: on plane vac_k every shear modulus*dt is vac_cdt
+  // (vac_k = -1: off)
+  int32_t vac_k;
+  float vac_cdt;
   // damping: PROFILE -> w (n+1) and mean-of-neighbours wh (n) per axis;
   // SEP -> tapers (n); W3D -> w3 in the padded layout.
   const float* wx; const float* wy; const float* wz;

@@ -12,7 +12,8 @@
  *
  * Further keys: nx= ny= nz= (instead of the NB and OVER keys), nt= (default 3000 iso,
  * 4500 ortho as OK4.cu:43 / ANISO.cu:43), mode=iso|ortho, pml=profile|
- * weights|taper|none, fs=0|1 (BC1), xsrc= ysrc= zsrc=, rec=C:i:j:k:name[,..]
+ * weights|taper|none, fs=0|1|2 (none, BC1, vacuum layer of the BC2 variant),
+ * xsrc= ysrc= zsrc=, rec=C:i:j:k:name[,..]
  * (C in x|y|z), dir=<path>, device=<n>, nout=<n> (progress every n steps),
  * snap=<n> (every n steps, all nine fields as <s>_0_<field>.res, s = 1, 2,
  * ..., and the step in <s>_0_step.inf), restart=<s> (start from snapshot s:
@@ -33,7 +34,7 @@
 static int usage(void) {
   fprintf(stderr,
           "usage: fwe_run [NBX=.. NBY=.. NBZ=.. OVERX=.. OVERY=.. OVERZ=..] [nx= ny= nz=] [nt=]\n"
-          "               [mode=iso|ortho] [pml=profile|weights|taper|none] [fs=0|1]\n"
+          "               [mode=iso|ortho] [pml=profile|weights|taper|none] [fs=0|1|2]\n"
           "               [xsrc= ysrc= zsrc=] [rec=x:i:j:k:name,...] [dir=.] [device=0] [nout=200]\n"
           "               [snap=<steps>] [restart=<snapshot>]\n");
   return 2;
@@ -251,7 +252,8 @@ int main(int argc, char** argv) {
   CHECK(fwe_set_receivers(ctx, nrec, rec));
 
   printf("fwe_run: %s law, %dx%dx%d cells, %d steps, dt=%.6g, free surface %s (%s)\n",
-         mode == FWE_MODE_ISO ? "iso" : "ortho", nx, ny, nz, nt, cfg.dt, fs ? "on" : "off", fwe_version());
+         mode == FWE_MODE_ISO ? "iso" : "ortho", nx, ny, nz, nt, cfg.dt,
+         fs == FWE_FS_BC1 ? "BC1" : (fs == FWE_FS_VACUUM ? "vacuum layer" : "off"), fwe_version());
   int done = 0, isnap = 0;
   if (restart >= 0) {
     if (read_snapshot(ctx, dir, restart, &cfg, &done)) { fwe_destroy(ctx); return 1; }

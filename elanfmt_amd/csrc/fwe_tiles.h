@@ -107,6 +107,11 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb) {
       m2 = ld(a.C[2] + c); m3 = ld(a.C[3] + c); m4 = ld(a.C[4] + c); m5 = ld(a.C[5] + c);
       m6 = ld(a.C[6] + c); m7 = ld(a.C[7] + c); m8 = ld(a.C[8] + c);
     }
+    // vacuum-layer top (diff_BC2_OK3/FastWaveED_GPU3D_v1.cu:309-314): the
+    // shear moduli of plane vac_k are replaced (uniform per plane)
+    const bool vac = k == a.vac_k;
+    const float c66k = vac ? a.vac_cdt : a.c66dt;
+    if (vac) { m6 = a.vac_cdt; m7 = a.vac_cdt; m8 = a.vac_cdt; }
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int64_t fo = (int64_t)b * a.fstride;
@@ -137,9 +142,9 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb) {
         sxx = iso_normal(fxh, sxx, m0, ex, m1, ey, m1, ez);
         syy = iso_normal(fyh, syy, m1, ex, m0, ey, m1, ez);
         szz = iso_normal(fzh, szz, m1, ex, m1, ey, m0, ez);
-        sxy = iso_shear(wzk * wyj * wxi, sxy, a.c66dt, dxy);
-        sxz = iso_shear(wyj * wzk * wxi, sxz, a.c66dt, dxz);
-        syz = iso_shear(wxi * wyj * wzk, syz, a.c66dt, dyz);
+        sxy = iso_shear(wzk * wyj * wxi, sxy, c66k, dxy);
+        sxz = iso_shear(wyj * wzk * wxi, sxz, c66k, dxz);
+        syz = iso_shear(wxi * wyj * wzk, syz, c66k, dyz);
       } else {
         // ANISO.cu:306-352 with Cij holding Cij*dt
         sxx = ortho_normal(w, sxx, m0, ex, m3, ey, m4, ez);

@@ -91,12 +91,22 @@ enum { FWE_SRC_LAYOUT_OK4 = 0, FWE_SRC_LAYOUT_ANISO = 1 };
  * schedule measured fastest on MI355X (currently TWO_PASS). */
 enum { FWE_KERNEL_AUTO = 0, FWE_KERNEL_TWO_PASS = 1, FWE_KERNEL_FUSED = 2 };
 
+/* Top boundary.  VACUUM: on the shear plane k = 1 every shear stress uses
+ * the modulus FWE_VACUUM_SHEAR_MODULUS (Pa) instead of the medium's, and BC1
+ * is off; the caller empties the top cell layer (zero moduli at k = 0), as
+ * the BC2 driver does (vizme_FastBiotED_GPU3D_OK_2024.m:238,243). */
+enum { FWE_FS_NONE = 0, FWE_FS_BC1 = 1, FWE_FS_VACUUM = 2 };
+#define FWE_VACUUM_SHEAR_MODULUS 1e-5
+
 typedef struct fwe_config {
   int32_t nx, ny, nz;     /* cells: OK4.cu:39-41 (nx = 64*NBX - OVERX ...) */
   int32_t nt;             /* trace length in steps (OK4.cu:43) */
   int32_t mode;           /* fwe_mode */
   int32_t pml;            /* fwe_pml */
-  int32_t free_surface;   /* 1: apply BC1 every step (OK4.cu:271-283,490) */
+  int32_t free_surface;   /* fwe_free_surface: FWE_FS_BC1 applies BC1 every
+                             step (OK4.cu:271-283,490); FWE_FS_VACUUM is the
+                             vacuum-layer top of the BC2 variant
+                             (diff_BC2_OK3/FastWaveED_GPU3D_v1.cu:309-314) */
   int32_t src_after_bc;   /* 0: inject before BC1 (OK4 order, OK4.cu:336-342);
                              1: after BC1 (ANISO order, ANISO.cu:536-538) */
   int32_t nbatch;         /* wavefields sharing one medium, 1..FWE_MAX_BATCH */

@@ -193,6 +193,15 @@ static inline REAL w3(const oracle_t* o, int i, int j, int k) {
   return (REAL)1;
 }
 
+/* Vacuum-layer top (free_surface = 2), the BC2 variant of the iso kernel
+ * (diff_BC2_OK3/FastWaveED_GPU3D_v1.cu:309-314): every shear stress on the
+ * plane k = 1 uses the modulus 1e-5 instead of the medium's, and BC1 is not
+ * applied.  The caller empties the top cell layer (zero moduli at k = 0), as
+ * that variant's driver does (vizme_FastBiotED_GPU3D_OK_2024.m:238,243). */
+static inline REAL shear_modulus(const oracle_t* o, REAL c, int k) {
+  return (o->fs == 2 && k == 1) ? (REAL)1e-5 : c;
+}
+
 /* Stress pass, iso law: OK4.cu:285-330 (compute_StressPrf).  Shear cells at
  * i=0, j=0 or k=0 read outside their neighbourhood in the reference and are
  * never consumed by the velocity pass (OK4.cu:351-378); they stay 0 here. */
@@ -202,7 +211,6 @@ static void stress_iso(oracle_t* o) {
   REAL *sxx = o->f[F_SXX], *syy = o->f[F_SYY], *szz = o->f[F_SZZ];
   REAL *sxy = o->f[F_SXY], *sxz = o->f[F_SXZ], *syz = o->f[F_SYZ];
   const REAL dt = o->dt, rdx = (REAL)1 / o->dx, rdy = (REAL)1 / o->dy, rdz = (REAL)1 / o->dz;
-  const REAL c66 = o->c66;
   const int prof = (o->pml == 1);
   const REAL *wx = o->wx, *wy = o->wy, *wz = o->wz;
 #pragma omp parallel for schedule(static)
@@ -210,6 +218,7 @@ static void stress_iso(oracle_t* o) {
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {
         const int64_t c = IC(i, j, k);
+        const REAL c66 = shear_modulus(o, o->c66, k);
         const REAL exx = (Vx[IVX(i + 1, j, k)] - Vx[IVX(i, j, k)]) * rdx;
         const REAL eyy = (Vy[IVY(i, j + 1, k)] - Vy[IVY(i, j, k)]) * rdy;
         const REAL ezz = (Vz[IVZ(i, j, k + 1)] - Vz[IVZ(i, j, k)]) * rdz;
@@ -267,19 +276,19 @@ static void stress_ortho(oracle_t* o) {
           const int64_t e = IXY(i, j, k);
           const REAL dVx_dy = (Vx[IVX(i, j, k)] - Vx[IVX(i, j - 1, k)]) / dy;
           const REAL dVy_dx = (Vy[IVY(i, j, k)] - Vy[IVY(i - 1, j, k)]) / dx;
-          sxy[e] = sxy[e] * w + dt * C66[e] * (dVy_dx + dVx_dy);
+          sxy[e] = sxy[e] * w + dt * shear_modulus(o, C66[e], k) * (dVy_dx + dVx_dy);
         }
         if (i >= 1 && k >= 1) { /* ANISO.cu:336-343 */
           const int64_t e = IXZ(i, j, k);
           const REAL dVx_dz = (Vx[IVX(i, j, k)] - Vx[IVX(i, j, k - 1)]) / dz;
           const REAL dVz_dx = (Vz[IVZ(i, j, k)] - Vz[IVZ(i - 1, j, k)]) / dx;
-          sxz[e] = sxz[e] * w + dt * C55[e] * (dVz_dx + dVx_dz);
+          sxz[e] = sxz[e] * w + dt * shear_modulus(o, C55[e], k) * (dVz_dx + dVx_dz);
         }
         if (j >= 1 && k >= 1) { /* ANISO.cu:345-352 */
           const int64_t e = IYZ(i, j, k);
           const REAL dVy_dz = (Vy[IVY(i, j, k)] - Vy[IVY(i, j, k - 1)]) / dz;
           const REAL dVz_dy = (Vz[IVZ(i, j, k)] - Vz[IVZ(i, j - 1, k)]) / dy;
-          syz[e] = syz[e] * w + dt * C44[e] * (dVy_dz + dVz_dy);
+          syz[e] = syz[e] * w + dt * shear_modulus(o, C44[e], k) * (dVy_dz + dVz_dy);
         }
       }
 }
@@ -403,7 +412,7 @@ void oracle_run(oracle_t* o, int nsteps, int nthreads) {
   for (int s = 0; s < nsteps; ++s) {
     if (o->mode == 0) stress_iso(o); else stress_ortho(o);
     if (!o->src_after_bc) inject(o);
-    if (o->fs) bc1(o);
+    if (o->fs == 1) bc1(o);
     if (o->src_after_bc) inject(o);
     velocity(o);
     record(o);

@@ -26,7 +26,9 @@ extern "C" {
 typedef struct oracle oracle_t;
 
 /* mode 0 = iso (OK4), 1 = ortho (ANISO); pml 0 none, 1 profile (iso),
- * 2 weights3d (ortho), 3 separable (ortho); src_after_bc as in fwe_config. */
+ * 2 weights3d (ortho), 3 separable (ortho); free_surface 0 none, 1 BC1,
+ * 2 vacuum layer (shear modulus 1e-5 on plane k = 1, no BC1); src_after_bc
+ * as in fwe_config. */
 oracle_t* oracle_create(int mode, int nx, int ny, int nz, double dx, double dy,
                         double dz, double dt, int pml, int free_surface,
                         int src_after_bc, int nt);

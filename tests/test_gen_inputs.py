@@ -67,3 +67,35 @@ def test_ortho_inputs_match_the_workload_builder(fwe, tmp_path, pml):
 def test_damping_that_does_not_fit_is_refused(tmp_path):
     r = subprocess.run([TOOL, f"dir={tmp_path}", "nx=30", "ny=30", "nz=30", "nt=10"], capture_output=True, text=True)
     assert r.returncode == 2 and "does not fit" in r.stderr
+
+
+@pytest.mark.parametrize("mode", ["iso", "ortho"])
+def test_vacuum_top_empties_the_first_cell_layer(fwe, tmp_path, mode):
+    """fs=2: zero moduli in the top cell layer (k=0) and nowhere else, as the
+    BC2 driver writes them (vizme_FastBiotED_GPU3D_OK_2024.m:238,243), and
+    an undamped top."""
+    nx, ny, nz = 24, 20, 16
+    for fs in (0, 2):
+        d = tmp_path / f"fs{fs}"
+        d.mkdir()
+        _run(d, f"nx={nx}", f"ny={ny}", f"nz={nz}", "nt=10", f"mode={mode}", "L=4", f"fs={fs}")
+    names = ["c11u_het.dat", "c12u_het.dat"] if mode == "iso" else ["C11m.dat", "C22m.dat", "C33m.dat", "C12m.dat",
+                                                                    "C13m.dat", "C23m.dat"]
+    for name in names:
+        a = _f32(tmp_path / "fs2", name).reshape(nz, ny, nx)
+        b = _f32(tmp_path / "fs0", name).reshape(nz, ny, nx)
+        assert np.all(a[0] == 0) and np.all(b[0] > 0), name
+        np.testing.assert_array_equal(a[1:], b[1:], err_msg=name)
+    if mode == "iso":
+        wz = _f32(tmp_path / "fs2", "weights_z.dat")
+        assert np.all(wz[: nz // 2] == 1) and wz[-1] < 1
+    else:
+        tz = _f32(tmp_path / "fs2", "taper_z.dat")
+        assert tz[0] == 1 and tz[-1] < 1
+        assert _f32(tmp_path / "fs0", "taper_z.dat")[0] < 1
+
+
+def test_bad_free_surface_key_is_refused(tmp_path):
+    r = subprocess.run([TOOL, f"dir={tmp_path}", "nx=10", "ny=10", "nz=10", "nt=5", "fs=3"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 2

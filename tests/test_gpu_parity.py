@@ -89,6 +89,63 @@ def test_ortho_free_surface_all_damping_forms(gpu, pml):
         assert rel_l2(gf[f], rf[f]) < TOL, f
 
 
+def test_iso_vacuum_layer_top(gpu):
+    """Vacuum-layer top (free_surface=2, the BC2 variant,
+    diff_BC2_OK3/FastWaveED_GPU3D_v1.cu:309-314): empty top cell layer and
+    shear modulus 1e-5 on plane k=1, BC1 off; source and receivers near the
+    top, including shear injections on plane 1."""
+    from elanfmt_amd import workloads as W
+
+    wl = W.iso_homogeneous(n=36, L=5, nt=60, rec_depth=2, nrec_side=3, free_surface=2)
+    wl.c11[:, :, 0] = 0
+    wl.c12[:, :, 0] = 0
+    c = 18
+    wl.injections = [(2, c, c, 2, 0, 3e14), (4, c, c, 1, 0, 2e14), (5, c, c + 1, 1, 0, -1e14), (3, c, c, 1, 0, 1e14)]
+    wl.receivers = [(q, c + a, c + b, k, 0) for q in range(3) for a, b in ((0, 0), (3, 1)) for k in (1, 2)]
+    g, r, gf, rf = _run_both(wl)
+    assert np.abs(r).max() > 0
+    assert rel_l2(g, r) < TOL
+    for f in range(9):
+        assert rel_l2(gf[f], rf[f]) < TOL, f
+
+
+@pytest.mark.parametrize("nb", [1, 3])
+def test_ortho_vacuum_layer_top(gpu, nb):
+    """The vacuum-layer top on the orthorhombic law (its three shear moduli on
+    plane k=1), single and batched wavefields, against the oracle."""
+    from elanfmt_amd import fwe, workloads as W
+
+    wl = W.ortho_layered(nx=66, ny=34, nz=28, L=5, nt=50, nrec_side=2)
+    wl.free_surface = 2
+    wl.injections = fwe.mt_injections(fwe.SRC_LAYOUT_ANISO, 30, 15, 2, [1e15, -2e14, 3e14, 4e14, -5e14, 2e14])
+    wl.receivers = [(q, 30 + a, 15, k, 0) for q in range(3) for a in (0, 4) for k in (1, 2)]
+    g0, r, gf, rf = _run_both(wl)
+    assert np.abs(r).max() > 0
+    assert rel_l2(g0, r) < TOL
+    for f in range(9):
+        assert rel_l2(gf[f], rf[f]) < TOL, f
+    if nb > 1:
+        wl.nbatch = nb
+        wl.stf = [wl.stf[0]] * nb
+        wl.injections = [(q, i, j, k, b, c) for b in range(nb) for (q, i, j, k, _, c) in wl.injections]
+        wl.receivers = [(q, i, j, k, b) for b in range(nb) for (q, i, j, k, _) in wl.receivers]
+        s = W.make_solver(wl)
+        s.run(wl.nt)
+        g = s.traces()
+        s.close()
+        n1 = len(g) // nb
+        for b in range(nb):
+            assert np.array_equal(g[b * n1:(b + 1) * n1], g0.astype(np.float32)), b
+
+
+def test_free_surface_mode_is_checked(gpu):
+    from elanfmt_amd import fwe
+
+    with pytest.raises(fwe.FweError):
+        fwe.Solver(nx=20, ny=20, nz=20, dx=50.0, dy=50.0, dz=50.0, dt=1e-3, nt=10, mode=fwe.MODE_ISO,
+                   pml=fwe.PML_PROFILE, free_surface=3)
+
+
 def test_batch_members_equal_single_runs(gpu):
     """nbatch wavefields sharing one medium give exactly the traces of
     independent single-wavefield runs."""

@@ -143,6 +143,45 @@ def test_bc1_invariants(mode):
 
 
 @pytest.mark.parametrize("mode", [0, 1])
+def test_vacuum_layer_shear_plane(mode):
+    """Vacuum-layer top (free_surface=2, diff_BC2_OK3/FastWaveED_GPU3D_v1.cu:
+    309-314): after one step from a random state, every shear stress on the
+    plane k=1 is old + dt*1e-5*(strain-rate sum) from the old velocities (no
+    damping here), every other stress equals the step without a free
+    surface, and BC1 is not applied (Szz at k=0 is not zeroed)."""
+    nx, ny, nz = 8, 9, 7
+    dt = 1e-3
+    runs = {}
+    for fs in (0, 2):
+        rng = np.random.default_rng(7)
+        o = _iso(nx, ny, nz, dt=dt, pml=0, fs=fs) if mode == 0 else _ortho(nx, ny, nz, dt=dt, pml=0, fs=fs, rng=rng)
+        before = _randomize(o, nx, ny, nz, rng)
+        o.run(1)
+        runs[fs] = [o.get_field(f) for f in range(9)]
+    vx, vy, vz = before[F_VX], before[F_VY], before[F_VZ]
+    k = 1
+    exp_xy = before[F_SXY][1:nx, 1:ny, k] + dt * 1e-5 * ((vy[1:nx, 1:ny, k] - vy[0:nx - 1, 1:ny, k]) +
+                                                         (vx[1:nx, 1:ny, k] - vx[1:nx, 0:ny - 1, k]))
+    exp_xz = before[F_SXZ][1:nx, :ny, k] + dt * 1e-5 * ((vz[1:nx, :, k] - vz[0:nx - 1, :, k]) +
+                                                        (vx[1:nx, :, k] - vx[1:nx, :, k - 1]))
+    exp_yz = before[F_SYZ][:nx, 1:ny, k] + dt * 1e-5 * ((vy[:, 1:ny, k] - vy[:, 1:ny, k - 1]) +
+                                                        (vz[:, 1:ny, k] - vz[:, 0:ny - 1, k]))
+    vac, free = runs[2], runs[0]
+    assert np.allclose(vac[F_SXY][1:nx, 1:ny, k], exp_xy, rtol=1e-13, atol=0)
+    assert np.allclose(vac[F_SXZ][1:nx, :ny, k], exp_xz, rtol=1e-13, atol=0)
+    assert np.allclose(vac[F_SYZ][:nx, 1:ny, k], exp_yz, rtol=1e-13, atol=0)
+    for f in (F_SXX, F_SYY, F_SZZ):
+        assert np.array_equal(vac[f], free[f]), f
+    for f in (F_SXY, F_SXZ, F_SYZ):
+        a, b = vac[f].copy(), free[f].copy()
+        assert not np.allclose(a[:, :, k], b[:, :, k])  # the medium's modulus is not 1e-5
+        a[:, :, k] = 0
+        b[:, :, k] = 0
+        assert np.array_equal(a, b), f
+    assert np.any(vac[F_SZZ][:, :, 0] != 0)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
 def test_injection_from_rest(mode):
     """From rest, one step leaves exactly coef*Src[0] in each injected stress
     cell and zero in every other stress cell (OK4.cu:336-342)."""
