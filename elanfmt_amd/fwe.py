@@ -1,4 +1,4 @@
-"""ctypes binding of libfwe.so (include/fwe.h and include/fwe_gen.h).
+"""ctypes binding of libfwe.so (include/fwe.h, include/fwe_gen.h and include/fwe_post.h).
 
 A thin Python mirror of the C ABI, used by the tests and bench.py.  All
 compute happens in libfwe.so on the GPU; there is no CPU path here: when the
@@ -147,6 +147,10 @@ SIGNATURES = {
                               [C.POINTER(_FP), _FP, _IP, _DP]),
     "fwe_gen_random_field": (None, [C.c_int32] * 3 + [C.c_double, C.c_uint64, _FP]),
     "fwe_gen_cfl_dt": (C.c_double, [C.c_double] * 5),
+    # post-processing (fwe_post.h)
+    "fwe_post_rotate_mt": (None, [C.c_double, _DP, _DP]),
+    "fwe_post_butter_bandpass": (C.c_int32, [C.c_int32, C.c_double, C.c_double, C.c_double, _DP]),
+    "fwe_post_sosfilt": (C.c_int32, [C.c_int32, _DP, _FP, C.c_int64, _FP]),
 }
 
 
@@ -188,6 +192,34 @@ def device_count() -> int:
     n = C.c_int32(0)
     _check(lib().fwe_device_count(C.byref(n)))
     return n.value
+
+
+def rotate_mt(azimuth_deg, m):
+    """Moment tensor (Mxx, Myy, Mzz, Mxy, Mxz, Myz) rotated by the station
+    azimuth, Q M Q^T (fwe_post_rotate_mt)."""
+    a = np.ascontiguousarray(m, dtype=np.float64)
+    out = np.zeros(6)
+    lib().fwe_post_rotate_mt(float(azimuth_deg), dptr(a), dptr(out))
+    return out
+
+
+def butter_bandpass(order, flo, fhi, fs):
+    """Second-order sections (order x 6) of the Butterworth band-pass
+    (fwe_post_butter_bandpass)."""
+    sos = np.zeros((order, 6))
+    if lib().fwe_post_butter_bandpass(int(order), float(flo), float(fhi), float(fs), dptr(sos)) != 0:
+        raise FweError(ERR_ARG, f"band {flo}..{fhi} Hz, order {order} at {fs} Hz is not a valid band-pass")
+    return sos
+
+
+def sosfilt(sos, x):
+    """Causal cascade of the sections over float32 samples (fwe_post_sosfilt)."""
+    sos = np.ascontiguousarray(sos, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.empty_like(x)
+    if lib().fwe_post_sosfilt(len(sos), dptr(sos), fptr(x), x.size, fptr(y)) != 0:
+        raise FweError(ERR_ARG, "bad sections")
+    return y
 
 
 def field_shape(nx, ny, nz, field):
