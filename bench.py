@@ -161,6 +161,7 @@ def bench_plain(args, steps):
     el, ms_a, ms_b, tune = _timed_run(s, steps, args.warmup, not args.no_autotune)
     finite = bool(np.isfinite(s.traces()).all())
     tiling = (s.tile_cols(),) + tuple(s.tiling())
+    order = "xcd" if s.wg_order() == fwe.ORDER_XCD else "linear"
     s.close()
     cells = wl.cells
     kernels, dom, bstep = _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells)
@@ -170,7 +171,7 @@ def bench_plain(args, steps):
             f"{wl.name}: isotropic homogeneous FP32, {wl.info.get('pml_cells')}-cell profile damping, "
             f"double couple, Ricker {wl.info.get('f0')} Hz, {len(wl.receivers)} receiver traces")
     config = {"workload": desc, "grid": [wl.nx, wl.ny, wl.nz], "wavefields": wl.nbatch, "nt": wl.nt,
-              "kernel": "fused single-pass" if fused else "two-pass", "tiling": list(tiling),
+              "kernel": "fused single-pass" if fused else "two-pass", "tiling": list(tiling), "wg_order": order,
               "autotune_ms_per_step": None if tune is None else round(tune, 4), "parallelism": "single GPU"}
     return _line(cells * wl.nbatch * steps / el / 1e9, steps, args.warmup, el, config,
                  hbm_pct_step=round(100 * bstep * cells * steps / el / 1e9 / PEAK_HBM_GBS, 2),

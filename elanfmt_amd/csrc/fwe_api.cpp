@@ -88,6 +88,7 @@ struct fwe_ctx {
   bool have_medium = false, have_damp = false;
   float c66dt = 0, bxdt = 0, bydt = 0, bzdt = 0;
   int tx = 64, ty = 4, kchunk = 8, ntx = 0, nty = 0, nkc = 0;
+  int wg_order = FWE_ORDER_LINEAR;
   int it = 0;
   hipStream_t stream = nullptr;
   std::vector<hipEvent_t> events;
@@ -160,6 +161,8 @@ static void choose_tiling(fwe_ctx* x) {
   x->ty = ty;
   const char* etx = getenv("FWE_TILE_X");
   x->tx = (etx && atoi(etx) == 128) ? 128 : 64;
+  const char* eo = getenv("FWE_WG_ORDER");
+  x->wg_order = (eo && atoi(eo) == FWE_ORDER_XCD) ? FWE_ORDER_XCD : FWE_ORDER_LINEAR;
   x->ntx = (c.nx + x->tx - 1) / x->tx;
   x->nty = (c.ny + ty - 1) / ty;
   int kc;
@@ -524,6 +527,7 @@ static StepArgs make_args(fwe_ctx* x, int from) {
   a.nx = c.nx; a.ny = c.ny; a.nz = c.nz; a.nb = c.nbatch;
   a.sy = x->sy; a.sz = x->sz; a.fstride = 9 * x->nelem;
   a.tile_x = x->tx;
+  a.xcd_order = x->wg_order == FWE_ORDER_XCD;
   for (int q = 0; q < 3; ++q) a.V[q] = x->fbuf[from] + (int64_t)q * x->nelem;
   for (int q = 0; q < 6; ++q) a.S[q] = x->fbuf[from] + (int64_t)(3 + q) * x->nelem;
   if (c.mode == FWE_MODE_ORTHO) {
@@ -785,6 +789,20 @@ extern "C" fwe_status fwe_set_tile_cols(fwe_ctx* x, int32_t cols) {
   return apply_tiling(x, cols, x->ty, x->kchunk);
 }
 
+extern "C" fwe_status fwe_set_wg_order(fwe_ctx* x, int32_t order) {
+  if (!x) return fail(FWE_ERR_ARG, "NULL ctx");
+  if (order != FWE_ORDER_LINEAR && order != FWE_ORDER_XCD) return fail(FWE_ERR_ARG, "bad workgroup order %d", order);
+  drop_graph(x);
+  x->wg_order = order;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_get_wg_order(const fwe_ctx* x, int32_t* order) {
+  if (!x || !order) return fail(FWE_ERR_ARG, "NULL argument");
+  *order = x->wg_order;
+  return FWE_OK;
+}
+
 extern "C" fwe_status fwe_get_tile_cols(const fwe_ctx* x, int32_t* cols) {
   if (!x || !cols) return fail(FWE_ERR_ARG, "NULL argument");
   *cols = x->tx;
@@ -803,26 +821,31 @@ extern "C" fwe_status fwe_autotune(fwe_ctx* x, int32_t trial_steps, double* best
   fwe_status st = ready(x);
   if (st) return st;
   const int nz = x->cfg.nz;
-  int best_tx = x->tx, best_ty = x->ty, best_kc = x->kchunk;
+  int best_tx = x->tx, best_ty = x->ty, best_kc = x->kchunk, best_order = x->wg_order;
   double best = 1e30;
   static const int txs[2] = {64, 128};
   static const int tys[3] = {2, 4, 8};
   static const int kcs[10] = {1, 2, 4, 8, 12, 16, 24, 32, 48, 64};
   const int kmin = (x->cfg.free_surface == FWE_FS_BC1 || use_fused(x)) ? 3 : 1;
-  for (int tx : txs) {
-    if (tx != 64 && use_fused(x)) continue;
-    for (int ty : tys) {
-      for (int kc : kcs) {
-        if (kc > nz || kc < kmin) continue;
-        if ((st = apply_tiling(x, tx, ty, kc))) return st;
-        if ((st = run_impl(x, 1, nullptr, nullptr, nullptr))) return st;  // warm-up
-        double a = 0, b = 0, w = 0;
-        if ((st = run_impl(x, trial_steps, &a, &b, &w))) return st;
-        const double per = (a + b) / trial_steps;
-        if (per < best) { best = per; best_tx = tx; best_ty = ty; best_kc = kc; }
+  drop_graph(x);
+  for (int order = FWE_ORDER_LINEAR; order <= FWE_ORDER_XCD; ++order) {
+    x->wg_order = order;
+    for (int tx : txs) {
+      if (tx != 64 && use_fused(x)) continue;
+      for (int ty : tys) {
+        for (int kc : kcs) {
+          if (kc > nz || kc < kmin) continue;
+          if ((st = apply_tiling(x, tx, ty, kc))) return st;
+          if ((st = run_impl(x, 1, nullptr, nullptr, nullptr))) return st;  // warm-up
+          double a = 0, b = 0, w = 0;
+          if ((st = run_impl(x, trial_steps, &a, &b, &w))) return st;
+          const double per = (a + b) / trial_steps;
+          if (per < best) { best = per; best_tx = tx; best_ty = ty; best_kc = kc; best_order = order; }
+        }
       }
     }
   }
+  x->wg_order = best_order;
   if ((st = apply_tiling(x, best_tx, best_ty, best_kc))) return st;
   if (best_ms_per_step) *best_ms_per_step = best;
   return fwe_reset(x);
@@ -1139,7 +1162,7 @@ extern "C" fwe_status fwe_copy_probe(int32_t device, int64_t bytes, int32_t reps
 // ---------------------------------------------------------------------------
 struct fwe_group {
   std::vector<fwe_ctx*> m;
-  int device = 0, mode = 0, pml = 0, fs = 0, ty = 4;
+  int device = 0, mode = 0, pml = 0, fs = 0, ty = 4, order = FWE_ORDER_LINEAR;
   hipStream_t stream = nullptr;
   StepArgs* tab = nullptr;
   int32_t* start = nullptr;
@@ -1235,9 +1258,9 @@ static fwe_status group_run_impl(fwe_group* g, int32_t nsteps, double* ms) {
   HIPCHK(fwe::launch_advance(g->it_dev, g->m[0]->it, 1, g->stream));
   if (ms) HIPCHK(hipEventRecord(e0, g->stream));
   for (int s = 0; s < nsteps; ++s) {
-    HIPCHK(fwe::launch_group(g->tab, g->start, (int)g->m.size(), g->nblk, g->ty, g->mode, g->pml, g->fs, false,
+    HIPCHK(fwe::launch_group(g->tab, g->start, (int)g->m.size(), g->nblk, g->ty, g->order, g->mode, g->pml, g->fs, false,
                              g->stream));
-    HIPCHK(fwe::launch_group(g->tab, g->start, (int)g->m.size(), g->nblk, g->ty, g->mode, g->pml, g->fs, true,
+    HIPCHK(fwe::launch_group(g->tab, g->start, (int)g->m.size(), g->nblk, g->ty, g->order, g->mode, g->pml, g->fs, true,
                              g->stream));
     HIPCHK(fwe::launch_advance(g->it_dev, 1, 0, g->stream));
   }
@@ -1279,20 +1302,23 @@ extern "C" fwe_status fwe_group_autotune(fwe_group* g, int32_t trial_steps, doub
   static const int tys[3] = {2, 4, 8};
   static const int kcs[8] = {1, 2, 4, 8, 16, 32, 64, 256};
   double best = 1e30;
-  int bty = g->ty, bkc = 8;
+  int bty = g->ty, bkc = 8, border = g->order;
   fwe_status st;
   int kmin = g->fs ? 3 : 1;
   for (fwe_ctx* x : g->m)
     if (use_fused(x)) kmin = 3;
-  for (int ty : tys)
-    for (int kc : kcs) {
-      if (kc < kmin) continue;
-      if ((st = fwe_group_set_tiling(g, ty, kc))) return st;
-      double ms = 0;
-      if ((st = group_run_impl(g, 1, &ms))) return st;
-      if ((st = group_run_impl(g, trial_steps, &ms))) return st;
-      if (ms / trial_steps < best) { best = ms / trial_steps; bty = ty; bkc = kc; }
-    }
+  for (int order = FWE_ORDER_LINEAR; order <= FWE_ORDER_XCD; ++order)
+    for (int ty : tys)
+      for (int kc : kcs) {
+        if (kc < kmin) continue;
+        g->order = order;
+        if ((st = fwe_group_set_tiling(g, ty, kc))) return st;
+        double ms = 0;
+        if ((st = group_run_impl(g, 1, &ms))) return st;
+        if ((st = group_run_impl(g, trial_steps, &ms))) return st;
+        if (ms / trial_steps < best) { best = ms / trial_steps; bty = ty; bkc = kc; border = order; }
+      }
+  g->order = border;
   if ((st = fwe_group_set_tiling(g, bty, bkc))) return st;
   for (fwe_ctx* x : g->m)
     if ((st = fwe_reset(x))) return st;

@@ -26,8 +26,12 @@
 
 namespace fwe {
 
-// Workgroup w is dispatched to XCD (w mod 8).  Give each XCD a contiguous
-// range of logical tiles so x/y neighbours share an L2.
+// Workgroup w is dispatched to XCD (w mod 8).  xcd_remap gives each XCD a
+// contiguous range of logical tiles so x/y neighbours share an L2; the
+// linear order instead keeps the workgroups in flight on one contiguous
+// stretch of every array, which HBM streams better.  Which wins depends on
+// the tiling (one-plane chunks: linear, measured), so the order is a tuned
+// parameter (StepArgs::xcd_order).
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
   const int q = nb >> 3, r = nb & 7;
   const int x = b & 7, w = b >> 3;
@@ -303,11 +307,11 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb) {
 // One model per launch: workgroup -> tile of that model.
 template <int MODE, int PML, bool FS, int TX, int TY, int NB>
 __global__ __launch_bounds__(TX * TY) void k_stress(const StepArgs a) {
-  stress_tile<MODE, PML, FS, TX, TY, NB>(a, xcd_remap(blockIdx.x, gridDim.x));
+  stress_tile<MODE, PML, FS, TX, TY, NB>(a, a.xcd_order ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x);
 }
 template <int MODE, int PML, int TX, int TY, int NB>
 __global__ __launch_bounds__(TX * TY) void k_velocity(const StepArgs a) {
-  velocity_tile<MODE, PML, TX, TY, NB>(a, xcd_remap(blockIdx.x, gridDim.x));
+  velocity_tile<MODE, PML, TX, TY, NB>(a, a.xcd_order ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x);
 }
 
 // Many models per launch (an uneven batch of independent grids): the
@@ -320,15 +324,17 @@ __device__ __forceinline__ int group_model(const int32_t* __restrict__ start, in
 }
 template <int MODE, int PML, bool FS, int TY>
 __global__ __launch_bounds__(64 * TY) void k_stress_group(const StepArgs* __restrict__ tab,
-                                                         const int32_t* __restrict__ start, int32_t nm) {
-  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+                                                         const int32_t* __restrict__ start, int32_t nm,
+                                                         int32_t order) {
+  const int lb = order ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int m = group_model(start, nm, lb);
   stress_tile<MODE, PML, FS, 64, TY, 1>(tab[m], lb - start[m]);
 }
 template <int MODE, int PML, int TY>
 __global__ __launch_bounds__(64 * TY) void k_velocity_group(const StepArgs* __restrict__ tab,
-                                                           const int32_t* __restrict__ start, int32_t nm) {
-  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+                                                           const int32_t* __restrict__ start, int32_t nm,
+                                                           int32_t order) {
+  const int lb = order ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int m = group_model(start, nm, lb);
   velocity_tile<MODE, PML, 64, TY, 1>(tab[m], lb - start[m]);
 }
@@ -411,17 +417,17 @@ static hipError_t go_vel(const StepArgs& a, hipStream_t s) {
 }
 
 template <int MODE, int PML, bool FS>
-static hipError_t go_group(const StepArgs* tab, const int32_t* start, int nm, int nblk, int ty, bool vel,
+static hipError_t go_group(const StepArgs* tab, const int32_t* start, int nm, int nblk, int ty, int order, bool vel,
                            hipStream_t s) {
   if (ty == 2) {
-    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, 2>), dim3(nblk), dim3(64, 2), 0, s, tab, start, nm);
-    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, 2>), dim3(nblk), dim3(64, 2), 0, s, tab, start, nm);
+    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, 2>), dim3(nblk), dim3(64, 2), 0, s, tab, start, nm, order);
+    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, 2>), dim3(nblk), dim3(64, 2), 0, s, tab, start, nm, order);
   } else if (ty == 4) {
-    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, 4>), dim3(nblk), dim3(64, 4), 0, s, tab, start, nm);
-    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, 4>), dim3(nblk), dim3(64, 4), 0, s, tab, start, nm);
+    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, 4>), dim3(nblk), dim3(64, 4), 0, s, tab, start, nm, order);
+    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, 4>), dim3(nblk), dim3(64, 4), 0, s, tab, start, nm, order);
   } else if (ty == 8) {
-    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, 8>), dim3(nblk), dim3(64, 8), 0, s, tab, start, nm);
-    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, 8>), dim3(nblk), dim3(64, 8), 0, s, tab, start, nm);
+    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, 8>), dim3(nblk), dim3(64, 8), 0, s, tab, start, nm, order);
+    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, 8>), dim3(nblk), dim3(64, 8), 0, s, tab, start, nm, order);
   } else {
     return hipErrorInvalidValue;
   }
@@ -439,10 +445,10 @@ hipError_t velocity_mp(const StepArgs& a, hipStream_t s) {
 }
 
 template <int MODE, int PML>
-hipError_t group_mp(const StepArgs* tab, const int32_t* start, int nm, int nblk, int ty, int fs, bool vel,
+hipError_t group_mp(const StepArgs* tab, const int32_t* start, int nm, int nblk, int ty, int order, int fs, bool vel,
                     hipStream_t s) {
-  return fs ? go_group<MODE, PML, true>(tab, start, nm, nblk, ty, vel, s)
-            : go_group<MODE, PML, false>(tab, start, nm, nblk, ty, vel, s);
+  return fs ? go_group<MODE, PML, true>(tab, start, nm, nblk, ty, order, vel, s)
+            : go_group<MODE, PML, false>(tab, start, nm, nblk, ty, order, vel, s);
 }
 
 }  // namespace fwe

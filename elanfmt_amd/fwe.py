@@ -18,6 +18,8 @@ RUN_PATH = os.path.join(_HERE, "fwe_run")
 # status codes (fwe_status)
 OK, ERR_ARG, ERR_IO, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_UNSUPPORTED, ERR_NONFINITE = range(8)
 KERNEL_AUTO, KERNEL_TWO_PASS, KERNEL_FUSED = range(3)
+ORDER_LINEAR, ORDER_XCD = 0, 1
+FS_NONE, FS_BC1, FS_VACUUM = 0, 1, 2
 MODE_ISO, MODE_ORTHO = 0, 1
 PML_NONE, PML_PROFILE, PML_WEIGHTS3D, PML_SEPARABLE = 0, 1, 2, 3
 SXX, SYY, SZZ, SXY, SXZ, SYZ = range(6)
@@ -124,6 +126,8 @@ SIGNATURES = {
     "fwe_lb_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_float, C.c_int32,
                                     _DP]),
     "fwe_set_learned_boundary": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float]),
+    "fwe_set_wg_order": (C.c_int, [C.c_void_p, C.c_int32]),
+    "fwe_get_wg_order": (C.c_int, [C.c_void_p, _IP]),
     # generators (fwe_gen.h)
     "fwe_rng_seed": (None, [C.POINTER(RNG), C.c_uint64]),
     "fwe_rng_next": (C.c_uint64, [C.POINTER(RNG)]),
@@ -345,6 +349,14 @@ class Solver:
     def tile_cols(self):
         c = C.c_int32()
         _check(lib().fwe_get_tile_cols(self._h, C.byref(c)))
+        return c.value
+
+    def set_wg_order(self, order):
+        _check(lib().fwe_set_wg_order(self._h, order))
+
+    def wg_order(self):
+        c = C.c_int32()
+        _check(lib().fwe_get_wg_order(self._h, C.byref(c)))
         return c.value
 
     def autotune(self, trial_steps=5):

@@ -235,6 +235,14 @@ fwe_status fwe_autotune(fwe_ctx* ctx, int32_t trial_steps, double* best_ms_per_s
  * and groups use 64.  fwe_autotune tries both. */
 fwe_status fwe_set_tile_cols(fwe_ctx* ctx, int32_t cols);
 fwe_status fwe_get_tile_cols(const fwe_ctx* ctx, int32_t* cols);
+/* Order in which the two-pass kernels map workgroup ids to tiles: LINEAR
+ * (tile b = workgroup b, so the workgroups in flight cover one contiguous
+ * stretch of every array) or XCD (each of the eight XCDs gets a contiguous
+ * range of tiles, so x/y neighbours share its L2).  Results do not depend on
+ * it; fwe_autotune tries both (and fwe_group_autotune for groups). */
+enum { FWE_ORDER_LINEAR = 0, FWE_ORDER_XCD = 1 };
+fwe_status fwe_set_wg_order(fwe_ctx* ctx, int32_t order);
+fwe_status fwe_get_wg_order(const fwe_ctx* ctx, int32_t* order);
 /* Steps advanced since create/reset. */
 fwe_status fwe_get_step(const fwe_ctx* ctx, int32_t* it);
 /* Set the step counter (restart from saved fields: the reference's unused

@@ -339,12 +339,14 @@ def test_autotune_and_tilings_agree(gpu):
 
     wl = W.ortho_layered(nx=70, ny=30, nz=26, L=4, nt=30, nrec_side=2)
     ref = None
-    for tx, ty, kc in ((64, 4, 8), (64, 2, 3), (64, 8, 26), (64, 4, 5), (64, 4, 1), (64, 8, 2), (128, 4, 8),
-                       (128, 8, 1), (128, 2, 26)):
+    for tx, ty, kc, order in ((64, 4, 8, 0), (64, 2, 3, 1), (64, 8, 26, 0), (64, 4, 5, 1), (64, 4, 1, 0),
+                              (64, 4, 1, 1), (64, 8, 2, 0), (128, 4, 8, 1), (128, 8, 1, 0), (128, 2, 1, 1),
+                              (128, 2, 26, 0)):
         s = W.make_solver(wl)
         s.set_tiling(ty, kc)
         s.set_tile_cols(tx)
-        assert s.tiling() == (ty, kc) and s.tile_cols() == tx
+        s.set_wg_order(order)
+        assert s.tiling() == (ty, kc) and s.tile_cols() == tx and s.wg_order() == order
         s.run(30)
         t = s.traces()
         s.close()
@@ -356,6 +358,8 @@ def test_autotune_and_tilings_agree(gpu):
     assert ms > 0 and s.step() == 0
     s.run(30)
     assert np.array_equal(s.traces(), ref)
+    with pytest.raises(fwe.FweError):
+        s.set_wg_order(2)
     s.close()
     # chunks below three planes are refused with a free surface
     wl.free_surface = 1
