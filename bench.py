@@ -195,6 +195,7 @@ def bench_batched(args, steps):
     s = W.make_solver(wl)
     el_b, ms_a, ms_b, tune = _timed_run(s, steps, args.warmup, not args.no_autotune)
     tiling = (s.tile_cols(),) + tuple(s.tiling())
+    order = "xcd" if s.wg_order() == fwe.ORDER_XCD else "linear"
     kernels, dom, bstep = _kernel_entries(wl, False, tiling, ms_a, ms_b, steps, cells)
     # 1000 lune tensors by linear combination of the six elementary runs
     mts = np.array([W.mt_lune_random(50000 + q, m0)[0] for q in range(1000)])
@@ -243,8 +244,8 @@ def bench_batched(args, steps):
     config = {"workload": f"{wl.name}: layered orthorhombic FP32, 24-cell separable damping, six elementary "
                           f"moment tensors (Mxx..Myz) as one batch of wavefields, {nr} receiver traces each",
               "grid": [wl.nx, wl.ny, wl.nz], "wavefields": 6, "nt": wl.nt, "kernel": "two-pass batched",
-              "tiling": list(tiling), "autotune_ms_per_step": None if tune is None else round(tune, 4),
-              "parallelism": "single GPU"}
+              "tiling": list(tiling), "wg_order": order,
+              "autotune_ms_per_step": None if tune is None else round(tune, 4), "parallelism": "single GPU"}
     return _line(cells * 6 * steps / el_b / 1e9, steps, args.warmup, el_b, config,
                  hbm_pct_step=round(100 * bstep * cells * steps / el_b / 1e9 / PEAK_HBM_GBS, 2),
                  kernels=kernels, roofline=_roofline(dom, steps, cells, wl.name + "x6"),
