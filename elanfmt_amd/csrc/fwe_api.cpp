@@ -1162,7 +1162,7 @@ extern "C" fwe_status fwe_copy_probe(int32_t device, int64_t bytes, int32_t reps
 // ---------------------------------------------------------------------------
 struct fwe_group {
   std::vector<fwe_ctx*> m;
-  int device = 0, mode = 0, pml = 0, fs = 0, ty = 4, order = FWE_ORDER_LINEAR;
+  int device = 0, mode = 0, pml = 0, fs = 0, tx = 64, ty = 4, order = FWE_ORDER_LINEAR;
   hipStream_t stream = nullptr;
   StepArgs* tab = nullptr;
   int32_t* start = nullptr;
@@ -1208,11 +1208,12 @@ extern "C" fwe_status fwe_group_create(int32_t n, fwe_ctx* const* members, fwe_g
   g->m.assign(members, members + n);
   g->device = c0.device; g->mode = c0.mode; g->pml = c0.pml; g->fs = c0.free_surface == FWE_FS_BC1;
   g->ty = members[0]->ty;
+  g->tx = members[0]->tx;
   HIPCHK(hipSetDevice(g->device));
   for (fwe_ctx* x : g->m) {
     HIPCHK(hipStreamSynchronize(x->stream));
-    if (x->ty != g->ty || x->tx != 64) {
-      fwe_status st = apply_tiling(x, 64, g->ty, std::min(x->kchunk, x->cfg.nz));
+    if (x->ty != g->ty || x->tx != g->tx) {
+      fwe_status st = apply_tiling(x, g->tx, g->ty, std::min(x->kchunk, x->cfg.nz));
       if (st) { delete g; return st; }
     }
   }
@@ -1245,7 +1246,7 @@ static fwe_status group_run_impl(fwe_group* g, int32_t nsteps, double* ms) {
   HIPCHK(hipSetDevice(g->device));
   for (fwe_ctx* x : g->m) {
     if (x->it != g->m[0]->it) return fail(FWE_ERR_STATE, "members were advanced separately");
-    if (x->ty != g->ty || x->tx != 64) return fail(FWE_ERR_STATE, "a member was re-tiled outside the group");
+    if (x->ty != g->ty || x->tx != g->tx) return fail(FWE_ERR_STATE, "a member was re-tiled outside the group");
     HIPCHK(hipStreamSynchronize(x->stream));
   }
   fwe_status st = group_table(g);
@@ -1258,9 +1259,9 @@ static fwe_status group_run_impl(fwe_group* g, int32_t nsteps, double* ms) {
   HIPCHK(fwe::launch_advance(g->it_dev, g->m[0]->it, 1, g->stream));
   if (ms) HIPCHK(hipEventRecord(e0, g->stream));
   for (int s = 0; s < nsteps; ++s) {
-    HIPCHK(fwe::launch_group(g->tab, g->start, (int)g->m.size(), g->nblk, g->ty, g->order, g->mode, g->pml, g->fs, false,
+    HIPCHK(fwe::launch_group(g->tab, g->start, (int)g->m.size(), g->nblk, g->tx, g->ty, g->order, g->mode, g->pml, g->fs, false,
                              g->stream));
-    HIPCHK(fwe::launch_group(g->tab, g->start, (int)g->m.size(), g->nblk, g->ty, g->order, g->mode, g->pml, g->fs, true,
+    HIPCHK(fwe::launch_group(g->tab, g->start, (int)g->m.size(), g->nblk, g->tx, g->ty, g->order, g->mode, g->pml, g->fs, true,
                              g->stream));
     HIPCHK(fwe::launch_advance(g->it_dev, 1, 0, g->stream));
   }
@@ -1290,10 +1291,21 @@ extern "C" fwe_status fwe_group_set_tiling(fwe_group* g, int32_t tile_rows, int3
     if (std::min<int>(zchunk, x->cfg.nz) < (x->cfg.free_surface == FWE_FS_BC1 ? 3 : 1))
       return fail(FWE_ERR_ARG, "z chunk too small for member with free surface");
   for (fwe_ctx* x : g->m) {
-    fwe_status st = apply_tiling(x, 64, tile_rows, std::min<int>(zchunk, x->cfg.nz));
+    fwe_status st = apply_tiling(x, g->tx, tile_rows, std::min<int>(zchunk, x->cfg.nz));
     if (st) return st;
   }
   g->ty = tile_rows;
+  return group_table(g);
+}
+
+extern "C" fwe_status fwe_group_set_tile_cols(fwe_group* g, int32_t cols) {
+  if (!g) return fail(FWE_ERR_ARG, "NULL group");
+  if (cols != 64 && cols != 128) return fail(FWE_ERR_ARG, "tile columns must be 64 or 128");
+  for (fwe_ctx* x : g->m) {
+    fwe_status st = apply_tiling(x, cols, g->ty, x->kchunk);
+    if (st) return st;
+  }
+  g->tx = cols;
   return group_table(g);
 }
 
@@ -1307,18 +1319,22 @@ extern "C" fwe_status fwe_group_autotune(fwe_group* g, int32_t trial_steps, doub
   int kmin = g->fs ? 3 : 1;
   for (fwe_ctx* x : g->m)
     if (use_fused(x)) kmin = 3;
+  int btx = g->tx;
   for (int order = FWE_ORDER_LINEAR; order <= FWE_ORDER_XCD; ++order)
-    for (int ty : tys)
-      for (int kc : kcs) {
-        if (kc < kmin) continue;
-        g->order = order;
-        if ((st = fwe_group_set_tiling(g, ty, kc))) return st;
-        double ms = 0;
-        if ((st = group_run_impl(g, 1, &ms))) return st;
-        if ((st = group_run_impl(g, trial_steps, &ms))) return st;
-        if (ms / trial_steps < best) { best = ms / trial_steps; bty = ty; bkc = kc; border = order; }
-      }
+    for (int tx : {64, 128})
+      for (int ty : tys)
+        for (int kc : kcs) {
+          if (kc < kmin) continue;
+          g->order = order;
+          g->tx = tx;
+          if ((st = fwe_group_set_tiling(g, ty, kc))) return st;
+          double ms = 0;
+          if ((st = group_run_impl(g, 1, &ms))) return st;
+          if ((st = group_run_impl(g, trial_steps, &ms))) return st;
+          if (ms / trial_steps < best) { best = ms / trial_steps; btx = tx; bty = ty; bkc = kc; border = order; }
+        }
   g->order = border;
+  g->tx = btx;
   if ((st = fwe_group_set_tiling(g, bty, bkc))) return st;
   for (fwe_ctx* x : g->m)
     if ((st = fwe_reset(x))) return st;

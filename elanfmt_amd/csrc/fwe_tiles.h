@@ -318,25 +318,31 @@ __global__ __launch_bounds__(TX * TY) void k_velocity(const StepArgs a) {
 // workgroups of all models form one grid; start[m] is the first workgroup
 // of model m and tab[m] its step arguments (read from device memory).
 __device__ __forceinline__ int group_model(const int32_t* __restrict__ start, int nm, int lb) {
-  int m = 0;
-  while (m + 1 < nm && lb >= start[m + 1]) ++m;
-  return m;
+  // binary search for the last m with start[m] <= lb: log2(nm) dependent
+  // scalar loads at the start of every workgroup instead of up to nm
+  int lo = 0, hi = nm;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (start[mid] <= lb) lo = mid;
+    else hi = mid;
+  }
+  return lo;
 }
-template <int MODE, int PML, bool FS, int TY>
-__global__ __launch_bounds__(64 * TY) void k_stress_group(const StepArgs* __restrict__ tab,
+template <int MODE, int PML, bool FS, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void k_stress_group(const StepArgs* __restrict__ tab,
                                                          const int32_t* __restrict__ start, int32_t nm,
                                                          int32_t order) {
   const int lb = order ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int m = group_model(start, nm, lb);
-  stress_tile<MODE, PML, FS, 64, TY, 1>(tab[m], lb - start[m]);
+  stress_tile<MODE, PML, FS, TX, TY, 1>(tab[m], lb - start[m]);
 }
-template <int MODE, int PML, int TY>
-__global__ __launch_bounds__(64 * TY) void k_velocity_group(const StepArgs* __restrict__ tab,
+template <int MODE, int PML, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void k_velocity_group(const StepArgs* __restrict__ tab,
                                                            const int32_t* __restrict__ start, int32_t nm,
                                                            int32_t order) {
   const int lb = order ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int m = group_model(start, nm, lb);
-  velocity_tile<MODE, PML, 64, TY, 1>(tab[m], lb - start[m]);
+  velocity_tile<MODE, PML, TX, TY, 1>(tab[m], lb - start[m]);
 }
 
 // ---------------------------------------------------------------------------
@@ -416,22 +422,30 @@ static hipError_t go_vel(const StepArgs& a, hipStream_t s) {
   }
 }
 
-template <int MODE, int PML, bool FS>
-static hipError_t go_group(const StepArgs* tab, const int32_t* start, int nm, int nblk, int ty, int order, bool vel,
-                           hipStream_t s) {
+template <int MODE, int PML, bool FS, int TX>
+static hipError_t go_group2(const StepArgs* tab, const int32_t* start, int nm, int nblk, int ty, int order, bool vel,
+                            hipStream_t s) {
   if (ty == 2) {
-    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, 2>), dim3(nblk), dim3(64, 2), 0, s, tab, start, nm, order);
-    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, 2>), dim3(nblk), dim3(64, 2), 0, s, tab, start, nm, order);
+    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, TX, 2>), dim3(nblk), dim3(TX, 2), 0, s, tab, start, nm, order);
+    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, TX, 2>), dim3(nblk), dim3(TX, 2), 0, s, tab, start, nm, order);
   } else if (ty == 4) {
-    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, 4>), dim3(nblk), dim3(64, 4), 0, s, tab, start, nm, order);
-    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, 4>), dim3(nblk), dim3(64, 4), 0, s, tab, start, nm, order);
+    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, TX, 4>), dim3(nblk), dim3(TX, 4), 0, s, tab, start, nm, order);
+    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, TX, 4>), dim3(nblk), dim3(TX, 4), 0, s, tab, start, nm, order);
   } else if (ty == 8) {
-    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, 8>), dim3(nblk), dim3(64, 8), 0, s, tab, start, nm, order);
-    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, 8>), dim3(nblk), dim3(64, 8), 0, s, tab, start, nm, order);
+    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, TX, 8>), dim3(nblk), dim3(TX, 8), 0, s, tab, start, nm, order);
+    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, TX, 8>), dim3(nblk), dim3(TX, 8), 0, s, tab, start, nm, order);
   } else {
     return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <int MODE, int PML, bool FS>
+static hipError_t go_group(const StepArgs* tab, const int32_t* start, int nm, int nblk, int tx, int ty, int order,
+                           bool vel, hipStream_t s) {
+  if (tx == 64) return go_group2<MODE, PML, FS, 64>(tab, start, nm, nblk, ty, order, vel, s);
+  if (tx == 128) return go_group2<MODE, PML, FS, 128>(tab, start, nm, nblk, ty, order, vel, s);
+  return hipErrorInvalidValue;
 }
 
 template <int MODE, int PML>
@@ -445,10 +459,10 @@ hipError_t velocity_mp(const StepArgs& a, hipStream_t s) {
 }
 
 template <int MODE, int PML>
-hipError_t group_mp(const StepArgs* tab, const int32_t* start, int nm, int nblk, int ty, int order, int fs, bool vel,
-                    hipStream_t s) {
-  return fs ? go_group<MODE, PML, true>(tab, start, nm, nblk, ty, order, vel, s)
-            : go_group<MODE, PML, false>(tab, start, nm, nblk, ty, order, vel, s);
+hipError_t group_mp(const StepArgs* tab, const int32_t* start, int nm, int nblk, int tx, int ty, int order, int fs,
+                    bool vel, hipStream_t s) {
+  return fs ? go_group<MODE, PML, true>(tab, start, nm, nblk, tx, ty, order, vel, s)
+            : go_group<MODE, PML, false>(tab, start, nm, nblk, tx, ty, order, vel, s);
 }
 
 }  // namespace fwe

@@ -127,6 +127,7 @@ SIGNATURES = {
                                     _DP]),
     "fwe_set_learned_boundary": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float]),
     "fwe_set_wg_order": (C.c_int, [C.c_void_p, C.c_int32]),
+    "fwe_group_set_tile_cols": (C.c_int, [C.c_void_p, C.c_int32]),
     "fwe_get_wg_order": (C.c_int, [C.c_void_p, _IP]),
     # generators (fwe_gen.h)
     "fwe_rng_seed": (None, [C.POINTER(RNG), C.c_uint64]),
@@ -448,6 +449,9 @@ class Group:
 
     def set_tiling(self, tile_rows, zchunk):
         _check(lib().fwe_group_set_tiling(self._h, tile_rows, zchunk))
+
+    def set_tile_cols(self, cols):
+        _check(lib().fwe_group_set_tile_cols(self._h, cols))
 
     def autotune(self, trial_steps=3):
         ms = C.c_double()

@@ -299,6 +299,8 @@ fwe_status fwe_group_run(fwe_group* g, int32_t nsteps);
 /* as fwe_group_run; *ms = GPU milliseconds of the nsteps steps */
 fwe_status fwe_group_run_timed(fwe_group* g, int32_t nsteps, double* ms);
 fwe_status fwe_group_set_tiling(fwe_group* g, int32_t tile_rows, int32_t zchunk);
+/* Columns per tile of every member, 64 or 128 (as fwe_set_tile_cols). */
+fwe_status fwe_group_set_tile_cols(fwe_group* g, int32_t cols);
 /* time each tile shape, keep the fastest, reset every member */
 fwe_status fwe_group_autotune(fwe_group* g, int32_t trial_steps, double* best_ms_per_step);
 

@@ -477,6 +477,17 @@ def test_group_equals_separate_runs(gpu):
     g.run(120)
     for s, r in zip(sol, ref):
         assert np.array_equal(s.traces(), r)
+    # 128-column tiles for every member
+    for s in sol:
+        s.reset()
+    g.set_tile_cols(128)
+    g.set_tiling(4, 7)
+    assert all(s.tile_cols() == 128 and s.tiling() == (4, 7) for s in sol)
+    g.run(120)
+    for s, r in zip(sol, ref):
+        assert np.array_equal(s.traces(), r)
+    with pytest.raises(fwe.FweError):
+        g.set_tile_cols(96)
     g.close()
     for s in sol:
         s.close()
