@@ -1404,9 +1404,14 @@ static void lb_cells_host(int nx, int ny, int nz, int L, std::vector<LbCell>& ou
   }
 }
 
+// the stencil gather of the learned boundary addresses a wavefield's nine
+// fields with 32-bit byte offsets
+static bool lb_fits(const fwe_ctx* x) { return (uint64_t)x->nelem * 9 * sizeof(float) < ((uint64_t)1 << 32); }
+
 static LbGeom lb_geom(const fwe_ctx* x, int from, float zinv) {
   LbGeom g;
   for (int f = 0; f < 9; ++f) g.F[f] = x->fbuf[from] + (int64_t)f * x->nelem;
+  g.fstride = x->nelem;
   g.sy = x->sy;
   g.sz = x->sz;
   g.nx = x->cfg.nx;
@@ -1666,6 +1671,8 @@ extern "C" fwe_status fwe_lb_train_step(fwe_lb* lb, fwe_ctx* thick, fwe_ctx* thi
   if (a.nx != b.nx + 2 * offset || a.ny != b.ny + 2 * offset || a.nz != b.nz + 2 * offset)
     return fail(FWE_ERR_ARG, "thick grid must be the thin grid plus %d cells on every face", offset);
   if ((st = ready(thick)) || (st = ready(thin))) return st;
+  if (!lb_fits(thick) || !lb_fits(thin))
+    return fail(FWE_ERR_UNSUPPORTED, "learned boundary: the nine fields of a wavefield exceed 4 GiB");
   HIPCHK(hipSetDevice(lb->device));
   if (lb->gx != b.nx || lb->gy != b.ny || lb->gz != b.nz) {
     std::vector<LbCell> h;
@@ -1744,6 +1751,7 @@ extern "C" fwe_status fwe_set_learned_boundary(fwe_ctx* x, fwe_lb* lb, float imp
   if (!(impedance > 0.f)) return fail(FWE_ERR_ARG, "impedance must be positive");
   if (lb->device != x->cfg.device) return fail(FWE_ERR_ARG, "learned boundary on another device");
   if (use_fused(x)) return fail(FWE_ERR_UNSUPPORTED, "the single-pass kernel has no learned boundary");
+  if (!lb_fits(x)) return fail(FWE_ERR_UNSUPPORTED, "learned boundary: the nine fields of a wavefield exceed 4 GiB");
   std::vector<LbCell> h;
   lb_cells_host(x->cfg.nx, x->cfg.ny, x->cfg.nz, lb->L, h, x->lb_face_off);
   HIPCHK(hipMalloc(&x->lb_cells, sizeof(LbCell) * std::max<size_t>(1, h.size())));

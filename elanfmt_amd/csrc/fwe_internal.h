@@ -151,7 +151,8 @@ struct LbCell {
 
 // one wavefield's field set in the padded layout
 struct LbGeom {
-  const float* F[9];
+  const float* F[9];  // field f at F[0] + f * fstride
+  int64_t fstride;
   int64_t sy, sz;
   int32_t nx, ny, nz;
   float zinv;

@@ -212,31 +212,31 @@ __global__ void k_lb_adam(float* __restrict__ P, float* __restrict__ m, float* _
 // stencil gather from a field set (one wavefield, padded layout)
 // ---------------------------------------------------------------------------
 
-// neighbour q of a cell whose layer face has normal axis ax
-__device__ __forceinline__ void lb_nbr(int q, int ax, int& di, int& dj, int& dk) {
-  di = dj = dk = 0;
-  switch (q) {
-    case 1: di = -1; break;
-    case 2: di = 1; break;
-    case 3: dj = -1; break;
-    case 4: dj = 1; break;
-    case 5: dk = -1; break;
-    case 6: dk = 1; break;
-    case 7: (ax == 0 ? di : ax == 1 ? dj : dk) = -2; break;
-    case 8: (ax == 0 ? di : ax == 1 ? dj : dk) = 2; break;
-    default: break;
-  }
-}
-
-// input e (0..80) of the cell: field e % 9 at neighbour e / 9
-__device__ __forceinline__ float lb_input(const LbGeom& g, const LbCell& c, int e) {
+// The nine stencil cells of a layer cell, q = 0..8: the cell itself; its
+// neighbours at -x, +x, -y, +y, -z, +z; and the cells two steps along the
+// face normal on the negative and positive side.
+//
+// Input e (0..80) of the cell is field e % 9 at stencil cell q = e / 9,
+// computed without branches: e differs between the lanes of a wavefront,
+// so a branch or a switch here would make every gathered load wait for the
+// previous one, whereas with selects all of a lane's loads issue together.
+// Neighbours outside the grid (and inputs with use = false) read element 0
+// and yield 0.
+__device__ __forceinline__ float lb_input(const LbGeom& g, const LbCell& c, int e, bool use = true) {
   const int q = e / 9, f = e - q * 9;
-  int di, dj, dk;
-  lb_nbr(q, (c.fd >> 8) >> 1, di, dj, dk);
-  const int i = c.i + di, j = c.j + dj, k = c.k + dk;
-  if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) return 0.f;
-  const float v = g.F[f][i + j * g.sy + (int64_t)k * g.sz];
-  return f >= 3 ? v * g.zinv : v;
+  const int ax = (c.fd >> 8) >> 1;
+  const int axis = q >= 7 ? ax : (q - 1) >> 1;  // q = 0: -1, no axis
+  const int mag = q >= 7 ? 2 : (q >= 1 ? 1 : 0);
+  const int d = (q & 1) ? -mag : mag;  // odd q: the negative side
+  const int i = c.i + (axis == 0 ? d : 0), j = c.j + (axis == 1 ? d : 0), k = c.k + (axis == 2 ? d : 0);
+  const bool ok = use && i >= 0 && j >= 0 && k >= 0 && i < g.nx && j < g.ny && k < g.nz;
+  // 32-bit byte offset from the field set's base (the set is below 4 GiB,
+  // checked on the host), masked arithmetically: written as a select of the
+  // offset, the compiler turns it back into a branch around the load
+  const uint32_t off = ((uint32_t)f * (uint32_t)g.fstride + (uint32_t)i + (uint32_t)j * (uint32_t)g.sy +
+                        (uint32_t)k * (uint32_t)g.sz) * 4u & (0u - (uint32_t)ok);
+  const float v = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(g.F[0]) + off);
+  return ok ? (f >= 3 ? v * g.zinv : v) : 0.f;
 }
 
 // Training features of n cells: X [n][81] normalised, R [n] = RMS, depth.
@@ -343,9 +343,12 @@ __device__ __forceinline__ float lb_shrink(float v, float d) {
 // (outputs padded 9 -> 16).  Operand layout of that instruction: A[i][k] in
 // lane i%16 + 16*(k/4), element k%4; B[k][n] in lane n + 16*(k/4); the
 // result D[m][n] in lane n + 16*(m/4), element m%4.  Inputs and hidden
-// activations are rounded to bfloat16, products accumulate in FP32; the
-// weights (B operands) stay in registers for the whole launch, and the
-// activations change layout through a per-wave LDS tile.
+// activations are rounded to bfloat16, products accumulate in FP32.  The
+// weights are rounded to bfloat16 once per block into LDS, transposed so
+// that a lane's B operand (four consecutive k of one n) is one 8-byte LDS
+// read; held in registers instead they took 88 VGPRs and left room for one
+// wavefront per SIMD.  The activations change layout through a per-wave LDS
+// tile.
 typedef short bf4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
@@ -357,34 +360,36 @@ __device__ __forceinline__ void wave_sync() {
 
 __device__ __forceinline__ float silu(float a) { return a * sigm(a); }
 
+constexpr int LB_K1 = 104;  // LDS row pitch (bf16) of W1^T: 96 padded inputs + 8
+constexpr int LB_K2 = 72;   // of W2^T and W3^T: 64 + 8
+
 __global__ __launch_bounds__(256) void k_lb_predict_mfma(const float* __restrict__ P, int32_t L, const LbGeom g,
                                                         const LbCell* __restrict__ cells, int32_t n,
                                                         float* __restrict__ D) {
   __shared__ uint16_t hs[LB_WAVES][16][LB_H + 4];
+  __shared__ __align__(16) uint16_t sw1[LB_H][LB_K1];
+  __shared__ __align__(16) uint16_t sw2[LB_H][LB_K2];
+  __shared__ __align__(16) uint16_t sw3[16][LB_K2];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int row = l & 15, grp = l >> 4;
-  // weights as B operands, rounded to bfloat16 once
-  bf4 w1[6][4], w2[4][4], w3[4];
-#pragma unroll
-  for (int s = 0; s < 6; ++s)
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int e = s * 16 + 4 * grp + j;
-        w1[s][t][j] = (short)(e < LB_IN ? to_bf16(P[lb_off_w1() + e * LB_H + t * 16 + row]) : 0);
-      }
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        w2[s][t][j] = (short)to_bf16(P[lb_off_w2(L) + (s * 16 + 4 * grp + j) * LB_H + t * 16 + row]);
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-      w3[s][j] = (short)(row < LB_OUT ? to_bf16(P[lb_off_w3(L) + (s * 16 + 4 * grp + j) * LB_OUT + row]) : 0);
+  // weights rounded to bfloat16, transposed: sw1[n][k] = W1[k][n]
+  for (int q = threadIdx.x; q < LB_H * 96; q += 256) {
+    const int nn = q / 96, k = q - nn * 96;
+    sw1[nn][k] = k < LB_IN ? to_bf16(P[lb_off_w1() + k * LB_H + nn]) : 0;
   }
+  for (int q = threadIdx.x; q < LB_H * LB_H; q += 256) {
+    const int nn = q / LB_H, k = q - nn * LB_H;
+    sw2[nn][k] = to_bf16(P[lb_off_w2(L) + k * LB_H + nn]);
+  }
+  for (int q = threadIdx.x; q < 16 * LB_H; q += 256) {
+    const int nn = q / LB_H, k = q - nn * LB_H;
+    sw3[nn][k] = nn < LB_OUT ? to_bf16(P[lb_off_w3(L) + k * LB_OUT + nn]) : 0;
+  }
+  __syncthreads();
+  // B operand of this lane: k = 16 s + 4 grp + (0..3), n = 16 t + row
+  auto b1op = [&](int s, int t) { return *reinterpret_cast<const bf4*>(&sw1[t * 16 + row][s * 16 + 4 * grp]); };
+  auto b2op = [&](int s, int t) { return *reinterpret_cast<const bf4*>(&sw2[t * 16 + row][s * 16 + 4 * grp]); };
+  auto b3op = [&](int s) { return *reinterpret_cast<const bf4*>(&sw3[row][s * 16 + 4 * grp]); };
   float b2v[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) b2v[t] = P[lb_off_b2(L) + t * 16 + row];
@@ -399,15 +404,21 @@ __global__ __launch_bounds__(256) void k_lb_predict_mfma(const float* __restrict
     const bool va = sa < n;
     const LbCell c = va ? cells[sa] : LbCell{0, 0, 0, 0};
     float v[6][4];
-    float ss = 0.f;
 #pragma unroll
     for (int s = 0; s < 6; ++s)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int e = s * 16 + 4 * grp + j;
-        v[s][j] = (va && e < LB_IN) ? lb_input(g, c, e) : 0.f;
-        ss = fmaf(v[s][j], v[s][j], ss);
+        v[s][j] = lb_input(g, c, e, va && e < LB_IN);
       }
+    // all 24 gathers in flight before the first use (the scheduler would
+    // otherwise place each use right after its load and wait for each)
+    __builtin_amdgcn_sched_barrier(0);
+    float ss = 0.f;
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) ss = fmaf(v[s][j], v[s][j], ss);
     ss += __shfl_xor(ss, 16);
     ss += __shfl_xor(ss, 32);
     const float r = sqrtf(ss * (1.f / LB_IN)) + 1e-30f;
@@ -432,7 +443,7 @@ __global__ __launch_bounds__(256) void k_lb_predict_mfma(const float* __restrict
     for (int t = 0; t < 4; ++t) {
       acc[t] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int s = 0; s < 6; ++s) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[s], w1[s][t], acc[t], 0, 0, 0);
+      for (int s = 0; s < 6; ++s) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[s], b1op(s, t), acc[t], 0, 0, 0);
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t)
@@ -451,7 +462,7 @@ __global__ __launch_bounds__(256) void k_lb_predict_mfma(const float* __restrict
     for (int t = 0; t < 4; ++t) {
       acc[t] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int s = 0; s < 4; ++s) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[s], w2[s][t], acc[t], 0, 0, 0);
+      for (int s = 0; s < 4; ++s) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[s], b2op(s, t), acc[t], 0, 0, 0);
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t)
@@ -466,7 +477,7 @@ __global__ __launch_bounds__(256) void k_lb_predict_mfma(const float* __restrict
     // layer 3: lane holds y[m = 4*grp + j][o = row]
     f4 y = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int s = 0; s < 4; ++s) y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[s], w3[s], y, 0, 0, 0);
+    for (int s = 0; s < 4; ++s) y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[s], b3op(s), y, 0, 0, 0);
     if (row < LB_OUT) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
