@@ -975,6 +975,12 @@ static std::string path_of(const char* dir, const char* name) {
 
 // read exactly n float32 values; FWE_ERR_IO on a missing or short file
 // (the reference would fread(NULL) and crash, OK4.cu:190)
+static bool file_exists(const char* dir, const char* name) {
+  FILE* f = fopen(path_of(dir, name).c_str(), "rb");
+  if (f) fclose(f);
+  return f != nullptr;
+}
+
 static fwe_status read_f32(const char* dir, const char* name, size_t n, std::vector<float>& out, size_t* got) {
   const std::string p = path_of(dir, name);
   FILE* f = fopen(p.c_str(), "rb");
@@ -1037,16 +1043,23 @@ extern "C" fwe_status fwe_load_legacy_inputs(fwe_ctx* x, const char* dir, const 
   } else {
     static const char* cn[9] = {"C11m.dat", "C22m.dat", "C33m.dat", "C12m.dat", "C13m.dat",
                                 "C23m.dat", "C44myz.dat", "C55mxz.dat", "C66mxy.dat"};
+    // the iso-het file set of the station solver (9km.cu:381-400): the same
+    // update with C11=C22=C33 = c11u_het, C12=C13=C23 = c12u_het and the
+    // shear moduli on their staggered edges
+    static const char* hn[9] = {"c11u_het.dat", "c11u_het.dat", "c11u_het.dat", "c12u_het.dat", "c12u_het.dat",
+                                "c12u_het.dat", "c44u_hetyz.dat", "c44u_hetxz.dat", "c44u_hetxy.dat"};
     static const int shp[9] = {FWE_F_SXX, FWE_F_SXX, FWE_F_SXX, FWE_F_SXX, FWE_F_SXX,
                                FWE_F_SXX, FWE_F_SYZ, FWE_F_SXZ, FWE_F_SXY};
     static const char* bn[3] = {"vrho_x_11_hc.dat", "vrho_y_11_hc.dat", "vrho_z_11_hc.dat"};
+    const bool isohet = !file_exists(dir, cn[0]) && file_exists(dir, hn[8]);
+    const char* const* names = isohet ? hn : cn;
     std::vector<std::vector<float>> cv(9), bv(3);
     const float* cp[9];
     const float* bp[3];
     for (int q = 0; q < 9; ++q) {
       int64_t d[3];
       field_shape(c, shp[q], d);
-      if ((st = read_f32(dir, cn[q], (size_t)(d[0] * d[1] * d[2]), cv[q], nullptr))) return st;
+      if ((st = read_f32(dir, names[q], (size_t)(d[0] * d[1] * d[2]), cv[q], nullptr))) return st;
       cp[q] = cv[q].data();
     }
     for (int q = 0; q < 3; ++q) {

@@ -8,9 +8,12 @@
  *   ortho (ANIS.m:378-533): pa1.dat pa2.dat pa3.dat MTS.dat Src.dat
  *                           C11m..C66mxy.dat vrho_{x,y,z}_11_hc.dat and
  *                           weights.dat (pml=weights) or taper_{x,y,z}.dat
+ *   isohet (9km.cu:381-400): as ortho with the medium as c11u_het.dat,
+ *                           c12u_het.dat and c44u_het{xy,xz,yz}.dat (layered
+ *                           isotropic: the ortho layers without anisotropy)
  *
  * Raw float32, x fastest, in the reference's staggered shapes.  Keys:
- *   dir= nx= ny= nz= dx=(50) nt= cfl=(0.5) mode=iso|ortho
+ *   dir= nx= ny= nz= dx=(50) nt= cfl=(0.5) mode=iso|ortho|isohet
  *   iso:   vp=(6000) vs=(3464) rho=(2700) L=(20) R=(1e-8)
  *   fs=(0|1|2): 1 and 2 leave the top undamped; 2 also empties the top cell
  *          layer (zero moduli at k = 0) for the vacuum-layer free surface, as
@@ -29,7 +32,7 @@
 
 static int usage(void) {
   fprintf(stderr,
-          "usage: fwe_gen_inputs nx= ny= nz= nt= [dir=.] [dx=50] [cfl=0.5] [mode=iso|ortho]\n"
+          "usage: fwe_gen_inputs nx= ny= nz= nt= [dir=.] [dx=50] [cfl=0.5] [mode=iso|ortho|isohet]\n"
           "       iso:   [vp=6000 vs=3464 rho=2700 L=20 R=1e-8]\n"
           "       ortho: [seed=1 layers=5 L=24 R=1e-8 pml=taper|weights]\n"
           "       [fs=0|1|2]\n"
@@ -70,7 +73,7 @@ static float* cell_taper(int n, int L, double R, double dt, int top_undamped) {
 }
 
 int main(int argc, char** argv) {
-  int nx = -1, ny = -1, nz = -1, nt = -1, ortho = 0, L = -1, fs = 0, layers = 5, weights3d = 0;
+  int nx = -1, ny = -1, nz = -1, nt = -1, ortho = 0, isohet = 0, L = -1, fs = 0, layers = 5, weights3d = 0;
   double dx = 50.0, cfl = 0.5, vp = 6000.0, vs = 3464.0, rho = 2700.0, R = 1e-8, m0 = 1e15, f0 = 4.0;
   double sdr[3] = {30.0, 60.0, 90.0};
   long long seed = 1, lune = -1;
@@ -87,7 +90,10 @@ int main(int argc, char** argv) {
     else if (KEY("nt")) nt = atoi(v);
     else if (KEY("dx")) dx = atof(v);
     else if (KEY("cfl")) cfl = atof(v);
-    else if (KEY("mode")) ortho = strcmp(v, "ortho") == 0;
+    else if (KEY("mode")) {
+      isohet = strcmp(v, "isohet") == 0;
+      ortho = isohet || strcmp(v, "ortho") == 0;
+    }
     else if (KEY("vp")) vp = atof(v);
     else if (KEY("vs")) vs = atof(v);
     else if (KEY("rho")) rho = atof(v);
@@ -125,7 +131,7 @@ int main(int argc, char** argv) {
     bcell = alloc(nc);
     int32_t* tops = (int32_t*)calloc((size_t)layers, sizeof(int32_t));
     if (!tops || fwe_gen_layered_ortho(nx, ny, nz, layers, (uint64_t)seed, 4000.0, 6500.0, 1.65, 1.85, 2400.0,
-                                       2900.0, 0.15, cij, bcell, tops, &vmax) != 0) {
+                                       2900.0, isohet ? 0.0 : 0.15, cij, bcell, tops, &vmax) != 0) {
       fprintf(stderr, "fwe_gen_inputs: layered medium generation failed\n");
       return 1;
     }
@@ -183,12 +189,16 @@ int main(int argc, char** argv) {
   } else {
     static const char* cn[9] = {"C11m.dat", "C22m.dat", "C33m.dat", "C12m.dat", "C13m.dat",
                                 "C23m.dat", "C44myz.dat", "C55mxz.dat", "C66mxy.dat"};
+    static const char* hn[9] = {"c11u_het.dat", NULL, NULL, "c12u_het.dat", NULL,
+                                NULL, "c44u_hetyz.dat", "c44u_hetxz.dat", "c44u_hetxy.dat"};
+    const char* const* names = isohet ? hn : cn;
     const size_t nyz = (size_t)nx * (ny + 1) * (nz + 1), nxz = (size_t)(nx + 1) * ny * (nz + 1),
                  nxy = (size_t)(nx + 1) * (ny + 1) * nz;
     float *gyz = alloc(nyz), *gxz = alloc(nxz), *gxy = alloc(nxy);
     fwe_gen_harmonic_edges(nx, ny, nz, cij[6], cij[7], cij[8], gyz, gxz, gxy);
-    for (int q = 0; q < 6 && !bad; ++q) bad = put(dir, cn[q], cij[q], nc);
-    bad = bad || put(dir, cn[6], gyz, nyz) || put(dir, cn[7], gxz, nxz) || put(dir, cn[8], gxy, nxy);
+    for (int q = 0; q < 6 && !bad; ++q)
+      if (names[q]) bad = put(dir, names[q], cij[q], nc);
+    bad = bad || put(dir, names[6], gyz, nyz) || put(dir, names[7], gxz, nxz) || put(dir, names[8], gxy, nxy);
     free(gyz); free(gxz); free(gxy);
     const size_t nbx = (size_t)(nx + 1) * ny * nz, nby = (size_t)nx * (ny + 1) * nz, nbz = (size_t)nx * ny * (nz + 1);
     float *bx = alloc(nbx), *by = alloc(nby), *bz = alloc(nbz);
@@ -214,6 +224,6 @@ int main(int argc, char** argv) {
   }
   if (bad) return 1;
   printf("fwe_gen_inputs: %s inputs for %dx%dx%d, dt=%.6g, %d steps, %d damping cells -> %s\n",
-         ortho ? "ortho" : "iso", nx, ny, nz, dt, nt, L, dir);
+         isohet ? "iso-het" : (ortho ? "ortho" : "iso"), nx, ny, nz, dt, nt, L, dir);
   return 0;
 }

@@ -11,7 +11,8 @@
  *   fwe_run NBX=12 NBY=12 NBZ=7 OVERX=0 OVERY=0 OVERZ=0 NPARS1=10 NPARS2=13 NPARS3=12
  *
  * Further keys: nx= ny= nz= (instead of the NB and OVER keys), nt= (default 3000 iso,
- * 4500 ortho as OK4.cu:43 / ANISO.cu:43), mode=iso|ortho, pml=profile|
+ * 4500 ortho as OK4.cu:43 / ANISO.cu:43), mode=iso|ortho|isohet (isohet: the
+ * ortho loop on the iso-het file set of 9km.cu:381-400), pml=profile|
  * weights|taper|none, fs=0|1|2 (none, BC1, vacuum layer of the BC2 variant),
  * xsrc= ysrc= zsrc=, rec=C:i:j:k:name[,..]
  * (C in x|y|z), dir=<path>, device=<n>, nout=<n> (progress every n steps),
@@ -34,7 +35,7 @@
 static int usage(void) {
   fprintf(stderr,
           "usage: fwe_run [NBX=.. NBY=.. NBZ=.. OVERX=.. OVERY=.. OVERZ=..] [nx= ny= nz=] [nt=]\n"
-          "               [mode=iso|ortho] [pml=profile|weights|taper|none] [fs=0|1|2]\n"
+          "               [mode=iso|ortho|isohet] [pml=profile|weights|taper|none] [fs=0|1|2]\n"
           "               [xsrc= ysrc= zsrc=] [rec=x:i:j:k:name,...] [dir=.] [device=0] [nout=200]\n"
           "               [snap=<steps>] [restart=<snapshot>]\n");
   return 2;
@@ -147,7 +148,8 @@ int main(int argc, char** argv) {
     else if (KEY("ny")) ny = atoi(v);
     else if (KEY("nz")) nz = atoi(v);
     else if (KEY("nt")) nt = atoi(v);
-    else if (KEY("mode")) mode = strcmp(v, "ortho") == 0 ? FWE_MODE_ORTHO : FWE_MODE_ISO;
+    else if (KEY("mode"))
+      mode = (strcmp(v, "ortho") == 0 || strcmp(v, "isohet") == 0) ? FWE_MODE_ORTHO : FWE_MODE_ISO;
     else if (KEY("pml")) pml = strcmp(v, "profile") == 0 ? FWE_PML_PROFILE
                             : strcmp(v, "weights") == 0 ? FWE_PML_WEIGHTS3D
                             : strcmp(v, "taper") == 0   ? FWE_PML_SEPARABLE

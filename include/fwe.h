@@ -14,6 +14,7 @@
  *   OK4.cu   = src/FastWaveED_GPU3D_v1_DC_Earthquake_FULL_OK4_final/FastWaveED_GPU3D_v1.cu
  *   ANISO.cu = src/FastWaveED_GPU3D_v1_9km_station_ANISO/FastWaveED_GPU3D_v1_9kmst_ANISO.cu
  *   ANIS.m   = src/FastWaveED_GPU3D_v1_9km_station_ANISO/vizme_FastWaveED_GPU3D_9kmst_ANIS.m
+ *   9km.cu   = src/FastWaveED_GPU3D_v1_9km_station/FastWaveED_GPU3D_v1_9kmst.cu
  *   OK4.m    = src/FastWaveED_GPU3D_v1_DC_Earthquake_FULL_OK4_final/vizme_FastBiotED_GPU3D_OK_2024_final.m
  *
  * Conventions
@@ -273,7 +274,11 @@ fwe_status fwe_read_legacy_params(const char* dir, fwe_legacy_params* out);
 /* Read the medium and damping files of the context's mode from dir:
  * iso: c11u_het.dat c12u_het.dat weights_{x,y,z}.dat (OK4.cu:415-425);
  * ortho: C11m..C23m, C44myz, C55mxz, C66mxy, vrho_{x,y,z}_11_hc.dat and
- * weights.dat (ANISO.cu:411-457); plus Src.dat (nt values, OK4.cu:454).
+ * weights.dat (ANISO.cu:411-457), or, when C11m.dat is absent and
+ * c44u_hetxy.dat present, the iso-het set of the station solver
+ * (9km.cu:381-400: c11u_het, c12u_het, c44u_het{xy,xz,yz}, the same
+ * buoyancy and damping files), run by the ortho law with C11=C22=C33 and
+ * C12=C13=C23; plus Src.dat (nt values, OK4.cu:454).
  * Scalars come from p (c66 = pa2[7], buoyancy = pa3[0..2]). */
 fwe_status fwe_load_legacy_inputs(fwe_ctx* ctx, const char* dir,
                                   const fwe_legacy_params* p);

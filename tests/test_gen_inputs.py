@@ -99,3 +99,22 @@ def test_bad_free_surface_key_is_refused(tmp_path):
     r = subprocess.run([TOOL, f"dir={tmp_path}", "nx=10", "ny=10", "nz=10", "nt=5", "fs=3"], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 2
+
+
+def test_isohet_set_is_isotropic_and_consistent(tmp_path):
+    """mode=isohet writes the station solver's file set (9km.cu:381-400) for
+    a layered isotropic medium: edge shear moduli equal (c11 - c12) / 2 of
+    the cells inside a layer, and no ortho moduli files."""
+    nx, ny, nz = 30, 26, 22
+    _run(tmp_path, f"nx={nx}", f"ny={ny}", f"nz={nz}", "nt=10", "mode=isohet", "L=4", "seed=9")
+    names = {p.name for p in tmp_path.iterdir()}
+    assert {"c11u_het.dat", "c12u_het.dat", "c44u_hetxy.dat", "c44u_hetxz.dat", "c44u_hetyz.dat",
+            "vrho_x_11_hc.dat", "vrho_y_11_hc.dat", "vrho_z_11_hc.dat"} <= names
+    assert not any(n.startswith("C11m") for n in names)
+    c11 = _f32(tmp_path, "c11u_het.dat").reshape(nz, ny, nx)
+    c12 = _f32(tmp_path, "c12u_het.dat").reshape(nz, ny, nx)
+    gxy = _f32(tmp_path, "c44u_hetxy.dat").reshape(nz, ny + 1, nx + 1)
+    mu = (c11.astype(np.float64) - c12) / 2
+    assert np.all(mu > 0) and np.all(c12 > 0)
+    # x-y edges of plane k inside the grid average four cells of the same layer
+    np.testing.assert_allclose(gxy[:, 1:ny, 1:nx], mu[:, 1:, 1:], rtol=2e-6)

@@ -544,6 +544,42 @@ def test_generated_inputs_drive_the_binary(gpu, tmp_path):
     assert tr.shape == (80,) and np.all(np.isfinite(tr)) and np.abs(tr).max() > 0
 
 
+def test_isohet_file_set_runs_as_the_ortho_law(gpu, tmp_path):
+    """The iso-het input set of the station solver (9km.cu:381-400:
+    c11u_het, c12u_het, c44u_het{xy,xz,yz}, staggered buoyancy, damping)
+    gives bit for bit the traces of the ortho file set that spells the same
+    medium out as C11m=C22m=C33m, C12m=C13m=C23m and the three edge moduli."""
+    import os
+    import shutil
+    import subprocess
+
+    from elanfmt_amd import fwe, legacy
+
+    tool = os.path.join(os.path.dirname(fwe.RUN_PATH), "fwe_gen_inputs")
+    a, b = tmp_path / "isohet", tmp_path / "ortho"
+    a.mkdir()
+    r = subprocess.run([tool, f"dir={a}", "nx=56", "ny=40", "nz=36", "nt=70", "mode=isohet", "L=6", "seed=5",
+                        "pml=weights", "lune=2", "f0=12"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    shutil.copytree(a, b)
+    for name in ("c11u_het", "c12u_het", "c44u_hetxy", "c44u_hetxz", "c44u_hetyz"):
+        os.remove(b / f"{name}.dat")
+    for src, dst in (("c11u_het", ("C11m", "C22m", "C33m")), ("c12u_het", ("C12m", "C13m", "C23m")),
+                     ("c44u_hetyz", ("C44myz",)), ("c44u_hetxz", ("C55mxz",)), ("c44u_hetxy", ("C66mxy",))):
+        for d in dst:
+            shutil.copy(a / f"{src}.dat", b / f"{d}.dat")
+    traces = []
+    for d, mode in ((a, "isohet"), (b, "ortho")):
+        r = subprocess.run([fwe.RUN_PATH, "nx=56", "ny=40", "nz=36", "nt=70", f"mode={mode}", "pml=weights", "fs=1",
+                            "xsrc=27", "ysrc=19", "zsrc=17", "rec=x:35:19:2:Rec,y:35:19:2:Rec_x,z:20:10:5:Rec_y",
+                            "nout=0"], cwd=str(d), capture_output=True, text=True, timeout=120)
+        print(mode, r.stdout[-200:], r.stderr[-200:], flush=True)
+        assert r.returncode == 0
+        traces.append(np.stack([legacy.read_res(str(d), n) for n in ("Rec", "Rec_x", "Rec_y")]))
+    assert np.abs(traces[0]).max() > 0
+    assert np.array_equal(traces[0], traces[1])
+
+
 @pytest.mark.parametrize("dims,tile", [((67, 45, 23), (128, 8, 5)), ((50, 19, 17), (64, 2, 1)), ((130, 9, 12), (128, 4, 3))])
 def test_ragged_grids_match_the_oracle(gpu, dims, tile):
     """Sizes that are not multiples of the tile (and narrower than one
