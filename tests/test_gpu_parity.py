@@ -451,6 +451,36 @@ def test_fused_batch_and_graph(gpu):
     s2.close()
 
 
+@pytest.mark.parametrize("fs", [1, 2])
+def test_group_with_a_free_surface_equals_separate_runs(gpu, fs):
+    """Groups with BC1 or the vacuum-layer top (receivers and a shear
+    injection near the top) give the members' separate traces bit for bit."""
+    from elanfmt_amd import fwe, workloads as W
+
+    wls = [W.iso_random(n, seed=q + 5, L=5, nt=60, f0=15.0, nrec_side=2) for q, n in enumerate((34, 40))]
+    for wl in wls:
+        wl.free_surface = fs
+        c = wl.nx // 2
+        wl.injections = wl.injections + [(fwe.SXZ, c, c, 1, 0, 2e14), (fwe.SZZ, c, c, 2, 0, 1e14)]
+        wl.receivers = wl.receivers + [(q, c + 3, c, k, 0) for q in range(3) for k in (1, 2)]
+    ref = []
+    for wl in wls:
+        s = W.make_solver(wl)
+        s.run(wl.nt)
+        ref.append(s.traces())
+        s.close()
+    sol = [W.make_solver(wl) for wl in wls]
+    g = fwe.Group(sol)
+    g.set_tiling(4, 3)
+    g.run(60)
+    for s, r in zip(sol, ref):
+        assert np.abs(r).max() > 0
+        assert np.array_equal(s.traces(), r)
+    g.close()
+    for s in sol:
+        s.close()
+
+
 def test_group_equals_separate_runs(gpu):
     """An uneven group of models advanced by one launch per half-step gives
     every member exactly the traces of its separate run; autotune over the
