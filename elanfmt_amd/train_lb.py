@@ -30,12 +30,18 @@ def main():
     ap.add_argument("--r-thin", type=float, default=1e-3)
     ap.add_argument("--seed0", type=int, default=1000)
     ap.add_argument("--sizes", default="40", help="interior edges drawn per run, comma separated")
+    ap.add_argument("--f0", default="3,8", help="Ricker peak-frequency range (Hz) drawn per run, lo,hi")
+    ap.add_argument("--init", default=None, help="start from the parameters of this .npz instead of a fresh init")
     ap.add_argument("--out", default=os.path.join(ROOT, "elanfmt_amd", "data", f"learned_boundary_L6.npz"))
     a = ap.parse_args()
-    lb = fwe.LearnedBoundary(a.layer)
+    if a.init:
+        lb, _ = learned.load(a.init)
+    else:
+        lb = fwe.LearnedBoundary(a.layer)
     t0 = time.time()
     sizes = [int(v) for v in a.sizes.split(",")]
-    log = learned.train(lb, runs=a.runs, n_int=sizes, lr=a.lr, resync_every=a.resync, R_thin=a.r_thin,
+    f0_range = tuple(float(v) for v in a.f0.split(","))
+    log = learned.train(lb, runs=a.runs, n_int=sizes, lr=a.lr, resync_every=a.resync, R_thin=a.r_thin, f0=f0_range,
                         seed0=a.seed0, log_every=200, progress_every=10)
     t_train = time.time() - t0
     print(f"trained {log.runs} runs, {log.steps} steps in {t_train:.1f} s; loss {log.first_loss:.3e} -> "
@@ -48,7 +54,7 @@ def main():
         evals[f"interior{n_int}"] = learned.evaluate(lb, seed=7, n_int=n_int, nt=nt, f0=f0, R_thin=a.r_thin)
         print(f"held-out medium 7, interior {n_int}: {evals[f'interior{n_int}']}", flush=True)
     learned.save(lb, a.out, runs=a.runs, layer=a.layer, lr=a.lr, resync_every=a.resync, R_thin=a.r_thin,
-                 seed0=a.seed0, sizes=sizes, train_seconds=round(t_train, 1), steps=log.steps,
+                 seed0=a.seed0, sizes=sizes, f0=list(f0_range), train_seconds=round(t_train, 1), steps=log.steps,
                  loss_curve=[(s, float(f"{l:.4e}")) for s, l in log.losses], evaluation=evals)
     print(json.dumps({"out": a.out, "evaluation": evals}), flush=True)
     lb.close()
