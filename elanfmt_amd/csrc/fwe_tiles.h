@@ -14,8 +14,9 @@
 //  * a workgroup owns a TX x TY column tile (TX = 64 or 128: one or two
 //    wavefronts per row) and marches a chunk of z planes,
 //    carrying the z-neighbours it needs in registers;
-//  * workgroup ids are remapped so that neighbouring tiles run on the same
-//    XCD and share its L2 for the x/y halo re-reads;
+//  * the order of workgroups over tiles is tuned: linear (the workgroups in
+//    flight stream one stretch of every array) or grouped per XCD
+//    (neighbouring tiles share an L2);
 //  * the reference's separate source/BC1/record launches are fused into the
 //    two stencil kernels (per-tile injection and receiver lists).
 // The stencil is HBM-bound (~0.8 flop/B), so there is no MFMA here.
