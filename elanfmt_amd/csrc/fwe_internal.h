@@ -171,8 +171,6 @@ hipError_t lb_launch_gather(const LbGeom& g, const LbCell* cells, int n, float* 
                             hipStream_t s);
 hipError_t lb_launch_targets(const LbGeom& thin, const LbGeom& thick, int off, const LbCell* cells, int n,
                              const float* R, float rmin, float wscale, float* T, float* M, hipStream_t s);
-hipError_t lb_launch_predict(const float* P, int L, const LbGeom& g, const LbCell* cells, int n, float* D,
-                             int nblk, hipStream_t s);
 // the same on the matrix cores: inputs and hidden activations in bfloat16
 hipError_t lb_launch_predict_mfma(const float* P, int L, const LbGeom& g, const LbCell* cells, int n, float* D,
                                   hipStream_t s);

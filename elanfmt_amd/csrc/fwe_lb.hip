@@ -293,41 +293,6 @@ __global__ __launch_bounds__(256) void k_lb_targets(const LbGeom thin, const LbG
   }
 }
 
-// Inference, before the step: corrections of n cells of one face into D [n][9].
-__global__ __launch_bounds__(256) void k_lb_predict(const float* __restrict__ P, int32_t L, const LbGeom g,
-                                                   const LbCell* __restrict__ cells, int32_t n,
-                                                   float* __restrict__ D) {
-  __shared__ LbShared sh;
-  lb_stage(sh, P, L, false);
-  __syncthreads();
-  const int w = threadIdx.x >> 6, h = threadIdx.x & 63;
-  const float z = 1.f / g.zinv;
-  for (int64_t base = (int64_t)blockIdx.x * LB_WAVES; base < n; base += (int64_t)gridDim.x * LB_WAVES) {
-    const int64_t s = base + w;
-    const bool valid = s < n;
-    LbCell c{0, 0, 0, 0};
-    float r = 1.f;
-    if (valid) {
-      c = cells[s];
-      const float v0 = lb_input(g, c, h);
-      const float v1 = h + 64 < LB_IN ? lb_input(g, c, h + 64) : 0.f;
-      float ss = fmaf(v0, v0, v1 * v1);
-      for (int o = 32; o >= 1; o >>= 1) ss += __shfl_xor(ss, o);
-      r = sqrtf(ss * (1.f / LB_IN)) + 1e-30f;
-      const float ri = 1.f / r;
-      sh.xs[w][h] = v0 * ri;
-      if (h + 64 < LB_IN) sh.xs[w][h + 64] = v1 * ri;
-    }
-    __syncthreads();
-    float a1, a2;
-    lb_hidden(sh, w, h, c.fd & 0xff, valid, nullptr, nullptr, a1, a2);
-    if (valid && h < LB_OUT) {
-      const float y = lb_output(sh, w, h);
-      D[s * LB_OUT + h] = lb_comp_live(c, h) ? y * r * (h >= 3 ? z : 1.f) : 0.f;
-    }
-    __syncthreads();
-  }
-}
 
 // The dissipative part of a correction: the stepped value v may move toward
 // zero by at most its own magnitude and never grow, so the layer acts as a
@@ -579,12 +544,6 @@ hipError_t lb_launch_targets(const LbGeom& thin, const LbGeom& thick, int off, c
   return hipGetLastError();
 }
 
-hipError_t lb_launch_predict(const float* P, int L, const LbGeom& g, const LbCell* cells, int n, float* D,
-                             int nblk, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_lb_predict, dim3(nblk), dim3(256), 0, s, P, L, g, cells, n, D);
-  return hipGetLastError();
-}
 
 hipError_t lb_launch_predict_mfma(const float* P, int L, const LbGeom& g, const LbCell* cells, int n, float* D,
                                   hipStream_t s) {
