@@ -165,17 +165,9 @@ static void choose_tiling(fwe_ctx* x) {
   x->wg_order = (eo && atoi(eo) == FWE_ORDER_XCD) ? FWE_ORDER_XCD : FWE_ORDER_LINEAR;
   x->ntx = (c.nx + x->tx - 1) / x->tx;
   x->nty = (c.ny + ty - 1) / ty;
-  int kc;
-  if (ekc) {
-    kc = atoi(ekc);
-  } else {
-    // enough workgroups to fill 256 CUs several times over
-    const int64_t cols = (int64_t)x->ntx * x->nty;
-    int64_t nkc = (4096 + cols - 1) / cols;
-    if (nkc < 1) nkc = 1;
-    kc = (int)((c.nz + nkc - 1) / nkc);
-    if (kc < 8) kc = 8;
-  }
+  // one plane per workgroup in linear order streams best on the measured
+  // cases (the autotuner confirms or replaces this)
+  int kc = ekc ? atoi(ekc) : 1;
   const int kmin = c.free_surface == FWE_FS_BC1 ? 3 : 1;  // BC1 needs planes 1 and 2 in one chunk
   if (kc < kmin) kc = kmin;
   if (kc > c.nz) kc = c.nz;

@@ -463,14 +463,20 @@ __global__ __launch_bounds__(256) void k_lb_add(const LbGeom g, const LbCell* __
   if (s >= n) return;
   const LbCell c = cells[s];
   const int64_t a = c.i + c.j * g.sy + (int64_t)c.k * g.sz;
+  // all loads first: the nine fields are distinct arrays, but the compiler
+  // cannot know it and would otherwise wait for each load after each store
+  float v[LB_OUT], d[LB_OUT];
+#pragma unroll
   for (int f = 0; f < LB_OUT; ++f) {
-    float* p = const_cast<float*>(g.F[f]) + a;
-    *p = lb_shrink(*p, D[s * LB_OUT + f]);
+    v[f] = g.F[f][a];
+    d[f] = D[s * LB_OUT + f];
   }
+#pragma unroll
+  for (int f = 0; f < LB_OUT; ++f) const_cast<float*>(g.F[f])[a] = lb_shrink(v[f], d[f]);
 }
 
 // Training rollout: the corrections predicted before the update, from Y
-// [n][9] and the stencil RMS R, applied as k_lb_predict + k_lb_add do.
+// [n][9] and the stencil RMS R, applied as k_lb_predict_mfma + k_lb_add do.
 __global__ __launch_bounds__(256) void k_lb_apply_y(const LbGeom g, const LbCell* __restrict__ cells, int32_t n,
                                                    const float* __restrict__ Y, const float* __restrict__ R) {
   const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
