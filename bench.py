@@ -105,10 +105,11 @@ def _timed_run(solver, steps, warmup, autotune):
     return time.perf_counter() - t0, ms_a, ms_b, tune
 
 
-def _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells):
+def _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells, table=False):
     from elanfmt_amd import workloads as W
 
-    ba, bb = W.alg_bytes(wl.mode, wl.pml, wl.nbatch, fused=fused)
+    ba, bb = W.alg_bytes(wl.mode, wl.pml, wl.nbatch, fused=fused, table=table)
+    km = 2 if table else wl.mode  # kernel law template argument (2: ortho through the material table)
     fs = "true" if wl.free_surface else "false"
     if fused:
         kernels = {"fused": {"ms_per_step": round(ms_a / steps, 5), "alg_bytes_per_cell": ba,
@@ -121,9 +122,9 @@ def _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells):
                      "GBs": round(bb * cells / (ms_b / steps * 1e-3) / 1e9, 1)},
     }
     if ms_a >= ms_b:
-        dom = ("stress", ms_a, ba, f"k_stress<{wl.mode}, {wl.pml}, {fs}, {tiling[0]}, {tiling[1]}, {wl.nbatch}>")
+        dom = ("stress", ms_a, ba, f"k_stress<{km}, {wl.pml}, {fs}, {tiling[0]}, {tiling[1]}, {wl.nbatch}>")
     else:
-        dom = ("velocity", ms_b, bb, f"k_velocity<{wl.mode}, {wl.pml}, {tiling[0]}, {tiling[1]}, {wl.nbatch}>")
+        dom = ("velocity", ms_b, bb, f"k_velocity<{km}, {wl.pml}, {tiling[0]}, {tiling[1]}, {wl.nbatch}>")
     return kernels, dom, ba + bb
 
 
@@ -137,6 +138,17 @@ def _roofline(dom, steps, cells, wname):
     if traffic is not None and prof_k != kname:
         out["traffic_profiled_as"] = prof_k
     return out
+
+
+MEDIUM = {"auto": 0, "arrays": 1, "table": 2}  # fwe.MEDIUM_*
+
+
+def _medium_desc(wl, table, ntab):
+    from elanfmt_amd import fwe
+
+    if wl.mode == fwe.MODE_ISO:
+        return "c11, c12 arrays"
+    return f"material table ({ntab} entries, one index byte per cell)" if table else "twelve streamed arrays"
 
 
 def _line(value, steps, warmup, el, config, **extra):
@@ -157,14 +169,17 @@ def bench_plain(args, steps):
     copy_gbs = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
     s = W.make_solver(wl)
     s.set_kernel(kind)
+    s.set_medium_encoding(MEDIUM[args.medium])
     fused = s.kernel() == fwe.KERNEL_FUSED
     el, ms_a, ms_b, tune = _timed_run(s, steps, args.warmup, not args.no_autotune)
     finite = bool(np.isfinite(s.traces()).all())
     tiling = (s.tile_cols(),) + tuple(s.tiling())
     order = "xcd" if s.wg_order() == fwe.ORDER_XCD else "linear"
+    enc, ntab = s.medium_encoding()
+    table = enc == fwe.MEDIUM_TABLE
     s.close()
     cells = wl.cells
-    kernels, dom, bstep = _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells)
+    kernels, dom, bstep = _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells, table)
     desc = (f"{wl.name}: layered orthorhombic FP32, {wl.info.get('pml_cells')}-cell separable damping, "
             f"lune MT, Ricker {wl.info.get('f0')} Hz, {len(wl.receivers)} receiver traces"
             if wl.mode == fwe.MODE_ORTHO else
@@ -172,6 +187,7 @@ def bench_plain(args, steps):
             f"double couple, Ricker {wl.info.get('f0')} Hz, {len(wl.receivers)} receiver traces")
     config = {"workload": desc, "grid": [wl.nx, wl.ny, wl.nz], "wavefields": wl.nbatch, "nt": wl.nt,
               "kernel": "fused single-pass" if fused else "two-pass", "tiling": list(tiling), "wg_order": order,
+              "medium": _medium_desc(wl, table, ntab),
               "autotune_ms_per_step": None if tune is None else round(tune, 4), "parallelism": "single GPU"}
     return _line(cells * wl.nbatch * steps / el / 1e9, steps, args.warmup, el, config,
                  hbm_pct_step=round(100 * bstep * cells * steps / el / 1e9 / PEAK_HBM_GBS, 2),
@@ -193,10 +209,13 @@ def bench_batched(args, steps):
     cells = wl.cells
     print(f"# batched: {wl.name} x{wl.nbatch}, {steps} steps", file=sys.stderr, flush=True)
     s = W.make_solver(wl)
+    s.set_medium_encoding(MEDIUM[args.medium])
     el_b, ms_a, ms_b, tune = _timed_run(s, steps, args.warmup, not args.no_autotune)
     tiling = (s.tile_cols(),) + tuple(s.tiling())
     order = "xcd" if s.wg_order() == fwe.ORDER_XCD else "linear"
-    kernels, dom, bstep = _kernel_entries(wl, False, tiling, ms_a, ms_b, steps, cells)
+    enc, ntab = s.medium_encoding()
+    table = enc == fwe.MEDIUM_TABLE
+    kernels, dom, bstep = _kernel_entries(wl, False, tiling, ms_a, ms_b, steps, cells, table)
     # 1000 lune tensors by linear combination of the six elementary runs
     mts = np.array([W.mt_lune_random(50000 + q, m0)[0] for q in range(1000)])
     t0 = time.perf_counter()
@@ -217,12 +236,14 @@ def bench_batched(args, steps):
         me[e] = m0
         base.injections = [t for t in fwe.mt_injections(fwe.SRC_LAYOUT_OK4, xs, ys, zs, me) if t[5] != 0]
         p = W.make_solver(base)
+        p.set_medium_encoding(MEDIUM[args.medium])
         if tuned is None:
             p.autotune(3)
-            tuned = (p.tile_cols(),) + tuple(p.tiling())
+            tuned = (p.tile_cols(),) + tuple(p.tiling()) + (p.wg_order(),)
         else:
-            p.set_tiling(*tuned[1:])
+            p.set_tiling(*tuned[1:3])
             p.set_tile_cols(tuned[0])
+            p.set_wg_order(tuned[3])
         el, _, _, _ = _timed_run(p, steps, args.warmup if e == 0 else 0, False)
         el_p += el
         plain_tiling = (p.tile_cols(),) + tuple(p.tiling())
@@ -234,7 +255,7 @@ def bench_batched(args, steps):
     for q in range(10):
         base.injections = fwe.mt_injections(fwe.SRC_LAYOUT_OK4, xs, ys, zs, mts[q])
         d = W.make_solver(base)
-        d.set_tiling(*tuned[1:])
+        d.set_tiling(*tuned[1:3])
         d.set_tile_cols(tuned[0])
         d.run(steps)
         tr = d.traces().astype(np.float64)
@@ -244,7 +265,7 @@ def bench_batched(args, steps):
     config = {"workload": f"{wl.name}: layered orthorhombic FP32, 24-cell separable damping, six elementary "
                           f"moment tensors (Mxx..Myz) as one batch of wavefields, {nr} receiver traces each",
               "grid": [wl.nx, wl.ny, wl.nz], "wavefields": 6, "nt": wl.nt, "kernel": "two-pass batched",
-              "tiling": list(tiling), "wg_order": order,
+              "tiling": list(tiling), "wg_order": order, "medium": _medium_desc(wl, table, ntab),
               "autotune_ms_per_step": None if tune is None else round(tune, 4), "parallelism": "single GPU"}
     return _line(cells * 6 * steps / el_b / 1e9, steps, args.warmup, el_b, config,
                  hbm_pct_step=round(100 * bstep * cells * steps / el_b / 1e9 / PEAK_HBM_GBS, 2),
@@ -401,6 +422,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--medium", default="auto", choices=sorted(MEDIUM),
+                    help="ortho medium: material table when it fits (auto), always the arrays, or the table")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

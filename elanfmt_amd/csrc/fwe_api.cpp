@@ -75,6 +75,12 @@ struct fwe_ctx {
   int32_t* injx_off = nullptr;
   PointEntry* injx = nullptr;
   float* medium = nullptr;  // 12 * nelem (ortho) or 2 * nelem (iso)
+  // ortho material table: mid[c] selects twelve floats at mtab + 12*mid[c]
+  // (ntab entries; 0 = the medium has too many distinct tuples)
+  uint8_t* mid = nullptr;
+  float* mtab = nullptr;
+  int ntab = 0;
+  int medium_enc = FWE_MEDIUM_AUTO;
   float* w3 = nullptr;      // nelem (W3D)
   float* prof = nullptr;    // wx wy wz whx why whz (profile / taper tables)
   float* stf = nullptr;     // nb * nstf
@@ -255,6 +261,7 @@ extern "C" fwe_status fwe_destroy(fwe_ctx* x) {
   hipFree(x->it_dev);
   for (auto ev : x->events) hipEventDestroy(ev);
   hipFree(x->fbuf[0]); hipFree(x->fbuf[1]); hipFree(x->medium); hipFree(x->w3); hipFree(x->prof);
+  hipFree(x->mid); hipFree(x->mtab);
   hipFree(x->lb_cells); hipFree(x->lb_D);
   hipFree(x->stf); hipFree(x->traces); hipFree(x->inj_off); hipFree(x->inj);
   hipFree(x->injx_off); hipFree(x->injx);
@@ -280,6 +287,57 @@ static fwe_status upload(fwe_ctx* x, float* dst, const std::vector<float>& h) {
 // ---------------------------------------------------------------------------
 // medium and damping
 // ---------------------------------------------------------------------------
+// Material table of an ortho medium: the distinct (bitwise) tuples
+// (C[0..8], B[0..2]) over every element of the padded layout, padding
+// included, in order of first appearance, and one byte per element naming
+// its tuple.  Built when there are at most FWE_MEDIUM_TABLE_MAX tuples;
+// otherwise ntab = 0 and the kernels stream the arrays.
+static fwe_status build_material_table(fwe_ctx* x, const std::vector<std::vector<float>>& h) {
+  constexpr int NSLOT = 1024;  // open addressing, > 2 * FWE_MEDIUM_TABLE_MAX
+  const int64_t n = x->nelem;
+  std::vector<uint8_t> idx((size_t)n);
+  std::vector<uint32_t> tab;
+  std::vector<int> slot(NSLOT, -1);
+  uint32_t t[12], prev[12];
+  int pe = -1;
+  bool fits = true;
+  for (int64_t c = 0; c < n && fits; ++c) {
+    for (int q = 0; q < 12; ++q) memcpy(&t[q], &h[q][(size_t)c], 4);
+    if (pe >= 0 && memcmp(t, prev, sizeof(t)) == 0) {
+      idx[(size_t)c] = (uint8_t)pe;
+      continue;
+    }
+    uint64_t hs = 1469598103934665603ull;  // FNV-1a over the 32-bit words
+    for (int q = 0; q < 12; ++q) hs = (hs ^ t[q]) * 1099511628211ull;
+    int sl = (int)(hs & (NSLOT - 1)), e;
+    for (;;) {
+      e = slot[sl];
+      if (e < 0) {
+        e = (int)(tab.size() / 12);
+        if (e >= FWE_MEDIUM_TABLE_MAX) { fits = false; break; }
+        tab.insert(tab.end(), t, t + 12);
+        slot[sl] = e;
+        break;
+      }
+      if (memcmp(&tab[(size_t)e * 12], t, sizeof(t)) == 0) break;
+      sl = (sl + 1) & (NSLOT - 1);
+    }
+    idx[(size_t)c] = (uint8_t)e;
+    memcpy(prev, t, sizeof(t));
+    pe = e;
+  }
+  x->ntab = 0;
+  if (!fits) return FWE_OK;
+  HIPCHK(hipSetDevice(x->cfg.device));
+  if (!x->mid) HIPCHK(hipMalloc(&x->mid, (size_t)n));
+  if (!x->mtab) HIPCHK(hipMalloc(&x->mtab, sizeof(float) * 12 * FWE_MEDIUM_TABLE_MAX));
+  HIPCHK(hipMemcpyAsync(x->mid, idx.data(), (size_t)n, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(x->mtab, tab.data(), sizeof(uint32_t) * tab.size(), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  x->ntab = (int)(tab.size() / 12);
+  return FWE_OK;
+}
+
 extern "C" fwe_status fwe_set_material_iso(fwe_ctx* x, const float* c11, const float* c12, float c66,
                                            float bx, float by, float bz) {
   if (!x || !c11 || !c12) return fail(FWE_ERR_ARG, "NULL argument");
@@ -311,16 +369,17 @@ extern "C" fwe_status fwe_set_material_ortho(fwe_ctx* x, const float* const cij[
   const double dt = x->cfg.dt;
   static const int shape_of[12] = {FWE_F_SXX, FWE_F_SXX, FWE_F_SXX, FWE_F_SXX, FWE_F_SXX, FWE_F_SXX,
                                    FWE_F_SYZ, FWE_F_SXZ, FWE_F_SXY, FWE_F_VX,  FWE_F_VY,  FWE_F_VZ};
-  std::vector<float> h;
+  drop_graph(x);
+  std::vector<std::vector<float>> h(12);
   for (int c = 0; c < 12; ++c) {
     int64_t d[3];
     field_shape(x->cfg, shape_of[c], d);
-    pack(x, c < 9 ? cij[c] : buoy[c - 9], d, dt, h);
-    fwe_status st = upload(x, x->medium + (int64_t)c * x->nelem, h);
+    pack(x, c < 9 ? cij[c] : buoy[c - 9], d, dt, h[c]);
+    fwe_status st = upload(x, x->medium + (int64_t)c * x->nelem, h[c]);
     if (st) return st;
   }
   x->have_medium = true;
-  return FWE_OK;
+  return build_material_table(x, h);
 }
 
 extern "C" fwe_status fwe_set_pml_profiles(fwe_ctx* x, const float* wx, const float* wy, const float* wz) {
@@ -525,6 +584,7 @@ static StepArgs make_args(fwe_ctx* x, int from) {
   if (c.mode == FWE_MODE_ORTHO) {
     for (int q = 0; q < 9; ++q) a.C[q] = x->medium + (int64_t)q * x->nelem;
     for (int q = 0; q < 3; ++q) a.B[q] = x->medium + (int64_t)(9 + q) * x->nelem;
+    if (x->ntab > 0) { a.mid = x->mid; a.mtab = x->mtab; }
   } else {
     a.C[0] = x->medium;
     a.C[1] = x->medium + x->nelem;
@@ -583,8 +643,16 @@ static bool use_fused(const fwe_ctx* x) { return x->kernel == FWE_KERNEL_FUSED; 
 static hipError_t lb_before(fwe_ctx* x, int from);
 static hipError_t lb_after(fwe_ctx* x, int from);
 
+// the two-pass kernels read the ortho medium through the material table
+// when one was built, unless the arrays were asked for
+static bool use_table(const fwe_ctx* x) {
+  return x->cfg.mode == FWE_MODE_ORTHO && x->ntab > 0 && x->medium_enc != FWE_MEDIUM_ARRAYS;
+}
+static int kernel_mode(const fwe_ctx* x) { return use_table(x) ? fwe::MODE_ORTHO_TAB : x->cfg.mode; }
+
 static hipError_t step_launch(fwe_ctx* x, int from, int it, const int32_t* it_base, hipEvent_t mid) {
   const fwe_config& c = x->cfg;
+  const int km = kernel_mode(x);
   hipError_t e = hipSuccess;
   if (x->lb) {
     // learned boundary: corrections from the fields before the step, added
@@ -594,9 +662,9 @@ static hipError_t step_launch(fwe_ctx* x, int from, int it, const int32_t* it_ba
       StepArgs a = make_args(x, from);
       a.it = it;
       a.it_base = it_base;
-      e = fwe::launch_stress(a, c.mode, c.pml, c.free_surface == FWE_FS_BC1, x->stream);
+      e = fwe::launch_stress(a, km, c.pml, c.free_surface == FWE_FS_BC1, x->stream);
       if (e == hipSuccess && mid) e = hipEventRecord(mid, x->stream);
-      if (e == hipSuccess) e = fwe::launch_velocity(a, c.mode, c.pml, x->stream);
+      if (e == hipSuccess) e = fwe::launch_velocity(a, km, c.pml, x->stream);
     }
     if (e == hipSuccess) e = lb_after(x, from);
     return e;
@@ -613,15 +681,18 @@ static hipError_t step_launch(fwe_ctx* x, int from, int it, const int32_t* it_ba
     StepArgs a = make_args(x, from);
     a.it = it;
     a.it_base = it_base;
-    e = fwe::launch_stress(a, c.mode, c.pml, c.free_surface == FWE_FS_BC1, x->stream);
+    e = fwe::launch_stress(a, km, c.pml, c.free_surface == FWE_FS_BC1, x->stream);
     if (e == hipSuccess && mid) e = hipEventRecord(mid, x->stream);
-    if (e == hipSuccess) e = fwe::launch_velocity(a, c.mode, c.pml, x->stream);
+    if (e == hipSuccess) e = fwe::launch_velocity(a, km, c.pml, x->stream);
   }
   return e;
 }
 
 static fwe_status ready(fwe_ctx* x) {
   if (!x->have_medium) return fail(FWE_ERR_STATE, "medium not set");
+  if (x->medium_enc == FWE_MEDIUM_TABLE && !use_table(x))
+    return fail(FWE_ERR_UNSUPPORTED, "material table asked for, but the medium has more than %d distinct tuples",
+                FWE_MEDIUM_TABLE_MAX);
   if (x->cfg.pml != FWE_PML_NONE && !x->have_damp) return fail(FWE_ERR_STATE, "damping not set");
   return FWE_OK;
 }
@@ -792,6 +863,25 @@ extern "C" fwe_status fwe_set_wg_order(fwe_ctx* x, int32_t order) {
 extern "C" fwe_status fwe_get_wg_order(const fwe_ctx* x, int32_t* order) {
   if (!x || !order) return fail(FWE_ERR_ARG, "NULL argument");
   *order = x->wg_order;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_set_medium_encoding(fwe_ctx* x, int32_t encoding) {
+  if (!x) return fail(FWE_ERR_ARG, "NULL ctx");
+  if (encoding < FWE_MEDIUM_AUTO || encoding > FWE_MEDIUM_TABLE) return fail(FWE_ERR_ARG, "bad medium encoding %d", encoding);
+  if (encoding == FWE_MEDIUM_TABLE && x->cfg.mode != FWE_MODE_ORTHO)
+    return fail(FWE_ERR_UNSUPPORTED, "the material table is for the ortho law");
+  if (encoding == FWE_MEDIUM_TABLE && x->have_medium && x->ntab == 0)
+    return fail(FWE_ERR_UNSUPPORTED, "the medium has more than %d distinct tuples", FWE_MEDIUM_TABLE_MAX);
+  drop_graph(x);
+  x->medium_enc = encoding;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_get_medium_encoding(const fwe_ctx* x, int32_t* encoding, int32_t* n_materials) {
+  if (!x || !encoding) return fail(FWE_ERR_ARG, "NULL argument");
+  *encoding = use_table(x) && !use_fused(x) ? FWE_MEDIUM_TABLE : FWE_MEDIUM_ARRAYS;
+  if (n_materials) *n_materials = x->ntab;
   return FWE_OK;
 }
 

@@ -15,7 +15,9 @@ namespace fwe {
 // workgroup marches a chunk of z planes.
 constexpr int TILE_X = 64;
 
-enum Mode { MODE_ISO = 0, MODE_ORTHO = 1 };
+// MODE_ORTHO_TAB: the ortho law with the medium read through a material
+// table (one byte per cell, see StepArgs::mid)
+enum Mode { MODE_ISO = 0, MODE_ORTHO = 1, MODE_ORTHO_TAB = 2 };
 enum Pml { PML_NONE = 0, PML_PROFILE = 1, PML_W3D = 2, PML_SEP = 3 };
 
 // One injection or receiver, sorted by xy tile on the host.
@@ -40,6 +42,10 @@ struct StepArgs {
   const float* C[9];
   // ortho buoyancy pre-multiplied by dt: bx by bz (staggered)
   const float* B[3];
+  // the same twelve values as a table (MODE_ORTHO_TAB): cell c uses entry
+  // mid[c], twelve floats at mtab + 12*mid[c] in the order C[0..8], B[0..2]
+  const uint8_t* mid;
+  const float* mtab;
   // iso medium: c11*dt, c12*dt arrays, scalars c66*dt and b*dt
   float c66dt, bxdt, bydt, bzdt;
   // vacuum-layer top: on plane vac_k every shear modulus*dt is vac_cdt

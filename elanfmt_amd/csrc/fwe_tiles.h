@@ -107,10 +107,22 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb) {
     if (PML == PML_SEP) w = wxi * wyj * wzk;
     // medium at this cell, shared by all wavefields of the batch
     float m0, m1, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f, m6 = 0.f, m7 = 0.f, m8 = 0.f;
-    m0 = ld(a.C[0] + c); m1 = ld(a.C[1] + c);
-    if (MODE == MODE_ORTHO) {
-      m2 = ld(a.C[2] + c); m3 = ld(a.C[3] + c); m4 = ld(a.C[4] + c); m5 = ld(a.C[5] + c);
-      m6 = ld(a.C[6] + c); m7 = ld(a.C[7] + c); m8 = ld(a.C[8] + c);
+    if (MODE == MODE_ORTHO_TAB) {
+      // one byte per cell selects the cell's twelve values in the table
+      // (48 B per entry, 16-B aligned; a layered medium keeps the entries in
+      // the caches)
+      const float* t = a.mtab + 12u * (uint32_t)a.mid[c];
+      const float4 q0 = *reinterpret_cast<const float4*>(t);
+      const float4 q1 = *reinterpret_cast<const float4*>(t + 4);
+      m0 = q0.x; m1 = q0.y; m2 = q0.z; m3 = q0.w;
+      m4 = q1.x; m5 = q1.y; m6 = q1.z; m7 = q1.w;
+      m8 = t[8];
+    } else {
+      m0 = ld(a.C[0] + c); m1 = ld(a.C[1] + c);
+      if (MODE == MODE_ORTHO) {
+        m2 = ld(a.C[2] + c); m3 = ld(a.C[3] + c); m4 = ld(a.C[4] + c); m5 = ld(a.C[5] + c);
+        m6 = ld(a.C[6] + c); m7 = ld(a.C[7] + c); m8 = ld(a.C[8] + c);
+      }
     }
     // vacuum-layer top (diff_BC2_OK3/FastWaveED_GPU3D_v1.cu:309-314): the
     // shear moduli of plane vac_k are replaced (uniform per plane)
@@ -259,6 +271,11 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb) {
     if (MODE == MODE_ORTHO) {
       // buoyancy through the caches (measured faster than streaming here)
       bx = a.B[0][c]; by = a.B[1][c]; bz = a.B[2][c];
+    }
+    if (MODE == MODE_ORTHO_TAB) {
+      // entry words 8..11: C66xy, bx, by, bz
+      const float4 q = *reinterpret_cast<const float4*>(a.mtab + 12u * (uint32_t)a.mid[c] + 8);
+      bx = q.y; by = q.z; bz = q.w;
     }
     float wzm = 1.f;
     if (PML == PML_PROFILE) wzm = k >= 1 ? a.wz[k - 1] : 1.f;

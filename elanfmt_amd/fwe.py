@@ -19,6 +19,7 @@ RUN_PATH = os.path.join(_HERE, "fwe_run")
 OK, ERR_ARG, ERR_IO, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_UNSUPPORTED, ERR_NONFINITE = range(8)
 KERNEL_AUTO, KERNEL_TWO_PASS, KERNEL_FUSED = range(3)
 ORDER_LINEAR, ORDER_XCD = 0, 1
+MEDIUM_AUTO, MEDIUM_ARRAYS, MEDIUM_TABLE = 0, 1, 2
 FS_NONE, FS_BC1, FS_VACUUM = 0, 1, 2
 MODE_ISO, MODE_ORTHO = 0, 1
 PML_NONE, PML_PROFILE, PML_WEIGHTS3D, PML_SEPARABLE = 0, 1, 2, 3
@@ -129,6 +130,8 @@ SIGNATURES = {
     "fwe_set_wg_order": (C.c_int, [C.c_void_p, C.c_int32]),
     "fwe_group_set_tile_cols": (C.c_int, [C.c_void_p, C.c_int32]),
     "fwe_get_wg_order": (C.c_int, [C.c_void_p, _IP]),
+    "fwe_set_medium_encoding": (C.c_int, [C.c_void_p, C.c_int32]),
+    "fwe_get_medium_encoding": (C.c_int, [C.c_void_p, _IP, _IP]),
     # generators (fwe_gen.h)
     "fwe_rng_seed": (None, [C.POINTER(RNG), C.c_uint64]),
     "fwe_rng_next": (C.c_uint64, [C.POINTER(RNG)]),
@@ -359,6 +362,17 @@ class Solver:
         c = C.c_int32()
         _check(lib().fwe_get_wg_order(self._h, C.byref(c)))
         return c.value
+
+    def set_medium_encoding(self, enc):
+        """MEDIUM_AUTO (material table when the medium has one), MEDIUM_ARRAYS
+        or MEDIUM_TABLE (fails when the medium has too many distinct tuples)."""
+        _check(lib().fwe_set_medium_encoding(self._h, enc))
+
+    def medium_encoding(self):
+        """(encoding fwe_run uses: MEDIUM_ARRAYS or MEDIUM_TABLE, table entries)"""
+        e, n = C.c_int32(), C.c_int32()
+        _check(lib().fwe_get_medium_encoding(self._h, C.byref(e), C.byref(n)))
+        return e.value, n.value
 
     def autotune(self, trial_steps=5):
         ms = C.c_double()

@@ -270,10 +270,15 @@ def config(idx, **kw):
 
 # algorithmic bytes per cell-update of each kernel formulation: every
 # streamed 3-D array counted once per pass (read and/or write), FP32.
-def alg_bytes(mode, pml, nbatch=1, fused=False):
+def alg_bytes(mode, pml, nbatch=1, fused=False, table=False):
     """(stress_bytes, velocity_bytes) per cell per step, all wavefields; for
-    the single-pass kernel the whole step is in the first entry."""
+    the single-pass kernel the whole step is in the first entry.  table:
+    the ortho medium read through the material table, one index byte per
+    cell and pass (the table entries stay in the caches)."""
     w = 1 if pml == fwe.PML_WEIGHTS3D else 0
+    if table and not fused:
+        # stress: R V3 S6 [w] + index byte, W S6 ; velocity: R S6 V3 [w] + index byte, W V3
+        return 4 * (nbatch * 15 + w) + 1, 4 * (nbatch * 12 + w) + 1
     if fused:
         # fields R9 W9 per wavefield; medium once per launch (one launch per
         # wavefield: iso c11 c12; ortho C9 b3 [w])

@@ -244,6 +244,21 @@ fwe_status fwe_get_tile_cols(const fwe_ctx* ctx, int32_t* cols);
 enum { FWE_ORDER_LINEAR = 0, FWE_ORDER_XCD = 1 };
 fwe_status fwe_set_wg_order(fwe_ctx* ctx, int32_t order);
 fwe_status fwe_get_wg_order(const fwe_ctx* ctx, int32_t* order);
+/* How the two-pass kernels read the ortho medium (9 Cij + 3 buoyancy
+ * values per cell; the reference streams all twelve arrays,
+ * ANISO.cu:306-352,374-386).  TABLE: one byte per cell indexes a table of
+ * the distinct 12-value tuples, read through the caches, so a cell update
+ * streams 1 byte of medium instead of 48 (the same values, so the same
+ * results bit for bit).  fwe_set_material_ortho builds the table when the
+ * medium has at most FWE_MEDIUM_TABLE_MAX distinct tuples (layered media:
+ * a few tens).  AUTO (default) uses the table whenever it was built; ARRAYS
+ * always streams the arrays.  Iso contexts, groups and the single-pass
+ * kernel use the arrays.  get returns the encoding fwe_run will use and the
+ * number of table entries (0 without a table). */
+enum { FWE_MEDIUM_AUTO = 0, FWE_MEDIUM_ARRAYS = 1, FWE_MEDIUM_TABLE = 2 };
+#define FWE_MEDIUM_TABLE_MAX 256
+fwe_status fwe_set_medium_encoding(fwe_ctx* ctx, int32_t encoding);
+fwe_status fwe_get_medium_encoding(const fwe_ctx* ctx, int32_t* encoding, int32_t* n_materials);
 /* Steps advanced since create/reset. */
 fwe_status fwe_get_step(const fwe_ctx* ctx, int32_t* it);
 /* Set the step counter (restart from saved fields: the reference's unused
