@@ -105,10 +105,11 @@ def _timed_run(solver, steps, warmup, autotune):
     return time.perf_counter() - t0, ms_a, ms_b, tune
 
 
-def _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells, table=False):
+def _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells, table=False, batch_tile=None):
     from elanfmt_amd import workloads as W
 
-    ba, bb = W.alg_bytes(wl.mode, wl.pml, wl.nbatch, fused=fused, table=table)
+    ba, bb = W.alg_bytes(wl.mode, wl.pml, wl.nbatch, fused=fused, table=table, batch_tile=batch_tile)
+    bt = batch_tile or wl.nbatch
     km = 2 if table else wl.mode  # kernel law template argument (2: ortho through the material table)
     fs = "true" if wl.free_surface else "false"
     if fused:
@@ -122,9 +123,9 @@ def _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells, table=False):
                      "GBs": round(bb * cells / (ms_b / steps * 1e-3) / 1e9, 1)},
     }
     if ms_a >= ms_b:
-        dom = ("stress", ms_a, ba, f"k_stress<{km}, {wl.pml}, {fs}, {tiling[0]}, {tiling[1]}, {wl.nbatch}>")
+        dom = ("stress", ms_a, ba, f"k_stress<{km}, {wl.pml}, {fs}, {tiling[0]}, {tiling[1]}, {bt}>")
     else:
-        dom = ("velocity", ms_b, bb, f"k_velocity<{km}, {wl.pml}, {tiling[0]}, {tiling[1]}, {wl.nbatch}>")
+        dom = ("velocity", ms_b, bb, f"k_velocity<{km}, {wl.pml}, {tiling[0]}, {tiling[1]}, {bt}>")
     return kernels, dom, ba + bb
 
 
@@ -141,6 +142,7 @@ def _roofline(dom, steps, cells, wname):
 
 
 MEDIUM = {"auto": 0, "arrays": 1, "table": 2}  # fwe.MEDIUM_*
+ORDER_NAMES = ("linear", "xcd", "plane split")  # fwe.ORDER_*
 
 
 def _medium_desc(wl, table, ntab):
@@ -174,7 +176,7 @@ def bench_plain(args, steps):
     el, ms_a, ms_b, tune = _timed_run(s, steps, args.warmup, not args.no_autotune)
     finite = bool(np.isfinite(s.traces()).all())
     tiling = (s.tile_cols(),) + tuple(s.tiling())
-    order = "xcd" if s.wg_order() == fwe.ORDER_XCD else "linear"
+    order = ORDER_NAMES[s.wg_order()]
     enc, ntab = s.medium_encoding()
     table = enc == fwe.MEDIUM_TABLE
     s.close()
@@ -212,10 +214,11 @@ def bench_batched(args, steps):
     s.set_medium_encoding(MEDIUM[args.medium])
     el_b, ms_a, ms_b, tune = _timed_run(s, steps, args.warmup, not args.no_autotune)
     tiling = (s.tile_cols(),) + tuple(s.tiling())
-    order = "xcd" if s.wg_order() == fwe.ORDER_XCD else "linear"
+    order = ORDER_NAMES[s.wg_order()]
     enc, ntab = s.medium_encoding()
     table = enc == fwe.MEDIUM_TABLE
-    kernels, dom, bstep = _kernel_entries(wl, False, tiling, ms_a, ms_b, steps, cells, table)
+    bt = s.batch_tile()
+    kernels, dom, bstep = _kernel_entries(wl, False, tiling, ms_a, ms_b, steps, cells, table, bt)
     # 1000 lune tensors by linear combination of the six elementary runs
     mts = np.array([W.mt_lune_random(50000 + q, m0)[0] for q in range(1000)])
     t0 = time.perf_counter()
@@ -266,6 +269,7 @@ def bench_batched(args, steps):
                           f"moment tensors (Mxx..Myz) as one batch of wavefields, {nr} receiver traces each",
               "grid": [wl.nx, wl.ny, wl.nz], "wavefields": 6, "nt": wl.nt, "kernel": "two-pass batched",
               "tiling": list(tiling), "wg_order": order, "medium": _medium_desc(wl, table, ntab),
+              "wavefields_per_workgroup": bt,
               "autotune_ms_per_step": None if tune is None else round(tune, 4), "parallelism": "single GPU"}
     return _line(cells * 6 * steps / el_b / 1e9, steps, args.warmup, el_b, config,
                  hbm_pct_step=round(100 * bstep * cells * steps / el_b / 1e9 / PEAK_HBM_GBS, 2),

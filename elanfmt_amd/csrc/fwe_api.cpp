@@ -81,6 +81,8 @@ struct fwe_ctx {
   float* mtab = nullptr;
   int ntab = 0;
   int medium_enc = FWE_MEDIUM_AUTO;
+  // wavefields per workgroup of a batch (a divisor of nbatch; 0 = default)
+  int batch_tile = 0;
   float* w3 = nullptr;      // nelem (W3D)
   float* prof = nullptr;    // wx wy wz whx why whz (profile / taper tables)
   float* stf = nullptr;     // nb * nstf
@@ -168,7 +170,8 @@ static void choose_tiling(fwe_ctx* x) {
   const char* etx = getenv("FWE_TILE_X");
   x->tx = (etx && atoi(etx) == 128) ? 128 : 64;
   const char* eo = getenv("FWE_WG_ORDER");
-  x->wg_order = (eo && atoi(eo) == FWE_ORDER_XCD) ? FWE_ORDER_XCD : FWE_ORDER_LINEAR;
+  x->wg_order = eo ? atoi(eo) : FWE_ORDER_LINEAR;
+  if (x->wg_order < FWE_ORDER_LINEAR || x->wg_order > FWE_ORDER_PLANE_SPLIT) x->wg_order = FWE_ORDER_LINEAR;
   x->ntx = (c.nx + x->tx - 1) / x->tx;
   x->nty = (c.ny + ty - 1) / ty;
   // one plane per workgroup in linear order streams best on the measured
@@ -571,14 +574,28 @@ extern "C" fwe_status fwe_mt_injections(int32_t layout, int32_t xs, int32_t ys, 
 // ---------------------------------------------------------------------------
 // time loop
 // ---------------------------------------------------------------------------
+static bool use_table(const fwe_ctx* x);
+
+// Wavefields each workgroup advances together.  Default: all of them with
+// the medium arrays (one medium load per cell serves the batch), one with
+// the material table (its medium costs a byte per cell, and single-field
+// workgroups stream at the plain kernel's rate).
+static int batch_tile(const fwe_ctx* x) {
+  if (x->batch_tile > 0) return x->batch_tile;
+  return use_table(x) ? 1 : x->cfg.nbatch;
+}
+
 static StepArgs make_args(fwe_ctx* x, int from) {
   const fwe_config& c = x->cfg;
   StepArgs a;
   memset(&a, 0, sizeof(a));
-  a.nx = c.nx; a.ny = c.ny; a.nz = c.nz; a.nb = c.nbatch;
+  a.nx = c.nx; a.ny = c.ny; a.nz = c.nz;
+  a.nb = batch_tile(x);
+  a.nbg = c.nbatch / a.nb;
   a.sy = x->sy; a.sz = x->sz; a.fstride = 9 * x->nelem;
   a.tile_x = x->tx;
-  a.xcd_order = x->wg_order == FWE_ORDER_XCD;
+  a.xcd_order = x->wg_order;
+  a.split = (x->ntx * x->nty + 7) / 8;
   for (int q = 0; q < 3; ++q) a.V[q] = x->fbuf[from] + (int64_t)q * x->nelem;
   for (int q = 0; q < 6; ++q) a.S[q] = x->fbuf[from] + (int64_t)(3 + q) * x->nelem;
   if (c.mode == FWE_MODE_ORTHO) {
@@ -854,7 +871,7 @@ extern "C" fwe_status fwe_set_tile_cols(fwe_ctx* x, int32_t cols) {
 
 extern "C" fwe_status fwe_set_wg_order(fwe_ctx* x, int32_t order) {
   if (!x) return fail(FWE_ERR_ARG, "NULL ctx");
-  if (order != FWE_ORDER_LINEAR && order != FWE_ORDER_XCD) return fail(FWE_ERR_ARG, "bad workgroup order %d", order);
+  if (order < FWE_ORDER_LINEAR || order > FWE_ORDER_PLANE_SPLIT) return fail(FWE_ERR_ARG, "bad workgroup order %d", order);
   drop_graph(x);
   x->wg_order = order;
   return FWE_OK;
@@ -863,6 +880,21 @@ extern "C" fwe_status fwe_set_wg_order(fwe_ctx* x, int32_t order) {
 extern "C" fwe_status fwe_get_wg_order(const fwe_ctx* x, int32_t* order) {
   if (!x || !order) return fail(FWE_ERR_ARG, "NULL argument");
   *order = x->wg_order;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_set_batch_tile(fwe_ctx* x, int32_t wavefields) {
+  if (!x) return fail(FWE_ERR_ARG, "NULL ctx");
+  if (wavefields < 0 || wavefields > x->cfg.nbatch || (wavefields > 0 && x->cfg.nbatch % wavefields))
+    return fail(FWE_ERR_ARG, "wavefields per workgroup must divide the batch of %d (0: default)", x->cfg.nbatch);
+  drop_graph(x);
+  x->batch_tile = wavefields;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_get_batch_tile(const fwe_ctx* x, int32_t* wavefields) {
+  if (!x || !wavefields) return fail(FWE_ERR_ARG, "NULL argument");
+  *wavefields = batch_tile(x);
   return FWE_OK;
 }
 
@@ -904,14 +936,20 @@ extern "C" fwe_status fwe_autotune(fwe_ctx* x, int32_t trial_steps, double* best
   if (st) return st;
   const int nz = x->cfg.nz;
   int best_tx = x->tx, best_ty = x->ty, best_kc = x->kchunk, best_order = x->wg_order;
+  int best_bt = x->batch_tile;
   double best = 1e30;
+  // batch layouts: all wavefields per workgroup, or one
+  std::vector<int> bts(1, x->cfg.nbatch);
+  if (x->cfg.nbatch > 1 && !use_fused(x)) bts.push_back(1);
   static const int txs[2] = {64, 128};
   static const int tys[3] = {2, 4, 8};
   static const int kcs[10] = {1, 2, 4, 8, 12, 16, 24, 32, 48, 64};
   const int kmin = (x->cfg.free_surface == FWE_FS_BC1 || use_fused(x)) ? 3 : 1;
   drop_graph(x);
-  for (int order = FWE_ORDER_LINEAR; order <= FWE_ORDER_XCD; ++order) {
+  for (int bt : bts)
+  for (int order = FWE_ORDER_LINEAR; order <= FWE_ORDER_PLANE_SPLIT; ++order) {
     x->wg_order = order;
+    x->batch_tile = bt;
     for (int tx : txs) {
       if (tx != 64 && use_fused(x)) continue;
       for (int ty : tys) {
@@ -922,12 +960,13 @@ extern "C" fwe_status fwe_autotune(fwe_ctx* x, int32_t trial_steps, double* best
           double a = 0, b = 0, w = 0;
           if ((st = run_impl(x, trial_steps, &a, &b, &w))) return st;
           const double per = (a + b) / trial_steps;
-          if (per < best) { best = per; best_tx = tx; best_ty = ty; best_kc = kc; best_order = order; }
+          if (per < best) { best = per; best_tx = tx; best_ty = ty; best_kc = kc; best_order = order; best_bt = bt; }
         }
       }
     }
   }
   x->wg_order = best_order;
+  x->batch_tile = best_bt;
   if ((st = apply_tiling(x, best_tx, best_ty, best_kc))) return st;
   if (best_ms_per_step) *best_ms_per_step = best;
   return fwe_reset(x);

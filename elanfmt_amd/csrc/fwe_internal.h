@@ -34,7 +34,11 @@ struct PointEntry {
 // All 3-D arrays share one padded layout: element (i,j,k) at
 // i + j*sy + k*sz with sy >= nx+1 (rounded to 32 floats) and sz = sy*(ny+1).
 struct StepArgs {
-  int32_t nx, ny, nz, nb;
+  int32_t nx, ny, nz;
+  // wavefields: each workgroup advances nb of them together (the template
+  // NB: one medium load per cell for all nb); a launch covers nbg such
+  // groups side by side (nb * nbg wavefields; the group kernels use 1, 1)
+  int32_t nb, nbg;
   int64_t sy, sz, fstride;
   float* V[3];        // Vx Vy Vz (batch 0)
   float* S[6];        // Sxx Syy Szz Sxy Sxz Syz (batch 0)
@@ -61,7 +65,9 @@ struct StepArgs {
   float dtdx, dtdy, dtdz; // dt/dx ... (iso stress)
   // tiling
   int32_t tile_x;       // columns per tile (64 or 128)
-  int32_t xcd_order;    // 1: workgroup ids grouped per XCD (xcd_remap); 0: linear
+  int32_t xcd_order;    // workgroup order: 0 linear, 1 grouped per XCD (xcd_remap),
+                        // 2 plane split (plane_split)
+  int32_t split;        // plane split: tiles per XCD part of a chunk, ceil(ntx*nty/8)
   int32_t ty_rows;      // TILE_Y
   int32_t ntx, nty, nkc, kchunk;
   // sources

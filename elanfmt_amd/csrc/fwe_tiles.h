@@ -15,8 +15,10 @@
 //    wavefronts per row) and marches a chunk of z planes,
 //    carrying the z-neighbours it needs in registers;
 //  * the order of workgroups over tiles is tuned: linear (the workgroups in
-//    flight stream one stretch of every array) or grouped per XCD
-//    (neighbouring tiles share an L2);
+//    flight stream one stretch of every array), grouped per XCD
+//    (neighbouring tiles share an L2) or plane split (both: every XCD owns
+//    one eighth of each chunk of planes, and all eight walk the chunks
+//    together);
 //  * the reference's separate source/BC1/record launches are fused into the
 //    two stencil kernels (per-tile injection and receiver lists).
 // The stencil is HBM-bound (~0.8 flop/B), so there is no MFMA here.
@@ -37,6 +39,41 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
   const int q = nb >> 3, r = nb & 7;
   const int x = b & 7, w = b >> 3;
   return x * q + (x < r ? x : r) + w;
+}
+
+// Plane split: the P = ntx*nty tiles of one z chunk, padded to 8*s, are
+// cut into eight contiguous runs of s tiles (x fastest, then y); XCD x
+// takes run x of every chunk, chunk after chunk.  A tile's x and y
+// neighbours (mostly) and its z neighbour (always) are then processed on
+// the same XCD shortly before or after it, so its halo lines hit that L2,
+// while the workgroups in flight still cover a few contiguous planes of
+// every array.  Padding slots return -1.
+__device__ __forceinline__ int plane_split(int b, int P, int s) {
+  const int x = b & 7, q = b >> 3;
+  const int kc = q / s;
+  const int t = x * s + (q - kc * s);
+  return t < P ? t + kc * P : -1;
+}
+
+__device__ __forceinline__ int tile_of(const StepArgs& a, int b, int nb) {
+  if (a.xcd_order == 1) return xcd_remap(b, nb);
+  if (a.xcd_order == 2) return plane_split(b, a.ntx * a.nty, a.split);
+  return b;
+}
+
+// workgroups of one group of NB wavefields (plane split pads every chunk to
+// 8*split tiles); a launch covers nbg such groups, one after the other
+__host__ __device__ inline int launch_blocks(const StepArgs& a) {
+  return a.xcd_order == 2 ? a.nkc * 8 * a.split : a.ntx * a.nty * a.nkc;
+}
+
+// workgroup -> (wavefield group g, tile lb or -1 for a padding slot).  Each
+// group's range is a multiple of 8 workgroups under plane split, so its
+// workgroups keep their XCD assignment.
+__device__ __forceinline__ void split_batch(const StepArgs& a, int& g, int& lb) {
+  const int per = launch_blocks(a);
+  g = a.nbg > 1 ? (int)blockIdx.x / per : 0;
+  lb = tile_of(a, (int)blockIdx.x - g * per, per);
 }
 
 __device__ __forceinline__ float ld(const float* p) { return __builtin_nontemporal_load(p); }
@@ -66,7 +103,7 @@ __device__ __forceinline__ float ldz(const float* p, bool use, int64_t idx) {
 // plane, so every field value is read from memory once per plane.
 // ---------------------------------------------------------------------------
 template <int MODE, int PML, bool FS, int TX, int TY, int NB>
-__device__ __forceinline__ void stress_tile(const StepArgs& a, int lb) {
+__device__ __forceinline__ void stress_tile(const StepArgs& a, int lb, const int wb) {
   const int lane = threadIdx.x, row = threadIdx.y;
   const int tx = lb % a.ntx;
   lb /= a.ntx;
@@ -90,7 +127,7 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb) {
   float cvx[NB], cvy[NB], cvz[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    const int64_t c0 = (int64_t)i + j * sy + (int64_t)k0 * sz + (int64_t)b * a.fstride;
+    const int64_t c0 = (int64_t)i + j * sy + (int64_t)k0 * sz + (int64_t)(wb + b) * a.fstride;
     cvx[b] = ldz(a.V[0], k0 >= 1, c0 - sz);
     cvy[b] = ldz(a.V[1], k0 >= 1, c0 - sz);
     cvz[b] = ldz(a.V[2], true, c0);
@@ -131,7 +168,7 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb) {
     if (vac) { m6 = a.vac_cdt; m7 = a.vac_cdt; m8 = a.vac_cdt; }
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      const int64_t fo = (int64_t)b * a.fstride;
+      const int64_t fo = (int64_t)(wb + b) * a.fstride;
       const float* Vx = a.V[0] + fo;
       const float* Vy = a.V[1] + fo;
       const float* Vz = a.V[2] + fo;
@@ -180,10 +217,10 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb) {
       float src_xz1 = 0.f, src_yz1 = 0.f;  // injections at (i,j,1), for BC1
       if (e1 > e0) {
         const int it = a.it + (a.it_base ? *a.it_base : 0);
-        const float amp = (it < a.nstf) ? a.stf[(int64_t)b * a.nstf + it] : 0.f;
+        const float amp = (it < a.nstf) ? a.stf[(int64_t)(wb + b) * a.nstf + it] : 0.f;
         for (int e = e0; e < e1; ++e) {
           const PointEntry p = a.inj[e];
-          if (p.i == i && p.j == j && p.batch == b) {
+          if (p.i == i && p.j == j && p.batch == wb + b) {
             if (p.k == k) src[p.comp] = fmaf(p.coef, amp, src[p.comp]);
             if (FS && k == 2 && p.k == 1) {
               if (p.comp == 4) src_xz1 = fmaf(p.coef, amp, src_xz1);
@@ -225,7 +262,7 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb) {
 // plane.
 // ---------------------------------------------------------------------------
 template <int MODE, int PML, int TX, int TY, int NB>
-__device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb) {
+__device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb, const int wb) {
   const int lane = threadIdx.x, row = threadIdx.y;
   const int tx = lb % a.ntx;
   lb /= a.ntx;
@@ -281,7 +318,7 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb) {
     if (PML == PML_PROFILE) wzm = k >= 1 ? a.wz[k - 1] : 1.f;
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      const int64_t fo = (int64_t)b * a.fstride;
+      const int64_t fo = (int64_t)(wb + b) * a.fstride;
       float* Vx = a.V[0] + fo; float* Vy = a.V[1] + fo; float* Vz = a.V[2] + fo;
       const float* Sxx = a.S[0] + fo; const float* Syy = a.S[1] + fo;
       const float* Szz = a.S[2] + fo; const float* Sxy = a.S[3] + fo;
@@ -313,7 +350,7 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb) {
       for (int r = r0; r < r1; ++r) {
         const PointEntry p = a.rec[r];
         const int it = a.it + (a.it_base ? *a.it_base : 0);
-        if (p.i == i && p.j == j && p.k == k && p.batch == b && it < a.nt)
+        if (p.i == i && p.j == j && p.k == k && p.batch == wb + b && it < a.nt)
           a.traces[(int64_t)p.slot * a.nt + it] = p.comp == 0 ? vx : (p.comp == 1 ? vy : vz);
       }
     }
@@ -325,11 +362,15 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb) {
 // One model per launch: workgroup -> tile of that model.
 template <int MODE, int PML, bool FS, int TX, int TY, int NB>
 __global__ __launch_bounds__(TX * TY) void k_stress(const StepArgs a) {
-  stress_tile<MODE, PML, FS, TX, TY, NB>(a, a.xcd_order ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x);
+  int g, lb;
+  split_batch(a, g, lb);
+  if (lb >= 0) stress_tile<MODE, PML, FS, TX, TY, NB>(a, lb, g * NB);
 }
 template <int MODE, int PML, int TX, int TY, int NB>
 __global__ __launch_bounds__(TX * TY) void k_velocity(const StepArgs a) {
-  velocity_tile<MODE, PML, TX, TY, NB>(a, a.xcd_order ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x);
+  int g, lb;
+  split_batch(a, g, lb);
+  if (lb >= 0) velocity_tile<MODE, PML, TX, TY, NB>(a, lb, g * NB);
 }
 
 // Many models per launch (an uneven batch of independent grids): the
@@ -352,7 +393,7 @@ __global__ __launch_bounds__(TX * TY) void k_stress_group(const StepArgs* __rest
                                                          int32_t order) {
   const int lb = order ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int m = group_model(start, nm, lb);
-  stress_tile<MODE, PML, FS, TX, TY, 1>(tab[m], lb - start[m]);
+  stress_tile<MODE, PML, FS, TX, TY, 1>(tab[m], lb - start[m], 0);
 }
 template <int MODE, int PML, int TX, int TY>
 __global__ __launch_bounds__(TX * TY) void k_velocity_group(const StepArgs* __restrict__ tab,
@@ -360,7 +401,7 @@ __global__ __launch_bounds__(TX * TY) void k_velocity_group(const StepArgs* __re
                                                            int32_t order) {
   const int lb = order ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int m = group_model(start, nm, lb);
-  velocity_tile<MODE, PML, TX, TY, 1>(tab[m], lb - start[m]);
+  velocity_tile<MODE, PML, TX, TY, 1>(tab[m], lb - start[m], 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -368,7 +409,7 @@ __global__ __launch_bounds__(TX * TY) void k_velocity_group(const StepArgs* __re
 // ---------------------------------------------------------------------------
 template <int MODE, int PML, bool FS, int TX, int NB>
 static hipError_t go_stress2(const StepArgs& a, hipStream_t s) {
-  const int nblk = a.ntx * a.nty * a.nkc;
+  const int nblk = launch_blocks(a) * a.nbg;
   if (a.ty_rows == 4) {
     hipLaunchKernelGGL((k_stress<MODE, PML, FS, TX, 4, NB>), dim3(nblk), dim3(TX, 4), 0, s, a);
   } else if (a.ty_rows == 8) {
@@ -405,7 +446,7 @@ static hipError_t go_stress(const StepArgs& a, hipStream_t s) {
 
 template <int MODE, int PML, int TX, int NB>
 static hipError_t go_vel2(const StepArgs& a, hipStream_t s) {
-  const int nblk = a.ntx * a.nty * a.nkc;
+  const int nblk = launch_blocks(a) * a.nbg;
   if (a.ty_rows == 4) {
     hipLaunchKernelGGL((k_velocity<MODE, PML, TX, 4, NB>), dim3(nblk), dim3(TX, 4), 0, s, a);
   } else if (a.ty_rows == 8) {

@@ -18,7 +18,7 @@ RUN_PATH = os.path.join(_HERE, "fwe_run")
 # status codes (fwe_status)
 OK, ERR_ARG, ERR_IO, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_UNSUPPORTED, ERR_NONFINITE = range(8)
 KERNEL_AUTO, KERNEL_TWO_PASS, KERNEL_FUSED = range(3)
-ORDER_LINEAR, ORDER_XCD = 0, 1
+ORDER_LINEAR, ORDER_XCD, ORDER_PLANE_SPLIT = 0, 1, 2
 MEDIUM_AUTO, MEDIUM_ARRAYS, MEDIUM_TABLE = 0, 1, 2
 FS_NONE, FS_BC1, FS_VACUUM = 0, 1, 2
 MODE_ISO, MODE_ORTHO = 0, 1
@@ -131,6 +131,8 @@ SIGNATURES = {
     "fwe_group_set_tile_cols": (C.c_int, [C.c_void_p, C.c_int32]),
     "fwe_get_wg_order": (C.c_int, [C.c_void_p, _IP]),
     "fwe_set_medium_encoding": (C.c_int, [C.c_void_p, C.c_int32]),
+    "fwe_set_batch_tile": (C.c_int, [C.c_void_p, C.c_int32]),
+    "fwe_get_batch_tile": (C.c_int, [C.c_void_p, _IP]),
     "fwe_get_medium_encoding": (C.c_int, [C.c_void_p, _IP, _IP]),
     # generators (fwe_gen.h)
     "fwe_rng_seed": (None, [C.POINTER(RNG), C.c_uint64]),
@@ -361,6 +363,15 @@ class Solver:
     def wg_order(self):
         c = C.c_int32()
         _check(lib().fwe_get_wg_order(self._h, C.byref(c)))
+        return c.value
+
+    def set_batch_tile(self, wavefields):
+        """Wavefields per workgroup (a divisor of nbatch; 0 = default)."""
+        _check(lib().fwe_set_batch_tile(self._h, wavefields))
+
+    def batch_tile(self):
+        c = C.c_int32()
+        _check(lib().fwe_get_batch_tile(self._h, C.byref(c)))
         return c.value
 
     def set_medium_encoding(self, enc):

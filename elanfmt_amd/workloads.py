@@ -270,15 +270,18 @@ def config(idx, **kw):
 
 # algorithmic bytes per cell-update of each kernel formulation: every
 # streamed 3-D array counted once per pass (read and/or write), FP32.
-def alg_bytes(mode, pml, nbatch=1, fused=False, table=False):
+def alg_bytes(mode, pml, nbatch=1, fused=False, table=False, batch_tile=None):
     """(stress_bytes, velocity_bytes) per cell per step, all wavefields; for
     the single-pass kernel the whole step is in the first entry.  table:
     the ortho medium read through the material table, one index byte per
-    cell and pass (the table entries stay in the caches)."""
+    cell and pass (the table entries stay in the caches).  batch_tile:
+    wavefields per workgroup (default all); the medium is read once per
+    workgroup group of wavefields, nbatch / batch_tile times per cell."""
     w = 1 if pml == fwe.PML_WEIGHTS3D else 0
+    reads = nbatch // (batch_tile or nbatch)  # medium reads per cell and pass
     if table and not fused:
         # stress: R V3 S6 [w] + index byte, W S6 ; velocity: R S6 V3 [w] + index byte, W V3
-        return 4 * (nbatch * 15 + w) + 1, 4 * (nbatch * 12 + w) + 1
+        return 4 * (nbatch * 15 + w * reads) + reads, 4 * (nbatch * 12 + w * reads) + reads
     if fused:
         # fields R9 W9 per wavefield; medium once per launch (one launch per
         # wavefield: iso c11 c12; ortho C9 b3 [w])
@@ -286,9 +289,9 @@ def alg_bytes(mode, pml, nbatch=1, fused=False, table=False):
         return 4 * nbatch * (18 + med), 0
     if mode == fwe.MODE_ISO:
         # stress: R V3 S6 c11 c12, W S6 ; velocity: R S6 V3, W V3
-        return 4 * (nbatch * 15 + 2), 4 * (nbatch * 12)
+        return 4 * (nbatch * 15 + 2 * reads), 4 * (nbatch * 12)
     # stress: R V3 S6 C9 [w], W S6 ; velocity: R S6 V3 b3 [w], W V3
-    return 4 * (nbatch * 15 + 9 + w), 4 * (nbatch * 12 + 3 + w)
+    return 4 * (nbatch * 15 + (9 + w) * reads), 4 * (nbatch * 12 + (3 + w) * reads)
 
 
 def random_field(nx, ny, nz, corr, seed):

@@ -238,10 +238,14 @@ fwe_status fwe_set_tile_cols(fwe_ctx* ctx, int32_t cols);
 fwe_status fwe_get_tile_cols(const fwe_ctx* ctx, int32_t* cols);
 /* Order in which the two-pass kernels map workgroup ids to tiles: LINEAR
  * (tile b = workgroup b, so the workgroups in flight cover one contiguous
- * stretch of every array) or XCD (each of the eight XCDs gets a contiguous
- * range of tiles, so x/y neighbours share its L2).  Results do not depend on
- * it; fwe_autotune tries both (and fwe_group_autotune for groups). */
-enum { FWE_ORDER_LINEAR = 0, FWE_ORDER_XCD = 1 };
+ * stretch of every array), XCD (each of the eight XCDs gets a contiguous
+ * range of tiles, so x/y neighbours share its L2) or PLANE_SPLIT (each
+ * chunk of planes is cut into eight contiguous runs of tiles, one per XCD,
+ * and the XCDs walk the chunks together: neighbours share an L2 and the
+ * workgroups in flight still cover a few contiguous planes).  Results do
+ * not depend on it; fwe_autotune tries all three, fwe_group_autotune the
+ * first two (groups take LINEAR or XCD). */
+enum { FWE_ORDER_LINEAR = 0, FWE_ORDER_XCD = 1, FWE_ORDER_PLANE_SPLIT = 2 };
 fwe_status fwe_set_wg_order(fwe_ctx* ctx, int32_t order);
 fwe_status fwe_get_wg_order(const fwe_ctx* ctx, int32_t* order);
 /* How the two-pass kernels read the ortho medium (9 Cij + 3 buoyancy
@@ -256,6 +260,16 @@ fwe_status fwe_get_wg_order(const fwe_ctx* ctx, int32_t* order);
  * kernel use the arrays.  get returns the encoding fwe_run will use and the
  * number of table entries (0 without a table). */
 enum { FWE_MEDIUM_AUTO = 0, FWE_MEDIUM_ARRAYS = 1, FWE_MEDIUM_TABLE = 2 };
+/* Wavefields of a batch that one workgroup of the two-pass kernels advances
+ * together (a divisor of nbatch).  nbatch: every cell's medium is loaded
+ * once for the whole batch; 1: the wavefields lie side by side in one grid
+ * of single-wavefield workgroups, each streaming like a plain run.  0 (the
+ * default) = nbatch with the medium arrays, 1 with the material table.  One
+ * launch per half-step either way, and the same results bit for bit.
+ * fwe_autotune tries nbatch and 1.  No counterpart in the reference, which
+ * runs one wavefield per process. */
+fwe_status fwe_set_batch_tile(fwe_ctx* ctx, int32_t wavefields);
+fwe_status fwe_get_batch_tile(const fwe_ctx* ctx, int32_t* wavefields);
 #define FWE_MEDIUM_TABLE_MAX 256
 fwe_status fwe_set_medium_encoding(fwe_ctx* ctx, int32_t encoding);
 fwe_status fwe_get_medium_encoding(const fwe_ctx* ctx, int32_t* encoding, int32_t* n_materials);

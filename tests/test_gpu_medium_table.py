@@ -72,7 +72,7 @@ def test_table_tilings_and_graph_equal_arrays(gpu):
 
     wl = W.ortho_layered(nx=70, ny=30, nz=26, L=4, nt=30, nrec_side=2)
     ref, _, _ = _run(wl, fwe.MEDIUM_ARRAYS)
-    for tiling in ((64, 2, 1, 0), (64, 8, 26, 1), (128, 4, 3, 0), (128, 2, 1, 1)):
+    for tiling in ((64, 2, 1, 0), (64, 8, 26, 1), (128, 4, 3, 0), (128, 2, 1, 1), (128, 2, 2, 2), (64, 4, 1, 2)):
         print(f"case ortho70x30x26 table tiling={tiling}", flush=True)
         t, _, e = _run(wl, fwe.MEDIUM_TABLE, tiling=tiling)
         assert e[0] == fwe.MEDIUM_TABLE

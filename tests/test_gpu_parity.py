@@ -178,6 +178,47 @@ def test_batch_members_equal_single_runs(gpu):
         assert np.array_equal(tb[b * n0:(b + 1) * n0], singles[b]), b
 
 
+@pytest.mark.parametrize("enc", [1, 2])
+def test_batch_layouts_agree(gpu, enc):
+    """Six wavefields advanced with 6, 3, 2 or 1 of them per workgroup (the
+    rest side by side in the grid), in every workgroup order, with the medium
+    arrays (enc 1) and the material table (enc 2): the same bits."""
+    from elanfmt_amd import fwe, workloads as W
+
+    wl = W.ortho_layered(nx=66, ny=34, nz=28, L=5, nt=40, nrec_side=2)
+    nb = 6
+    rng = np.random.default_rng(4)
+    rec0 = [t for t in wl.receivers if t[4] == 0]
+    wl.nbatch, wl.stf = nb, [wl.stf[0] * (1 + 0.25 * b) for b in range(nb)]
+    wl.injections = sum((fwe.mt_injections(fwe.SRC_LAYOUT_OK4, 20 + 3 * b, 15, 10 + b, rng.normal(0, 1e15, 6), batch=b)
+                         for b in range(nb)), [])
+    wl.receivers = [t[:4] + (b,) for b in range(nb) for t in rec0]
+    ref = None
+    for bt, order in ((6, 0), (3, 2), (2, 1), (1, 0), (1, 2), (0, 0)):
+        print(f"case ortho66x34x28 x6 enc={enc} batch_tile={bt} order={order}", flush=True)
+        s = W.make_solver(wl)
+        s.set_medium_encoding(enc)
+        s.set_batch_tile(bt)
+        s.set_wg_order(order)
+        assert s.batch_tile() == (bt if bt else (1 if enc == fwe.MEDIUM_TABLE else 6))
+        s.run(wl.nt)
+        t = s.traces()
+        s.close()
+        if ref is None:
+            ref = t
+            assert np.abs(ref).max() > 0
+        assert np.array_equal(t, ref), (bt, order)
+    s = W.make_solver(wl)
+    for bad in (4, 7, -1):
+        with pytest.raises(fwe.FweError):
+            s.set_batch_tile(bad)
+    s.autotune(1)
+    assert s.batch_tile() in (1, 6)
+    s.run(wl.nt)
+    assert np.array_equal(s.traces(), ref)
+    s.close()
+
+
 def test_restart_and_field_roundtrip(gpu):
     """Saving all nine fields after N steps, resetting, restoring them and
     the step counter, and running N more equals 2N steps bit for bit."""
@@ -341,7 +382,7 @@ def test_autotune_and_tilings_agree(gpu):
     ref = None
     for tx, ty, kc, order in ((64, 4, 8, 0), (64, 2, 3, 1), (64, 8, 26, 0), (64, 4, 5, 1), (64, 4, 1, 0),
                               (64, 4, 1, 1), (64, 8, 2, 0), (128, 4, 8, 1), (128, 8, 1, 0), (128, 2, 1, 1),
-                              (128, 2, 26, 0)):
+                              (128, 2, 26, 0), (64, 2, 1, 2), (64, 8, 3, 2), (128, 4, 26, 2), (128, 2, 2, 2)):
         s = W.make_solver(wl)
         s.set_tiling(ty, kc)
         s.set_tile_cols(tx)
@@ -359,7 +400,7 @@ def test_autotune_and_tilings_agree(gpu):
     s.run(30)
     assert np.array_equal(s.traces(), ref)
     with pytest.raises(fwe.FweError):
-        s.set_wg_order(2)
+        s.set_wg_order(3)
     s.close()
     # chunks below three planes are refused with a free surface
     wl.free_surface = 1
