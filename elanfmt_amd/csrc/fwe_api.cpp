@@ -172,6 +172,9 @@ static void choose_tiling(fwe_ctx* x) {
   const char* eo = getenv("FWE_WG_ORDER");
   x->wg_order = eo ? atoi(eo) : FWE_ORDER_LINEAR;
   if (x->wg_order < FWE_ORDER_LINEAR || x->wg_order > FWE_ORDER_PLANE_SPLIT) x->wg_order = FWE_ORDER_LINEAR;
+  const char* eb = getenv("FWE_BATCH_TILE");
+  const int bt = eb ? atoi(eb) : 0;
+  x->batch_tile = (bt > 0 && bt <= c.nbatch && c.nbatch % bt == 0) ? bt : 0;
   x->ntx = (c.nx + x->tx - 1) / x->tx;
   x->nty = (c.ny + ty - 1) / ty;
   // one plane per workgroup in linear order streams best on the measured
