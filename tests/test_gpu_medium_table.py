@@ -137,3 +137,27 @@ def test_table_refused_for_iso(gpu):
     with pytest.raises(fwe.FweError):
         s.set_medium_encoding(3)
     s.close()
+
+
+def test_group_reads_the_arrays_and_equals_table_runs(gpu):
+    """Groups read the medium arrays; members whose separate runs use the
+    material table get the same bits from the group."""
+    from elanfmt_amd import fwe, workloads as W
+
+    wls = [W.ortho_layered(nx=n, ny=n - 6, nz=n - 10, L=4, nt=50, nrec_side=2, seed=3 + q)
+           for q, n in enumerate((36, 44, 52))]
+    ref = []
+    for wl in wls:
+        print(f"case {wl.name} table single run", flush=True)
+        t, _, e = _run(wl, fwe.MEDIUM_AUTO)
+        assert e[0] == fwe.MEDIUM_TABLE
+        ref.append(t)
+    sol = [W.make_solver(wl) for wl in wls]
+    g = fwe.Group(sol)
+    g.run(50)
+    for s, r in zip(sol, ref):
+        assert np.abs(r).max() > 0
+        assert np.array_equal(s.traces(), r)
+    g.close()
+    for s in sol:
+        s.close()
