@@ -299,48 +299,21 @@ static fwe_status upload(fwe_ctx* x, float* dst, const std::vector<float>& h) {
 // its tuple.  Built when there are at most FWE_MEDIUM_TABLE_MAX tuples;
 // otherwise ntab = 0 and the kernels stream the arrays.
 static fwe_status build_material_table(fwe_ctx* x, const std::vector<std::vector<float>>& h) {
-  constexpr int NSLOT = 1024;  // open addressing, > 2 * FWE_MEDIUM_TABLE_MAX
   const int64_t n = x->nelem;
   std::vector<uint8_t> idx((size_t)n);
-  std::vector<uint32_t> tab;
-  std::vector<int> slot(NSLOT, -1);
-  uint32_t t[12], prev[12];
-  int pe = -1;
-  bool fits = true;
-  for (int64_t c = 0; c < n && fits; ++c) {
-    for (int q = 0; q < 12; ++q) memcpy(&t[q], &h[q][(size_t)c], 4);
-    if (pe >= 0 && memcmp(t, prev, sizeof(t)) == 0) {
-      idx[(size_t)c] = (uint8_t)pe;
-      continue;
-    }
-    uint64_t hs = 1469598103934665603ull;  // FNV-1a over the 32-bit words
-    for (int q = 0; q < 12; ++q) hs = (hs ^ t[q]) * 1099511628211ull;
-    int sl = (int)(hs & (NSLOT - 1)), e;
-    for (;;) {
-      e = slot[sl];
-      if (e < 0) {
-        e = (int)(tab.size() / 12);
-        if (e >= FWE_MEDIUM_TABLE_MAX) { fits = false; break; }
-        tab.insert(tab.end(), t, t + 12);
-        slot[sl] = e;
-        break;
-      }
-      if (memcmp(&tab[(size_t)e * 12], t, sizeof(t)) == 0) break;
-      sl = (sl + 1) & (NSLOT - 1);
-    }
-    idx[(size_t)c] = (uint8_t)e;
-    memcpy(prev, t, sizeof(t));
-    pe = e;
-  }
+  std::vector<float> tab((size_t)12 * FWE_MEDIUM_TABLE_MAX);
+  const float* arrays[12];
+  for (int q = 0; q < 12; ++q) arrays[q] = h[q].data();
+  const int32_t nt = fwe_gen_material_table(n, 12, arrays, FWE_MEDIUM_TABLE_MAX, idx.data(), tab.data());
   x->ntab = 0;
-  if (!fits) return FWE_OK;
+  if (nt < 1) return FWE_OK;
   HIPCHK(hipSetDevice(x->cfg.device));
   if (!x->mid) HIPCHK(hipMalloc(&x->mid, (size_t)n));
   if (!x->mtab) HIPCHK(hipMalloc(&x->mtab, sizeof(float) * 12 * FWE_MEDIUM_TABLE_MAX));
   HIPCHK(hipMemcpyAsync(x->mid, idx.data(), (size_t)n, hipMemcpyHostToDevice, x->stream));
-  HIPCHK(hipMemcpyAsync(x->mtab, tab.data(), sizeof(uint32_t) * tab.size(), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(x->mtab, tab.data(), sizeof(float) * 12 * (size_t)nt, hipMemcpyHostToDevice, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
-  x->ntab = (int)(tab.size() / 12);
+  x->ntab = nt;
   return FWE_OK;
 }
 

@@ -331,3 +331,43 @@ void fwe_gen_random_field(int32_t nx, int32_t ny, int32_t nz, double corr, uint6
 double fwe_gen_cfl_dt(double vmax, double dx, double dy, double dz, double cfl) {
   return cfl / (vmax * sqrt(1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz)));
 }
+
+int32_t fwe_gen_material_table(int64_t n, int32_t m, const float* const* arrays, int32_t max_entries, uint8_t* idx,
+                               float* tab) {
+  enum { NSLOT = 1024 };  /* open addressing, more than twice the largest table */
+  if (n < 0 || m < 1 || m > 16 || max_entries < 1 || max_entries > 256 || !arrays || (n > 0 && (!idx || !tab)))
+    return -1;
+  for (int q = 0; q < m; ++q)
+    if (!arrays[q] && n > 0) return -1;
+  int slot[NSLOT];
+  for (int q = 0; q < NSLOT; ++q) slot[q] = -1;
+  uint32_t t[16], prev[16];
+  int ntab = 0, pe = -1;
+  const size_t bytes = sizeof(uint32_t) * (size_t)m;
+  for (int64_t e = 0; e < n; ++e) {
+    for (int q = 0; q < m; ++q) memcpy(&t[q], &arrays[q][e], 4);
+    if (pe >= 0 && memcmp(t, prev, bytes) == 0) {  /* runs of one material are the common case */
+      idx[e] = (uint8_t)pe;
+      continue;
+    }
+    uint64_t h = 1469598103934665603ull;  /* FNV-1a over the 32-bit words */
+    for (int q = 0; q < m; ++q) h = (h ^ t[q]) * 1099511628211ull;
+    int sl = (int)(h & (NSLOT - 1)), id;
+    for (;;) {
+      id = slot[sl];
+      if (id < 0) {
+        if (ntab >= max_entries) return -1;
+        id = ntab++;
+        memcpy(tab + (size_t)id * m, t, bytes);
+        slot[sl] = id;
+        break;
+      }
+      if (memcmp(tab + (size_t)id * m, t, bytes) == 0) break;
+      sl = (sl + 1) & (NSLOT - 1);
+    }
+    idx[e] = (uint8_t)id;
+    memcpy(prev, t, bytes);
+    pe = id;
+  }
+  return ntab;
+}

@@ -157,6 +157,8 @@ SIGNATURES = {
                               [C.POINTER(_FP), _FP, _IP, _DP]),
     "fwe_gen_random_field": (None, [C.c_int32] * 3 + [C.c_double, C.c_uint64, _FP]),
     "fwe_gen_cfl_dt": (C.c_double, [C.c_double] * 5),
+    "fwe_gen_material_table": (C.c_int32, [C.c_int64, C.c_int32, C.POINTER(_FP), C.c_int32,
+                                           C.POINTER(C.c_uint8), _FP]),
     # post-processing (fwe_post.h)
     "fwe_post_rotate_mt": (None, [C.c_double, _DP, _DP]),
     "fwe_post_butter_bandpass": (C.c_int32, [C.c_int32, C.c_double, C.c_double, C.c_double, _DP]),

@@ -100,6 +100,18 @@ void fwe_gen_random_field(int32_t nx, int32_t ny, int32_t nz, double corr, uint6
 /* dt = cfl / (vmax * sqrt(1/dx^2 + 1/dy^2 + 1/dz^2)) (OK4.m:165-169). */
 double fwe_gen_cfl_dt(double vmax, double dx, double dy, double dz, double cfl);
 
+/* Material table: the distinct m-tuples (arrays[0][e], ..., arrays[m-1][e])
+ * over n elements, compared bit for bit (so -0.0 and 0.0 differ), numbered
+ * in order of first appearance.  idx[e] receives the number of element e's
+ * tuple and tab[t*m + q] the tuples.  Returns the number of distinct tuples,
+ * or -1 when there are more than max_entries (idx and tab then hold partial
+ * results) or an argument is out of range (1 <= m <= 16,
+ * 1 <= max_entries <= 256).  libfwe uses it to read layered media as one
+ * byte per cell (fwe_set_medium_encoding); no counterpart in the reference,
+ * which streams every array. */
+int32_t fwe_gen_material_table(int64_t n, int32_t m, const float* const* arrays, int32_t max_entries, uint8_t* idx,
+                               float* tab);
+
 #ifdef __cplusplus
 }
 #endif
