@@ -123,17 +123,14 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb, const int
   if (PML == PML_PROFILE) { fxh = a.whx[i]; fyh = a.why[j]; }
   if (PML == PML_PROFILE || PML == PML_SEP) { wxi = a.wx[i]; wyj = a.wy[j]; }
 
-  // carried z-neighbours, per wavefield
+  // carried z-neighbours, per wavefield: Vx(k-1), Vy(k-1), Vz(k)
   float cvx[NB], cvy[NB], cvz[NB];
-#pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    const int64_t c0 = (int64_t)i + j * sy + (int64_t)k0 * sz + (int64_t)(wb + b) * a.fstride;
-    cvx[b] = ldz(a.V[0], k0 >= 1, c0 - sz);
-    cvy[b] = ldz(a.V[1], k0 >= 1, c0 - sz);
-    cvz[b] = ldz(a.V[2], true, c0);
-  }
 
-  for (int k = k0; k < k1; ++k) {
+  // The first plane of the chunk loads its z-neighbours with its other
+  // loads (a prologue before the loop would cost the workgroup one more
+  // memory round trip); later planes use the carries.
+  auto plane = [&](const int k, auto first_tag) __attribute__((always_inline)) {
+    constexpr bool FIRST = decltype(first_tag)::value;
     const int64_t c = (int64_t)i + j * sy + k * sz;
     const bool km = k >= 1;
     float wzk = 1.f, fzh = 1.f;
@@ -177,7 +174,9 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb, const int
       // all loads of this cell first, branch-free
       const float vx = Vx[c], vy = Vy[c];
       const float vx_ip = Vx[c + 1], vy_jp = Vy[c + sy], vz_kp = Vz[c + sz];
-      const float vz = cvz[b], vx_km = cvx[b], vy_km = cvy[b];
+      const float vz = FIRST ? Vz[c] : cvz[b];
+      const float vx_km = FIRST ? ldz(Vx, k >= 1, c - sz) : cvx[b];
+      const float vy_km = FIRST ? ldz(Vy, k >= 1, c - sz) : cvy[b];
       const float vx_jm = ldz(Vx, jm, c - sy);
       const float vy_im = ldz(Vy, im, c - 1);
       const float vz_im = ldz(Vz, im, c - 1);
@@ -249,7 +248,9 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb, const int
       st(Sxx + c, sxx); st(Syy + c, syy); st(Szz + c, szz);
       st(Sxy + c, sxy); st(Sxz + c, sxz); st(Syz + c, syz);
     }
-  }
+  };
+  plane(k0, std::true_type{});
+  for (int k = k0 + 1; k < k1; ++k) plane(k, std::false_type{});
 }
 
 // ---------------------------------------------------------------------------
@@ -303,17 +304,15 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb, const i
     const bool uz = uz_ij && k >= 1;
     float w = 1.f;
     if (PML == PML_W3D) w = ld(a.w3 + c);
-    if (PML == PML_SEP) w = txi * tyj * a.wz[k];
+    const float wzk = PML == PML_SEP ? a.wz[k] : 1.f;
     float bx = a.bxdt, by = a.bydt, bz = a.bzdt;
     if (MODE == MODE_ORTHO) {
       // buoyancy through the caches (measured faster than streaming here)
       bx = a.B[0][c]; by = a.B[1][c]; bz = a.B[2][c];
     }
-    if (MODE == MODE_ORTHO_TAB) {
-      // entry words 8..11: C66xy, bx, by, bz
-      const float4 q = *reinterpret_cast<const float4*>(a.mtab + 12u * (uint32_t)a.mid[c] + 8);
-      bx = q.y; by = q.z; bz = q.w;
-    }
+    // material-table index first: its entry is loaded after the field
+    // loads of the first wavefield, so those are in flight meanwhile
+    const uint32_t mi = MODE == MODE_ORTHO_TAB ? (uint32_t)a.mid[c] : 0u;
     float wzm = 1.f;
     if (PML == PML_PROFILE) wzm = k >= 1 ? a.wz[k - 1] : 1.f;
 #pragma unroll
@@ -336,6 +335,15 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb, const i
       const float syz = FIRST ? Syz[c] : csyz[b];
       const float syz_kp = Syz[c + sz], syz_jp = Syz[c + sy];
       cszz[b] = szz; csxz[b] = sxz_kp; csyz[b] = syz_kp;
+      if (MODE == MODE_ORTHO_TAB && b == 0) {
+        // entry words 8..11: C66xy, bx, by, bz
+        const float4 q = *reinterpret_cast<const float4*>(a.mtab + 12u * mi + 8);
+        bx = q.y; by = q.z; bz = q.w;
+      }
+      // one memory round trip per plane: nothing below is scheduled before
+      // every load above is issued
+      if (NB == 1) __builtin_amdgcn_sched_barrier(0);
+      if (PML == PML_SEP && b == 0) w = txi * tyj * wzk;
       const float dSx = rate3(sxx - sxx_im, a.rdx, sxy_jp - sxy, a.rdy, sxz_kp - sxz, a.rdz);
       const float dSy = rate3(syy - syy_jm, a.rdy, sxy_ip - sxy, a.rdx, syz_kp - syz, a.rdz);
       const float dSz = rate3(szz - szz_km, a.rdz, sxz_ip - sxz, a.rdx, syz_jp - syz, a.rdy);
