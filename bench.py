@@ -105,12 +105,13 @@ def _timed_run(solver, steps, warmup, autotune):
     return time.perf_counter() - t0, ms_a, ms_b, tune
 
 
-def _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells, table=False, batch_tile=None):
+def _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells, medium=1, batch_tile=None):
     from elanfmt_amd import workloads as W
 
-    ba, bb = W.alg_bytes(wl.mode, wl.pml, wl.nbatch, fused=fused, table=table, batch_tile=batch_tile)
+    ba, bb = W.alg_bytes(wl.mode, wl.pml, wl.nbatch, fused=fused, medium=medium, batch_tile=batch_tile)
     bt = batch_tile or wl.nbatch
-    km = 2 if table else wl.mode  # kernel law template argument (2: ortho through the material table)
+    # kernel law template argument: 2 ortho through the material table, 3 ortho on isotropic arrays
+    km = medium if medium in (2, 3) and not fused else wl.mode
     fs = "true" if wl.free_surface else "false"
     if fused:
         kernels = {"fused": {"ms_per_step": round(ms_a / steps, 5), "alg_bytes_per_cell": ba,
@@ -141,16 +142,20 @@ def _roofline(dom, steps, cells, wname):
     return out
 
 
-MEDIUM = {"auto": 0, "arrays": 1, "table": 2}  # fwe.MEDIUM_*
+MEDIUM = {"auto": 0, "arrays": 1, "table": 2, "isotropic": 3}  # fwe.MEDIUM_*
 ORDER_NAMES = ("linear", "xcd", "plane split")  # fwe.ORDER_*
 
 
-def _medium_desc(wl, table, ntab):
+def _medium_desc(wl, medium, ntab):
     from elanfmt_amd import fwe
 
     if wl.mode == fwe.MODE_ISO:
         return "c11, c12 arrays"
-    return f"material table ({ntab} entries, one index byte per cell)" if table else "twelve streamed arrays"
+    if medium == fwe.MEDIUM_TABLE:
+        return f"material table ({ntab} entries, one index byte per cell)"
+    if medium == fwe.MEDIUM_ISOTROPIC:
+        return "isotropic: C11, C12, three edge moduli and three buoyancy arrays"
+    return "twelve streamed arrays"
 
 
 def _line(value, steps, warmup, el, config, **extra):
@@ -178,10 +183,9 @@ def bench_plain(args, steps):
     tiling = (s.tile_cols(),) + tuple(s.tiling())
     order = ORDER_NAMES[s.wg_order()]
     enc, ntab = s.medium_encoding()
-    table = enc == fwe.MEDIUM_TABLE
     s.close()
     cells = wl.cells
-    kernels, dom, bstep = _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells, table)
+    kernels, dom, bstep = _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells, enc)
     desc = (f"{wl.name}: layered orthorhombic FP32, {wl.info.get('pml_cells')}-cell separable damping, "
             f"lune MT, Ricker {wl.info.get('f0')} Hz, {len(wl.receivers)} receiver traces"
             if wl.mode == fwe.MODE_ORTHO else
@@ -189,7 +193,7 @@ def bench_plain(args, steps):
             f"double couple, Ricker {wl.info.get('f0')} Hz, {len(wl.receivers)} receiver traces")
     config = {"workload": desc, "grid": [wl.nx, wl.ny, wl.nz], "wavefields": wl.nbatch, "nt": wl.nt,
               "kernel": "fused single-pass" if fused else "two-pass", "tiling": list(tiling), "wg_order": order,
-              "medium": _medium_desc(wl, table, ntab),
+              "medium": _medium_desc(wl, enc, ntab),
               "autotune_ms_per_step": None if tune is None else round(tune, 4), "parallelism": "single GPU"}
     return _line(cells * wl.nbatch * steps / el / 1e9, steps, args.warmup, el, config,
                  hbm_pct_step=round(100 * bstep * cells * steps / el / 1e9 / PEAK_HBM_GBS, 2),
@@ -216,9 +220,8 @@ def bench_batched(args, steps):
     tiling = (s.tile_cols(),) + tuple(s.tiling())
     order = ORDER_NAMES[s.wg_order()]
     enc, ntab = s.medium_encoding()
-    table = enc == fwe.MEDIUM_TABLE
     bt = s.batch_tile()
-    kernels, dom, bstep = _kernel_entries(wl, False, tiling, ms_a, ms_b, steps, cells, table, bt)
+    kernels, dom, bstep = _kernel_entries(wl, False, tiling, ms_a, ms_b, steps, cells, enc, bt)
     # 1000 lune tensors by linear combination of the six elementary runs
     mts = np.array([W.mt_lune_random(50000 + q, m0)[0] for q in range(1000)])
     t0 = time.perf_counter()
@@ -268,7 +271,7 @@ def bench_batched(args, steps):
     config = {"workload": f"{wl.name}: layered orthorhombic FP32, 24-cell separable damping, six elementary "
                           f"moment tensors (Mxx..Myz) as one batch of wavefields, {nr} receiver traces each",
               "grid": [wl.nx, wl.ny, wl.nz], "wavefields": 6, "nt": wl.nt, "kernel": "two-pass batched",
-              "tiling": list(tiling), "wg_order": order, "medium": _medium_desc(wl, table, ntab),
+              "tiling": list(tiling), "wg_order": order, "medium": _medium_desc(wl, enc, ntab),
               "wavefields_per_workgroup": bt,
               "autotune_ms_per_step": None if tune is None else round(tune, 4), "parallelism": "single GPU"}
     return _line(cells * 6 * steps / el_b / 1e9, steps, args.warmup, el_b, config,
@@ -290,6 +293,10 @@ def bench_uneven(args, steps):
     cells = sum(w.cells for w in wls)
     print(f"# uneven: {len(wls)} models, {cells / 1e6:.1f} M cells, {steps} steps", file=sys.stderr, flush=True)
     sol = [W.make_solver(w) for w in wls]
+    for s in sol:
+        s.set_medium_encoding(MEDIUM[args.medium])
+    enc = sol[0].medium_encoding()[0] if all(s.medium_encoding()[0] == fwe.MEDIUM_ISOTROPIC for s in sol) \
+        else fwe.MEDIUM_ARRAYS
     g = fwe.Group(sol)
     tune = g.autotune(3) if not args.no_autotune else None
     g.run(args.warmup)
@@ -316,11 +323,12 @@ def bench_uneven(args, steps):
         s.close()
     from elanfmt_amd import workloads as Wm
 
-    ba, bb = Wm.alg_bytes(fwe.MODE_ORTHO, fwe.PML_SEPARABLE)
+    ba, bb = Wm.alg_bytes(fwe.MODE_ORTHO, fwe.PML_SEPARABLE, medium=enc)
     config = {"workload": f"uneven batch: {len(wls)} isotropic random-field cubes (edges "
                           f"{sorted(set(w.nx for w in wls))}), ortho law, 16-cell separable damping, own lune MT "
                           f"and Ricker f0 each", "models": len(wls), "cells": cells, "nt": steps,
-              "kernel": "two-pass group", "tiling": list(tiling),
+              "kernel": "two-pass group", "tiling": list(tiling), "medium": _medium_desc(wls[0], enc, 0),
+              "alg_bytes_per_cell": ba + bb,
               "autotune_ms_per_step": None if tune is None else round(tune, 4), "parallelism": "single GPU"}
     return _line(cells * steps / el_g / 1e9, steps, args.warmup, el_g, config,
                  hbm_pct_step=round(100 * (ba + bb) * cells * steps / el_g / 1e9 / PEAK_HBM_GBS, 2),
@@ -427,7 +435,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--medium", default="auto", choices=sorted(MEDIUM),
-                    help="ortho medium: material table when it fits (auto), always the arrays, or the table")
+                    help="ortho medium: the material table when it fits, else the isotropic arrays when they "
+                         "apply (auto); always the twelve arrays; the table; the isotropic arrays")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

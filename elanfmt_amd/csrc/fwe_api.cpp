@@ -81,6 +81,8 @@ struct fwe_ctx {
   float* mtab = nullptr;
   int ntab = 0;
   int medium_enc = FWE_MEDIUM_AUTO;
+  // ortho medium with C11 = C22 = C33 and C12 = C13 = C23 everywhere
+  bool iso_medium = false;
   // wavefields per workgroup of a batch (a divisor of nbatch; 0 = default)
   int batch_tile = 0;
   float* w3 = nullptr;      // nelem (W3D)
@@ -358,6 +360,12 @@ extern "C" fwe_status fwe_set_material_ortho(fwe_ctx* x, const float* const cij[
     if (st) return st;
   }
   x->have_medium = true;
+  // an isotropic medium run through the ortho law (the station solver's
+  // file set, random-field cubes): the kernels can read two of the six
+  // normal-moduli arrays
+  const size_t nb = sizeof(float) * (size_t)x->nelem;
+  x->iso_medium = !memcmp(h[0].data(), h[1].data(), nb) && !memcmp(h[0].data(), h[2].data(), nb) &&
+                  !memcmp(h[3].data(), h[4].data(), nb) && !memcmp(h[3].data(), h[5].data(), nb);
   return build_material_table(x, h);
 }
 
@@ -639,9 +647,19 @@ static hipError_t lb_after(fwe_ctx* x, int from);
 // the two-pass kernels read the ortho medium through the material table
 // when one was built, unless the arrays were asked for
 static bool use_table(const fwe_ctx* x) {
-  return x->cfg.mode == FWE_MODE_ORTHO && x->ntab > 0 && x->medium_enc != FWE_MEDIUM_ARRAYS;
+  return x->cfg.mode == FWE_MODE_ORTHO && x->ntab > 0 &&
+         (x->medium_enc == FWE_MEDIUM_AUTO || x->medium_enc == FWE_MEDIUM_TABLE);
 }
-static int kernel_mode(const fwe_ctx* x) { return use_table(x) ? fwe::MODE_ORTHO_TAB : x->cfg.mode; }
+// otherwise an isotropic medium reads eight of the twelve arrays
+static bool use_iso(const fwe_ctx* x) {
+  return x->cfg.mode == FWE_MODE_ORTHO && x->iso_medium && !use_table(x) &&
+         (x->medium_enc == FWE_MEDIUM_AUTO || x->medium_enc == FWE_MEDIUM_ISOTROPIC);
+}
+static int kernel_mode(const fwe_ctx* x) {
+  if (use_table(x)) return fwe::MODE_ORTHO_TAB;
+  if (use_iso(x)) return fwe::MODE_ORTHO_ISO;
+  return x->cfg.mode;
+}
 
 static hipError_t step_launch(fwe_ctx* x, int from, int it, const int32_t* it_base, hipEvent_t mid) {
   const fwe_config& c = x->cfg;
@@ -686,6 +704,8 @@ static fwe_status ready(fwe_ctx* x) {
   if (x->medium_enc == FWE_MEDIUM_TABLE && !use_table(x))
     return fail(FWE_ERR_UNSUPPORTED, "material table asked for, but the medium has more than %d distinct tuples",
                 FWE_MEDIUM_TABLE_MAX);
+  if (x->medium_enc == FWE_MEDIUM_ISOTROPIC && !use_iso(x))
+    return fail(FWE_ERR_UNSUPPORTED, "isotropic medium arrays asked for, but the medium is not isotropic");
   if (x->cfg.pml != FWE_PML_NONE && !x->have_damp) return fail(FWE_ERR_STATE, "damping not set");
   return FWE_OK;
 }
@@ -876,11 +896,14 @@ extern "C" fwe_status fwe_get_batch_tile(const fwe_ctx* x, int32_t* wavefields) 
 
 extern "C" fwe_status fwe_set_medium_encoding(fwe_ctx* x, int32_t encoding) {
   if (!x) return fail(FWE_ERR_ARG, "NULL ctx");
-  if (encoding < FWE_MEDIUM_AUTO || encoding > FWE_MEDIUM_TABLE) return fail(FWE_ERR_ARG, "bad medium encoding %d", encoding);
-  if (encoding == FWE_MEDIUM_TABLE && x->cfg.mode != FWE_MODE_ORTHO)
-    return fail(FWE_ERR_UNSUPPORTED, "the material table is for the ortho law");
+  if (encoding < FWE_MEDIUM_AUTO || encoding > FWE_MEDIUM_ISOTROPIC)
+    return fail(FWE_ERR_ARG, "bad medium encoding %d", encoding);
+  if ((encoding == FWE_MEDIUM_TABLE || encoding == FWE_MEDIUM_ISOTROPIC) && x->cfg.mode != FWE_MODE_ORTHO)
+    return fail(FWE_ERR_UNSUPPORTED, "the material table and the isotropic arrays are for the ortho law");
   if (encoding == FWE_MEDIUM_TABLE && x->have_medium && x->ntab == 0)
     return fail(FWE_ERR_UNSUPPORTED, "the medium has more than %d distinct tuples", FWE_MEDIUM_TABLE_MAX);
+  if (encoding == FWE_MEDIUM_ISOTROPIC && x->have_medium && !x->iso_medium)
+    return fail(FWE_ERR_UNSUPPORTED, "the medium is not isotropic (C11 = C22 = C33, C12 = C13 = C23)");
   drop_graph(x);
   x->medium_enc = encoding;
   return FWE_OK;
@@ -888,7 +911,10 @@ extern "C" fwe_status fwe_set_medium_encoding(fwe_ctx* x, int32_t encoding) {
 
 extern "C" fwe_status fwe_get_medium_encoding(const fwe_ctx* x, int32_t* encoding, int32_t* n_materials) {
   if (!x || !encoding) return fail(FWE_ERR_ARG, "NULL argument");
-  *encoding = use_table(x) && !use_fused(x) ? FWE_MEDIUM_TABLE : FWE_MEDIUM_ARRAYS;
+  *encoding = use_fused(x) ? FWE_MEDIUM_ARRAYS
+              : use_table(x) ? FWE_MEDIUM_TABLE
+              : use_iso(x)   ? FWE_MEDIUM_ISOTROPIC
+                             : FWE_MEDIUM_ARRAYS;
   if (n_materials) *n_materials = x->ntab;
   return FWE_OK;
 }
@@ -1366,12 +1392,20 @@ static fwe_status group_run_impl(fwe_group* g, int32_t nsteps, double* ms) {
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
   }
+  // groups read the medium arrays: the eight of an isotropic medium when
+  // every member's would be read that way, else the full set
+  int gm = g->mode;
+  if (gm == FWE_MODE_ORTHO) {
+    bool iso = true;
+    for (fwe_ctx* x : g->m) iso = iso && x->iso_medium && (x->medium_enc == FWE_MEDIUM_AUTO || x->medium_enc == FWE_MEDIUM_ISOTROPIC);
+    if (iso) gm = fwe::MODE_ORTHO_ISO;
+  }
   HIPCHK(fwe::launch_advance(g->it_dev, g->m[0]->it, 1, g->stream));
   if (ms) HIPCHK(hipEventRecord(e0, g->stream));
   for (int s = 0; s < nsteps; ++s) {
-    HIPCHK(fwe::launch_group(g->tab, g->start, (int)g->m.size(), g->nblk, g->tx, g->ty, g->order, g->mode, g->pml, g->fs, false,
+    HIPCHK(fwe::launch_group(g->tab, g->start, (int)g->m.size(), g->nblk, g->tx, g->ty, g->order, gm, g->pml, g->fs, false,
                              g->stream));
-    HIPCHK(fwe::launch_group(g->tab, g->start, (int)g->m.size(), g->nblk, g->tx, g->ty, g->order, g->mode, g->pml, g->fs, true,
+    HIPCHK(fwe::launch_group(g->tab, g->start, (int)g->m.size(), g->nblk, g->tx, g->ty, g->order, gm, g->pml, g->fs, true,
                              g->stream));
     HIPCHK(fwe::launch_advance(g->it_dev, 1, 0, g->stream));
   }

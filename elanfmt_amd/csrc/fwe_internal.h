@@ -16,8 +16,10 @@ namespace fwe {
 constexpr int TILE_X = 64;
 
 // MODE_ORTHO_TAB: the ortho law with the medium read through a material
-// table (one byte per cell, see StepArgs::mid)
-enum Mode { MODE_ISO = 0, MODE_ORTHO = 1, MODE_ORTHO_TAB = 2 };
+// table (one byte per cell, see StepArgs::mid).  MODE_ORTHO_ISO: the ortho
+// law on an isotropic medium (C11 = C22 = C33 and C12 = C13 = C23 in every
+// cell), which reads C[0] and C[3] for the six normal moduli.
+enum Mode { MODE_ISO = 0, MODE_ORTHO = 1, MODE_ORTHO_TAB = 2, MODE_ORTHO_ISO = 3 };
 enum Pml { PML_NONE = 0, PML_PROFILE = 1, PML_W3D = 2, PML_SEP = 3 };
 
 // One injection or receiver, sorted by xy tile on the host.

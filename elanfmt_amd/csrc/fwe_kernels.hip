@@ -62,6 +62,14 @@ hipError_t launch_stress(const StepArgs& a, int mode, int pml, int fs, hipStream
     if (pml == PML_NONE) return stress_mp<MODE_ISO, PML_NONE>(a, fs, s);
     return hipErrorInvalidValue;
   }
+  if (mode == MODE_ORTHO_ISO) {
+    switch (pml) {
+      case PML_NONE: return stress_mp<MODE_ORTHO_ISO, PML_NONE>(a, fs, s);
+      case PML_W3D: return stress_mp<MODE_ORTHO_ISO, PML_W3D>(a, fs, s);
+      case PML_SEP: return stress_mp<MODE_ORTHO_ISO, PML_SEP>(a, fs, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
   if (mode == MODE_ORTHO_TAB) {
     switch (pml) {
       case PML_NONE: return stress_mp<MODE_ORTHO_TAB, PML_NONE>(a, fs, s);
@@ -84,6 +92,14 @@ hipError_t launch_velocity(const StepArgs& a, int mode, int pml, hipStream_t s) 
     if (pml == PML_NONE) return velocity_mp<MODE_ISO, PML_NONE>(a, s);
     return hipErrorInvalidValue;
   }
+  if (mode == MODE_ORTHO_ISO) {
+    switch (pml) {
+      case PML_NONE: return velocity_mp<MODE_ORTHO_ISO, PML_NONE>(a, s);
+      case PML_W3D: return velocity_mp<MODE_ORTHO_ISO, PML_W3D>(a, s);
+      case PML_SEP: return velocity_mp<MODE_ORTHO_ISO, PML_SEP>(a, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
   if (mode == MODE_ORTHO_TAB) {
     switch (pml) {
       case PML_NONE: return velocity_mp<MODE_ORTHO_TAB, PML_NONE>(a, s);
@@ -103,6 +119,14 @@ hipError_t launch_velocity(const StepArgs& a, int mode, int pml, hipStream_t s) 
 hipError_t launch_group(const StepArgs* tab, const int32_t* start, int nm, int nblk, int tx, int ty, int order, int mode, int pml,
                         int fs, bool vel, hipStream_t s) {
   if (mode == MODE_ORTHO_TAB) return hipErrorInvalidValue;
+  if (mode == MODE_ORTHO_ISO) {
+    switch (pml) {
+      case PML_NONE: return group_mp<MODE_ORTHO_ISO, PML_NONE>(tab, start, nm, nblk, tx, ty, order, fs, vel, s);
+      case PML_W3D: return group_mp<MODE_ORTHO_ISO, PML_W3D>(tab, start, nm, nblk, tx, ty, order, fs, vel, s);
+      case PML_SEP: return group_mp<MODE_ORTHO_ISO, PML_SEP>(tab, start, nm, nblk, tx, ty, order, fs, vel, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
   if (mode == MODE_ISO) {
     if (pml == PML_PROFILE) return group_mp<MODE_ISO, PML_PROFILE>(tab, start, nm, nblk, tx, ty, order, fs, vel, s);
     if (pml == PML_NONE) return group_mp<MODE_ISO, PML_NONE>(tab, start, nm, nblk, tx, ty, order, fs, vel, s);

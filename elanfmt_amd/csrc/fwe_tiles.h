@@ -151,6 +151,12 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb, const int
       m0 = q0.x; m1 = q0.y; m2 = q0.z; m3 = q0.w;
       m4 = q1.x; m5 = q1.y; m6 = q1.z; m7 = q1.w;
       m8 = t[8];
+    } else if (MODE == MODE_ORTHO_ISO) {
+      // isotropic medium (as the station solver's c11/c12 arrays,
+      // FastWaveED_GPU3D_v1_9kmst.cu:381-400): two normal moduli per cell
+      m0 = ld(a.C[0] + c); m3 = ld(a.C[3] + c);
+      m1 = m0; m2 = m0; m4 = m3; m5 = m3;
+      m6 = ld(a.C[6] + c); m7 = ld(a.C[7] + c); m8 = ld(a.C[8] + c);
     } else {
       m0 = ld(a.C[0] + c); m1 = ld(a.C[1] + c);
       if (MODE == MODE_ORTHO) {
@@ -306,7 +312,7 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb, const i
     if (PML == PML_W3D) w = ld(a.w3 + c);
     const float wzk = PML == PML_SEP ? a.wz[k] : 1.f;
     float bx = a.bxdt, by = a.bydt, bz = a.bzdt;
-    if (MODE == MODE_ORTHO) {
+    if (MODE == MODE_ORTHO || MODE == MODE_ORTHO_ISO) {
       // buoyancy through the caches (measured faster than streaming here)
       bx = a.B[0][c]; by = a.B[1][c]; bz = a.B[2][c];
     }

@@ -19,7 +19,7 @@ RUN_PATH = os.path.join(_HERE, "fwe_run")
 OK, ERR_ARG, ERR_IO, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_UNSUPPORTED, ERR_NONFINITE = range(8)
 KERNEL_AUTO, KERNEL_TWO_PASS, KERNEL_FUSED = range(3)
 ORDER_LINEAR, ORDER_XCD, ORDER_PLANE_SPLIT = 0, 1, 2
-MEDIUM_AUTO, MEDIUM_ARRAYS, MEDIUM_TABLE = 0, 1, 2
+MEDIUM_AUTO, MEDIUM_ARRAYS, MEDIUM_TABLE, MEDIUM_ISOTROPIC = 0, 1, 2, 3
 FS_NONE, FS_BC1, FS_VACUUM = 0, 1, 2
 MODE_ISO, MODE_ORTHO = 0, 1
 PML_NONE, PML_PROFILE, PML_WEIGHTS3D, PML_SEPARABLE = 0, 1, 2, 3
@@ -377,12 +377,13 @@ class Solver:
         return c.value
 
     def set_medium_encoding(self, enc):
-        """MEDIUM_AUTO (material table when the medium has one), MEDIUM_ARRAYS
-        or MEDIUM_TABLE (fails when the medium has too many distinct tuples)."""
+        """MEDIUM_AUTO (material table when the medium has one, else the
+        isotropic arrays when they apply), MEDIUM_ARRAYS (all twelve),
+        MEDIUM_TABLE or MEDIUM_ISOTROPIC (fail when they do not apply)."""
         _check(lib().fwe_set_medium_encoding(self._h, enc))
 
     def medium_encoding(self):
-        """(encoding fwe_run uses: MEDIUM_ARRAYS or MEDIUM_TABLE, table entries)"""
+        """(encoding fwe_run uses: MEDIUM_ARRAYS, MEDIUM_TABLE or MEDIUM_ISOTROPIC, table entries)"""
         e, n = C.c_int32(), C.c_int32()
         _check(lib().fwe_get_medium_encoding(self._h, C.byref(e), C.byref(n)))
         return e.value, n.value

@@ -161,3 +161,60 @@ def test_group_reads_the_arrays_and_equals_table_runs(gpu):
     g.close()
     for s in sol:
         s.close()
+
+
+@pytest.mark.parametrize("fs", [0, 1])
+def test_isotropic_arrays_equal_the_twelve(gpu, fs):
+    """An isotropic random-field medium through the ortho law (C11 = C22 =
+    C33, C12 = C13 = C23 in every cell, too many distinct tuples for a
+    table): the eight-array form gives the bits of the twelve arrays, and
+    both match the oracle."""
+    from elanfmt_amd import fwe, workloads as W
+    from oracle import Oracle
+
+    wl = W.iso_random(40, seed=5, L=5, nt=60, f0=12.0, nrec_side=2)
+    wl.free_surface = fs
+    print(f"case isorand40 fs={fs}", flush=True)
+    ta, fa, ea = _run(wl, fwe.MEDIUM_ARRAYS)
+    ti, fi, ei = _run(wl, fwe.MEDIUM_AUTO)
+    assert ea == (fwe.MEDIUM_ARRAYS, 0) and ei == (fwe.MEDIUM_ISOTROPIC, 0)
+    assert np.abs(ta).max() > 0
+    assert np.array_equal(ti, ta)
+    for q, (x, y) in enumerate(zip(fi, fa)):
+        assert np.array_equal(x, y), q
+    o = Oracle(wl.nx, wl.ny, wl.nz, wl.dx, wl.dy, wl.dz, wl.dt, wl.nt, mode=wl.mode, pml=wl.pml, free_surface=fs)
+    W.apply(o, wl)
+    o.run(wl.nt)
+    assert rel_l2(ti.astype(np.float64), o.traces()) < 1e-5
+
+
+def test_isotropic_arrays_in_groups_and_refusals(gpu):
+    """Groups of isotropic members read the eight arrays and equal the
+    members' own runs; a member held to the twelve arrays puts the group on
+    the twelve, with the same bits; a layered orthorhombic medium refuses
+    the isotropic form."""
+    from elanfmt_amd import fwe, workloads as W
+
+    wls = [W.iso_random(n, seed=q, L=4, nt=40, f0=15.0, nrec_side=2) for q, n in enumerate((32, 40))]
+    ref = []
+    for wl in wls:
+        t, _, e = _run(wl, fwe.MEDIUM_ISOTROPIC)
+        assert e[0] == fwe.MEDIUM_ISOTROPIC
+        ref.append(t)
+    for held in (None, 1):
+        print(f"case isorand group, member {held} on the twelve arrays", flush=True)
+        sol = [W.make_solver(wl) for wl in wls]
+        if held is not None:
+            sol[held].set_medium_encoding(fwe.MEDIUM_ARRAYS)
+        g = fwe.Group(sol)
+        g.run(40)
+        for s, r in zip(sol, ref):
+            assert np.array_equal(s.traces(), r)
+        g.close()
+        for s in sol:
+            s.close()
+    wl = W.ortho_layered(nx=40, ny=36, nz=30, L=4, nt=5, nrec_side=2)
+    s = W.make_solver(wl)
+    with pytest.raises(fwe.FweError):
+        s.set_medium_encoding(fwe.MEDIUM_ISOTROPIC)
+    s.close()
