@@ -64,14 +64,15 @@ def _traffic(kernel_name, workload):
     return None, None
 
 
-def cpu_baseline(wl, steps, threads):
+def cpu_baseline(wl, steps, threads, fp32=False):
     """The FP64 oracle (CPU restatement of the reference step, OpenMP over z
-    planes) on a bounded number of steps of one wavefield of the workload."""
+    planes), or its FP32 build, on a bounded number of steps of one
+    wavefield of the workload."""
     from elanfmt_amd import workloads as W
     from oracle import Oracle
 
     o = Oracle(wl.nx, wl.ny, wl.nz, wl.dx, wl.dy, wl.dz, wl.dt, max(wl.nt, steps + 2), mode=wl.mode,
-               pml=wl.pml, free_surface=wl.free_surface)
+               pml=wl.pml, free_surface=wl.free_surface, fp32=fp32)
     W.apply(o, wl)
     o.run(2, threads)  # first touch
     t0 = time.perf_counter()
@@ -86,8 +87,12 @@ def _cpu_entry(args, wl_single, name):
         return None
     threads = min(16, os.cpu_count() or 1)
     v, el = cpu_baseline(wl_single, args.cpu_steps, threads)
+    steps32 = max(1, args.cpu_steps // 2)
+    v32, el32 = cpu_baseline(wl_single, steps32, threads, fp32=True)
     return {"value": round(v, 5), "unit": "Gcell/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle, {args.cpu_steps} steps of {name} ({el:.1f} s), OpenMP over z planes"}
+            "sample": f"FP64 oracle, {args.cpu_steps} steps of {name} ({el:.1f} s), OpenMP over z planes",
+            "fp32_value": round(v32, 5),
+            "fp32_sample": f"FP32 build of the same oracle, {steps32} steps ({el32:.1f} s)"}
 
 
 def _timed_run(solver, steps, warmup, autotune):
