@@ -78,8 +78,11 @@ struct StepArgs {
   const float* stf;         // [nb][nstf]
   int32_t nstf;
   int32_t src_after_bc;
-  // receivers
+  // receivers, sorted by (xy tile, plane): rec_off per tile, rec_koff per
+  // (tile, plane) pair, index tile*nz + k (the two-pass velocity kernel
+  // scans only the entries of its own plane)
   const int32_t* rec_off;
+  const int32_t* rec_koff;
   const PointEntry* rec;
   float* traces;            // [nrec][nt]
   int32_t nt;

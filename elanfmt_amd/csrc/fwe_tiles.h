@@ -282,7 +282,6 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb, const i
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int64_t sy = a.sy, sz = a.sz;
   const int tile = tx + tyi * a.ntx;
-  const int r0 = a.rec_off[tile], r1 = a.rec_off[tile + 1];
   const bool im = i >= 1, jm = j >= 1;
   const bool xin = im && jm;  // all three need i>=1, j>=1, k>=1
   const bool ux_ij = xin && j <= ny - 2;
@@ -311,6 +310,8 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb, const i
     float w = 1.f;
     if (PML == PML_W3D) w = ld(a.w3 + c);
     const float wzk = PML == PML_SEP ? a.wz[k] : 1.f;
+    const int64_t tk = (int64_t)tile * nz + k;
+    const int r0 = a.rec_koff[tk], r1 = a.rec_koff[tk + 1];
     float bx = a.bxdt, by = a.bydt, bz = a.bzdt;
     if (MODE == MODE_ORTHO || MODE == MODE_ORTHO_ISO) {
       // buoyancy through the caches (measured faster than streaming here)
@@ -361,7 +362,7 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb, const i
       st(Vx + c, vx);
       st(Vy + c, vy);
       st(Vz + c, vz);
-      for (int r = r0; r < r1; ++r) {
+      for (int r = r0; r < r1; ++r) {  // this tile's receivers in plane k
         const PointEntry p = a.rec[r];
         const int it = a.it + (a.it_base ? *a.it_base : 0);
         if (p.i == i && p.j == j && p.k == k && p.batch == wb + b && it < a.nt)
