@@ -1519,32 +1519,50 @@ extern "C" fwe_status fwe_group_autotune(fwe_group* g, int32_t trial_steps, doub
   if (!g || trial_steps < 1) return fail(FWE_ERR_ARG, "bad autotune arguments");
   static const int tys[3] = {2, 4, 8};
   static const int kcs[8] = {1, 2, 4, 8, 16, 32, 64, 256};
-  double best = 1e30;
-  int bty = g->ty, bkc = 8, border = g->order;
   fwe_status st;
   int kmin = g->fs ? 3 : 1;
   for (fwe_ctx* x : g->m)
     if (use_fused(x)) kmin = 3;
-  int btx = g->tx;
+  struct Cand {
+    double ms;
+    int order, tx, ty, kc;
+  };
+  auto trial = [&](const Cand& c, int steps, double* ms) -> fwe_status {
+    g->order = c.order;
+    g->tx = c.tx;
+    fwe_status s2 = fwe_group_set_tiling(g, c.ty, c.kc);
+    if (s2) return s2;
+    double t = 0;
+    if ((s2 = group_run_impl(g, 1, &t))) return s2;
+    if ((s2 = group_run_impl(g, steps, &t))) return s2;
+    *ms = t / steps;
+    return FWE_OK;
+  };
+  std::vector<Cand> cands;
   for (int order = FWE_ORDER_LINEAR; order <= FWE_ORDER_XCD; ++order)
     for (int tx : {64, 128})
       for (int ty : tys)
         for (int kc : kcs) {
           if (kc < kmin) continue;
-          g->order = order;
-          g->tx = tx;
-          if ((st = fwe_group_set_tiling(g, ty, kc))) return st;
-          double ms = 0;
-          if ((st = group_run_impl(g, 1, &ms))) return st;
-          if ((st = group_run_impl(g, trial_steps, &ms))) return st;
-          if (ms / trial_steps < best) { best = ms / trial_steps; btx = tx; bty = ty; bkc = kc; border = order; }
+          Cand c{0.0, order, tx, ty, kc};
+          if ((st = trial(c, trial_steps, &c.ms))) return st;
+          cands.push_back(c);
         }
-  g->order = border;
-  g->tx = btx;
-  if ((st = fwe_group_set_tiling(g, bty, bkc))) return st;
+  // as fwe_autotune: the best few again with longer trials
+  std::sort(cands.begin(), cands.end(), [](const Cand& p, const Cand& q) { return p.ms < q.ms; });
+  Cand best = cands[0];
+  double best_ms = 1e30;
+  for (size_t q = 0; q < std::min<size_t>(4, cands.size()); ++q) {
+    double ms = 0;
+    if ((st = trial(cands[q], 4 * trial_steps, &ms))) return st;
+    if (ms < best_ms) { best_ms = ms; best = cands[q]; }
+  }
+  g->order = best.order;
+  g->tx = best.tx;
+  if ((st = fwe_group_set_tiling(g, best.ty, best.kc))) return st;
   for (fwe_ctx* x : g->m)
     if ((st = fwe_reset(x))) return st;
-  if (best_ms_per_step) *best_ms_per_step = best;
+  if (best_ms_per_step) *best_ms_per_step = best_ms;
   return FWE_OK;
 }
 
