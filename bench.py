@@ -148,6 +148,11 @@ def _roofline(dom, steps, cells, wname):
 
 
 MEDIUM = {"auto": 0, "arrays": 1, "table": 2, "isotropic": 3}  # fwe.MEDIUM_*
+# the reference's only published figure for this path (Performance/PerfWE.m:3):
+# 0.0244 s per iteration at 512^3 on an A100, precision and variant not
+# stated; another grid than any line here, so vs_baseline stays null
+PUBLISHED_A100 = {"value": 5.50, "unit": "Gcell/s", "hardware": "NVIDIA A100", "grid": [512, 512, 512],
+                  "source": "Performance/PerfWE.m:3 (0.0244 s/iteration; Gcell/s = cells / time)"}
 ORDER_NAMES = ("linear", "xcd", "plane split")  # fwe.ORDER_*
 
 
@@ -188,9 +193,29 @@ def bench_plain(args, steps):
     tiling = (s.tile_cols(),) + tuple(s.tiling())
     order = ORDER_NAMES[s.wg_order()]
     enc, ntab = s.medium_encoding()
+    tr_main = s.traces()
     s.close()
     cells = wl.cells
     kernels, dom, bstep = _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells, enc)
+    # the same workload with the medium streamed from its twelve arrays, in
+    # the same process (the encoding changes which bytes stream, not results)
+    arrays_run = None
+    if enc in (fwe.MEDIUM_TABLE, fwe.MEDIUM_ISOTROPIC) and not fused:
+        n2 = min(steps, 1000)
+        s2 = W.make_solver(wl)
+        s2.set_medium_encoding(fwe.MEDIUM_ARRAYS)
+        el2, a2, b2, _ = _timed_run(s2, n2, min(args.warmup, 20), not args.no_autotune)
+        nr = len(wl.receivers)
+        tr2 = s2.traces()
+        # the first n2 samples of every trace, bit for bit
+        same = bool(np.array_equal(tr2.reshape(nr, -1)[:, :n2], tr_main.reshape(nr, -1)[:, :n2]))
+        t2 = (s2.tile_cols(),) + tuple(s2.tiling())
+        s2.close()
+        _, _, bst2 = _kernel_entries(wl, False, t2, a2, b2, n2, cells, fwe.MEDIUM_ARRAYS)
+        arrays_run = {"value": round(cells * wl.nbatch * n2 / el2 / 1e9, 3), "unit": "Gcell/s", "steps": n2,
+                      "ms_per_step": round(el2 / n2 * 1e3, 4), "tiling": list(t2), "alg_bytes_per_cell": bst2,
+                      "hbm_pct_step": round(100 * bst2 * cells * n2 / el2 / 1e9 / PEAK_HBM_GBS, 2),
+                      "traces_identical_to_main_run": same}
     desc = (f"{wl.name}: layered orthorhombic FP32, {wl.info.get('pml_cells')}-cell separable damping, "
             f"lune MT, Ricker {wl.info.get('f0')} Hz, {len(wl.receivers)} receiver traces"
             if wl.mode == fwe.MODE_ORTHO else
@@ -203,7 +228,8 @@ def bench_plain(args, steps):
     return _line(cells * wl.nbatch * steps / el / 1e9, steps, args.warmup, el, config,
                  hbm_pct_step=round(100 * bstep * cells * steps / el / 1e9 / PEAK_HBM_GBS, 2),
                  kernels=kernels, roofline=_roofline(dom, steps, cells, wl.name),
-                 copy_probe_GBs=round(copy_gbs, 1), traces_finite=finite,
+                 copy_probe_GBs=round(copy_gbs, 1), traces_finite=finite, medium_arrays_run=arrays_run,
+                 published_reference=PUBLISHED_A100,
                  cpu_baseline=_cpu_entry(args, W.config(args.config, nt=args.cpu_steps + 2), wl.name))
 
 
