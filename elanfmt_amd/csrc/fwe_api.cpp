@@ -1011,8 +1011,11 @@ extern "C" fwe_status fwe_autotune(fwe_ctx* x, int32_t trial_steps, double* best
       for (int tx : txs) {
         if (tx != 64 && use_fused(x)) continue;
         for (int ty : tys)
-          for (int kc : kcs) {
-            if (kc > nz || kc < kmin) continue;
+          for (int q = 0; q <= 10; ++q) {
+            // the listed chunk heights, then the whole column (so a short
+            // grid always has a candidate)
+            const int kc = q < 10 ? kcs[q] : nz;
+            if (kc > nz || kc < kmin || (q == 10 && std::find(kcs, kcs + 10, nz) != kcs + 10)) continue;
             Cand c{0.0, bt, order, tx, ty, kc};
             if ((st = trial(c, trial_steps, &c.ms))) return st;
             cands.push_back(c);

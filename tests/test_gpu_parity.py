@@ -411,6 +411,27 @@ def test_autotune_and_tilings_agree(gpu):
     s.close()
 
 
+def test_autotune_on_a_three_plane_grid(gpu):
+    """A grid of three planes with BC1 (chunks of at least three planes):
+    the only z chunk is the whole column; autotune must find it."""
+    from elanfmt_amd import fwe, workloads as W
+
+    wl = W.ortho_layered(nx=40, ny=30, nz=3, nl=1, L=2, nt=12, nrec_side=2)
+    wl.free_surface = 1
+    wl.injections = fwe.mt_injections(fwe.SRC_LAYOUT_ANISO, 20, 15, 0, [1e15, -2e14, 3e14, 4e14, -5e14, 2e14])
+    wl.receivers = [(q, 22, 15, 1, 0) for q in range(3)]
+    s = W.make_solver(wl)
+    s.run(wl.nt)
+    ref = s.traces()
+    s.close()
+    s = W.make_solver(wl)
+    assert s.autotune(1) > 0
+    assert s.tiling()[1] == 3
+    s.run(wl.nt)
+    assert np.array_equal(s.traces(), ref) and np.abs(ref).max() > 0
+    s.close()
+
+
 def _run_kernel(wl, kind, tiling=None, steps=None):
     from elanfmt_amd import workloads as W
 
