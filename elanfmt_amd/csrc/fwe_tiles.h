@@ -20,7 +20,14 @@
 //    one eighth of each chunk of planes, and all eight walk the chunks
 //    together);
 //  * the reference's separate source/BC1/record launches are fused into the
-//    two stencil kernels (per-tile injection and receiver lists).
+//    two stencil kernels (per-tile injection lists, per-(tile, plane)
+//    receiver lists);
+//  * the ortho medium is read in one of three forms with the same values
+//    (the law template argument): twelve arrays (MODE_ORTHO), a material
+//    table indexed by one byte per cell (MODE_ORTHO_TAB), or the eight
+//    arrays of an isotropic medium (MODE_ORTHO_ISO);
+//  * a plane's loads are all issued before its arithmetic, one memory round
+//    trip per plane.
 // The stencil is HBM-bound (~0.8 flop/B), so there is no MFMA here.
 #pragma once
 #include <type_traits>
