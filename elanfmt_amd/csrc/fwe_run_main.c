@@ -19,7 +19,10 @@
  * snap=<n> (every n steps, all nine fields as <s>_0_<field>.res, s = 1, 2,
  * ..., and the step in <s>_0_step.inf), restart=<s> (start from snapshot s:
  * the read_data convention "%d_%d_%s.%s" of OK4.cu:166-175, for all nine
- * fields; traces before the restart step stay zero).
+ * fields; traces before the restart step stay zero), tune=<n> (autotune the
+ * tiling with n-step trials before the run, default 3; 0 keeps the default
+ * tiling), graph=<g> (replay g-step HIP graphs instead of launching every
+ * kernel, default 0).
  * Outputs: 0_infos.inf and 0_0_{Src,Rec,Rec_x,Rec_y,Rec_y2,Vx,Vy,Vz}.res.
  */
 #include <math.h>
@@ -37,7 +40,7 @@ static int usage(void) {
           "usage: fwe_run [NBX=.. NBY=.. NBZ=.. OVERX=.. OVERY=.. OVERZ=..] [nx= ny= nz=] [nt=]\n"
           "               [mode=iso|ortho|isohet] [pml=profile|weights|taper|none] [fs=0|1|2]\n"
           "               [xsrc= ysrc= zsrc=] [rec=x:i:j:k:name,...] [dir=.] [device=0] [nout=200]\n"
-          "               [snap=<steps>] [restart=<snapshot>]\n");
+          "               [snap=<steps>] [restart=<snapshot>] [tune=3] [graph=0]\n");
   return 2;
 }
 
@@ -125,6 +128,7 @@ static int read_snapshot(fwe_ctx* ctx, const char* dir, int s, const fwe_config*
 int main(int argc, char** argv) {
   int nbx = -1, nby = -1, nbz = -1, ovx = 0, ovy = 0, ovz = 0, nx = -1, ny = -1, nz = -1;
   int nt = -1, mode = FWE_MODE_ISO, pml = -1, fs = 1, device = 0, nout = 200, snap = 0, restart = -1;
+  int tune = 3, graph = 0;
   int xsrc = -1, ysrc = -1, zsrc = -1;
   int npars[3] = {-1, -1, -1};
   const char* dir = ".";
@@ -164,6 +168,8 @@ int main(int argc, char** argv) {
     else if (KEY("nout")) nout = atoi(v);
     else if (KEY("snap")) snap = atoi(v);
     else if (KEY("restart")) restart = atoi(v);
+    else if (KEY("tune")) tune = atoi(v);
+    else if (KEY("graph")) graph = atoi(v);
     else { fprintf(stderr, "fwe_run: unknown key in '%s'\n", argv[a]); return usage(); }
 #undef KEY
   }
@@ -256,6 +262,22 @@ int main(int argc, char** argv) {
   printf("fwe_run: %s law, %dx%dx%d cells, %d steps, dt=%.6g, free surface %s (%s)\n",
          mode == FWE_MODE_ISO ? "iso" : "ortho", nx, ny, nz, nt, cfg.dt,
          fs == FWE_FS_BC1 ? "BC1" : (fs == FWE_FS_VACUUM ? "vacuum layer" : "off"), fwe_version());
+  /* tiling: autotuned (the context is reset afterwards, before any restart) */
+  if (tune > 0) CHECK(fwe_autotune(ctx, tune, NULL));
+  if (graph > 0) CHECK(fwe_set_graph_steps(ctx, graph));
+  {
+    int32_t rows = 0, zc = 0, cols = 0, order = 0, enc = 0, ntab = 0;
+    CHECK(fwe_get_tiling(ctx, &rows, &zc));
+    CHECK(fwe_get_tile_cols(ctx, &cols));
+    CHECK(fwe_get_wg_order(ctx, &order));
+    CHECK(fwe_get_medium_encoding(ctx, &enc, &ntab));
+    static const char* const ENC[4] = {"auto", "arrays", "material table", "isotropic arrays"};
+    printf("fwe_run: tile %dx%d, %d-plane chunks, order %d%s, medium %s", cols, rows, zc, order,
+           tune > 0 ? " (autotuned)" : "", ENC[enc & 3]);
+    if (enc == FWE_MEDIUM_TABLE) printf(" (%d entries)", ntab);
+    if (graph > 0) printf(", %d-step graphs", graph);
+    printf("\n");
+  }
   int done = 0, isnap = 0;
   if (restart >= 0) {
     if (read_snapshot(ctx, dir, restart, &cfg, &done)) { fwe_destroy(ctx); return 1; }

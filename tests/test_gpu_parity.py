@@ -307,6 +307,16 @@ def test_legacy_drop_in_binary(gpu, tmp_path):
     assert np.array_equal(legacy.read_res(str(tmp_path), "Src"), wl.stf[0])
     vx = legacy.read_res(str(tmp_path), "Vx", (41, 40, 40))
     assert rel_l2(vx, o.get_field(0)) < TOL
+    assert "(autotuned)" in r.stdout
+    # the default tiling with 9-step graph replays: the same files bit for bit
+    first = {n: legacy.read_res(str(tmp_path), n) for n in ["Rec", "Rec_x", "Rec_y", "Rec_y2", "Vx"]}
+    r = subprocess.run([fwe.RUN_PATH, "nx=40", "ny=40", "nz=40", "nt=70", "mode=iso", "fs=1",
+                        f"xsrc={xs}", f"ysrc={xs}", f"zsrc={xs}", f"rec={spec}", "tune=0", "graph=9", "nout=35"],
+                       cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "9-step graphs" in r.stdout and "(autotuned)" not in r.stdout
+    for n, v in first.items():
+        assert np.array_equal(legacy.read_res(str(tmp_path), n), v), n
 
 
 def test_legacy_binary_snapshots_and_restart(gpu, tmp_path):
