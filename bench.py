@@ -115,8 +115,11 @@ def _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells, medium=1, batch
 
     ba, bb = W.alg_bytes(wl.mode, wl.pml, wl.nbatch, fused=fused, medium=medium, batch_tile=batch_tile)
     bt = batch_tile or wl.nbatch
-    # kernel law template argument: 2 ortho through the material table, 3 ortho on isotropic arrays
-    km = medium if medium in (2, 3) and not fused else wl.mode
+    # kernel law template argument: 2 ortho through the material table, 3 ortho on
+    # isotropic arrays, 4 iso through the material table
+    km = wl.mode
+    if medium in (2, 3) and not fused:
+        km = 4 if wl.mode == 0 else medium
     fs = "true" if wl.free_surface else "false"
     if fused:
         kernels = {"fused": {"ms_per_step": round(ms_a / steps, 5), "alg_bytes_per_cell": ba,
@@ -160,6 +163,8 @@ def _medium_desc(wl, medium, ntab):
     from elanfmt_amd import fwe
 
     if wl.mode == fwe.MODE_ISO:
+        if medium == fwe.MEDIUM_TABLE:
+            return f"material table of (c11, c12) ({ntab} entries, one index byte per cell)"
         return "c11, c12 arrays"
     if medium == fwe.MEDIUM_TABLE:
         return f"material table ({ntab} entries, one index byte per cell)"
@@ -197,8 +202,8 @@ def bench_plain(args, steps):
     s.close()
     cells = wl.cells
     kernels, dom, bstep = _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells, enc)
-    # the same workload with the medium streamed from its twelve arrays, in
-    # the same process (the encoding changes which bytes stream, not results)
+    # the same workload with the medium streamed from its arrays, in the
+    # same process (the encoding changes which bytes stream, not results)
     arrays_run = None
     if enc in (fwe.MEDIUM_TABLE, fwe.MEDIUM_ISOTROPIC) and not fused:
         n2 = min(steps, 1000)

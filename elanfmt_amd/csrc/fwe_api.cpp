@@ -296,25 +296,26 @@ static fwe_status upload(fwe_ctx* x, float* dst, const std::vector<float>& h) {
 // ---------------------------------------------------------------------------
 // medium and damping
 // ---------------------------------------------------------------------------
-// Material table of an ortho medium: the distinct (bitwise) tuples
-// (C[0..8], B[0..2]) over every element of the padded layout, padding
-// included, in order of first appearance, and one byte per element naming
-// its tuple.  Built when there are at most FWE_MEDIUM_TABLE_MAX tuples;
+// Material table of a medium: the distinct (bitwise) tuples over every
+// element of the padded layout, padding included, in order of first
+// appearance, and one byte per element naming its tuple.  A tuple is
+// (C[0..8], B[0..2]) for the ortho law and (c11, c12) for the iso law.  Built when there are at most FWE_MEDIUM_TABLE_MAX tuples;
 // otherwise ntab = 0 and the kernels stream the arrays.
 static fwe_status build_material_table(fwe_ctx* x, const std::vector<std::vector<float>>& h) {
   const int64_t n = x->nelem;
+  const int m = (int)h.size();  // 12 (ortho) or 2 (iso: c11, c12)
   std::vector<uint8_t> idx((size_t)n);
   std::vector<float> tab((size_t)12 * FWE_MEDIUM_TABLE_MAX);
   const float* arrays[12];
-  for (int q = 0; q < 12; ++q) arrays[q] = h[q].data();
-  const int32_t nt = fwe_gen_material_table(n, 12, arrays, FWE_MEDIUM_TABLE_MAX, idx.data(), tab.data());
+  for (int q = 0; q < m; ++q) arrays[q] = h[q].data();
+  const int32_t nt = fwe_gen_material_table(n, m, arrays, FWE_MEDIUM_TABLE_MAX, idx.data(), tab.data());
   x->ntab = 0;
   if (nt < 1) return FWE_OK;
   HIPCHK(hipSetDevice(x->cfg.device));
   if (!x->mid) HIPCHK(hipMalloc(&x->mid, (size_t)n));
   if (!x->mtab) HIPCHK(hipMalloc(&x->mtab, sizeof(float) * 12 * FWE_MEDIUM_TABLE_MAX));
   HIPCHK(hipMemcpyAsync(x->mid, idx.data(), (size_t)n, hipMemcpyHostToDevice, x->stream));
-  HIPCHK(hipMemcpyAsync(x->mtab, tab.data(), sizeof(float) * 12 * (size_t)nt, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(x->mtab, tab.data(), sizeof(float) * m * (size_t)nt, hipMemcpyHostToDevice, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   x->ntab = nt;
   return FWE_OK;
@@ -327,19 +328,20 @@ extern "C" fwe_status fwe_set_material_iso(fwe_ctx* x, const float* c11, const f
   const double dt = x->cfg.dt;
   int64_t d[3];
   field_shape(x->cfg, FWE_F_SXX, d);
-  std::vector<float> h;
-  pack(x, c11, d, dt, h);
-  fwe_status st = upload(x, x->medium, h);
+  drop_graph(x);
+  std::vector<std::vector<float>> h(2);
+  pack(x, c11, d, dt, h[0]);
+  fwe_status st = upload(x, x->medium, h[0]);
   if (st) return st;
-  pack(x, c12, d, dt, h);
-  st = upload(x, x->medium + x->nelem, h);
+  pack(x, c12, d, dt, h[1]);
+  st = upload(x, x->medium + x->nelem, h[1]);
   if (st) return st;
   x->c66dt = (float)((double)c66 * dt);
   x->bxdt = (float)((double)bx * dt);
   x->bydt = (float)((double)by * dt);
   x->bzdt = (float)((double)bz * dt);
   x->have_medium = true;
-  return FWE_OK;
+  return build_material_table(x, h);
 }
 
 extern "C" fwe_status fwe_set_material_ortho(fwe_ctx* x, const float* const cij[9],
@@ -628,11 +630,11 @@ static StepArgs make_args(fwe_ctx* x, int from) {
   if (c.mode == FWE_MODE_ORTHO) {
     for (int q = 0; q < 9; ++q) a.C[q] = x->medium + (int64_t)q * x->nelem;
     for (int q = 0; q < 3; ++q) a.B[q] = x->medium + (int64_t)(9 + q) * x->nelem;
-    if (x->ntab > 0) { a.mid = x->mid; a.mtab = x->mtab; }
   } else {
     a.C[0] = x->medium;
     a.C[1] = x->medium + x->nelem;
   }
+  if (x->ntab > 0) { a.mid = x->mid; a.mtab = x->mtab; }
   a.c66dt = x->c66dt; a.bxdt = x->bxdt; a.bydt = x->bydt; a.bzdt = x->bzdt;
   a.vac_k = c.free_surface == FWE_FS_VACUUM ? 1 : -1;
   a.vac_cdt = (float)(FWE_VACUUM_SHEAR_MODULUS * c.dt);
@@ -687,11 +689,10 @@ static bool use_fused(const fwe_ctx* x) { return x->kernel == FWE_KERNEL_FUSED; 
 static hipError_t lb_before(fwe_ctx* x, int from);
 static hipError_t lb_after(fwe_ctx* x, int from);
 
-// the two-pass kernels read the ortho medium through the material table
-// when one was built, unless the arrays were asked for
+// the two-pass kernels read the medium through the material table when one
+// was built, unless the arrays were asked for
 static bool use_table(const fwe_ctx* x) {
-  return x->cfg.mode == FWE_MODE_ORTHO && x->ntab > 0 &&
-         (x->medium_enc == FWE_MEDIUM_AUTO || x->medium_enc == FWE_MEDIUM_TABLE);
+  return x->ntab > 0 && (x->medium_enc == FWE_MEDIUM_AUTO || x->medium_enc == FWE_MEDIUM_TABLE);
 }
 // otherwise an isotropic medium reads eight of the twelve arrays
 static bool use_iso(const fwe_ctx* x) {
@@ -699,7 +700,7 @@ static bool use_iso(const fwe_ctx* x) {
          (x->medium_enc == FWE_MEDIUM_AUTO || x->medium_enc == FWE_MEDIUM_ISOTROPIC);
 }
 static int kernel_mode(const fwe_ctx* x) {
-  if (use_table(x)) return fwe::MODE_ORTHO_TAB;
+  if (use_table(x)) return x->cfg.mode == FWE_MODE_ISO ? fwe::MODE_ISO_TAB : fwe::MODE_ORTHO_TAB;
   if (use_iso(x)) return fwe::MODE_ORTHO_ISO;
   return x->cfg.mode;
 }
@@ -941,8 +942,8 @@ extern "C" fwe_status fwe_set_medium_encoding(fwe_ctx* x, int32_t encoding) {
   if (!x) return fail(FWE_ERR_ARG, "NULL ctx");
   if (encoding < FWE_MEDIUM_AUTO || encoding > FWE_MEDIUM_ISOTROPIC)
     return fail(FWE_ERR_ARG, "bad medium encoding %d", encoding);
-  if ((encoding == FWE_MEDIUM_TABLE || encoding == FWE_MEDIUM_ISOTROPIC) && x->cfg.mode != FWE_MODE_ORTHO)
-    return fail(FWE_ERR_UNSUPPORTED, "the material table and the isotropic arrays are for the ortho law");
+  if (encoding == FWE_MEDIUM_ISOTROPIC && x->cfg.mode != FWE_MODE_ORTHO)
+    return fail(FWE_ERR_UNSUPPORTED, "the isotropic arrays are a form of the ortho law's medium");
   if (encoding == FWE_MEDIUM_TABLE && x->have_medium && x->ntab == 0)
     return fail(FWE_ERR_UNSUPPORTED, "the medium has more than %d distinct tuples", FWE_MEDIUM_TABLE_MAX);
   if (encoding == FWE_MEDIUM_ISOTROPIC && x->have_medium && !x->iso_medium)

@@ -18,8 +18,12 @@ constexpr int TILE_X = 64;
 // MODE_ORTHO_TAB: the ortho law with the medium read through a material
 // table (one byte per cell, see StepArgs::mid).  MODE_ORTHO_ISO: the ortho
 // law on an isotropic medium (C11 = C22 = C33 and C12 = C13 = C23 in every
-// cell), which reads C[0] and C[3] for the six normal moduli.
-enum Mode { MODE_ISO = 0, MODE_ORTHO = 1, MODE_ORTHO_TAB = 2, MODE_ORTHO_ISO = 3 };
+// cell), which reads C[0] and C[3] for the six normal moduli.  MODE_ISO_TAB:
+// the iso law with c11, c12 read through a material table (two floats per
+// entry).
+enum Mode { MODE_ISO = 0, MODE_ORTHO = 1, MODE_ORTHO_TAB = 2, MODE_ORTHO_ISO = 3, MODE_ISO_TAB = 4 };
+// the iso law's update formulas (OK4.cu) rather than the ortho law's
+__host__ __device__ constexpr bool iso_law(int mode) { return mode == MODE_ISO || mode == MODE_ISO_TAB; }
 enum Pml { PML_NONE = 0, PML_PROFILE = 1, PML_W3D = 2, PML_SEP = 3 };
 
 // One injection or receiver, sorted by xy tile on the host.
@@ -48,8 +52,9 @@ struct StepArgs {
   const float* C[9];
   // ortho buoyancy pre-multiplied by dt: bx by bz (staggered)
   const float* B[3];
-  // the same twelve values as a table (MODE_ORTHO_TAB): cell c uses entry
-  // mid[c], twelve floats at mtab + 12*mid[c] in the order C[0..8], B[0..2]
+  // the same values as a table: cell c uses entry mid[c], twelve floats at
+  // mtab + 12*mid[c] in the order C[0..8], B[0..2] (MODE_ORTHO_TAB), or two
+  // at mtab + 2*mid[c], c11 and c12 (MODE_ISO_TAB)
   const uint8_t* mid;
   const float* mtab;
   // iso medium: c11*dt, c12*dt arrays, scalars c66*dt and b*dt

@@ -57,6 +57,11 @@ __global__ __launch_bounds__(256) void k_combine(const float* __restrict__ tr, i
 // dispatch: law x damping form -> instantiation unit
 // ---------------------------------------------------------------------------
 hipError_t launch_stress(const StepArgs& a, int mode, int pml, int fs, hipStream_t s) {
+  if (mode == MODE_ISO_TAB) {
+    if (pml == PML_PROFILE) return stress_mp<MODE_ISO_TAB, PML_PROFILE>(a, fs, s);
+    if (pml == PML_NONE) return stress_mp<MODE_ISO_TAB, PML_NONE>(a, fs, s);
+    return hipErrorInvalidValue;
+  }
   if (mode == MODE_ISO) {
     if (pml == PML_PROFILE) return stress_mp<MODE_ISO, PML_PROFILE>(a, fs, s);
     if (pml == PML_NONE) return stress_mp<MODE_ISO, PML_NONE>(a, fs, s);
@@ -87,6 +92,11 @@ hipError_t launch_stress(const StepArgs& a, int mode, int pml, int fs, hipStream
 }
 
 hipError_t launch_velocity(const StepArgs& a, int mode, int pml, hipStream_t s) {
+  if (mode == MODE_ISO_TAB) {
+    if (pml == PML_PROFILE) return velocity_mp<MODE_ISO_TAB, PML_PROFILE>(a, s);
+    if (pml == PML_NONE) return velocity_mp<MODE_ISO_TAB, PML_NONE>(a, s);
+    return hipErrorInvalidValue;
+  }
   if (mode == MODE_ISO) {
     if (pml == PML_PROFILE) return velocity_mp<MODE_ISO, PML_PROFILE>(a, s);
     if (pml == PML_NONE) return velocity_mp<MODE_ISO, PML_NONE>(a, s);
@@ -118,7 +128,7 @@ hipError_t launch_velocity(const StepArgs& a, int mode, int pml, hipStream_t s) 
 
 hipError_t launch_group(const StepArgs* tab, const int32_t* start, int nm, int nblk, int tx, int ty, int order, int mode, int pml,
                         int fs, bool vel, hipStream_t s) {
-  if (mode == MODE_ORTHO_TAB) return hipErrorInvalidValue;
+  if (mode == MODE_ORTHO_TAB || mode == MODE_ISO_TAB) return hipErrorInvalidValue;
   if (mode == MODE_ORTHO_ISO) {
     switch (pml) {
       case PML_NONE: return group_mp<MODE_ORTHO_ISO, PML_NONE>(tab, start, nm, nblk, tx, ty, order, fs, vel, s);

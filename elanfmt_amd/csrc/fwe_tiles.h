@@ -25,7 +25,8 @@
 //  * the ortho medium is read in one of three forms with the same values
 //    (the law template argument): twelve arrays (MODE_ORTHO), a material
 //    table indexed by one byte per cell (MODE_ORTHO_TAB), or the eight
-//    arrays of an isotropic medium (MODE_ORTHO_ISO);
+//    arrays of an isotropic medium (MODE_ORTHO_ISO); the iso law's c11 and
+//    c12 as two arrays (MODE_ISO) or through a table (MODE_ISO_TAB);
 //  * a plane's loads are all issued before its arithmetic, one memory round
 //    trip per plane.
 // The stencil is HBM-bound (~0.8 flop/B), so there is no MFMA here.
@@ -158,6 +159,9 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb, const int
       m0 = q0.x; m1 = q0.y; m2 = q0.z; m3 = q0.w;
       m4 = q1.x; m5 = q1.y; m6 = q1.z; m7 = q1.w;
       m8 = t[8];
+    } else if (MODE == MODE_ISO_TAB) {
+      const float2 q = *reinterpret_cast<const float2*>(a.mtab + 2u * (uint32_t)a.mid[c]);
+      m0 = q.x; m1 = q.y;
     } else if (MODE == MODE_ORTHO_ISO) {
       // isotropic medium (as the station solver's c11/c12 arrays,
       // FastWaveED_GPU3D_v1_9kmst.cu:381-400): two normal moduli per cell
@@ -203,7 +207,7 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb, const int
       const float dxy = rate2(vy - vy_im, a.rdx, vx - vx_jm, a.rdy);
       const float dxz = rate2(vz - vz_im, a.rdx, vx - vx_km, a.rdz);
       const float dyz = rate2(vy - vy_km, a.rdz, vz - vz_jm, a.rdy);
-      if (MODE == MODE_ISO) {
+      if (iso_law(MODE)) {
         // OK4.cu:307-309,323-329 with c11,c12 arrays holding c*dt
         sxx = iso_normal(fxh, sxx, m0, ex, m1, ey, m1, ez);
         syy = iso_normal(fyh, syy, m1, ex, m0, ey, m1, ez);
@@ -361,9 +365,9 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb, const i
       const float dSx = rate3(sxx - sxx_im, a.rdx, sxy_jp - sxy, a.rdy, sxz_kp - sxz, a.rdz);
       const float dSy = rate3(syy - syy_jm, a.rdy, sxy_ip - sxy, a.rdx, syz_kp - syz, a.rdz);
       const float dSz = rate3(szz - szz_km, a.rdz, sxz_ip - sxz, a.rdx, syz_jp - syz, a.rdy);
-      if (ux) vx = (MODE == MODE_ISO) ? iso_vel(wxm, vx, bx, dSx) : ortho_vel(w, vx, bx, dSx);
-      if (uy) vy = (MODE == MODE_ISO) ? iso_vel(wym, vy, by, dSy) : ortho_vel(w, vy, by, dSy);
-      if (uz) vz = (MODE == MODE_ISO) ? iso_vel(wzm, vz, bz, dSz) : ortho_vel(w, vz, bz, dSz);
+      if (ux) vx = iso_law(MODE) ? iso_vel(wxm, vx, bx, dSx) : ortho_vel(w, vx, bx, dSx);
+      if (uy) vy = iso_law(MODE) ? iso_vel(wym, vy, by, dSy) : ortho_vel(w, vy, by, dSy);
+      if (uz) vz = iso_law(MODE) ? iso_vel(wzm, vz, bz, dSz) : ortho_vel(w, vz, bz, dSz);
       // every cell is written (unchanged outside the update ranges) with
       // streaming stores, so whole lines leave the CU
       st(Vx + c, vx);

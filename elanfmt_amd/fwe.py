@@ -377,9 +377,10 @@ class Solver:
         return c.value
 
     def set_medium_encoding(self, enc):
-        """MEDIUM_AUTO (material table when the medium has one, else the
-        isotropic arrays when they apply), MEDIUM_ARRAYS (all twelve),
-        MEDIUM_TABLE or MEDIUM_ISOTROPIC (fail when they do not apply)."""
+        """MEDIUM_AUTO (material table when the medium has one, either law;
+        else the isotropic arrays when they apply), MEDIUM_ARRAYS (all the
+        arrays), MEDIUM_TABLE or MEDIUM_ISOTROPIC (fail when they do not
+        apply)."""
         _check(lib().fwe_set_medium_encoding(self._h, enc))
 
     def medium_encoding(self):

@@ -272,14 +272,19 @@ def config(idx, **kw):
 # streamed 3-D array counted once per pass (read and/or write), FP32.
 def alg_bytes(mode, pml, nbatch=1, fused=False, medium=fwe.MEDIUM_ARRAYS, batch_tile=None):
     """(stress_bytes, velocity_bytes) per cell per step, all wavefields; for
-    the single-pass kernel the whole step is in the first entry.  medium
-    (ortho law): MEDIUM_ARRAYS, the twelve arrays; MEDIUM_TABLE, one index
-    byte per cell and pass (the table entries stay in the caches);
-    MEDIUM_ISOTROPIC, C11 and C12 for the six normal moduli.  batch_tile:
+    the single-pass kernel the whole step is in the first entry.  medium:
+    MEDIUM_ARRAYS, the arrays (ortho twelve, iso c11 and c12); MEDIUM_TABLE,
+    one index byte per cell and pass that reads the medium (the table
+    entries stay in the caches; the iso velocity pass reads no medium);
+    MEDIUM_ISOTROPIC (ortho law), C11 and C12 for the six normal moduli.
+    batch_tile:
     wavefields per workgroup (default all); the medium is read once per
     workgroup group of wavefields, nbatch / batch_tile times per cell."""
     w = 1 if pml == fwe.PML_WEIGHTS3D else 0
     reads = nbatch // (batch_tile or nbatch)  # medium reads per cell and pass
+    if medium == fwe.MEDIUM_TABLE and not fused and mode == fwe.MODE_ISO:
+        # stress: R V3 S6 + index byte, W S6 ; velocity: R S6 V3, W V3
+        return 4 * nbatch * 15 + reads, 4 * nbatch * 12
     if medium == fwe.MEDIUM_TABLE and not fused:
         # stress: R V3 S6 [w] + index byte, W S6 ; velocity: R S6 V3 [w] + index byte, W V3
         return 4 * (nbatch * 15 + w * reads) + reads, 4 * (nbatch * 12 + w * reads) + reads

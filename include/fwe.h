@@ -255,14 +255,16 @@ fwe_status fwe_get_wg_order(const fwe_ctx* ctx, int32_t* order);
  * streams 1 byte of medium instead of 48 (the same values, so the same
  * results bit for bit).  fwe_set_material_ortho builds the table when the
  * medium has at most FWE_MEDIUM_TABLE_MAX distinct tuples (layered media:
- * a few tens).  ISOTROPIC: when C11 = C22 = C33 and C12 = C13 = C23 in
+ * a few tens).  The iso law's (c11, c12) pairs get the same table (8 B per
+ * entry; fwe_set_material_iso builds it), so its stress pass streams 1
+ * byte of medium instead of 8.  ISOTROPIC (ortho law only): when C11 = C22 = C33 and C12 = C13 = C23 in
  * every cell (the station solver's c11/c12 file set,
  * FastWaveED_GPU3D_v1_9kmst.cu:381-400, or any isotropic medium), the
  * stress pass reads C11 and C12 for the six, eight arrays in all.  AUTO
  * (default) uses the table when it was built, else the isotropic arrays
- * when they apply, else all twelve; ARRAYS always streams the twelve.  Iso
- * contexts and the single-pass kernel use their arrays; groups use the
- * isotropic arrays when every member may, else the twelve.  get returns
+ * when they apply, else all the arrays; ARRAYS always streams them all.
+ * The single-pass kernel uses the arrays; groups use the arrays too (ortho
+ * groups the isotropic arrays when every member may, else the twelve).  get returns
  * the encoding fwe_run will use and the number of table entries (0
  * without a table). */
 enum { FWE_MEDIUM_AUTO = 0, FWE_MEDIUM_ARRAYS = 1, FWE_MEDIUM_TABLE = 2, FWE_MEDIUM_ISOTROPIC = 3 };

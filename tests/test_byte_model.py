@@ -22,6 +22,8 @@ def test_ortho_single_wavefield():
 def test_iso_law():
     # stress: R V3 S6 c11 c12, W S6; velocity: R S6 V3, W V3 (scalar buoyancy)
     assert W.alg_bytes(fwe.MODE_ISO, fwe.PML_PROFILE) == (68, 48)
+    # material table: the index byte replaces c11 and c12; velocity reads no medium
+    assert W.alg_bytes(fwe.MODE_ISO, fwe.PML_PROFILE, medium=fwe.MEDIUM_TABLE) == (F * 15 + 1, F * 12)
 
 
 def test_batches():

@@ -126,17 +126,68 @@ def test_table_limit_and_heterogeneous_fallback(gpu, nvalues):
     s.close()
 
 
-def test_table_refused_for_iso(gpu):
+@pytest.mark.parametrize("fs,pml", [(0, 1), (1, 1), (0, 0)])
+def test_iso_table_equals_arrays(gpu, fs, pml):
+    """The iso law reads (c11, c12) through the table: a two-layer medium
+    gives the bits of the two streamed arrays, on both damping forms and
+    free-surface modes, and matches the oracle."""
+    from elanfmt_amd import fwe, workloads as W
+    from oracle import Oracle
+
+    wl = W.iso_homogeneous(n=36, L=5, nt=60, rec_depth=3, nrec_side=2, free_surface=fs)
+    # a faster lower half: two tuples in the cells plus the zero padding
+    wl.c11 = wl.c11.copy()
+    wl.c12 = wl.c12.copy()
+    wl.c11[:, :, 18:] *= np.float32(1.3)
+    wl.c12[:, :, 18:] *= np.float32(1.2)
+    if pml == 0:
+        wl.pml, wl.prof = fwe.PML_NONE, None
+    print(f"case iso36 two layers fs={fs} pml={pml}", flush=True)
+    ta, fa, ea = _run(wl, fwe.MEDIUM_ARRAYS)
+    tt, ft, et = _run(wl, fwe.MEDIUM_AUTO)
+    assert ea == (fwe.MEDIUM_ARRAYS, 3) and et == (fwe.MEDIUM_TABLE, 3)
+    assert np.abs(ta).max() > 0
+    assert np.array_equal(tt, ta)
+    for q, (x, y) in enumerate(zip(ft, fa)):
+        assert np.array_equal(x, y), q
+    o = Oracle(wl.nx, wl.ny, wl.nz, wl.dx, wl.dy, wl.dz, wl.dt, wl.nt, mode=wl.mode, pml=wl.pml, free_surface=fs)
+    W.apply(o, wl)
+    o.run(wl.nt)
+    assert rel_l2(tt.astype(np.float64), o.traces()) < 1e-5
+
+
+def test_iso_table_limit_groups_and_refusals(gpu):
+    """An iso medium with more distinct (c11, c12) pairs than the table holds
+    streams the arrays and refuses the table; iso members on the table get
+    the same bits in a group (groups read the arrays); the ortho law's
+    isotropic form is refused for the iso law."""
     from elanfmt_amd import fwe, workloads as W
 
-    wl = W.iso_homogeneous(n=24, L=4, nt=5, rec_depth=3, nrec_side=2)
+    wl = W.iso_homogeneous(n=24, L=4, nt=30, rec_depth=3, nrec_side=2)
+    wl.c11 = wl.c11 * (1.0 + 0.01 * np.arange(24, dtype=np.float32))[:, None, None]
+    wl.c11 = wl.c11 * (1.0 + 0.001 * np.arange(24, dtype=np.float32))[None, :, None]
     s = W.make_solver(wl)
     assert s.medium_encoding() == (fwe.MEDIUM_ARRAYS, 0)
     with pytest.raises(fwe.FweError):
         s.set_medium_encoding(fwe.MEDIUM_TABLE)
     with pytest.raises(fwe.FweError):
-        s.set_medium_encoding(3)
+        s.set_medium_encoding(fwe.MEDIUM_ISOTROPIC)
     s.close()
+    wls = [W.iso_homogeneous(n=n, L=4, nt=40, rec_depth=3, nrec_side=2) for n in (28, 36)]
+    ref = []
+    for w in wls:
+        t, _, e = _run(w, fwe.MEDIUM_AUTO)
+        assert e[0] == fwe.MEDIUM_TABLE
+        ref.append(t)
+    print("case iso group of two table members", flush=True)
+    sol = [W.make_solver(w) for w in wls]
+    g = fwe.Group(sol)
+    g.run(40)
+    for s, r in zip(sol, ref):
+        assert np.array_equal(s.traces(), r)
+    g.close()
+    for s in sol:
+        s.close()
 
 
 def test_group_reads_the_arrays_and_equals_table_runs(gpu):
