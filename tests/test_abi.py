@@ -90,3 +90,44 @@ def test_legacy_params_reader(fwe, tmp_path):
     os.remove(tmp_path / "pa2.dat")
     assert fwe.lib().fwe_read_legacy_params(str(tmp_path).encode(), C.byref(p)) == fwe.ERR_IO
     assert b"pa2.dat" in fwe.lib().fwe_last_error()
+
+
+@pytest.mark.parametrize("field,value,status", [
+    ("nx", 2, "ERR_ARG"), ("nz", 2, "ERR_ARG"), ("nt", -1, "ERR_ARG"), ("nbatch", 0, "ERR_ARG"),
+    ("nbatch", 9, "ERR_ARG"), ("dx", 0.0, "ERR_ARG"), ("dt", -1.0, "ERR_ARG"), ("free_surface", 3, "ERR_ARG"),
+    ("pml", 4, "ERR_ARG"), ("mode", 5, "ERR_ARG")])
+def test_create_rejects_bad_configs(fwe, field, value, status):
+    """Configuration errors are reported before any device is touched, with
+    the status the header documents and no context returned."""
+    import ctypes as C
+
+    cfg = fwe.Config()
+    assert fwe.lib().fwe_config_default(C.byref(cfg), fwe.MODE_ORTHO) == fwe.OK
+    cfg.nx, cfg.ny, cfg.nz, cfg.nt = 16, 12, 10, 5
+    cfg.dx = cfg.dy = cfg.dz = 10.0
+    cfg.dt = 1e-3
+    cfg.pml = fwe.PML_SEPARABLE
+    setattr(cfg, field, value)
+    h = C.c_void_p(1)
+    assert fwe.lib().fwe_create(C.byref(cfg), C.byref(h)) == getattr(fwe, status)
+    assert not h.value
+
+
+def test_create_rejects_unsupported_law_damping_pairs(fwe):
+    import ctypes as C
+
+    for mode, pml in ((fwe.MODE_ISO, fwe.PML_WEIGHTS3D), (fwe.MODE_ISO, fwe.PML_SEPARABLE),
+                      (fwe.MODE_ORTHO, fwe.PML_PROFILE)):
+        cfg = fwe.Config()
+        assert fwe.lib().fwe_config_default(C.byref(cfg), mode) == fwe.OK
+        cfg.nx = cfg.ny = cfg.nz = 8
+        cfg.nt = 3
+        cfg.dx = cfg.dy = cfg.dz = 10.0
+        cfg.dt = 1e-3
+        cfg.pml = pml
+        h = C.c_void_p()
+        assert fwe.lib().fwe_create(C.byref(cfg), C.byref(h)) == fwe.ERR_UNSUPPORTED
+    cfg.nx = cfg.ny = 4096
+    cfg.nz = 1024  # 2^34 cells
+    cfg.pml = fwe.PML_SEPARABLE
+    assert fwe.lib().fwe_create(C.byref(cfg), C.byref(h)) == fwe.ERR_ARG
