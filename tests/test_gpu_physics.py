@@ -72,3 +72,38 @@ def test_rayleigh_wave_speed_under_a_vacuum_layer(gpu):
     c_r = 0.9194 * vs
     print(f"vacuum layer: surface wave speed {speed:.1f} m/s ({speed / c_r - 1:+.2%})", flush=True)
     assert abs(speed / c_r - 1) < 0.04
+
+
+def test_double_couple_on_the_gpu_matches_the_whole_space_solution(gpu):
+    """The HIP path (the iso law through its material table) against the
+    closed-form whole-space field of a double couple Mxy: the bounds the
+    oracle meets, at 10-18 cells from the source (receiver geometry and
+    sign convention as in test_oracle_green)."""
+    from elanfmt_amd import fwe, workloads as W
+    from test_oracle_green import DC_F0, DC_M0, DC_N, DC_OFFS, DC_S, DX, RHO, VP, VS, _check, _ricker, \
+        whole_space_velocity
+
+    nt = 540
+    dt = W.cfl_dt(VP, DX, DX, DX, 0.5)
+    n, s = DC_N, DC_S
+    wl = W.Workload("dc112", n, n, n, DX, DX, DX, dt, nt, fwe.MODE_ISO, fwe.PML_PROFILE,
+                    c11=np.full((n, n, n), RHO * VP * VP, np.float32),
+                    c12=np.full((n, n, n), RHO * (VP * VP - 2 * VS * VS), np.float32), c66=RHO * VS * VS,
+                    bscal=(1 / RHO,) * 3, prof=tuple(W.pml_profile(n, 16, dt) for _ in range(3)),
+                    stf=[(W.ricker(nt, dt, DC_F0, 1.2 / DC_F0) * np.float32(dt / DX ** 3)).astype(np.float32)],
+                    injections=[(fwe.SXY, s, s, s, 0, DC_M0)],
+                    receivers=[(1, s + m, s, s, 0) for m in DC_OFFS] + [(0, s, s + m, s, 0) for m in DC_OFFS])
+    print("case dc112 whole-space double couple", flush=True)
+    sol = W.make_solver(wl)
+    assert sol.medium_encoding()[0] == fwe.MEDIUM_TABLE
+    sol.run(nt)
+    tr = sol.traces().astype(np.float64)
+    sol.close()
+    t = (np.arange(nt) + 1) * dt
+    rate, drate = _ricker(DC_F0, 1.2 / DC_F0)
+    mxy = np.zeros((3, 3))
+    mxy[0, 1] = mxy[1, 0] = -DC_M0
+    for q, m in enumerate(DC_OFFS):
+        r = (m + 0.5) * DX
+        _check(tr[q], whole_space_velocity(mxy, (1, 0, 0), r, t, rate, drate)[1], 0.06, 0.02)
+        _check(tr[len(DC_OFFS) + q], whole_space_velocity(mxy, (0, 1, 0), r, t, rate, drate)[0], 0.06, 0.02)
