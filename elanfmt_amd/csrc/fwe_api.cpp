@@ -1367,7 +1367,9 @@ struct fwe_group {
   int device = 0, mode = 0, pml = 0, fs = 0, tx = 64, ty = 4, order = FWE_ORDER_LINEAR;
   hipStream_t stream = nullptr;
   StepArgs* tab = nullptr;
-  int32_t* start = nullptr;
+  int32_t* wg = nullptr;  // per workgroup: (model, workgroup within the model)
+  int64_t wg_cap = 0;     // workgroups wg has room for
+  std::vector<int32_t> wg_start;  // first workgroup of every model, as wg holds them
   int32_t* it_dev = nullptr;
   int nblk = 0;
 };
@@ -1386,7 +1388,23 @@ static fwe_status group_table(fwe_group* g) {
   g->nblk = start.back();
   HIPCHK(hipSetDevice(g->device));
   HIPCHK(hipMemcpy(g->tab, tab.data(), sizeof(StepArgs) * tab.size(), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(g->start, start.data(), sizeof(int32_t) * start.size(), hipMemcpyHostToDevice));
+  if (start == g->wg_start) return FWE_OK;  // the tilings did not change
+  std::vector<int32_t> wg(2 * (size_t)g->nblk);
+  for (size_t m = 0; m + 1 < start.size(); ++m)
+    for (int32_t b = start[m]; b < start[m + 1]; ++b) {
+      wg[2 * (size_t)b] = (int32_t)m;
+      wg[2 * (size_t)b + 1] = b - start[m];
+    }
+  g->wg_start.clear();
+  if (g->nblk > g->wg_cap) {
+    HIPCHK(hipFree(g->wg));
+    g->wg = nullptr;
+    g->wg_cap = 0;
+    HIPCHK(hipMalloc(&g->wg, sizeof(int32_t) * wg.size()));
+    g->wg_cap = g->nblk;
+  }
+  HIPCHK(hipMemcpy(g->wg, wg.data(), sizeof(int32_t) * wg.size(), hipMemcpyHostToDevice));
+  g->wg_start = start;
   return FWE_OK;
 }
 
@@ -1421,7 +1439,6 @@ extern "C" fwe_status fwe_group_create(int32_t n, fwe_ctx* const* members, fwe_g
   }
   hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&g->tab, sizeof(StepArgs) * n);
-  if (e == hipSuccess) e = hipMalloc(&g->start, sizeof(int32_t) * (n + 1));
   if (e == hipSuccess) e = hipMalloc(&g->it_dev, sizeof(int32_t));
   if (e != hipSuccess) {
     fwe_group_destroy(g);
@@ -1437,7 +1454,7 @@ extern "C" fwe_status fwe_group_destroy(fwe_group* g) {
   if (!g) return FWE_OK;
   hipSetDevice(g->device);
   if (g->stream) hipStreamSynchronize(g->stream);
-  hipFree(g->tab); hipFree(g->start); hipFree(g->it_dev);
+  hipFree(g->tab); hipFree(g->wg); hipFree(g->it_dev);
   if (g->stream) hipStreamDestroy(g->stream);
   delete g;
   return FWE_OK;
@@ -1469,9 +1486,9 @@ static fwe_status group_run_impl(fwe_group* g, int32_t nsteps, double* ms) {
   HIPCHK(fwe::launch_advance(g->it_dev, g->m[0]->it, 1, g->stream));
   if (ms) HIPCHK(hipEventRecord(e0, g->stream));
   for (int s = 0; s < nsteps; ++s) {
-    HIPCHK(fwe::launch_group(g->tab, g->start, (int)g->m.size(), g->nblk, g->tx, g->ty, g->order, gm, g->pml, g->fs, false,
+    HIPCHK(fwe::launch_group(g->tab, g->wg, g->nblk, g->tx, g->ty, g->order, gm, g->pml, g->fs, false,
                              g->stream));
-    HIPCHK(fwe::launch_group(g->tab, g->start, (int)g->m.size(), g->nblk, g->tx, g->ty, g->order, gm, g->pml, g->fs, true,
+    HIPCHK(fwe::launch_group(g->tab, g->wg, g->nblk, g->tx, g->ty, g->order, gm, g->pml, g->fs, true,
                              g->stream));
     HIPCHK(fwe::launch_advance(g->it_dev, 1, 0, g->stream));
   }

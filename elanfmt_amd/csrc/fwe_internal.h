@@ -136,14 +136,15 @@ hipError_t stress_mp(const StepArgs& a, int fs, hipStream_t s);
 template <int MODE, int PML>
 hipError_t velocity_mp(const StepArgs& a, hipStream_t s);
 template <int MODE, int PML>
-hipError_t group_mp(const StepArgs* tab, const int32_t* start, int nm, int nblk, int tx, int ty, int order, int fs, bool vel,
+hipError_t group_mp(const StepArgs* tab, const int32_t* wg, int nblk, int tx, int ty, int order, int fs, bool vel,
                     hipStream_t s);
 
 hipError_t launch_stress(const StepArgs& a, int mode, int pml, int fs, hipStream_t s);
 hipError_t launch_velocity(const StepArgs& a, int mode, int pml, hipStream_t s);
-// one stress (vel=false) or velocity (vel=true) launch over the tiles of nm
-// single-wavefield models; tab/start in device memory, nblk = start[nm]
-hipError_t launch_group(const StepArgs* tab, const int32_t* start, int nm, int nblk, int tx, int ty, int order, int mode, int pml,
+// one stress (vel=false) or velocity (vel=true) launch over the tiles of a
+// group of single-wavefield models: nblk workgroups, tab (one entry per
+// model) and wg (per workgroup: model, workgroup within it) in device memory
+hipError_t launch_group(const StepArgs* tab, const int32_t* wg, int nblk, int tx, int ty, int order, int mode, int pml,
                         int fs, bool vel, hipStream_t s);
 // *it_base = g (set) or *it_base += g
 hipError_t launch_advance(int32_t* it_base, int32_t g, int32_t set, hipStream_t s);

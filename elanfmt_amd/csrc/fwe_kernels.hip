@@ -126,26 +126,26 @@ hipError_t launch_velocity(const StepArgs& a, int mode, int pml, hipStream_t s) 
   }
 }
 
-hipError_t launch_group(const StepArgs* tab, const int32_t* start, int nm, int nblk, int tx, int ty, int order, int mode, int pml,
+hipError_t launch_group(const StepArgs* tab, const int32_t* wg, int nblk, int tx, int ty, int order, int mode, int pml,
                         int fs, bool vel, hipStream_t s) {
   if (mode == MODE_ORTHO_TAB || mode == MODE_ISO_TAB) return hipErrorInvalidValue;
   if (mode == MODE_ORTHO_ISO) {
     switch (pml) {
-      case PML_NONE: return group_mp<MODE_ORTHO_ISO, PML_NONE>(tab, start, nm, nblk, tx, ty, order, fs, vel, s);
-      case PML_W3D: return group_mp<MODE_ORTHO_ISO, PML_W3D>(tab, start, nm, nblk, tx, ty, order, fs, vel, s);
-      case PML_SEP: return group_mp<MODE_ORTHO_ISO, PML_SEP>(tab, start, nm, nblk, tx, ty, order, fs, vel, s);
+      case PML_NONE: return group_mp<MODE_ORTHO_ISO, PML_NONE>(tab, wg, nblk, tx, ty, order, fs, vel, s);
+      case PML_W3D: return group_mp<MODE_ORTHO_ISO, PML_W3D>(tab, wg, nblk, tx, ty, order, fs, vel, s);
+      case PML_SEP: return group_mp<MODE_ORTHO_ISO, PML_SEP>(tab, wg, nblk, tx, ty, order, fs, vel, s);
       default: return hipErrorInvalidValue;
     }
   }
   if (mode == MODE_ISO) {
-    if (pml == PML_PROFILE) return group_mp<MODE_ISO, PML_PROFILE>(tab, start, nm, nblk, tx, ty, order, fs, vel, s);
-    if (pml == PML_NONE) return group_mp<MODE_ISO, PML_NONE>(tab, start, nm, nblk, tx, ty, order, fs, vel, s);
+    if (pml == PML_PROFILE) return group_mp<MODE_ISO, PML_PROFILE>(tab, wg, nblk, tx, ty, order, fs, vel, s);
+    if (pml == PML_NONE) return group_mp<MODE_ISO, PML_NONE>(tab, wg, nblk, tx, ty, order, fs, vel, s);
     return hipErrorInvalidValue;
   }
   switch (pml) {
-    case PML_NONE: return group_mp<MODE_ORTHO, PML_NONE>(tab, start, nm, nblk, tx, ty, order, fs, vel, s);
-    case PML_W3D: return group_mp<MODE_ORTHO, PML_W3D>(tab, start, nm, nblk, tx, ty, order, fs, vel, s);
-    case PML_SEP: return group_mp<MODE_ORTHO, PML_SEP>(tab, start, nm, nblk, tx, ty, order, fs, vel, s);
+    case PML_NONE: return group_mp<MODE_ORTHO, PML_NONE>(tab, wg, nblk, tx, ty, order, fs, vel, s);
+    case PML_W3D: return group_mp<MODE_ORTHO, PML_W3D>(tab, wg, nblk, tx, ty, order, fs, vel, s);
+    case PML_SEP: return group_mp<MODE_ORTHO, PML_SEP>(tab, wg, nblk, tx, ty, order, fs, vel, s);
     default: return hipErrorInvalidValue;
   }
 }

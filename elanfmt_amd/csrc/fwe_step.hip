@@ -13,7 +13,7 @@ template hipError_t stress_mp<FWE_MODE, FWE_PML>(const StepArgs&, int, hipStream
 template hipError_t velocity_mp<FWE_MODE, FWE_PML>(const StepArgs&, hipStream_t);
 #if FWE_MODE != 2 && FWE_MODE != 4
 // groups read the medium arrays (no material table)
-template hipError_t group_mp<FWE_MODE, FWE_PML>(const StepArgs*, const int32_t*, int, int, int, int, int, int, bool,
+template hipError_t group_mp<FWE_MODE, FWE_PML>(const StepArgs*, const int32_t*, int, int, int, int, int, bool,
                                                 hipStream_t);
 #endif
 }  // namespace fwe

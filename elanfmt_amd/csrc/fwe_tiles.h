@@ -400,34 +400,22 @@ __global__ __launch_bounds__(TX * TY) void k_velocity(const StepArgs a) {
 }
 
 // Many models per launch (an uneven batch of independent grids): the
-// workgroups of all models form one grid; start[m] is the first workgroup
-// of model m and tab[m] its step arguments (read from device memory).
-__device__ __forceinline__ int group_model(const int32_t* __restrict__ start, int nm, int lb) {
-  // binary search for the last m with start[m] <= lb: log2(nm) dependent
-  // scalar loads at the start of every workgroup instead of up to nm
-  int lo = 0, hi = nm;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (start[mid] <= lb) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
+// workgroups of all models form one grid; wg[2*lb] is the model of logical
+// workgroup lb and wg[2*lb+1] its workgroup within that model (one 8-byte
+// scalar load), tab[m] the model's step arguments (read from device memory).
 template <int MODE, int PML, bool FS, int TX, int TY>
 __global__ __launch_bounds__(TX * TY) void k_stress_group(const StepArgs* __restrict__ tab,
-                                                         const int32_t* __restrict__ start, int32_t nm,
-                                                         int32_t order) {
+                                                         const int32_t* __restrict__ wg, int32_t order) {
   const int lb = order ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-  const int m = group_model(start, nm, lb);
-  stress_tile<MODE, PML, FS, TX, TY, 1>(tab[m], lb - start[m], 0);
+  const int2 ml = *reinterpret_cast<const int2*>(wg + 2 * (int64_t)lb);
+  stress_tile<MODE, PML, FS, TX, TY, 1>(tab[ml.x], ml.y, 0);
 }
 template <int MODE, int PML, int TX, int TY>
 __global__ __launch_bounds__(TX * TY) void k_velocity_group(const StepArgs* __restrict__ tab,
-                                                           const int32_t* __restrict__ start, int32_t nm,
-                                                           int32_t order) {
+                                                           const int32_t* __restrict__ wg, int32_t order) {
   const int lb = order ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-  const int m = group_model(start, nm, lb);
-  velocity_tile<MODE, PML, TX, TY, 1>(tab[m], lb - start[m], 0);
+  const int2 ml = *reinterpret_cast<const int2*>(wg + 2 * (int64_t)lb);
+  velocity_tile<MODE, PML, TX, TY, 1>(tab[ml.x], ml.y, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -508,17 +496,17 @@ static hipError_t go_vel(const StepArgs& a, hipStream_t s) {
 }
 
 template <int MODE, int PML, bool FS, int TX>
-static hipError_t go_group2(const StepArgs* tab, const int32_t* start, int nm, int nblk, int ty, int order, bool vel,
+static hipError_t go_group2(const StepArgs* tab, const int32_t* wg, int nblk, int ty, int order, bool vel,
                             hipStream_t s) {
   if (ty == 2) {
-    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, TX, 2>), dim3(nblk), dim3(TX, 2), 0, s, tab, start, nm, order);
-    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, TX, 2>), dim3(nblk), dim3(TX, 2), 0, s, tab, start, nm, order);
+    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, TX, 2>), dim3(nblk), dim3(TX, 2), 0, s, tab, wg, order);
+    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, TX, 2>), dim3(nblk), dim3(TX, 2), 0, s, tab, wg, order);
   } else if (ty == 4) {
-    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, TX, 4>), dim3(nblk), dim3(TX, 4), 0, s, tab, start, nm, order);
-    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, TX, 4>), dim3(nblk), dim3(TX, 4), 0, s, tab, start, nm, order);
+    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, TX, 4>), dim3(nblk), dim3(TX, 4), 0, s, tab, wg, order);
+    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, TX, 4>), dim3(nblk), dim3(TX, 4), 0, s, tab, wg, order);
   } else if (ty == 8) {
-    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, TX, 8>), dim3(nblk), dim3(TX, 8), 0, s, tab, start, nm, order);
-    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, TX, 8>), dim3(nblk), dim3(TX, 8), 0, s, tab, start, nm, order);
+    if (vel) hipLaunchKernelGGL((k_velocity_group<MODE, PML, TX, 8>), dim3(nblk), dim3(TX, 8), 0, s, tab, wg, order);
+    else hipLaunchKernelGGL((k_stress_group<MODE, PML, FS, TX, 8>), dim3(nblk), dim3(TX, 8), 0, s, tab, wg, order);
   } else {
     return hipErrorInvalidValue;
   }
@@ -526,10 +514,10 @@ static hipError_t go_group2(const StepArgs* tab, const int32_t* start, int nm, i
 }
 
 template <int MODE, int PML, bool FS>
-static hipError_t go_group(const StepArgs* tab, const int32_t* start, int nm, int nblk, int tx, int ty, int order,
+static hipError_t go_group(const StepArgs* tab, const int32_t* wg, int nblk, int tx, int ty, int order,
                            bool vel, hipStream_t s) {
-  if (tx == 64) return go_group2<MODE, PML, FS, 64>(tab, start, nm, nblk, ty, order, vel, s);
-  if (tx == 128) return go_group2<MODE, PML, FS, 128>(tab, start, nm, nblk, ty, order, vel, s);
+  if (tx == 64) return go_group2<MODE, PML, FS, 64>(tab, wg, nblk, ty, order, vel, s);
+  if (tx == 128) return go_group2<MODE, PML, FS, 128>(tab, wg, nblk, ty, order, vel, s);
   return hipErrorInvalidValue;
 }
 
@@ -544,10 +532,10 @@ hipError_t velocity_mp(const StepArgs& a, hipStream_t s) {
 }
 
 template <int MODE, int PML>
-hipError_t group_mp(const StepArgs* tab, const int32_t* start, int nm, int nblk, int tx, int ty, int order, int fs,
+hipError_t group_mp(const StepArgs* tab, const int32_t* wg, int nblk, int tx, int ty, int order, int fs,
                     bool vel, hipStream_t s) {
-  return fs ? go_group<MODE, PML, true>(tab, start, nm, nblk, tx, ty, order, vel, s)
-            : go_group<MODE, PML, false>(tab, start, nm, nblk, tx, ty, order, vel, s);
+  return fs ? go_group<MODE, PML, true>(tab, wg, nblk, tx, ty, order, vel, s)
+            : go_group<MODE, PML, false>(tab, wg, nblk, tx, ty, order, vel, s);
 }
 
 }  // namespace fwe
