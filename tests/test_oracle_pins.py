@@ -318,3 +318,95 @@ def test_zero_source_stays_zero_and_cfl_limit():
         o.run(400)
         growth.append(o.sum_sq_velocity() / e0)
     assert growth[0] < 10 and growth[1] > 1e6
+
+
+def _single_element_medium(nx, ny, nz, where):
+    """Uniform ortho medium whose nine moduli and three buoyancies have
+    distinct background values, with one element of every array raised by
+    its own factor at staggered index `where`."""
+    sh = _shapes(nx, ny, nz)
+    # C11 C22 C33 C12 C13 C23 on cells, C44yz C55xz C66xy on edges
+    base_c = [2.0, 2.5, 3.0, 0.5, 0.7, 0.9, 1.1, 1.3, 1.7]
+    base_b = [0.8, 0.6, 0.4]
+    cell_of_c = [(nx, ny, nz)] * 6 + [sh[F_SYZ], sh[F_SXZ], sh[F_SXY]]
+    cij = [np.full(s, v) for s, v in zip(cell_of_c, base_c)]
+    buoy = [np.full(sh[f], v) for f, v in zip((F_VX, F_VY, F_VZ), base_b)]
+    for n, a in enumerate(cij):
+        a[where] *= 1.5 + 0.25 * n
+    for n, a in enumerate(buoy):
+        a[where] *= 3.0 + 0.5 * n
+    return cij, buoy
+
+
+def test_ortho_heterogeneous_staggered_medium_indexing():
+    """Which element of each staggered medium array a cell update reads.
+
+    ANISO.cu:327-352: Sxy(i,j,k) reads C66xy at its own edge index
+    i+j(nx+1)+k(nx+1)(ny+1) (:333), Sxz C55xz at i+j(nx+1)+k(nx+1)ny (:342),
+    Syz C44yz at i+j nx+k nx(ny+1) (:351); ANISO.cu:374-385: Vx, Vy, Vz read
+    vrhox11het/vrhoy11het/vrhoz11het at the velocity's own staggered index.
+    Each array holds a uniform background value with one element at (3,4,2)
+    raised by its own factor.  A linear velocity field (constant strain
+    rates) then makes every shear stress dt*C*(rate) with C the background,
+    except exactly the element at (3,4,2) of the stress whose modulus was
+    raised; a linear stress field does the same for the velocities through
+    the buoyancies.  Reading an edge array with the cell index, or swapping
+    two moduli or buoyancies, moves or rescales the changed element."""
+    nx, ny, nz = 9, 8, 7
+    dx, dy, dz, dt = 2.0, 3.0, 5.0, 1e-3
+    where = (3, 4, 2)
+    from oracle import Oracle
+
+    cij, buoy = _single_element_medium(nx, ny, nz, where)
+    sh = _shapes(nx, ny, nz)
+    o = Oracle(nx, ny, nz, dx, dy, dz, dt, 2, mode=1, pml=0, free_surface=0)
+    o.set_material_ortho(cij, buoy)
+    # float32 images of the medium, as the oracle stores them
+    cij = [np.float32(a).astype(np.float64) for a in cij]
+    buoy = [np.float32(a).astype(np.float64) for a in buoy]
+
+    # --- shear moduli: V linear, S zero ---------------------------------
+    ax, az = 0.3, -0.2    # Vx = ax*y + az*z
+    by_, bz_ = 0.7, 0.1   # Vy = by_*x + bz_*z
+    cx, cy = -0.4, 0.9    # Vz = cx*x + cy*y
+    gx = lambda s: np.arange(s[0])[:, None, None] * dx
+    gy = lambda s: np.arange(s[1])[None, :, None] * dy
+    gz = lambda s: np.arange(s[2])[None, None, :] * dz
+    o.set_field(F_VX, np.broadcast_to(ax * gy(sh[F_VX]) + az * gz(sh[F_VX]), sh[F_VX]))
+    o.set_field(F_VY, np.broadcast_to(by_ * gx(sh[F_VY]) + bz_ * gz(sh[F_VY]), sh[F_VY]))
+    o.set_field(F_VZ, np.broadcast_to(cx * gx(sh[F_VZ]) + cy * gy(sh[F_VZ]), sh[F_VZ]))
+    for f in range(F_SXX, F_SYZ + 1):
+        o.set_field(f, np.zeros(sh[f]))
+    o.run(1)
+    m = _ranges(nx, ny, nz)
+    # (field, modulus array index, background modulus, strain-rate sum)
+    for f, ci, rate in ((F_SXY, 8, by_ + ax), (F_SXZ, 7, cx + az), (F_SYZ, 6, bz_ + cy)):
+        got = o.get_field(f)
+        bg = dt * cij[ci].flat[0 if where != (0, 0, 0) else 1] * rate
+        want = np.where(m[f], bg, 0.0)
+        want[where] = dt * cij[ci][where] * rate
+        assert np.allclose(got, want, rtol=1e-12, atol=1e-18), f
+        # exactly one element differs from the background
+        changed = np.argwhere(np.abs(got - bg) > 1e-9 * abs(bg) + 1e-18)
+        changed = [tuple(c) for c in changed if m[f][tuple(c)]]
+        assert changed == [where], (f, changed)
+    # the normal stresses see no strain: they stay zero
+    for f in (F_SXX, F_SYY, F_SZZ):
+        assert np.all(o.get_field(f) == 0.0), f
+
+    # --- buoyancies: V zero, normal stresses linear ----------------------
+    o.reset()
+    p, q, r = 0.6, -0.8, 1.2  # Sxx = p*x, Syy = q*y, Szz = r*z
+    for f in (F_VX, F_VY, F_VZ, F_SXY, F_SXZ, F_SYZ):
+        o.set_field(f, np.zeros(sh[f]))
+    o.set_field(F_SXX, np.broadcast_to(p * gx(sh[F_SXX]), sh[F_SXX]))
+    o.set_field(F_SYY, np.broadcast_to(q * gy(sh[F_SYY]), sh[F_SYY]))
+    o.set_field(F_SZZ, np.broadcast_to(r * gz(sh[F_SZZ]), sh[F_SZZ]))
+    o.run(1)  # stress pass: zero strain rates leave S unchanged (no damping)
+    for f, bi, grad in ((F_VX, 0, p), (F_VY, 1, q), (F_VZ, 2, r)):
+        got = o.get_field(f)
+        want = np.where(m[f], buoy[bi] * grad * dt, 0.0)
+        assert np.allclose(got, want, rtol=1e-12, atol=1e-18), f
+        bg = buoy[bi].flat[0] * grad * dt
+        changed = [tuple(c) for c in np.argwhere(np.abs(got - bg) > 1e-9 * abs(bg)) if m[f][tuple(c)]]
+        assert changed == [where], (f, changed)
