@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1085,9 +1086,11 @@ extern "C" fwe_status fwe_get_traces(fwe_ctx* x, float* out) {
   return FWE_OK;
 }
 
-extern "C" fwe_status fwe_combine_traces(fwe_ctx* x, int32_t n, const float* coef, float* out) {
+// Arguments and receiver layout of a trace combination: the receiver list
+// must be nbatch repeats of the same nr receivers.  *nr_out = nr.
+static fwe_status combine_check(const fwe_ctx* x, int32_t n, const float* coef, const float* out, int* nr_out) {
   if (!x || n < 0 || (n > 0 && (!coef || !out))) return fail(FWE_ERR_ARG, "bad combine arguments");
-  const int nb = x->cfg.nbatch, nt = x->cfg.nt;
+  const int nb = x->cfg.nbatch;
   if (x->nrec == 0 || x->nrec % nb) return fail(FWE_ERR_STATE, "receivers must be %d equal per-wavefield sets", nb);
   const int nr = x->nrec / nb;
   for (int b = 0; b < nb; ++b)
@@ -1098,6 +1101,15 @@ extern "C" fwe_status fwe_combine_traces(fwe_ctx* x, int32_t n, const float* coe
         return fail(FWE_ERR_STATE, "receiver %d of wavefield %d does not repeat receiver %d of wavefield 0",
                     b * nr + r, b, r);
     }
+  *nr_out = nr;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_combine_traces(fwe_ctx* x, int32_t n, const float* coef, float* out) {
+  int nr = 0;
+  const fwe_status cs = combine_check(x, n, coef, out, &nr);
+  if (cs != FWE_OK) return cs;
+  const int nb = x->cfg.nbatch, nt = x->cfg.nt;
   if (n == 0 || nt == 0) return FWE_OK;
   HIPCHK(hipSetDevice(x->cfg.device));
   const int64_t per = (int64_t)nr * nt;
@@ -1121,6 +1133,73 @@ extern "C" fwe_status fwe_combine_traces(fwe_ctx* x, int32_t n, const float* coe
   hipFree(dc);
   hipFree(dout);
   return st;
+}
+
+// ---------------------------------------------------------------------------
+// trace combination without allocation
+// ---------------------------------------------------------------------------
+// Staging buffer of fwe_combine_traces_noalloc: the coefficients of a chunk
+// of sources, then (from a 256-byte boundary) their combined traces.  Each
+// device holds its own copy at its own address; host code never uses the
+// array's name as a device address, only the address hipGetSymbolAddress
+// returned for the current device (noalloc_scratch below).
+static constexpr int64_t NOALLOC_FLOATS = FWE_NOALLOC_SCRATCH_BYTES / sizeof(float);
+static __device__ __align__(256) float scratch[NOALLOC_FLOATS];
+
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static float* g_scratch_ptr[NOALLOC_MAX_DEVICES];
+
+// Device address of `scratch` on `dev`, looked up once per device with
+// that device current (the caller has made it current).
+static fwe_status noalloc_scratch(int dev, float** ptr) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(FWE_ERR_ARG, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(FWE_ERR_HIP, "static staging buffer on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<float*>(p);
+  }
+  *ptr = g_scratch_ptr[dev];
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_combine_traces_noalloc(fwe_ctx* x, int32_t n, const float* coef, float* out) {
+  int nr = 0;
+  const fwe_status cs = combine_check(x, n, coef, out, &nr);
+  if (cs != FWE_OK) return cs;
+  const int nb = x->cfg.nbatch, nt = x->cfg.nt;
+  if (n == 0 || nt == 0) return FWE_OK;
+  const int64_t per = (int64_t)nr * nt;
+  // a chunk of mc sources: coefficients at [0, cb), traces at [cb, cb + mc*per)
+  auto need = [&](int64_t mc) { return round_up(mc * nb, 64) + mc * per; };
+  if (need(1) > NOALLOC_FLOATS)
+    return fail(FWE_ERR_ARG, "one combined source needs %lld floats; the static staging buffer holds %lld",
+                (long long)need(1), (long long)NOALLOC_FLOATS);
+  int64_t mc = std::min<int64_t>(n, NOALLOC_FLOATS / (nb + per));
+  while (mc > 1 && need(mc) > NOALLOC_FLOATS) --mc;
+  HIPCHK(hipSetDevice(x->cfg.device));
+  float* base = nullptr;
+  const fwe_status ss = noalloc_scratch(x->cfg.device, &base);
+  if (ss != FWE_OK) return ss;
+  float* dc = base;
+  float* dout = base + round_up(mc * nb, 64);
+  hipError_t e = hipSuccess;
+  for (int m0 = 0; m0 < n && e == hipSuccess; m0 += (int)mc) {
+    const int m = (int)std::min<int64_t>(mc, n - m0);
+    e = hipMemcpyAsync(dc, coef + (size_t)m0 * nb, sizeof(float) * (size_t)m * nb, hipMemcpyHostToDevice,
+                       x->stream);
+    if (e == hipSuccess) e = fwe::launch_combine(x->traces, nb, per, dc, m, dout, x->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(out + (size_t)m0 * per, dout, sizeof(float) * (size_t)m * per, hipMemcpyDeviceToHost,
+                         x->stream);
+    // the next chunk overwrites the staging buffer
+    if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
+  }
+  if (e != hipSuccess) return fail(FWE_ERR_HIP, "combine (no allocation): %s", hipGetErrorString(e));
+  return FWE_OK;
 }
 
 extern "C" fwe_status fwe_get_field(fwe_ctx* x, int32_t batch, int32_t field, float* out) {

@@ -103,6 +103,7 @@ SIGNATURES = {
     "fwe_set_step": (C.c_int, [C.c_void_p, C.c_int32]),
     "fwe_get_traces": (C.c_int, [C.c_void_p, _FP]),
     "fwe_combine_traces": (C.c_int, [C.c_void_p, C.c_int32, _FP, _FP]),
+    "fwe_combine_traces_noalloc": (C.c_int, [C.c_void_p, C.c_int32, _FP, _FP]),
     "fwe_get_field": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _FP]),
     "fwe_set_field": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _FP]),
     "fwe_read_legacy_params": (C.c_int, [C.c_char_p, C.POINTER(LegacyParams)]),
@@ -410,17 +411,20 @@ class Solver:
             _check(st)
         return out[: self.nrec * self.cfg.nt].reshape(self.nrec, self.cfg.nt)
 
-    def combine_traces(self, coef):
+    def combine_traces(self, coef, noalloc=False):
         """Traces of len(coef) sources, each a linear combination (coef row,
         one weight per wavefield) of the nbatch elementary-source wavefields;
-        shape (n, receivers per wavefield, nt)."""
+        shape (n, receivers per wavefield, nt).  noalloc: stage through the
+        library's static device buffer (fwe_combine_traces_noalloc) instead
+        of per-call device allocations."""
         cf = np.ascontiguousarray(np.atleast_2d(coef), dtype=np.float32)  # row m = source m
         n, nb = cf.shape
         if nb != self.cfg.nbatch:
             raise ValueError(f"coef needs {self.cfg.nbatch} columns")
         nr = self.nrec // self.cfg.nbatch
         out = np.empty(max(1, n * nr * self.cfg.nt), np.float32)
-        _check(lib().fwe_combine_traces(self._h, n, fptr(cf), fptr(out)))
+        fn = lib().fwe_combine_traces_noalloc if noalloc else lib().fwe_combine_traces
+        _check(fn(self._h, n, fptr(cf), fptr(out)))
         return out[: n * nr * self.cfg.nt].reshape(n, nr, self.cfg.nt)
 
     def get_field(self, field, batch=0):

@@ -299,6 +299,19 @@ fwe_status fwe_get_traces(fwe_ctx* ctx, float* out);
  * (OK4.cu:336-342 injects the single MTS.dat tensor); this is the batched
  * elementary-tensor form of that injection. */
 fwe_status fwe_combine_traces(fwe_ctx* ctx, int32_t n, const float* coef, float* out);
+/* As fwe_combine_traces (same arguments, same results bit for bit), but it
+ * allocates nothing: the coefficients and the combined traces are staged
+ * through a static device buffer of FWE_NOALLOC_SCRATCH_BYTES that the
+ * library holds on every device it runs on until the process ends, instead
+ * of device memory allocated per call.  One combined source
+ * (nbatch + receivers-per-wavefield * nt floats) must fit in it, else
+ * FWE_ERR_ARG.  Every call on one device shares that buffer: calls that
+ * overlap on one device (from several host threads) race, and callers must
+ * serialise them.  No counterpart in the reference; it is for callers that
+ * cannot allocate device memory (serving loops, code around captured
+ * graphs, allocation-free harnesses). */
+#define FWE_NOALLOC_SCRATCH_BYTES (32u << 20)
+fwe_status fwe_combine_traces_noalloc(fwe_ctx* ctx, int32_t n, const float* coef, float* out);
 /* Copy one field of wavefield `batch` to/from a host array of its reference
  * shape (gather/SaveArray, OK4.cu:75,164,530-532; read_data, OK4.cu:166-175). */
 fwe_status fwe_get_field(fwe_ctx* ctx, int32_t batch, int32_t field, float* out);
