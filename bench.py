@@ -7,7 +7,7 @@ velocity-stress update of all 28.3M cells.  Inputs are resident in HBM
 before the timed region.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup 50] [--config 1]
-                  [--kernel auto|two_pass|fused] [--cpu-steps 300] [--no-cpu]
+                  [--kernel auto|two_pass|fused] [--cpu-steps 100] [--no-cpu]
 
 Other configurations:
   --config 0  isotropic homogeneous 128^3, 600 steps (a parity case; fits
@@ -43,31 +43,23 @@ DEFAULT_STEPS = {0: 600, 1: 3000, 2: 1000, 3: 1500, 4: 200}
 def _traffic(kernel_name, workload):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE, each scaled by the same counter on the copy probe), kept in
-    profiles/traffic.json.  Falls back to the same kernel at another tile
-    shape when the autotuner picked one that was not profiled.  Returns
-    (bytes, profiled kernel name) or (None, None)."""
+    profiles/traffic.json, for exactly this kernel (template arguments
+    included) on this workload; None when that kernel was not profiled."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p) as f:
             d = json.load(f)[workload]
     except Exception:
-        return None, None
-    if kernel_name in d:
-        return d[kernel_name]["bytes_per_launch"], kernel_name
-    stem, args = kernel_name.split("<")[0], kernel_name.split("<")[1].rstrip(">").split(", ")
-    for k, v in d.items():
-        if k.startswith(stem + "<"):
-            a = k.split("<")[1].rstrip(">").split(", ")
-            # same law, damping, free surface (stress) and wavefield count
-            if a[:2] == args[:2] and a[-1] == args[-1] and (stem != "k_stress" or a[2] == args[2]):
-                return v["bytes_per_launch"], k
-    return None, None
+        return None
+    v = d.get(kernel_name)
+    return None if v is None else v["bytes_per_launch"]
 
 
-def cpu_baseline(wl, steps, threads, fp32=False):
+def cpu_baseline(wl, steps, threads, fp32=False, repeats=3):
     """The FP64 oracle (CPU restatement of the reference step, OpenMP over z
-    planes), or its FP32 build, on a bounded number of steps of one
-    wavefield of the workload."""
+    planes), or its FP32 build, timed on `steps` steps of one wavefield of
+    the workload, `repeats` times; returns the median rate (Gcell/s) and
+    the seconds of each timing (time loop only, no setup or I/O)."""
     from elanfmt_amd import workloads as W
     from oracle import Oracle
 
@@ -75,24 +67,50 @@ def cpu_baseline(wl, steps, threads, fp32=False):
                pml=wl.pml, free_surface=wl.free_surface, fp32=fp32)
     W.apply(o, wl)
     o.run(2, threads)  # first touch
-    t0 = time.perf_counter()
-    o.run(steps, threads)
-    el = time.perf_counter() - t0
+    els = []
+    for _ in range(repeats):
+        o.reset()
+        t0 = time.perf_counter()
+        o.run(steps, threads)
+        els.append(time.perf_counter() - t0)
     o.close()
-    return wl.cells * steps / el / 1e9, el
+    return wl.cells * steps / float(np.median(els)) / 1e9, els
+
+
+def host_threads():
+    """Threads for the CPU baseline: the host CPUs this process may use,
+    limited to OMP_NUM_THREADS when that is set (a GPU host's share of its
+    CPUs); returns (threads, CPUs visible)."""
+    visible = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return (min(visible, omp) if omp > 0 else visible), visible
 
 
 def _cpu_entry(args, wl_single, name):
+    """CPU baseline beside the GPU figure: the FP64 oracle on `cpu_steps`
+    steps of this workload (median of three timings) and its FP32 build;
+    for the 384x384x192 case also the 128^3 isotropic case in full (600
+    steps), as the baseline plan asks."""
     if args.no_cpu or args.cpu_steps <= 0:
         return None
-    threads = min(16, os.cpu_count() or 1)
-    v, el = cpu_baseline(wl_single, args.cpu_steps, threads)
-    steps32 = max(1, args.cpu_steps // 2)
-    v32, el32 = cpu_baseline(wl_single, steps32, threads, fp32=True)
-    return {"value": round(v, 5), "unit": "Gcell/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle, {args.cpu_steps} steps of {name} ({el:.1f} s), OpenMP over z planes",
-            "fp32_value": round(v32, 5),
-            "fp32_sample": f"FP32 build of the same oracle, {steps32} steps ({el32:.1f} s)"}
+    from elanfmt_amd import workloads as W
+
+    threads, visible = host_threads()
+    v, els = cpu_baseline(wl_single, args.cpu_steps, threads)
+    v32, els32 = cpu_baseline(wl_single, args.cpu_steps, threads, fp32=True)
+    out = {"value": round(v, 5), "unit": "Gcell/s", "cores": threads, "host_cpus_visible": visible, "kind": "port",
+           "sample": f"FP64 oracle, {args.cpu_steps} steps of {name}, median of 3 timings "
+                     f"({', '.join(f'{e:.2f}' for e in els)} s), OpenMP over z planes",
+           "fp32_value": round(v32, 5),
+           "fp32_sample": f"FP32 build of the same oracle, {args.cpu_steps} steps, median of 3 "
+                          f"({', '.join(f'{e:.2f}' for e in els32)} s)"}
+    if args.config == 1:
+        c0 = W.config(0)
+        v0, els0 = cpu_baseline(c0, c0.nt, threads)
+        out["config0_full"] = {"value": round(v0, 5), "unit": "Gcell/s", "steps": c0.nt,
+                               "sample": f"FP64 oracle, {c0.name} all {c0.nt} steps, median of 3 "
+                                         f"({', '.join(f'{e:.2f}' for e in els0)} s)"}
+    return out
 
 
 GRAPH_STEPS = 25  # steps per captured time-loop graph when replay is faster
@@ -137,7 +155,10 @@ def _timed_run(solver, steps, warmup, autotune, graph=True):
     return el, ms_a * steps / n_ev, ms_b * steps / n_ev, tune, gsteps
 
 
-def _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells, medium=1, batch_tile=None):
+def _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells, medium=1, batch_tile=None, sweep=None):
+    """Per-kernel durations and algorithmic bandwidth, and the dominant
+    kernel (name as rocprofv3 prints it).  fused: the single-pass sweep,
+    sweep = its (rows per lane, z chunk)."""
     from elanfmt_amd import workloads as W
 
     ba, bb = W.alg_bytes(wl.mode, wl.pml, wl.nbatch, fused=fused, medium=medium, batch_tile=batch_tile)
@@ -145,13 +166,13 @@ def _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells, medium=1, batch
     # kernel law template argument: 2 ortho through the material table, 3 ortho on
     # isotropic arrays, 4 iso through the material table
     km = wl.mode
-    if medium in (2, 3) and not fused:
+    if medium in (2, 3):
         km = 4 if wl.mode == 0 else medium
-    fs = "true" if wl.free_surface else "false"
+    fs = "true" if wl.free_surface == 1 else "false"
     if fused:
-        kernels = {"fused": {"ms_per_step": round(ms_a / steps, 5), "alg_bytes_per_cell": ba,
+        kernels = {"sweep": {"ms_per_launch": round(ms_a / steps, 5), "alg_bytes_per_cell": ba,
                              "GBs": round(ba * cells / (ms_a / steps * 1e-3) / 1e9, 1)}}
-        return kernels, ("fused", ms_a, ba, f"k_fused<{wl.mode}, {wl.pml}, {tiling[1]}>"), ba + bb
+        return kernels, ("sweep", ms_a, ba, f"k_sweep<{km}, {wl.pml}, {fs}, {sweep[0]}>"), ba + bb
     kernels = {
         "stress": {"ms_per_launch": round(ms_a / steps, 5), "alg_bytes_per_cell": ba,
                    "GBs": round(ba * cells / (ms_a / steps * 1e-3) / 1e9, 1)},
@@ -168,13 +189,10 @@ def _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells, medium=1, batch
 def _roofline(dom, steps, cells, wname):
     name, ms, b, kname = dom
     ach = b * cells / (ms / steps * 1e-3) / 1e9
-    traffic, prof_k = _traffic(kname, wname)
-    out = {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-           "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel_name": kname,
-           "alg_bytes_per_launch": b * cells}
-    if traffic is not None and prof_k != kname:
-        out["traffic_profiled_as"] = prof_k
-    return out
+    traffic = _traffic(kname, wname)
+    return {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel_name": kname,
+            "alg_bytes_per_launch": b * cells}
 
 
 MEDIUM = {"auto": 0, "arrays": 1, "table": 2, "isotropic": 3}  # fwe.MEDIUM_*
@@ -200,6 +218,14 @@ def _medium_desc(wl, medium, ntab):
     return "twelve streamed arrays"
 
 
+# what each configuration's line measures, beside the metric's full text
+MEASURES = {0: "single-wavefield Gcell-updates/s and % of 8 TB/s HBM (a cache-resident parity case)",
+            1: "single-wavefield Gcell-updates/s and % of 8 TB/s HBM (the plain-stencil half of the metric)",
+            2: "batched six elementary-MT wavefields against six plain runs (wavefield Gcell-updates/s)",
+            3: "uneven batch of 32 models: Gcell-updates/s of the whole batch",
+            4: "learned thin boundary: batched-6 interior cell-updates/s and errors against a padded reference"}
+
+
 def _line(value, steps, warmup, el, config, **extra):
     out = {"metric": METRIC, "value": round(value, 3), "unit": "Gcell/s", "n_gpus": 1, "steps": steps,
            "warmup": warmup, "ms_per_step": round(el / steps * 1e3, 4), "higher_is_better": True,
@@ -219,8 +245,10 @@ def bench_plain(args, steps):
     s = W.make_solver(wl)
     s.set_kernel(kind)
     s.set_medium_encoding(MEDIUM[args.medium])
-    fused = s.kernel() == fwe.KERNEL_FUSED
     el, ms_a, ms_b, tune, gsteps = _timed_run(s, steps, args.warmup, not args.no_autotune)
+    # the schedule the run used (under AUTO the autotuner times both)
+    fused = s.kernel() == fwe.KERNEL_FUSED
+    sweep = s.sweep_tiling()
     finite = bool(np.isfinite(s.traces()).all())
     tiling = (s.tile_cols(),) + tuple(s.tiling())
     order = ORDER_NAMES[s.wg_order()]
@@ -228,13 +256,14 @@ def bench_plain(args, steps):
     tr_main = s.traces()
     s.close()
     cells = wl.cells
-    kernels, dom, bstep = _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells, enc)
+    kernels, dom, bstep = _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells, enc, sweep=sweep)
     # the same workload with the medium streamed from its arrays, in the
     # same process (the encoding changes which bytes stream, not results)
     arrays_run = None
-    if enc in (fwe.MEDIUM_TABLE, fwe.MEDIUM_ISOTROPIC) and not fused:
+    if enc in (fwe.MEDIUM_TABLE, fwe.MEDIUM_ISOTROPIC):
         n2 = min(steps, 1000)
         s2 = W.make_solver(wl)
+        s2.set_kernel(kind)
         s2.set_medium_encoding(fwe.MEDIUM_ARRAYS)
         el2, a2, b2, _, _ = _timed_run(s2, n2, min(args.warmup, 20), not args.no_autotune)
         nr = len(wl.receivers)
@@ -242,10 +271,13 @@ def bench_plain(args, steps):
         # the first n2 samples of every trace, bit for bit
         same = bool(np.array_equal(tr2.reshape(nr, -1)[:, :n2], tr_main.reshape(nr, -1)[:, :n2]))
         t2 = (s2.tile_cols(),) + tuple(s2.tiling())
+        f2 = s2.kernel() == fwe.KERNEL_FUSED
+        sw2 = s2.sweep_tiling()
         s2.close()
-        _, _, bst2 = _kernel_entries(wl, False, t2, a2, b2, n2, cells, fwe.MEDIUM_ARRAYS)
+        _, _, bst2 = _kernel_entries(wl, f2, t2, a2, b2, n2, cells, fwe.MEDIUM_ARRAYS, sweep=sw2)
         arrays_run = {"value": round(cells * wl.nbatch * n2 / el2 / 1e9, 3), "unit": "Gcell/s", "steps": n2,
-                      "ms_per_step": round(el2 / n2 * 1e3, 4), "tiling": list(t2), "alg_bytes_per_cell": bst2,
+                      "ms_per_step": round(el2 / n2 * 1e3, 4), "kernel": "sweep" if f2 else "two-pass",
+                      "tiling": list(sw2) if f2 else list(t2), "alg_bytes_per_cell": bst2,
                       "hbm_pct_step": round(100 * bst2 * cells * n2 / el2 / 1e9 / PEAK_HBM_GBS, 2),
                       "traces_identical_to_main_run": same}
     desc = (f"{wl.name}: layered orthorhombic FP32, {wl.info.get('pml_cells')}-cell separable damping, "
@@ -254,7 +286,9 @@ def bench_plain(args, steps):
             f"{wl.name}: isotropic homogeneous FP32, {wl.info.get('pml_cells')}-cell profile damping, "
             f"double couple, Ricker {wl.info.get('f0')} Hz, {len(wl.receivers)} receiver traces")
     config = {"workload": desc, "grid": [wl.nx, wl.ny, wl.nz], "wavefields": wl.nbatch, "nt": wl.nt,
-              "kernel": "fused single-pass" if fused else "two-pass", "tiling": list(tiling), "wg_order": order,
+              "kernel": "single-pass sweep" if fused else "two-pass",
+              "tiling": ({"rows_per_lane": sweep[0], "zchunk": sweep[1]} if fused else list(tiling)),
+              "wg_order": None if fused else order,
               "medium": _medium_desc(wl, enc, ntab),
               "autotune_ms_per_step": None if tune is None else round(tune, 4), "graph_steps": gsteps,
               "parallelism": "single GPU"}
@@ -502,7 +536,7 @@ def main():
     ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 3, 4])
     ap.add_argument("--eval-steps", type=int, default=1500)
     ap.add_argument("--kernel", default="auto", choices=["auto", "two_pass", "fused"])
-    ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--medium", default="auto", choices=sorted(MEDIUM),
@@ -529,6 +563,7 @@ def main():
         out = bench_uneven(args, steps)
     else:
         out = bench_learned(args, steps)
+    out["measures"] = MEASURES[args.config]
     print(json.dumps(out), flush=True)
 
 

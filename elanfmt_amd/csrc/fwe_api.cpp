@@ -70,11 +70,20 @@ struct fwe_ctx {
   // single-pass kernel writes the next level into fbuf[cur^1]
   float* fbuf[2] = {nullptr, nullptr};
   int cur = 0;
-  int kernel = FWE_KERNEL_TWO_PASS;
-  // per-tile injection lists grown by one cell (halo values of the
-  // single-pass kernel)
-  int32_t* injx_off = nullptr;
-  PointEntry* injx = nullptr;
+  // requested schedule (FWE_KERNEL_*); under AUTO, auto_sweep says which
+  // one fwe_autotune measured faster
+  int kernel = FWE_KERNEL_AUTO;
+  bool auto_sweep = false;
+  // single-pass (sweep) kernel: output rows per lane, z chunk, tile counts
+  // (62-column tiles) and its point lists -- injections per tile grown by
+  // one cell in x and y, receivers per (tile, plane) -- rebuilt before the
+  // next sweep launch when points or the sweep tiling change
+  int sw_rows = 2, sw_kchunk = 32, sw_ntx = 0, sw_nty = 0, sw_nkc = 0;
+  bool sw_dirty = true;
+  int32_t* sw_inj_off = nullptr;
+  PointEntry* sw_inj = nullptr;
+  int32_t* sw_rec_koff = nullptr;
+  PointEntry* sw_rec = nullptr;
   float* medium = nullptr;  // 12 * nelem (ortho) or 2 * nelem (iso)
   // ortho material table: mid[c] selects twelve floats at mtab + 12*mid[c]
   // (ntab entries; 0 = the medium has too many distinct tuples)
@@ -189,6 +198,18 @@ static void choose_tiling(fwe_ctx* x) {
   if (kc > c.nz) kc = c.nz;
   x->kchunk = kc;
   x->nkc = (c.nz + kc - 1) / kc;
+  // sweep kernel: 2 rows per lane, 12-plane chunks (3 at least with BC1)
+  const char* esr = getenv("FWE_SWEEP_ROWS");
+  const char* esk = getenv("FWE_SWEEP_KCHUNK");
+  x->sw_rows = (esr && atoi(esr) == 4) ? 4 : 2;
+  int skc = esk ? atoi(esk) : 12;
+  if (skc < kmin) skc = kmin;
+  if (skc > c.nz) skc = c.nz;
+  x->sw_kchunk = skc;
+  x->sw_ntx = (c.nx + fwe::SWEEP_X - 1) / fwe::SWEEP_X;
+  x->sw_nty = (c.ny + x->sw_rows - 1) / x->sw_rows;
+  x->sw_nkc = (c.nz + skc - 1) / skc;
+  x->sw_dirty = true;
 }
 
 extern "C" fwe_status fwe_config_default(fwe_config* cfg, int32_t mode) {
@@ -274,7 +295,7 @@ extern "C" fwe_status fwe_destroy(fwe_ctx* x) {
   hipFree(x->mid); hipFree(x->mtab);
   hipFree(x->lb_cells); hipFree(x->lb_D);
   hipFree(x->stf); hipFree(x->traces); hipFree(x->inj_off); hipFree(x->inj);
-  hipFree(x->injx_off); hipFree(x->injx);
+  hipFree(x->sw_inj_off); hipFree(x->sw_inj); hipFree(x->sw_rec_koff); hipFree(x->sw_rec);
   hipFree(x->rec_off); hipFree(x->rec_koff); hipFree(x->rec);
   if (x->stream) hipStreamDestroy(x->stream);
   delete x;
@@ -517,23 +538,6 @@ static fwe_status upload_receivers(fwe_ctx* x) {
   return FWE_OK;
 }
 
-// each injection in the list of every tile whose one-cell-grown footprint
-// holds it (the single-pass kernel recomputes those halo stresses)
-static fwe_status upload_grown(fwe_ctx* x) {
-  std::vector<std::pair<int, PointEntry>> tv;
-  for (auto& p : x->inj_host) {
-    for (int tx = 0; tx < x->ntx; ++tx) {
-      const int lo = tx * x->tx - 1, hi = tx * x->tx + x->tx;
-      if (p.i < lo || p.i > hi) continue;
-      for (int ty = 0; ty < x->nty; ++ty) {
-        const int jl = ty * x->ty - 1, jh = ty * x->ty + x->ty;
-        if (p.j >= jl && p.j <= jh) tv.push_back({tx + ty * x->ntx, p});
-      }
-    }
-  }
-  return upload_tiled(x, tv, &x->injx_off, &x->injx);
-}
-
 extern "C" fwe_status fwe_set_injections(fwe_ctx* x, int32_t n, const fwe_injection* inj) {
   if (!x || n < 0 || (n > 0 && !inj)) return fail(FWE_ERR_ARG, "bad injection arguments");
   const fwe_config& c = x->cfg;
@@ -551,7 +555,8 @@ extern "C" fwe_status fwe_set_injections(fwe_ctx* x, int32_t n, const fwe_inject
   if (st) return st;
   x->inj_host = v;
   x->ninj = n;
-  return upload_grown(x);
+  x->sw_dirty = true;
+  return FWE_OK;
 }
 
 extern "C" fwe_status fwe_set_receivers(fwe_ctx* x, int32_t n, const fwe_receiver* rec) {
@@ -568,6 +573,7 @@ extern "C" fwe_status fwe_set_receivers(fwe_ctx* x, int32_t n, const fwe_receive
   }
   const std::vector<PointEntry> old = x->rec_host;
   x->rec_host = v;
+  x->sw_dirty = true;
   fwe_status st = upload_receivers(x);
   if (st) {
     x->rec_host = old;
@@ -652,38 +658,106 @@ static StepArgs make_args(fwe_ctx* x, int from) {
   return a;
 }
 
-// single-pass arguments for wavefield b, time level in fbuf[from]
-static fwe::FusedArgs make_fused(fwe_ctx* x, int from, int b) {
+// Point lists of the sweep kernel's tiles (62 columns x sw_rows rows): an
+// injection goes to every tile whose footprint grown by one cell in x and y
+// holds it (those tiles recompute its halo stresses), a receiver to the
+// (tile, plane) that owns its cell.  Both keep the list order of the host
+// copies, so sums of several injections into one cell round as in the
+// two-pass kernels.
+static fwe_status sweep_upload(fwe_ctx* x) {
+  if (!x->sw_dirty) return FWE_OK;
+  drop_graph(x);
+  const int W = fwe::SWEEP_X, R = x->sw_rows, nz = x->cfg.nz;
+  const int64_t ntile = (int64_t)x->sw_ntx * x->sw_nty;
+  std::vector<std::vector<PointEntry>> per((size_t)ntile);
+  for (const PointEntry& p : x->inj_host)
+    for (int tx = std::max(0, (p.i - W) / W); tx < x->sw_ntx; ++tx) {
+      const int lo = tx * W - 1, hi = tx * W + W;
+      if (p.i < lo) break;
+      if (p.i > hi) continue;
+      for (int ty = std::max(0, (p.j - R) / R); ty < x->sw_nty; ++ty) {
+        const int jl = ty * R - 1, jh = ty * R + R;
+        if (p.j < jl) break;
+        if (p.j <= jh) per[(size_t)(tx + (int64_t)ty * x->sw_ntx)].push_back(p);
+      }
+    }
+  std::vector<int32_t> ioff((size_t)ntile + 1, 0);
+  std::vector<PointEntry> iv;
+  for (int64_t t = 0; t < ntile; ++t) {
+    iv.insert(iv.end(), per[(size_t)t].begin(), per[(size_t)t].end());
+    ioff[(size_t)t + 1] = (int32_t)iv.size();
+  }
+  std::vector<std::pair<int64_t, PointEntry>> rv;
+  for (const PointEntry& p : x->rec_host) rv.push_back({(int64_t)(p.i / W + (p.j / R) * x->sw_ntx) * nz + p.k, p});
+  std::stable_sort(rv.begin(), rv.end(), [](const std::pair<int64_t, PointEntry>& a,
+                                            const std::pair<int64_t, PointEntry>& b) { return a.first < b.first; });
+  std::vector<int32_t> roff((size_t)(ntile * nz + 1), 0);
+  std::vector<PointEntry> rl;
+  for (auto& q : rv) {
+    roff[(size_t)q.first + 1]++;
+    rl.push_back(q.second);
+  }
+  for (size_t t = 1; t < roff.size(); ++t) roff[t] += roff[t - 1];
+  HIPCHK(hipSetDevice(x->cfg.device));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  hipFree(x->sw_inj_off); hipFree(x->sw_inj); hipFree(x->sw_rec_koff); hipFree(x->sw_rec);
+  x->sw_inj_off = x->sw_rec_koff = nullptr;
+  x->sw_inj = x->sw_rec = nullptr;
+  HIPCHK(hipMalloc(&x->sw_inj_off, sizeof(int32_t) * ioff.size()));
+  HIPCHK(hipMemcpy(x->sw_inj_off, ioff.data(), sizeof(int32_t) * ioff.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&x->sw_inj, sizeof(PointEntry) * std::max<size_t>(1, iv.size())));
+  if (!iv.empty()) HIPCHK(hipMemcpy(x->sw_inj, iv.data(), sizeof(PointEntry) * iv.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&x->sw_rec_koff, sizeof(int32_t) * roff.size()));
+  HIPCHK(hipMemcpy(x->sw_rec_koff, roff.data(), sizeof(int32_t) * roff.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&x->sw_rec, sizeof(PointEntry) * std::max<size_t>(1, rl.size())));
+  if (!rl.empty()) HIPCHK(hipMemcpy(x->sw_rec, rl.data(), sizeof(PointEntry) * rl.size(), hipMemcpyHostToDevice));
+  x->sw_dirty = false;
+  return FWE_OK;
+}
+
+// the sweep kernel addresses one array with 32-bit byte offsets
+static bool sweep_fits(const fwe_ctx* x) { return (uint64_t)x->nelem * sizeof(float) < ((uint64_t)1 << 32); }
+
+// sweep arguments: all wavefields, time level fbuf[from] -> fbuf[from ^ 1]
+static fwe::SweepArgs make_sweep(fwe_ctx* x, int from) {
   const StepArgs s = make_args(x, from);
-  fwe::FusedArgs f;
+  fwe::SweepArgs f;
   memset(&f, 0, sizeof(f));
-  f.nx = s.nx; f.ny = s.ny; f.nz = s.nz; f.sy = s.sy; f.sz = s.sz;
-  const int64_t fo = (int64_t)b * 9 * x->nelem;
+  f.nx = s.nx; f.ny = s.ny; f.nz = s.nz;
+  f.sy = (int32_t)s.sy; f.sz = (int32_t)s.sz;
+  f.fstride = 9 * x->nelem;
+  f.nbatch = x->cfg.nbatch;
   for (int q = 0; q < 3; ++q) {
-    f.Vi[q] = x->fbuf[from] + fo + (int64_t)q * x->nelem;
-    f.Vo[q] = x->fbuf[from ^ 1] + fo + (int64_t)q * x->nelem;
+    f.Vi[q] = x->fbuf[from] + (int64_t)q * x->nelem;
+    f.Vo[q] = x->fbuf[from ^ 1] + (int64_t)q * x->nelem;
   }
   for (int q = 0; q < 6; ++q) {
-    f.Si[q] = x->fbuf[from] + fo + (int64_t)(3 + q) * x->nelem;
-    f.So[q] = x->fbuf[from ^ 1] + fo + (int64_t)(3 + q) * x->nelem;
+    f.Si[q] = x->fbuf[from] + (int64_t)(3 + q) * x->nelem;
+    f.So[q] = x->fbuf[from ^ 1] + (int64_t)(3 + q) * x->nelem;
   }
   for (int q = 0; q < 9; ++q) f.C[q] = s.C[q];
   for (int q = 0; q < 3; ++q) f.B[q] = s.B[q];
+  f.mid = s.mid; f.mtab = s.mtab;
+  f.ntab = s.mtab ? x->ntab : 0;
   f.c66dt = s.c66dt; f.bxdt = s.bxdt; f.bydt = s.bydt; f.bzdt = s.bzdt;
+  f.vac_k = s.vac_k; f.vac_cdt = s.vac_cdt;
   f.wx = s.wx; f.wy = s.wy; f.wz = s.wz; f.whx = s.whx; f.why = s.why; f.whz = s.whz; f.w3 = s.w3;
   f.rdx = s.rdx; f.rdy = s.rdy; f.rdz = s.rdz;
-  f.ty_rows = s.ty_rows; f.ntx = s.ntx; f.nty = s.nty; f.nkc = s.nkc; f.kchunk = s.kchunk;
-  f.inj_off = x->injx_off; f.inj = x->injx;
-  f.stf = x->stf ? x->stf + (int64_t)b * x->nstf : nullptr;
-  f.nstf = x->nstf;
-  f.batch = b;
-  f.rec_off = s.rec_off; f.rec = s.rec; f.traces = s.traces; f.nt = s.nt;
+  f.rows = x->sw_rows;
+  f.ntx = x->sw_ntx; f.nty = x->sw_nty; f.nkc = x->sw_nkc; f.kchunk = x->sw_kchunk;
+  f.nyb = (x->sw_nty + 7) / 8;
+  f.inj_off = x->sw_inj_off; f.inj = x->sw_inj;
+  f.stf = x->stf; f.nstf = x->nstf;
+  f.src_after_bc = s.src_after_bc;
+  f.rec_koff = x->sw_rec_koff; f.rec = x->sw_rec; f.traces = x->traces; f.nt = s.nt;
   return f;
 }
 
-// AUTO picks the schedule measured fastest on MI355X: the two-pass kernels
-// (the single-pass kernel moves fewer bytes but runs at lower bandwidth).
-static bool use_fused(const fwe_ctx* x) { return x->kernel == FWE_KERNEL_FUSED; }
+// the single-pass sweep kernel runs when asked for, or under AUTO when
+// fwe_autotune measured it faster than the two passes
+static bool use_fused(const fwe_ctx* x) {
+  return x->kernel == FWE_KERNEL_FUSED || (x->kernel == FWE_KERNEL_AUTO && x->auto_sweep);
+}
 
 // one step from time level fbuf[from]; step index it (+ *it_base); `mid`
 // (optional) is recorded between the stress and velocity passes
@@ -726,12 +800,12 @@ static hipError_t step_launch(fwe_ctx* x, int from, int it, const int32_t* it_ba
     return e;
   }
   if (use_fused(x)) {
-    for (int b = 0; b < c.nbatch && e == hipSuccess; ++b) {
-      fwe::FusedArgs f = make_fused(x, from, b);
-      f.it = it;
-      f.it_base = it_base;
-      e = fwe::launch_fused(f, c.mode, c.pml, x->stream);
-    }
+    // one launch for the whole step and every wavefield; `mid` (the event
+    // between the passes of the two-pass schedule) follows it
+    fwe::SweepArgs f = make_sweep(x, from);
+    f.it = it;
+    f.it_base = it_base;
+    e = fwe::launch_sweep(f, km, c.pml, c.free_surface == FWE_FS_BC1, x->stream);
     if (e == hipSuccess && mid) e = hipEventRecord(mid, x->stream);
   } else {
     StepArgs a = make_args(x, from);
@@ -774,8 +848,9 @@ static fwe_status run_impl(fwe_ctx* x, int32_t nsteps, double* ms_s, double* ms_
   if (st) return st;
   HIPCHK(hipSetDevice(x->cfg.device));
   const bool fused = use_fused(x);
-  if (fused && x->cfg.free_surface)
-    return fail(FWE_ERR_UNSUPPORTED, "the single-pass kernel has no free surface; use the two-pass kernels");
+  if (fused && !sweep_fits(x))
+    return fail(FWE_ERR_UNSUPPORTED, "the single-pass kernel needs arrays under 4 GiB; use the two-pass kernels");
+  if (fused && (st = sweep_upload(x))) return st;
   if (fused && !x->fbuf[1]) {
     // second time level, starting as a copy so cells no kernel updates agree
     HIPCHK(hipMalloc(&x->fbuf[1], field_bytes(x)));
@@ -862,10 +937,9 @@ static fwe_status apply_tiling(fwe_ctx* x, int tx, int ty, int kchunk) {
   const fwe_config& c = x->cfg;
   if (ty != 2 && ty != 4 && ty != 8) return fail(FWE_ERR_ARG, "tile rows must be 2, 4 or 8");
   if (tx != 64 && tx != 128) return fail(FWE_ERR_ARG, "tile columns must be 64 or 128");
-  if (tx != 64 && use_fused(x)) return fail(FWE_ERR_ARG, "the single-pass kernel takes 64-column tiles");
-  // the free surface (BC1 of plane 1 written after plane 2) and the
-  // single-pass kernel's halo planes need planes 0..2 in one chunk
-  const int kmin = (c.free_surface == FWE_FS_BC1 || use_fused(x)) ? 3 : 1;
+  // the free surface (BC1 of plane 1 written after plane 2) needs planes
+  // 0..2 in one chunk
+  const int kmin = c.free_surface == FWE_FS_BC1 ? 3 : 1;
   if (kchunk < kmin || kchunk > c.nz) return fail(FWE_ERR_ARG, "z chunk must be in %d..nz", kmin);
   x->tx = tx;
   x->ty = ty;
@@ -875,7 +949,6 @@ static fwe_status apply_tiling(fwe_ctx* x, int tx, int ty, int kchunk) {
   x->nkc = (c.nz + kchunk - 1) / kchunk;
   fwe_status st = upload_points(x, x->inj_host, &x->inj_off, &x->inj);
   if (st) return st;
-  if ((st = upload_grown(x))) return st;
   return upload_receivers(x);
 }
 
@@ -883,20 +956,46 @@ extern "C" fwe_status fwe_set_kernel(fwe_ctx* x, int32_t kind) {
   if (!x) return fail(FWE_ERR_ARG, "NULL ctx");
   if (kind != FWE_KERNEL_AUTO && kind != FWE_KERNEL_TWO_PASS && kind != FWE_KERNEL_FUSED)
     return fail(FWE_ERR_ARG, "bad kernel kind %d", kind);
-  if (kind == FWE_KERNEL_FUSED && x->cfg.free_surface)
-    return fail(FWE_ERR_UNSUPPORTED, "the single-pass kernel has no free surface");
   if (kind == FWE_KERNEL_FUSED && x->lb)
     return fail(FWE_ERR_UNSUPPORTED, "the single-pass kernel has no learned boundary");
+  if (kind == FWE_KERNEL_FUSED && !sweep_fits(x))
+    return fail(FWE_ERR_UNSUPPORTED, "the single-pass kernel needs arrays under 4 GiB");
   drop_graph(x);
   x->kernel = kind;
-  if (use_fused(x) && (x->kchunk < 3 || x->tx != 64))
-    return apply_tiling(x, 64, x->ty, std::max(x->kchunk, std::min(3, x->cfg.nz)));
   return FWE_OK;
 }
 
 extern "C" fwe_status fwe_get_kernel(const fwe_ctx* x, int32_t* kind) {
   if (!x || !kind) return fail(FWE_ERR_ARG, "NULL argument");
   *kind = use_fused(x) ? FWE_KERNEL_FUSED : FWE_KERNEL_TWO_PASS;
+  return FWE_OK;
+}
+
+static fwe_status apply_sweep_tiling(fwe_ctx* x, int rows, int kchunk) {
+  const fwe_config& c = x->cfg;
+  if (rows != 2 && rows != 4) return fail(FWE_ERR_ARG, "sweep rows per lane must be 2 or 4");
+  const int kmin = c.free_surface == FWE_FS_BC1 ? 3 : 1;
+  if (kchunk < std::min(kmin, c.nz) || kchunk > c.nz)
+    return fail(FWE_ERR_ARG, "sweep z chunk must be in %d..nz", kmin);
+  drop_graph(x);
+  x->sw_rows = rows;
+  x->sw_kchunk = kchunk;
+  x->sw_ntx = (c.nx + fwe::SWEEP_X - 1) / fwe::SWEEP_X;
+  x->sw_nty = (c.ny + rows - 1) / rows;
+  x->sw_nkc = (c.nz + kchunk - 1) / kchunk;
+  x->sw_dirty = true;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_set_sweep_tiling(fwe_ctx* x, int32_t rows, int32_t zchunk) {
+  if (!x) return fail(FWE_ERR_ARG, "NULL ctx");
+  return apply_sweep_tiling(x, rows, zchunk);
+}
+
+extern "C" fwe_status fwe_get_sweep_tiling(const fwe_ctx* x, int32_t* rows, int32_t* zchunk) {
+  if (!x || !rows || !zchunk) return fail(FWE_ERR_ARG, "NULL argument");
+  *rows = x->sw_rows;
+  *zchunk = x->sw_kchunk;
   return FWE_OK;
 }
 
@@ -956,8 +1055,7 @@ extern "C" fwe_status fwe_set_medium_encoding(fwe_ctx* x, int32_t encoding) {
 
 extern "C" fwe_status fwe_get_medium_encoding(const fwe_ctx* x, int32_t* encoding, int32_t* n_materials) {
   if (!x || !encoding) return fail(FWE_ERR_ARG, "NULL argument");
-  *encoding = use_fused(x) ? FWE_MEDIUM_ARRAYS
-              : use_table(x) ? FWE_MEDIUM_TABLE
+  *encoding = use_table(x) ? FWE_MEDIUM_TABLE
               : use_iso(x)   ? FWE_MEDIUM_ISOTROPIC
                              : FWE_MEDIUM_ARRAYS;
   if (n_materials) *n_materials = x->ntab;
@@ -982,47 +1080,72 @@ extern "C" fwe_status fwe_autotune(fwe_ctx* x, int32_t trial_steps, double* best
   fwe_status st = ready(x);
   if (st) return st;
   const int nz = x->cfg.nz;
+  // a candidate: the two-pass schedule (sweep = false: batch layout,
+  // workgroup order, tile and z chunk) or the single-pass sweep (rows per
+  // lane in ty, z chunk)
   struct Cand {
     double ms;
+    bool sweep;
     int bt, order, tx, ty, kc;
   };
   std::vector<Cand> cands;
+  const bool try_two = x->kernel != FWE_KERNEL_FUSED;
+  const bool try_sweep = x->kernel != FWE_KERNEL_TWO_PASS && !x->lb && sweep_fits(x);
   // batch layouts: all wavefields per workgroup, or one
   std::vector<int> bts(1, x->cfg.nbatch);
-  if (x->cfg.nbatch > 1 && !use_fused(x)) bts.push_back(1);
+  if (x->cfg.nbatch > 1) bts.push_back(1);
   static const int txs[2] = {64, 128};
   static const int tys[3] = {2, 4, 8};
   static const int kcs[10] = {1, 2, 4, 8, 12, 16, 24, 32, 48, 64};
-  const int kmin = (x->cfg.free_surface == FWE_FS_BC1 || use_fused(x)) ? 3 : 1;
+  static const int skcs[7] = {4, 8, 12, 16, 24, 32, 64};
+  const int kmin = x->cfg.free_surface == FWE_FS_BC1 ? 3 : 1;
   drop_graph(x);
+  const int kernel0 = x->kernel;
   // time one candidate: a warm-up step, then `steps` timed steps; ms per step
   // is the sum of the kernel times (launch gaps do not depend on the tiling)
   auto trial = [&](const Cand& c, int steps, double* ms) -> fwe_status {
-    x->wg_order = c.order;
-    x->batch_tile = c.bt;
-    fwe_status s2 = apply_tiling(x, c.tx, c.ty, c.kc);
-    if (s2) return s2;
-    if ((s2 = run_impl(x, 1, nullptr, nullptr, nullptr))) return s2;
+    fwe_status s2;
+    if (c.sweep) {
+      x->kernel = FWE_KERNEL_FUSED;
+      s2 = apply_sweep_tiling(x, c.ty, c.kc);
+    } else {
+      x->kernel = FWE_KERNEL_TWO_PASS;
+      x->wg_order = c.order;
+      x->batch_tile = c.bt;
+      s2 = apply_tiling(x, c.tx, c.ty, c.kc);
+    }
+    if (!s2) s2 = run_impl(x, 1, nullptr, nullptr, nullptr);
     double a = 0, b = 0, w = 0;
-    if ((s2 = run_impl(x, steps, &a, &b, &w))) return s2;
+    if (!s2) s2 = run_impl(x, steps, &a, &b, &w);
+    x->kernel = kernel0;
+    if (s2) return s2;
     *ms = (a + b) / steps;
     return FWE_OK;
   };
-  for (int bt : bts)
-    for (int order = FWE_ORDER_LINEAR; order <= FWE_ORDER_PLANE_SPLIT; ++order)
-      for (int tx : txs) {
-        if (tx != 64 && use_fused(x)) continue;
-        for (int ty : tys)
-          for (int q = 0; q <= 10; ++q) {
-            // the listed chunk heights, then the whole column (so a short
-            // grid always has a candidate)
-            const int kc = q < 10 ? kcs[q] : nz;
-            if (kc > nz || kc < kmin || (q == 10 && std::find(kcs, kcs + 10, nz) != kcs + 10)) continue;
-            Cand c{0.0, bt, order, tx, ty, kc};
-            if ((st = trial(c, trial_steps, &c.ms))) return st;
-            cands.push_back(c);
-          }
+  if (try_two)
+    for (int bt : bts)
+      for (int order = FWE_ORDER_LINEAR; order <= FWE_ORDER_PLANE_SPLIT; ++order)
+        for (int tx : txs)
+          for (int ty : tys)
+            for (int q = 0; q <= 10; ++q) {
+              // the listed chunk heights, then the whole column (so a short
+              // grid always has a candidate)
+              const int kc = q < 10 ? kcs[q] : nz;
+              if (kc > nz || kc < kmin || (q == 10 && std::find(kcs, kcs + 10, nz) != kcs + 10)) continue;
+              Cand c{0.0, false, bt, order, tx, ty, kc};
+              if ((st = trial(c, trial_steps, &c.ms))) return st;
+              cands.push_back(c);
+            }
+  if (try_sweep)
+    for (int rows = 2; rows <= 4; rows += 2)
+      for (int q = 0; q <= 7; ++q) {
+        const int kc = q < 7 ? skcs[q] : nz;
+        if (kc > nz || kc < kmin || (q == 7 && std::find(skcs, skcs + 7, nz) != skcs + 7)) continue;
+        Cand c{0.0, true, 0, 0, 64, rows, kc};
+        if ((st = trial(c, trial_steps, &c.ms))) return st;
+        cands.push_back(c);
       }
+  if (cands.empty()) return fail(FWE_ERR_UNSUPPORTED, "no kernel schedule fits this context");
   // short trials rank candidates within a few percent of each other by
   // noise: time the best few again, longer, and keep the fastest of those
   std::sort(cands.begin(), cands.end(), [](const Cand& p, const Cand& q) { return p.ms < q.ms; });
@@ -1034,9 +1157,14 @@ extern "C" fwe_status fwe_autotune(fwe_ctx* x, int32_t trial_steps, double* best
     if ((st = trial(cands[q], 4 * trial_steps, &ms))) return st;
     if (ms < best_ms) { best_ms = ms; best = cands[q]; }
   }
-  x->wg_order = best.order;
-  x->batch_tile = best.bt;
-  if ((st = apply_tiling(x, best.tx, best.ty, best.kc))) return st;
+  if (best.sweep) {
+    if ((st = apply_sweep_tiling(x, best.ty, best.kc))) return st;
+  } else {
+    x->wg_order = best.order;
+    x->batch_tile = best.bt;
+    if ((st = apply_tiling(x, best.tx, best.ty, best.kc))) return st;
+  }
+  if (x->kernel == FWE_KERNEL_AUTO) x->auto_sweep = best.sweep;
   if (best_ms_per_step) *best_ms_per_step = best_ms;
   return fwe_reset(x);
 }
@@ -1620,9 +1748,7 @@ extern "C" fwe_status fwe_group_autotune(fwe_group* g, int32_t trial_steps, doub
   static const int tys[3] = {2, 4, 8};
   static const int kcs[8] = {1, 2, 4, 8, 16, 32, 64, 256};
   fwe_status st;
-  int kmin = g->fs ? 3 : 1;
-  for (fwe_ctx* x : g->m)
-    if (use_fused(x)) kmin = 3;
+  const int kmin = g->fs ? 3 : 1;
   struct Cand {
     double ms;
     int order, tx, ty, kc;
@@ -2074,7 +2200,9 @@ extern "C" fwe_status fwe_set_learned_boundary(fwe_ctx* x, fwe_lb* lb, float imp
     return fail(FWE_ERR_UNSUPPORTED, "learned boundary: it replaces damping on all six faces; no free surface");
   if (!(impedance > 0.f)) return fail(FWE_ERR_ARG, "impedance must be positive");
   if (lb->device != x->cfg.device) return fail(FWE_ERR_ARG, "learned boundary on another device");
-  if (use_fused(x)) return fail(FWE_ERR_UNSUPPORTED, "the single-pass kernel has no learned boundary");
+  if (x->kernel == FWE_KERNEL_FUSED)
+    return fail(FWE_ERR_UNSUPPORTED, "the single-pass kernel has no learned boundary");
+  x->auto_sweep = false;  // AUTO keeps the two-pass kernels while it is attached
   if (!lb_fits(x)) return fail(FWE_ERR_UNSUPPORTED, "learned boundary: the nine fields of a wavefield exceed 4 GiB");
   std::vector<LbCell> h;
   lb_cells_host(x->cfg.nx, x->cfg.ny, x->cfg.nz, lb->L, h, x->lb_face_off);

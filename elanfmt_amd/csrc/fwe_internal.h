@@ -9,11 +9,9 @@
 
 namespace fwe {
 
-// Tile of one workgroup in the xy plane: TILE_X lanes along x (one
-// wavefront row, coalesced x-fastest access) for the single-pass and group
-// kernels, 64 or 128 for the two-pass kernels; TILE_Y rows along y.  Each
-// workgroup marches a chunk of z planes.
-constexpr int TILE_X = 64;
+// Columns of a single-pass (sweep) tile: a wavefront's 64 lanes cover the
+// tile plus one halo column on each side.
+constexpr int SWEEP_X = 62;
 
 // MODE_ORTHO_TAB: the ortho law with the medium read through a material
 // table (one byte per cell, see StepArgs::mid).  MODE_ORTHO_ISO: the ortho
@@ -97,30 +95,43 @@ struct StepArgs {
   const int32_t* it_base;
 };
 
-// Single-pass step: stress of plane m and velocity of plane m-1 in one
-// z-march, reading time level n (Vi, Si) and writing n+1 (Vo, So).  Free
-// surface is not handled here (the two-pass kernels cover it).
-struct FusedArgs {
+// Single-pass step of the sweep kernel (fwe_sweep.h): every wavefront owns
+// a tile of 62 columns x `rows` rows x `kchunk` planes of one wavefield and
+// reads time level n (Vi, Si), writes n+1 (Vo, So).  The medium fields are
+// those of StepArgs.  Offsets inside one array are 32-bit byte offsets.
+struct SweepArgs {
   int32_t nx, ny, nz;
-  int64_t sy, sz;
+  int32_t sy, sz;           // padded pitches in elements (sz*(nz+1)*4 < 4 GiB)
+  int64_t fstride;          // elements between wavefields
+  int32_t nbatch;
   const float* Vi[3];
   const float* Si[6];
   float* Vo[3];
   float* So[6];
   const float* C[9];
   const float* B[3];
+  const uint8_t* mid;
+  const float* mtab;
+  int32_t ntab;             // table entries (copied to LDS by each workgroup)
   float c66dt, bxdt, bydt, bzdt;
+  int32_t vac_k;
+  float vac_cdt;
   const float* wx; const float* wy; const float* wz;
   const float* whx; const float* why; const float* whz;
   const float* w3;
   float rdx, rdy, rdz;
-  int32_t ty_rows, ntx, nty, nkc, kchunk;
-  const int32_t* inj_off;   // per tile, entries within the tile grown by one cell
+  int32_t rows;             // output rows per lane (2 or 4)
+  int32_t ntx, nty, nkc, kchunk;
+  int32_t nyb;              // tile rows per XCD band, ceil(nty / 8)
+  // injections: per xy tile, every entry whose cell lies in the tile grown
+  // by one cell in x and y (the halo stresses include them)
+  const int32_t* inj_off;
   const PointEntry* inj;
-  const float* stf;         // this wavefield's source time function
+  const float* stf;         // [nbatch][nstf]
   int32_t nstf;
-  int32_t batch;            // wavefield index (selects injections and receivers)
-  const int32_t* rec_off;
+  int32_t src_after_bc;
+  // receivers per (xy tile, plane): rec_koff[tile*nz + k]
+  const int32_t* rec_koff;
   const PointEntry* rec;
   float* traces;
   int32_t nt;
@@ -128,7 +139,9 @@ struct FusedArgs {
   const int32_t* it_base;
 };
 
-hipError_t launch_fused(const FusedArgs& a, int mode, int pml, hipStream_t s);
+template <int MODE, int PML>
+hipError_t sweep_mp(const SweepArgs& a, int fs, hipStream_t s);
+hipError_t launch_sweep(const SweepArgs& a, int mode, int pml, int fs, hipStream_t s);
 
 // per (law, damping form) launchers, instantiated in fwe_step.hip
 template <int MODE, int PML>

@@ -126,6 +126,37 @@ hipError_t launch_velocity(const StepArgs& a, int mode, int pml, hipStream_t s) 
   }
 }
 
+hipError_t launch_sweep(const SweepArgs& a, int mode, int pml, int fs, hipStream_t s) {
+  if (mode == MODE_ISO_TAB || mode == MODE_ISO) {
+    const bool tab = mode == MODE_ISO_TAB;
+    if (pml == PML_PROFILE) return tab ? sweep_mp<MODE_ISO_TAB, PML_PROFILE>(a, fs, s) : sweep_mp<MODE_ISO, PML_PROFILE>(a, fs, s);
+    if (pml == PML_NONE) return tab ? sweep_mp<MODE_ISO_TAB, PML_NONE>(a, fs, s) : sweep_mp<MODE_ISO, PML_NONE>(a, fs, s);
+    return hipErrorInvalidValue;
+  }
+  if (mode == MODE_ORTHO_ISO) {
+    switch (pml) {
+      case PML_NONE: return sweep_mp<MODE_ORTHO_ISO, PML_NONE>(a, fs, s);
+      case PML_W3D: return sweep_mp<MODE_ORTHO_ISO, PML_W3D>(a, fs, s);
+      case PML_SEP: return sweep_mp<MODE_ORTHO_ISO, PML_SEP>(a, fs, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  if (mode == MODE_ORTHO_TAB) {
+    switch (pml) {
+      case PML_NONE: return sweep_mp<MODE_ORTHO_TAB, PML_NONE>(a, fs, s);
+      case PML_W3D: return sweep_mp<MODE_ORTHO_TAB, PML_W3D>(a, fs, s);
+      case PML_SEP: return sweep_mp<MODE_ORTHO_TAB, PML_SEP>(a, fs, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  switch (pml) {
+    case PML_NONE: return sweep_mp<MODE_ORTHO, PML_NONE>(a, fs, s);
+    case PML_W3D: return sweep_mp<MODE_ORTHO, PML_W3D>(a, fs, s);
+    case PML_SEP: return sweep_mp<MODE_ORTHO, PML_SEP>(a, fs, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_group(const StepArgs* tab, const int32_t* wg, int nblk, int tx, int ty, int order, int mode, int pml,
                         int fs, bool vel, hipStream_t s) {
   if (mode == MODE_ORTHO_TAB || mode == MODE_ISO_TAB) return hipErrorInvalidValue;

@@ -97,6 +97,8 @@ SIGNATURES = {
     "fwe_set_kernel": (C.c_int, [C.c_void_p, C.c_int32]),
     "fwe_get_kernel": (C.c_int, [C.c_void_p, _IP]),
     "fwe_set_tiling": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "fwe_set_sweep_tiling": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "fwe_get_sweep_tiling": (C.c_int, [C.c_void_p, _IP, _IP]),
     "fwe_get_tiling": (C.c_int, [C.c_void_p, _IP, _IP]),
     "fwe_autotune": (C.c_int, [C.c_void_p, C.c_int32, _DP]),
     "fwe_get_step": (C.c_int, [C.c_void_p, _IP]),
@@ -350,6 +352,14 @@ class Solver:
     def tiling(self):
         a, b = C.c_int32(), C.c_int32()
         _check(lib().fwe_get_tiling(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def set_sweep_tiling(self, rows, zchunk):
+        _check(lib().fwe_set_sweep_tiling(self._h, rows, zchunk))
+
+    def sweep_tiling(self):
+        a, b = C.c_int32(), C.c_int32()
+        _check(lib().fwe_get_sweep_tiling(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
     def set_tile_cols(self, cols):

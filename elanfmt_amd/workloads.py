@@ -289,8 +289,14 @@ def alg_bytes(mode, pml, nbatch=1, fused=False, medium=fwe.MEDIUM_ARRAYS, batch_
         # stress: R V3 S6 [w] + index byte, W S6 ; velocity: R S6 V3 [w] + index byte, W V3
         return 4 * (nbatch * 15 + w * reads) + reads, 4 * (nbatch * 12 + w * reads) + reads
     if fused:
-        # fields R9 W9 per wavefield; medium once per launch (one launch per
-        # wavefield: iso c11 c12; ortho C9 b3 [w])
+        # single-pass sweep: fields R9 W9 per wavefield and the medium once
+        # per wavefield (each wavefront advances one wavefield): the index
+        # byte (table), iso c11 c12, ortho C9 b3, isotropic arrays C11 C12
+        # C44 C55 C66 b3; plus the 3-D weight
+        if medium == fwe.MEDIUM_TABLE:
+            return nbatch * (4 * (18 + w) + 1), 0
+        if medium == fwe.MEDIUM_ISOTROPIC:
+            return 4 * nbatch * (18 + 8 + w), 0
         med = 2 if mode == fwe.MODE_ISO else 12 + w
         return 4 * nbatch * (18 + med), 0
     if mode == fwe.MODE_ISO:

@@ -87,9 +87,11 @@ enum { FWE_REC_VX = 0, FWE_REC_VY = 1, FWE_REC_VZ = 2 };
 /* Injection offset layouts of the reference's moment-tensor source. */
 enum { FWE_SRC_LAYOUT_OK4 = 0, FWE_SRC_LAYOUT_ANISO = 1 };
 /* Kernel schedule: TWO_PASS = a stress launch and a velocity launch per step
- * (in place, the reference's two-kernel structure); FUSED = one z-marching
- * launch per step over two time-level buffers (no free surface); AUTO = the
- * schedule measured fastest on MI355X (currently TWO_PASS). */
+ * (in place, the reference's two-kernel structure); FUSED = the single-pass
+ * sweep: one z-marching launch per step (stress of a plane, then velocity of
+ * the plane behind it) over two time-level buffers, every law, damping form
+ * and top boundary, the same results bit for bit; AUTO = TWO_PASS until
+ * fwe_autotune has timed both, then the faster one. */
 enum { FWE_KERNEL_AUTO = 0, FWE_KERNEL_TWO_PASS = 1, FWE_KERNEL_FUSED = 2 };
 
 /* Top boundary.  VACUUM: on the shear plane k = 1 every shear stress uses
@@ -231,9 +233,17 @@ fwe_status fwe_get_tiling(const fwe_ctx* ctx, int32_t* tile_rows, int32_t* zchun
 /* Time trial_steps steps for each tiling candidate, keep the fastest, and
  * reset the context (fields, traces, step counter). */
 fwe_status fwe_autotune(fwe_ctx* ctx, int32_t trial_steps, double* best_ms_per_step);
+/* Tiling of the single-pass sweep kernel: every wavefront owns 62 columns x
+ * `rows` rows (2 or 4) and marches z chunks of `zchunk` planes (1..nz; 3..nz
+ * with BC1), recomputing one plane of stresses on each side of a chunk.
+ * fwe_autotune tries both row counts and several chunk heights.  Replaces
+ * the fixed 32x2x8 block of OK4.cu:26-31 for this schedule. */
+fwe_status fwe_set_sweep_tiling(fwe_ctx* ctx, int32_t rows, int32_t zchunk);
+fwe_status fwe_get_sweep_tiling(const fwe_ctx* ctx, int32_t* rows, int32_t* zchunk);
 /* Columns per tile of the two-pass kernels: 64 (one wavefront per tile row)
- * or 128 (two; halves the x-halo lines per tile).  The single-pass kernel
- * and groups use 64.  fwe_autotune tries both. */
+ * or 128 (two; halves the x-halo lines per tile).  Groups take 64 or 128 as
+ * well (fwe_group_set_tile_cols, fwe_group_autotune).  fwe_autotune tries
+ * both. */
 fwe_status fwe_set_tile_cols(fwe_ctx* ctx, int32_t cols);
 fwe_status fwe_get_tile_cols(const fwe_ctx* ctx, int32_t* cols);
 /* Order in which the two-pass kernels map workgroup ids to tiles: LINEAR
@@ -263,7 +273,7 @@ fwe_status fwe_get_wg_order(const fwe_ctx* ctx, int32_t* order);
  * stress pass reads C11 and C12 for the six, eight arrays in all.  AUTO
  * (default) uses the table when it was built, else the isotropic arrays
  * when they apply, else all the arrays; ARRAYS always streams them all.
- * The single-pass kernel uses the arrays; groups use the arrays too (ortho
+ * The single-pass kernel reads the same encoding; groups use the arrays (ortho
  * groups the isotropic arrays when every member may, else the twelve).  get returns
  * the encoding fwe_run will use and the number of table entries (0
  * without a table). */

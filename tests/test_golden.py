@@ -41,7 +41,7 @@ def test_fixture_hip(gpu, name):
     from elanfmt_amd import workloads as W
 
     wl, fx, _ = _load(name)
-    kinds = [gpu.KERNEL_TWO_PASS] + ([] if wl.free_surface else [gpu.KERNEL_FUSED])
+    kinds = [gpu.KERNEL_TWO_PASS, gpu.KERNEL_FUSED]
     for kind in kinds:
         print(f"fixture {name} kernel {kind}", flush=True)
         s = W.make_solver(wl)

@@ -442,87 +442,6 @@ def test_autotune_on_a_three_plane_grid(gpu):
     s.close()
 
 
-def _run_kernel(wl, kind, tiling=None, steps=None):
-    from elanfmt_amd import workloads as W
-
-    s = W.make_solver(wl)
-    s.set_kernel(kind)
-    if tiling:
-        s.set_tiling(*tiling)
-    s.run(wl.nt if steps is None else steps)
-    out = s.traces().astype(np.float64), [s.get_field(f) for f in range(9)]
-    s.close()
-    return out
-
-
-@pytest.mark.parametrize("mode,pml", [(0, 1), (0, 0), (1, 0), (1, 2), (1, 3)])
-@pytest.mark.parametrize("tiling", [(4, 7), (8, 26), (2, 5)])
-def test_fused_kernel_matches_oracle(gpu, mode, pml, tiling):
-    """The single-pass kernel on a grid whose x/y/z sizes are not multiples
-    of the tile, with injections and receivers on tile, halo and z-chunk
-    edges, agrees with the FP64 oracle and with the two-pass kernels."""
-    from elanfmt_amd import fwe, workloads as W
-    from oracle import Oracle
-
-    if mode == 0:
-        wl = W.iso_homogeneous(n=40, L=6, nt=50, rec_depth=2, nrec_side=2)
-        wl.nx, wl.ny, wl.nz = 40, 40, 40
-        if pml == 0:
-            wl.pml, wl.prof = fwe.PML_NONE, None
-    else:
-        wl = W.ortho_layered(nx=131, ny=37, nz=29, L=5, nt=50, nrec_side=2,
-                             pml=fwe.PML_WEIGHTS3D if pml == 2 else fwe.PML_SEPARABLE)
-        if pml == 0:
-            wl.pml, wl.prof, wl.w3 = fwe.PML_NONE, None, None
-    ty, kc = tiling
-    nx, ny, nz = wl.nx, wl.ny, wl.nz
-    ic = [63, 64, 65, 127, 128] if nx > 100 else [1, 31, 32, 33, 38]
-    jc = [ty - 1, ty, 2 * ty - 1, 2 * ty]
-    kcs = [kc - 1, kc, 2 * kc - 1]
-    inj = []
-    for n, (i, j, k) in enumerate((i, j, k) for i in ic for j in jc for k in kcs):
-        if i < nx and j < ny and k < nz:
-            inj.append((n % 6, i, j, k, 0, 1e14 * (1 + (n % 5))))
-    wl.injections = inj
-    wl.receivers = [(q, i, j, k, 0) for q in range(3) for i in ic for j in jc for k in kcs
-                    if i < nx and j < ny and k < nz]
-    tf, ff = _run_kernel(wl, fwe.KERNEL_FUSED, tiling)
-    t2, f2 = _run_kernel(wl, fwe.KERNEL_TWO_PASS, tiling)
-    o = Oracle(nx, ny, nz, wl.dx, wl.dy, wl.dz, wl.dt, wl.nt, mode=wl.mode, pml=wl.pml)
-    W.apply(o, wl)
-    o.run(wl.nt)
-    r = o.traces()
-    assert np.abs(r).max() > 0
-    assert rel_l2(tf, r) < TOL
-    assert rel_l2(tf, t2) < 2e-6
-    for f in range(9):
-        assert rel_l2(ff[f], o.get_field(f)) < TOL, f
-
-
-def test_fused_batch_and_graph(gpu):
-    """Batched wavefields and the captured loop on the single-pass kernel."""
-    from elanfmt_amd import fwe, workloads as W
-
-    wl = W.ortho_layered(nx=70, ny=30, nz=26, L=4, nt=41, nrec_side=2, elementary=True)
-    t2, f2 = _run_kernel(wl, fwe.KERNEL_TWO_PASS)
-    s = W.make_solver(wl)
-    s.set_kernel(fwe.KERNEL_FUSED)
-    s.set_graph_steps(6)
-    s.run(20)
-    s.run(21)
-    tf = s.traces().astype(np.float64)
-    fb = [[s.get_field(f, b) for f in range(9)] for b in range(6)]
-    s.close()
-    assert s is not None and rel_l2(tf, t2) < 2e-6
-    s2 = W.make_solver(wl)
-    s2.set_kernel(fwe.KERNEL_TWO_PASS)
-    s2.run(41)
-    for b in range(6):
-        for f in range(9):
-            assert rel_l2(fb[b][f], s2.get_field(f, b)) < 2e-6, (b, f)
-    s2.close()
-
-
 @pytest.mark.parametrize("fs", [1, 2])
 def test_group_with_a_free_surface_equals_separate_runs(gpu, fs):
     """Groups with BC1 or the vacuum-layer top (receivers and a shear
