@@ -284,10 +284,13 @@ extern "C" fwe_status fwe_create(const fwe_config* cfg, fwe_ctx** out) {
   return FWE_OK;
 }
 
+static void lb_detach(fwe_ctx* x);
+
 extern "C" fwe_status fwe_destroy(fwe_ctx* x) {
   if (!x) return FWE_OK;
   hipSetDevice(x->cfg.device);
   if (x->stream) hipStreamSynchronize(x->stream);
+  lb_detach(x);
   drop_graph(x);
   hipFree(x->it_dev);
   for (auto ev : x->events) hipEventDestroy(ev);
@@ -414,8 +417,9 @@ extern "C" fwe_status fwe_set_pml_profiles(fwe_ctx* x, const float* wx, const fl
       for (int q = 0; q < n[a]; ++q) h[o + q] = 0.5f * (w[a][q + 1] + w[a][q]);
     o += n[a] + 1;
   }
-  HIPCHK(hipSetDevice(c.device));
-  HIPCHK(hipMemcpy(x->prof, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+  // on the context stream, after the steps already queued there
+  fwe_status st = upload(x, x->prof, h);
+  if (st) return st;
   x->have_damp = true;
   return FWE_OK;
 }
@@ -1626,6 +1630,7 @@ extern "C" fwe_status fwe_group_create(int32_t n, fwe_ctx* const* members, fwe_g
     if (c.device != c0.device || c.mode != c0.mode || c.pml != c0.pml || c.free_surface != c0.free_surface)
       return fail(FWE_ERR_ARG, "member %d differs in device, law, damping or free surface", q);
     if (c.nbatch != 1) return fail(FWE_ERR_UNSUPPORTED, "member %d has %d wavefields; groups take 1", q, c.nbatch);
+    if (x->lb) return fail(FWE_ERR_UNSUPPORTED, "member %d has a learned boundary; groups run plain damping", q);
     if (x->it != members[0]->it) return fail(FWE_ERR_STATE, "member %d is at another step", q);
     fwe_status st = ready(x);
     if (st) return st;
@@ -1673,6 +1678,7 @@ static fwe_status group_run_impl(fwe_group* g, int32_t nsteps, double* ms) {
   for (fwe_ctx* x : g->m) {
     if (x->it != g->m[0]->it) return fail(FWE_ERR_STATE, "members were advanced separately");
     if (x->ty != g->ty || x->tx != g->tx) return fail(FWE_ERR_STATE, "a member was re-tiled outside the group");
+    if (x->lb) return fail(FWE_ERR_UNSUPPORTED, "a member has a learned boundary attached; groups run plain damping");
     HIPCHK(hipStreamSynchronize(x->stream));
   }
   fwe_status st = group_table(g);
@@ -1817,7 +1823,17 @@ struct fwe_lb {
   int32_t* depth = nullptr;
   float* part = nullptr;
   float* loss = nullptr;
+  // contexts the boundary is attached to (their queued steps read P)
+  std::vector<fwe_ctx*> attached;
 };
+
+// forget x's learned boundary (both directions)
+static void lb_detach(fwe_ctx* x) {
+  if (!x->lb) return;
+  auto& v = x->lb->attached;
+  v.erase(std::remove(v.begin(), v.end(), x), v.end());
+  x->lb = nullptr;
+}
 
 static const int LB_MLP_BLOCKS = 1024;  // resident blocks of the MLP kernels
 static const int LB_CHUNK = 512;        // samples per gradient partial sum
@@ -1968,6 +1984,9 @@ extern "C" fwe_status fwe_lb_init(fwe_lb* lb, uint64_t seed) {
 
 extern "C" fwe_status fwe_lb_destroy(fwe_lb* lb) {
   if (!lb) return FWE_OK;
+  if (!lb->attached.empty())
+    return fail(FWE_ERR_STATE, "learned boundary still attached to %zu context(s): detach it first",
+                lb->attached.size());
   hipSetDevice(lb->device);
   if (lb->stream) hipStreamSynchronize(lb->stream);
   lb_free_scratch(lb);
@@ -2011,6 +2030,8 @@ extern "C" fwe_status fwe_lb_set_params(fwe_lb* lb, int32_t face, const float* p
   if (!lb || !p || face < 0 || face > 5) return fail(FWE_ERR_ARG, "bad learned-boundary parameter arguments");
   HIPCHK(hipSetDevice(lb->device));
   HIPCHK(hipStreamSynchronize(lb->stream));
+  // steps queued on the contexts it is attached to read the parameters
+  for (fwe_ctx* x : lb->attached) HIPCHK(hipStreamSynchronize(x->stream));
   HIPCHK(hipMemcpy(lb->P + (int64_t)face * lb->np, p, sizeof(float) * lb->np, hipMemcpyHostToDevice));
   return FWE_OK;
 }
@@ -2194,7 +2215,7 @@ extern "C" fwe_status fwe_set_learned_boundary(fwe_ctx* x, fwe_lb* lb, float imp
   hipFree(x->lb_D);
   x->lb_cells = nullptr;
   x->lb_D = nullptr;
-  x->lb = nullptr;
+  lb_detach(x);
   if (!lb) return FWE_OK;
   if (x->cfg.free_surface)
     return fail(FWE_ERR_UNSUPPORTED, "learned boundary: it replaces damping on all six faces; no free surface");
@@ -2212,6 +2233,7 @@ extern "C" fwe_status fwe_set_learned_boundary(fwe_ctx* x, fwe_lb* lb, float imp
   HIPCHK(hipMalloc(&x->lb_D, sizeof(float) * nd));
   HIPCHK(hipMemset(x->lb_D, 0, sizeof(float) * nd));
   x->lb = lb;
+  lb->attached.push_back(x);
   x->lb_zinv = 1.f / impedance;
   return FWE_OK;
 }

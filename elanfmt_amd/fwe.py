@@ -269,8 +269,9 @@ class Solver:
     # context management
     def close(self):
         if self._h:
-            lib().fwe_destroy(self._h)
+            lib().fwe_destroy(self._h)  # also detaches a learned boundary
             self._h = C.c_void_p()
+        self._lb = None
 
     def __del__(self):
         try:
@@ -515,8 +516,10 @@ class LearnedBoundary:
         _check(lib().fwe_lb_create(device, layer, C.byref(self._h)))
 
     def close(self):
+        """Free the networks; refused (FweError) while a solver still has
+        the boundary attached (detach it, or close that solver, first)."""
         if self._h:
-            lib().fwe_lb_destroy(self._h)
+            _check(lib().fwe_lb_destroy(self._h))
             self._h = C.c_void_p()
 
     def __del__(self):
@@ -566,7 +569,9 @@ class LearnedBoundary:
 
     def attach(self, solver, impedance):
         _check(lib().fwe_set_learned_boundary(solver._h, self._h, impedance))
+        solver._lb = self  # the solver's steps read these networks: keep them alive
 
     @staticmethod
     def detach(solver):
         _check(lib().fwe_set_learned_boundary(solver._h, None, 1.0))
+        solver._lb = None

@@ -60,3 +60,155 @@ def test_batch_members_equal_single_runs_at_512x512x256(gpu):
         s.close()
         assert np.abs(single).max() > 0
         assert np.array_equal(single, batch[b * nr:(b + 1) * nr]), b
+
+
+@pytest.mark.timeout(900)
+def test_uneven_batch_of_32_cubes_as_one_group(gpu):
+    """The configured uneven batch (32 random-field cubes of edge 96..256,
+    196M cells): one group run of 200 steps gives every model exactly the
+    traces of its own run, and the two first 96^3 members agree with the
+    FP64 oracle to 1e-5 relative L2.  Each model also records three cells
+    next to its source (its configured receivers, two cells below the top
+    damping layer, are reached later on the large cubes)."""
+    from elanfmt_amd import fwe, workloads as W
+    from oracle import Oracle
+
+    steps = 200
+    wls = W.config(3, nt=steps)
+    for w in wls:
+        c = w.nx // 2 - 1
+        w.receivers = w.receivers + [(q, c + 3, c - 2, c + 1, 0) for q in range(3)]
+    print(f"uneven batch: {len(wls)} models, {sum(w.cells for w in wls) / 1e6:.1f} M cells, {steps} steps",
+          flush=True)
+    sol = [W.make_solver(w) for w in wls]
+    g = fwe.Group(sol)
+    g.run(steps)
+    tr_g = [s.traces() for s in sol]
+    g.close()
+    for q, (s, t) in enumerate(zip(sol, tr_g)):
+        s.reset()
+        s.run(steps)
+        assert np.array_equal(s.traces(), t), q
+        assert np.isfinite(t).all() and np.abs(t).max() > 0, q
+    for s in sol:
+        s.close()
+    small = [q for q, w in enumerate(wls) if w.nx == 96][:2]
+    assert len(small) == 2
+    for q in small:
+        w = wls[q]
+        o = Oracle(w.nx, w.ny, w.nz, w.dx, w.dy, w.dz, w.dt, w.nt, mode=w.mode, pml=w.pml)
+        W.apply(o, w)
+        o.run(steps)
+        r = o.traces()
+        err = rel_l2(tr_g[q], r)
+        print(f"model {q} (96^3) vs FP64 oracle: rel-L2 {err:.2e}", flush=True)
+        assert np.abs(r).max() > 0 and err < 1e-5
+
+
+def _c4_workload(nbatch, steps, f0=15.0):
+    """The learned-boundary grid: 512x512x256 layered orthorhombic with a
+    6-cell layer (the committed networks' reflection target), an explosive
+    source 6 cells inside the layer's inner edge on the x- face and
+    receivers between the source and the layer, so the layer is active
+    within `steps` steps."""
+    from elanfmt_amd import learned as Lr, workloads as W
+
+    thin = (512, 512, 256)
+    med = Lr.layered_medium(thin, 4242)
+    dt = W.cfl_dt(med["vmax"], Lr.DX, Lr.DX, Lr.DX, 0.5)
+    z = Lr.impedance(med["cells"][:1], med["bcell"])
+    src = (12, 256, 128)
+    rec = [(c, i, 256 + dj, 128 + dk, 0) for c in range(3) for i in (3, 7, 9) for (dj, dk) in ((0, 0), (4, -3))]
+    ms = [np.eye(6)[e] * 1e15 for e in range(nbatch)] if nbatch > 1 else np.array([1e15, 1e15, 1e15, 0, 0, 0])
+    return Lr, med, z, Lr.workload("c4", thin, 6, med, dt, src, ms, f0, steps, rec, R=1e-3, nbatch=nbatch)
+
+
+@pytest.mark.timeout(900)
+def test_learned_boundary_batch_at_512x512x256(gpu):
+    """The six elementary-tensor wavefields with the committed learned layer
+    attached, on the full 512x512x256 grid: each wavefield equals its own
+    single run bit for bit (wavefields 0, 3 and 5), replay of a captured
+    graph equals direct launches bit for bit, and every trace is finite and
+    reached by the wave."""
+    import os
+
+    from elanfmt_amd import workloads as W
+
+    steps = 90
+    Lr, med, z, wl = _c4_workload(6, steps)
+    lb, meta = Lr.load(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "elanfmt_amd",
+                                    "data", "learned_boundary_L6.npz"))
+    print(f"c4: 512x512x256 x6 wavefields with the learned layer, {steps} steps", flush=True)
+    s = W.make_solver(wl)
+    lb.attach(s, z)
+    s.run(steps)
+    batch = s.traces()
+    s.close()
+    assert np.isfinite(batch).all()
+    nr = len(batch) // 6
+    s = W.make_solver(wl)
+    lb.attach(s, z)
+    s.set_graph_steps(10)
+    s.run(40)
+    s.run(steps - 40)
+    assert np.array_equal(s.traces(), batch)
+    s.close()
+    stf, inj, rec = wl.stf, wl.injections, wl.receivers
+    for b in (0, 3, 5):
+        wl.nbatch = 1
+        wl.stf = [stf[b]]
+        wl.injections = [(q, i, j, k, 0, c) for (q, i, j, k, bb, c) in inj if bb == b]
+        wl.receivers = [(q, i, j, k, 0) for (q, i, j, k, bb) in rec if bb == b]
+        s = W.make_solver(wl)
+        lb.attach(s, z)
+        s.run(steps)
+        single = s.traces()
+        s.close()
+        # the receivers inside the layer (i = 3) are reached
+        assert np.abs(single[[0, 1, 6, 7, 12, 13]]).max() > 0, b
+        assert np.array_equal(single, batch[b * nr:(b + 1) * nr]), b
+    lb.close()
+
+
+@pytest.mark.timeout(900)
+def test_learned_corrections_only_shrink_values_at_512x512x256(gpu):
+    """The applied corrections are dissipative: from the same state, one step
+    with the learned layer attached leaves every layer value between zero
+    and the plain step's value (same sign, no larger magnitude), and every
+    cell outside the layer equal to the plain step."""
+    import os
+
+    from elanfmt_amd import workloads as W
+
+    warm = 80
+    Lr, med, z, wl = _c4_workload(1, warm + 1)
+    lb, meta = Lr.load(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "elanfmt_amd",
+                                    "data", "learned_boundary_L6.npz"))
+    print(f"c4 shrink: 512x512x256, {warm} warm-up steps with the layer, then one step with and without", flush=True)
+    s = W.make_solver(wl)
+    lb.attach(s, z)
+    s.run(warm)
+    state = [s.get_field(f) for f in range(9)]
+    s.run(1)
+    learned = [s.get_field(f) for f in range(9)]
+    s.close()
+    p = W.make_solver(wl)
+    for f in range(9):
+        p.set_field(f, state[f])
+    p.set_step(warm)
+    p.run(1)
+    plain = [p.get_field(f) for f in range(9)]
+    p.close()
+    lb.close()
+    L = 6
+    moved = 0
+    for f in range(9):
+        a, b = learned[f], plain[f]
+        inner = np.zeros(a.shape, bool)
+        inner[L:-L - 1, L:-L - 1, L:-L - 1] = True  # off the layer cells of every staggered shape
+        assert np.array_equal(a[inner], b[inner]), f
+        assert np.all(a * b >= 0), f
+        assert np.all(np.abs(a) <= np.abs(b)), f
+        moved += int(np.count_nonzero(a != b))
+    print(f"c4 shrink: {moved} layer values corrected", flush=True)
+    assert moved > 0

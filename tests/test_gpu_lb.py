@@ -192,3 +192,40 @@ def test_learned_boundary_in_a_captured_time_loop(gpu):
         assert np.array_equal(out[0][0][f], out[1][0][f]), f
     assert np.array_equal(out[0][1], out[1][1]) and np.abs(out[0][1]).max() > 0
     lb.close()
+
+
+def test_learned_members_are_refused_by_groups_and_attached_boundaries_stay_alive(gpu):
+    """A group runs plain damping only: a member with the learned boundary
+    attached is refused when the group is made, and a group whose member
+    gets one attached later refuses to run; an attached boundary cannot be
+    freed until it is detached (or its solver closed)."""
+    from elanfmt_amd import fwe, workloads as W
+
+    wls = [W.ortho_layered(nx=40, ny=36, nz=32, L=6, nt=12, nrec_side=2, seed=q + 1) for q in range(2)]
+    sol = [W.make_solver(w) for w in wls]
+    lb = fwe.LearnedBoundary(6)
+    lb.init(3)
+    lb.attach(sol[0], 1e7)
+    with pytest.raises(fwe.FweError) as ei:
+        fwe.Group(sol)
+    assert ei.value.status == fwe.ERR_UNSUPPORTED
+    fwe.LearnedBoundary.detach(sol[0])
+    g = fwe.Group(sol)
+    g.run(3)
+    lb.attach(sol[1], 1e7)
+    with pytest.raises(fwe.FweError) as ei:
+        g.run(1)
+    assert ei.value.status == fwe.ERR_UNSUPPORTED
+    # attached: freeing is refused and the solver holds the boundary
+    assert sol[1]._lb is lb
+    with pytest.raises(fwe.FweError) as ei:
+        lb.close()
+    assert ei.value.status == fwe.ERR_STATE
+    fwe.LearnedBoundary.detach(sol[1])
+    assert sol[1]._lb is None
+    g.run(2)
+    g.close()
+    lb.attach(sol[0], 1e7)
+    sol[0].close()  # closing the solver detaches the boundary
+    lb.close()
+    sol[1].close()
