@@ -1808,6 +1808,7 @@ struct fwe_lb {
   int device = 0, L = 6, np = 0;
   hipStream_t stream = nullptr;
   float* P = nullptr;     // 6 faces x np, FP32 masters
+  uint8_t* pk = nullptr;  // 6 faces x LB_PK_BYTES: P packed for inference (lb_repack after every change)
   float* Am = nullptr;    // Adam first moments
   float* Av = nullptr;    // Adam second moments
   float* grad = nullptr;  // np (one face at a time)
@@ -1826,6 +1827,13 @@ struct fwe_lb {
   // contexts the boundary is attached to (their queued steps read P)
   std::vector<fwe_ctx*> attached;
 };
+
+// P changed: rewrite the inference copy.  Callers have drained the streams
+// of the attached contexts, whose queued steps read it.
+static hipError_t lb_repack(fwe_lb* lb) {
+  hipError_t e = fwe::lb_launch_pack(lb->P, lb->np, lb->L, lb->pk, lb->stream);
+  return e == hipSuccess ? hipStreamSynchronize(lb->stream) : e;
+}
 
 // forget x's learned boundary (both directions)
 static void lb_detach(fwe_ctx* x) {
@@ -1887,35 +1895,33 @@ static LbGeom lb_geom(const fwe_ctx* x, int from, float zinv) {
   return g;
 }
 
-// field set of wavefield b
-static LbGeom lb_geom_b(const fwe_ctx* x, int from, int b, float zinv) {
-  LbGeom g = lb_geom(x, from, zinv);
-  for (int f = 0; f < 9; ++f) g.F[f] += (int64_t)b * 9 * x->nelem;
-  return g;
+// one inference (or add) launch over every layer cell of every wavefield
+static fwe::LbInfer lb_infer_args(const fwe_ctx* x, int from) {
+  fwe::LbInfer a{};
+  a.pk = x->lb->pk;
+  a.cells = x->lb_cells;
+  for (int f = 0; f < 7; ++f) a.face_off[f] = (int32_t)x->lb_face_off[f];
+  a.F = x->fbuf[from];
+  a.wstride = (int64_t)9 * x->nelem;
+  a.fstride = (uint32_t)x->nelem;
+  a.sy = (int32_t)x->sy;
+  a.sz = (int32_t)x->sz;
+  a.nx = x->cfg.nx;
+  a.ny = x->cfg.ny;
+  a.nz = x->cfg.nz;
+  a.nbatch = x->cfg.nbatch;
+  a.zinv = x->lb_zinv;
+  a.D = x->lb_D;
+  a.trash = x->lb_D + (int64_t)x->lb_face_off[6] * fwe::LB_OUT * x->cfg.nbatch;
+  return a;
 }
 
 static hipError_t lb_before(fwe_ctx* x, int from) {
-  hipError_t e = hipSuccess;
-  const int64_t nc = x->lb_face_off[6];
-  for (int b = 0; b < x->cfg.nbatch && e == hipSuccess; ++b) {
-    const LbGeom g = lb_geom_b(x, from, b, x->lb_zinv);
-    for (int f = 0; f < 6 && e == hipSuccess; ++f) {
-      const int64_t o = x->lb_face_off[f], n = x->lb_face_off[f + 1] - o;
-      e = fwe::lb_launch_predict_mfma(x->lb->P + (int64_t)f * x->lb->np, x->lb->L, g, x->lb_cells + o, (int)n,
-                                      x->lb_D + (b * nc + o) * fwe::LB_OUT, x->stream);
-    }
-  }
-  return e;
+  fwe::LbInfer a = lb_infer_args(x, from);
+  return fwe::lb_launch_infer(a, x->stream);
 }
 
-static hipError_t lb_after(fwe_ctx* x, int from) {
-  hipError_t e = hipSuccess;
-  const int64_t nc = x->lb_face_off[6];
-  for (int b = 0; b < x->cfg.nbatch && e == hipSuccess; ++b)
-    e = fwe::lb_launch_add(lb_geom_b(x, from, b, x->lb_zinv), x->lb_cells, (int)nc, x->lb_D + b * nc * fwe::LB_OUT,
-                           x->stream);
-  return e;
-}
+static hipError_t lb_after(fwe_ctx* x, int from) { return fwe::lb_launch_add(lb_infer_args(x, from), x->stream); }
 
 extern "C" fwe_status fwe_lb_param_count(int32_t layer, int32_t* n) {
   if (!n || layer < 1 || layer > fwe::LB_MAXL) return fail(FWE_ERR_ARG, "layer must be in 1..%d", fwe::LB_MAXL);
@@ -1975,10 +1981,12 @@ extern "C" fwe_status fwe_lb_init(fwe_lb* lb, uint64_t seed) {
     for (int q = 0; q < fwe::LB_H * fwe::LB_H; ++q) p[fwe::lb_off_w2(L) + q] = (float)(s2 * fwe_rng_normal(&r));
   }
   HIPCHK(hipSetDevice(lb->device));
+  for (fwe_ctx* x : lb->attached) HIPCHK(hipStreamSynchronize(x->stream));
   HIPCHK(hipMemcpy(lb->P, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemset(lb->Am, 0, sizeof(float) * h.size()));
   HIPCHK(hipMemset(lb->Av, 0, sizeof(float) * h.size()));
   for (int f = 0; f < 6; ++f) lb->adam_t[f] = 0;
+  HIPCHK(lb_repack(lb));
   return FWE_OK;
 }
 
@@ -1990,7 +1998,7 @@ extern "C" fwe_status fwe_lb_destroy(fwe_lb* lb) {
   hipSetDevice(lb->device);
   if (lb->stream) hipStreamSynchronize(lb->stream);
   lb_free_scratch(lb);
-  hipFree(lb->P); hipFree(lb->Am); hipFree(lb->Av); hipFree(lb->grad); hipFree(lb->cells); hipFree(lb->loss);
+  hipFree(lb->P); hipFree(lb->pk); hipFree(lb->Am); hipFree(lb->Av); hipFree(lb->grad); hipFree(lb->cells); hipFree(lb->loss);
   if (lb->stream) hipStreamDestroy(lb->stream);
   delete lb;
   return FWE_OK;
@@ -2012,6 +2020,7 @@ extern "C" fwe_status fwe_lb_create(int32_t device, int32_t layer, fwe_lb** out)
   const size_t pb = sizeof(float) * (size_t)lb->np * 6;
   hipError_t e = hipStreamCreateWithFlags(&lb->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&lb->P, pb);
+  if (e == hipSuccess) e = hipMalloc(&lb->pk, (size_t)fwe::LB_PK_BYTES * 6);
   if (e == hipSuccess) e = hipMalloc(&lb->Am, pb);
   if (e == hipSuccess) e = hipMalloc(&lb->Av, pb);
   if (e == hipSuccess) e = hipMalloc(&lb->grad, sizeof(float) * lb->np);
@@ -2033,6 +2042,7 @@ extern "C" fwe_status fwe_lb_set_params(fwe_lb* lb, int32_t face, const float* p
   // steps queued on the contexts it is attached to read the parameters
   for (fwe_ctx* x : lb->attached) HIPCHK(hipStreamSynchronize(x->stream));
   HIPCHK(hipMemcpy(lb->P + (int64_t)face * lb->np, p, sizeof(float) * lb->np, hipMemcpyHostToDevice));
+  HIPCHK(lb_repack(lb));
   return FWE_OK;
 }
 
@@ -2187,6 +2197,9 @@ extern "C" fwe_status fwe_lb_train_step(fwe_lb* lb, fwe_ctx* thick, fwe_ctx* thi
                                 (int)ncell, lb->R, (float)(1e-4 * sqrt(r2)), (float)(1.0 / r2), lb->T, lb->M,
                                 lb->stream));
   double lsum = 0;
+  // the update rewrites P: steps queued on attached contexts read it
+  if (lr > 0.f)
+    for (fwe_ctx* x : lb->attached) HIPCHK(hipStreamSynchronize(x->stream));
   for (int f = 0; f < 6; ++f) {
     const int64_t o = lb->face_off[f], n = lb->face_off[f + 1] - o;
     if (n == 0) continue;
@@ -2201,6 +2214,7 @@ extern "C" fwe_status fwe_lb_train_step(fwe_lb* lb, fwe_ctx* thick, fwe_ctx* thi
   }
   // the thin run continues as it would with the boundary attached
   HIPCHK(fwe::lb_launch_apply_y(lb_geom(thin, thin->cur, zinv), lb->cells, (int)ncell, lb->Y, lb->R, lb->stream));
+  if (lr > 0.f) HIPCHK(fwe::lb_launch_pack(lb->P, lb->np, lb->L, lb->pk, lb->stream));
   HIPCHK(hipStreamSynchronize(lb->stream));
   if (loss) *loss = lsum / (9.0 * (double)std::max<int64_t>(1, ncell));
   return FWE_OK;
@@ -2229,7 +2243,8 @@ extern "C" fwe_status fwe_set_learned_boundary(fwe_ctx* x, fwe_lb* lb, float imp
   lb_cells_host(x->cfg.nx, x->cfg.ny, x->cfg.nz, lb->L, h, x->lb_face_off);
   HIPCHK(hipMalloc(&x->lb_cells, sizeof(LbCell) * std::max<size_t>(1, h.size())));
   HIPCHK(hipMemcpy(x->lb_cells, h.data(), sizeof(LbCell) * h.size(), hipMemcpyHostToDevice));
-  const size_t nd = std::max<size_t>(1, h.size() * fwe::LB_OUT * x->cfg.nbatch);
+  // corrections, then the inference kernel's 64-float trash slot
+  const size_t nd = h.size() * fwe::LB_OUT * x->cfg.nbatch + 64;
   HIPCHK(hipMalloc(&x->lb_D, sizeof(float) * nd));
   HIPCHK(hipMemset(x->lb_D, 0, sizeof(float) * nd));
   x->lb = lb;

@@ -207,10 +207,40 @@ hipError_t lb_launch_gather(const LbGeom& g, const LbCell* cells, int n, float* 
                             hipStream_t s);
 hipError_t lb_launch_targets(const LbGeom& thin, const LbGeom& thick, int off, const LbCell* cells, int n,
                              const float* R, float rmin, float wscale, float* T, float* M, hipStream_t s);
-// the same on the matrix cores: inputs and hidden activations in bfloat16
-hipError_t lb_launch_predict_mfma(const float* P, int L, const LbGeom& g, const LbCell* cells, int n, float* D,
-                                  hipStream_t s);
-hipError_t lb_launch_add(const LbGeom& g, const LbCell* cells, int n, const float* D, hipStream_t s);
+
+// Inference weights of one face in the operand order of k_lb_infer,
+// rewritten by lb_launch_pack whenever the FP32 parameters change: bfloat16
+// W1 [4 tiles][3 k-steps][64 lanes][8], W2 [4][2][64][8], W3 [2][64][8],
+// then FP32 b1 [LB_MAXL][64] (rows >= L zero), b2 [64], b3 [16].
+constexpr int LB_PK_W1 = 0;
+constexpr int LB_PK_W2 = LB_PK_W1 + 4 * 3 * 64 * 8 * 2;
+constexpr int LB_PK_W3 = LB_PK_W2 + 4 * 2 * 64 * 8 * 2;
+constexpr int LB_PK_B1 = LB_PK_W3 + 2 * 64 * 8 * 2;
+constexpr int LB_PK_B2 = LB_PK_B1 + LB_MAXL * LB_H * 4;
+constexpr int LB_PK_B3 = LB_PK_B2 + LB_H * 4;
+constexpr int LB_PK_BYTES = LB_PK_B3 + 16 * 4;
+
+// One inference launch: every layer cell of every wavefield.
+struct LbInfer {
+  const uint8_t* pk;      // 6 faces x LB_PK_BYTES
+  const LbCell* cells;    // layer cells grouped by face
+  int32_t face_off[7];    // cells of face f: [face_off[f], face_off[f+1])
+  int32_t blk_off[7];     // workgroups of face f (set by lb_launch_infer)
+  float* F;               // wavefield 0: field q at F + q * fstride
+  int64_t wstride;        // elements between wavefields
+  uint32_t fstride;       // elements between the fields of a wavefield
+  int32_t sy, sz, nx, ny, nz;
+  int32_t nbatch;
+  float zinv;
+  float* D;               // corrections [nbatch][9][face_off[6]]
+  float* trash;           // 64 floats, written by lanes that have no output
+};
+hipError_t lb_launch_pack(const float* P, int np, int L, uint8_t* pk, hipStream_t s);
+// corrections D from the fields before the step (inputs and hidden
+// activations rounded to bfloat16, sums in FP32)
+hipError_t lb_launch_infer(LbInfer& a, hipStream_t s);
+// fields <- shrink(fields, D) after the step
+hipError_t lb_launch_add(const LbInfer& a, hipStream_t s);
 hipError_t lb_launch_apply_y(const LbGeom& g, const LbCell* cells, int n, const float* Y, const float* R,
                              hipStream_t s);
 hipError_t lb_launch_crop(const LbGeom& thick, const LbGeom& thin, int off, hipStream_t s);

@@ -13,8 +13,9 @@
 // W1, W2, W3 are used rounded to bfloat16 (kept as FP32 master copies for
 // training); all sums accumulate in FP32.
 //
-// One wavefront evaluates one sample: lane h owns hidden unit h.  Weights
-// are staged in LDS as bfloat16 for the whole block.
+// Training (k_lb_mlp): one wavefront evaluates one sample, lane h owns
+// hidden unit h, weights staged in LDS as bfloat16 for the whole block.
+// Inference (k_lb_infer): 16 cells per wavefront on the matrix cores.
 #include <algorithm>
 
 #include "fwe_internal.h"
@@ -301,182 +302,317 @@ __device__ __forceinline__ float lb_shrink(float v, float d) {
   return v > 0.f ? v + fminf(fmaxf(d, -v), 0.f) : (v < 0.f ? v + fmaxf(fminf(d, -v), 0.f) : v);
 }
 
-// Inference on the matrix cores, before the step: corrections of n cells of
-// one face into D [n][9].  A wavefront evaluates 16 cells at a time with
-// v_mfma_f32_16x16x16_bf16: layer 1 is X[16 x 96] W1[96 x 64] (inputs padded
-// 81 -> 96), layer 2 H1[16 x 64] W2[64 x 64], layer 3 H2[16 x 64] W3[64 x 16]
-// (outputs padded 9 -> 16).  Operand layout of that instruction: A[i][k] in
-// lane i%16 + 16*(k/4), element k%4; B[k][n] in lane n + 16*(k/4); the
-// result D[m][n] in lane n + 16*(m/4), element m%4.  Inputs and hidden
-// activations are rounded to bfloat16, products accumulate in FP32.  The
-// weights are rounded to bfloat16 once per block into LDS, transposed so
-// that a lane's B operand (four consecutive k of one n) is one 8-byte LDS
-// read; held in registers instead they took 88 VGPRs and left room for one
-// wavefront per SIMD.  The activations change layout through a per-wave LDS
-// tile.
-typedef short bf4 __attribute__((ext_vector_type(4)));
+// ---------------------------------------------------------------------------
+// inference on the matrix cores
+// ---------------------------------------------------------------------------
+// k_lb_infer evaluates the networks of every layer cell of every wavefield
+// in one launch, before the step.  A wavefront takes 16 cells of one face
+// at a time and forms the transposed products
+//   H1^T = W1^T X^T,   H2^T = W2^T H1^T,   Y^T = W3^T H2^T
+// with v_mfma_f32_16x16x32_bf16, the weights as the A operand (rows: hidden
+// units or outputs) and the cells as the B operand (columns).  Lane
+// (j, g) = (lane % 16, lane / 16) holds eight k values of cell j in each
+// B operand and receives rows 4g..4g+3 of each 16-row result tile, i.e.
+// hidden units 16t + 4g + e of cell j: these are the next layer's B operand
+// once the next weights are packed in the matching k order, so no
+// activation leaves the registers.  Element m of lane group g may stand for
+// any k as long as A and B use the same assignment: the instruction pairs
+// A and B elements of equal (g, m).
+//
+// Layer-1 k order (lb_slot1): lane group g gathers stencil points g and
+// g + 4 with all nine fields, and a share of point 8 (fields 0..2 for g = 0,
+// fields 1+2g and 2+2g otherwise): 21 of its 24 slots, the other 3 zero.
+// Layers 2 and 3 (lb_k2): slot n of group g is hidden unit
+// 16 (n / 4) + 4 g + n % 4, which is where layer 1 (2) left it.
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+__host__ __device__ constexpr int lb_slot1(int g, int n) {
+  if (n < 9) return 9 * g + n;
+  if (n < 18) return 9 * (g + 4) + (n - 9);
+  const int r = n - 18, cnt = g == 0 ? 3 : 2, f0 = g == 0 ? 0 : 1 + 2 * g;
+  return r < cnt ? 72 + f0 + r : -1;
+}
+__host__ __device__ constexpr int lb_k2(int g, int n) { return 16 * (n >> 2) + 4 * g + (n & 3); }
+
+__global__ __launch_bounds__(256) void k_lb_pack(const float* __restrict__ P, int32_t np, int32_t L,
+                                                uint8_t* __restrict__ pk) {
+  constexpr int NW1 = 4 * 3 * 64 * 8, NW2 = 4 * 2 * 64 * 8, NW3 = 2 * 64 * 8;
+  constexpr int NB = LB_MAXL * LB_H + LB_H + 16;
+  const float* __restrict__ p = P + (int64_t)blockIdx.y * np;
+  uint8_t* o = pk + (int64_t)blockIdx.y * LB_PK_BYTES;
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q < NW1 + NW2 + NW3) {
+    // element m of lane (i, g), tile t, k-step s
+    const int m = q & 7, lane = (q >> 3) & 63, i = lane & 15, g = lane >> 4;
+    float v;
+    if (q < NW1) {
+      const int ts = q >> 9, t = ts / 3, s = ts - 3 * t;
+      const int e = lb_slot1(g, 8 * s + m);
+      v = e >= 0 ? p[lb_off_w1() + e * LB_H + 16 * t + i] : 0.f;
+    } else if (q < NW1 + NW2) {
+      const int ts = (q - NW1) >> 9, t = ts >> 1, s = ts & 1;
+      v = p[lb_off_w2(L) + lb_k2(g, 8 * s + m) * LB_H + 16 * t + i];
+    } else {
+      const int s = (q - NW1 - NW2) >> 9;
+      v = i < LB_OUT ? p[lb_off_w3(L) + lb_k2(g, 8 * s + m) * LB_OUT + i] : 0.f;
+    }
+    reinterpret_cast<uint16_t*>(o)[q] = to_bf16(v);
+  } else if (q < NW1 + NW2 + NW3 + NB) {
+    const int r = q - (NW1 + NW2 + NW3);
+    float v;
+    if (r < LB_MAXL * LB_H) v = r < L * LB_H ? p[lb_off_b1() + r] : 0.f;
+    else if (r < LB_MAXL * LB_H + LB_H) v = p[lb_off_b2(L) + r - LB_MAXL * LB_H];
+    else v = r - LB_MAXL * LB_H - LB_H < LB_OUT ? p[lb_off_b3(L) + r - LB_MAXL * LB_H - LB_H] : 0.f;
+    reinterpret_cast<float*>(o + LB_PK_B1)[r] = v;
+  }
 }
 
-__device__ __forceinline__ float silu(float a) { return a * sigm(a); }
+// a * sigmoid(a) from the hardware exp2 and reciprocal (about 2 ulp before
+// the result is rounded to bfloat16)
+__device__ __forceinline__ float silu_hw(float a) {
+  return a * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-1.44269504f * a));
+}
 
-constexpr int LB_K1 = 104;  // LDS row pitch (bf16) of W1^T: 96 padded inputs + 8
-constexpr int LB_K2 = 72;   // of W2^T and W3^T: 64 + 8
-
-__global__ __launch_bounds__(256) void k_lb_predict_mfma(const float* __restrict__ P, int32_t L, const LbGeom g,
-                                                        const LbCell* __restrict__ cells, int32_t n,
-                                                        float* __restrict__ D) {
-  __shared__ uint16_t hs[LB_WAVES][16][LB_H + 4];
-  __shared__ __align__(16) uint16_t sw1[LB_H][LB_K1];
-  __shared__ __align__(16) uint16_t sw2[LB_H][LB_K2];
-  __shared__ __align__(16) uint16_t sw3[16][LB_K2];
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int row = l & 15, grp = l >> 4;
-  // weights rounded to bfloat16, transposed: sw1[n][k] = W1[k][n]
-  for (int q = threadIdx.x; q < LB_H * 96; q += 256) {
-    const int nn = q / 96, k = q - nn * 96;
-    sw1[nn][k] = k < LB_IN ? to_bf16(P[lb_off_w1() + k * LB_H + nn]) : 0;
-  }
-  for (int q = threadIdx.x; q < LB_H * LB_H; q += 256) {
-    const int nn = q / LB_H, k = q - nn * LB_H;
-    sw2[nn][k] = to_bf16(P[lb_off_w2(L) + k * LB_H + nn]);
-  }
-  for (int q = threadIdx.x; q < 16 * LB_H; q += 256) {
-    const int nn = q / LB_H, k = q - nn * LB_H;
-    sw3[nn][k] = nn < LB_OUT ? to_bf16(P[lb_off_w3(L) + k * LB_OUT + nn]) : 0;
+__global__ __launch_bounds__(256) void k_lb_infer(const LbInfer a) {
+  __shared__ __align__(16) uint8_t wsh[LB_PK_BYTES];
+  const int blk = blockIdx.x;
+  int f = 0;
+#pragma unroll
+  for (int q = 1; q < 6; ++q) f += blk >= a.blk_off[q] ? 1 : 0;
+  {
+    const uint4* __restrict__ src = reinterpret_cast<const uint4*>(a.pk + (int64_t)f * LB_PK_BYTES);
+    for (int q = threadIdx.x; q < LB_PK_BYTES / 16; q += 256) reinterpret_cast<uint4*>(wsh)[q] = src[q];
   }
   __syncthreads();
-  // B operand of this lane: k = 16 s + 4 grp + (0..3), n = 16 t + row
-  auto b1op = [&](int s, int t) { return *reinterpret_cast<const bf4*>(&sw1[t * 16 + row][s * 16 + 4 * grp]); };
-  auto b2op = [&](int s, int t) { return *reinterpret_cast<const bf4*>(&sw2[t * 16 + row][s * 16 + 4 * grp]); };
-  auto b3op = [&](int s) { return *reinterpret_cast<const bf4*>(&sw3[row][s * 16 + 4 * grp]); };
-  float b2v[4];
-#pragma unroll
-  for (int t = 0; t < 4; ++t) b2v[t] = P[lb_off_b2(L) + t * 16 + row];
-  const float b3v = row < LB_OUT ? P[lb_off_b3(L) + row] : 0.f;
-  const float* __restrict__ b1 = P + lb_off_b1();
-  const float z = 1.f / g.zinv;
+  // (w through readfirstlane: the compiler then knows it, and every loop
+  // bound and branch derived from it, to be uniform in the wavefront)
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int jl = lane & 15, g = lane >> 4;
+  const int c0 = a.face_off[f], c1 = a.face_off[f + 1], nc = a.face_off[6];
+  const int ngrp = (c1 - c0 + 15) >> 4;
+  const int nw = (a.blk_off[f + 1] - a.blk_off[f]) * 4;
+  // stencil point g of this lane group: the cell, -x, +x, -y; point g + 4:
+  // +y, -z, +z, two cells in along the normal; point 8: two cells out.  A
+  // workgroup's cells lie on one face, so every axis is a lane constant:
+  // the coordinate it moves (0 i, 1 j, 2 k), the grid's extent along it and
+  // the offset of the step.
+  const int nax = f >> 1;
+  const int ax0 = g == 3 ? 1 : 0, d0 = g == 0 ? 0 : (g == 2 ? 1 : -1);
+  const int ax1 = g == 0 ? 1 : (g == 3 ? nax : 2), d1 = g == 0 ? 1 : (g == 1 ? -1 : (g == 2 ? 1 : -2));
+  auto extent = [&](int ax) { return ax == 0 ? a.nx : (ax == 1 ? a.ny : a.nz); };
+  auto pitch = [&](int ax) { return ax == 0 ? 1 : (ax == 1 ? a.sy : a.sz); };
+  const int n0 = extent(ax0), n1 = extent(ax1), n2 = extent(nax);
+  const uint32_t dd0 = (uint32_t)(d0 * pitch(ax0)), dd1 = (uint32_t)(d1 * pitch(ax1)), dd2 = (uint32_t)(2 * pitch(nax));
+  const uint32_t fs4 = a.fstride * 4u;
+  const uint32_t fo8 = (uint32_t)(g == 0 ? 0 : 1 + 2 * g) * fs4;  // first field of the point-8 share
+  const bool vel8 = g == 0;                                      // that share: velocities or stresses
+  const float zinv = a.zinv, z = 1.f / a.zinv;
+  const bf8* __restrict__ w1 = reinterpret_cast<const bf8*>(wsh + LB_PK_W1) + lane;
+  const bf8* __restrict__ w2 = reinterpret_cast<const bf8*>(wsh + LB_PK_W2) + lane;
+  const bf8* __restrict__ w3 = reinterpret_cast<const bf8*>(wsh + LB_PK_W3) + lane;
+  const f4* __restrict__ b2p = reinterpret_cast<const f4*>(wsh + LB_PK_B2) + g;
+  const f4 b3v = reinterpret_cast<const f4*>(wsh + LB_PK_B3)[g];
 
-  for (int64_t base = ((int64_t)blockIdx.x * LB_WAVES + w) * 16; base < n;
-       base += (int64_t)gridDim.x * LB_WAVES * 16) {
-    // the A row of this lane: cell base + row
-    const int64_t sa = base + row;
-    const bool va = sa < n;
-    const LbCell c = va ? cells[sa] : LbCell{0, 0, 0, 0};
-    float v[6][4];
+  const int64_t drow = (int64_t)nc;
+
+  // A wavefront's items are (16-cell group, wavefield) pairs, wavefields
+  // innermost.  The gathers of the next item are issued before the current
+  // item's network is evaluated, and the cell records of the next group one
+  // group ahead, so the loads' latency hides behind the arithmetic.
+  struct Geo {
+    int cell, depth, live;  // live: outputs 6, 7, 8 kept (bits 0..2)
+    bool valid, k0, k1, k2;
+    uint32_t o0, o1, o2;    // byte offsets of the lane's stencil points
+  };
+  // (lanes past the face's end read its last record and are masked by
+  // Geo::valid: an unconditional load is not waited for until it is used)
+  auto cell_of = [&](int grp) { return a.cells[min(c0 + grp * 16 + jl, c1 - 1)]; };
+  auto geo = [&](int grp, const LbCell& c) {
+    Geo q;
+    q.cell = c0 + grp * 16 + jl;
+    q.valid = q.cell < c1;
+    q.depth = c.fd & 0xff;
+    q.live = (c.i > 0 && c.j > 0) | ((c.i > 0 && c.k > 0) << 1) | ((c.j > 0 && c.k > 0) << 2);
+    const uint32_t base = (uint32_t)c.i + (uint32_t)c.j * (uint32_t)a.sy + (uint32_t)c.k * (uint32_t)a.sz;
+    auto coord = [&](int ax) { return ax == 0 ? c.i : (ax == 1 ? c.j : c.k); };
+    // byte offset of a stencil point from the wavefield's base, 0 off the
+    // grid (the loads then read element 0 and are discarded)
+    auto point = [&](int cc, int d, int n, uint32_t dd, bool& ok) -> uint32_t {
+      ok = q.valid && (unsigned)(cc + d) < (unsigned)n;
+      return ((base + dd) * 4u) & (0u - (uint32_t)ok);
+    };
+    q.o0 = point(coord(ax0), d0, n0, dd0, q.k0);
+    q.o1 = point(coord(ax1), d1, n1, dd1, q.k1);
+    q.o2 = point(coord(nax), 2, n2, dd2, q.k2);
+    return q;
+  };
+  // buffer loads: the wavefield's base in scalar registers, the field's
+  // offset as the scalar offset, the stencil point's as the vector one (the
+  // nine fields of a wavefield span less than 4 GiB, checked on the host),
+  // so no per-load address arithmetic
+  auto gather = [&](const Geo& q, int b, float* x) {
+    const __amdgpu_buffer_rsrc_t fb =
+        __builtin_amdgcn_make_buffer_rsrc(a.F + (int64_t)b * a.wstride, (short)0, (int)(9u * fs4), 0x00020000);
+    auto ld = [&](uint32_t voff, uint32_t soff) {
+      return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(fb, voff, soff, 0));
+    };
 #pragma unroll
-    for (int s = 0; s < 6; ++s)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int e = s * 16 + 4 * grp + j;
-        v[s][j] = lb_input(g, c, e, va && e < LB_IN);
-      }
-    // all 24 gathers in flight before the first use (the scheduler would
-    // otherwise place each use right after its load and wait for each)
-    __builtin_amdgcn_sched_barrier(0);
-    float ss = 0.f;
-#pragma unroll
-    for (int s = 0; s < 6; ++s)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) ss = fmaf(v[s][j], v[s][j], ss);
-    ss += __shfl_xor(ss, 16);
-    ss += __shfl_xor(ss, 32);
-    const float r = sqrtf(ss * (1.f / LB_IN)) + 1e-30f;
-    const float ri = 1.f / r;
-    bf4 a[6];
-#pragma unroll
-    for (int s = 0; s < 6; ++s)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) a[s][j] = (short)to_bf16(v[s][j] * ri);
-    // per result row m = 4*grp + j: depth, RMS and shear flags of cell base + m
-    const int flags = (c.fd & 0xff) | ((c.i > 0) << 8) | ((c.j > 0) << 9) | ((c.k > 0) << 10);
-    int fm[4];
-    float rm[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      fm[j] = __shfl(flags, 4 * grp + j);
-      rm[j] = __shfl(r, 4 * grp + j);
+    for (int m = 0; m < 9; ++m) {
+      x[m] = ld(q.o0, m * fs4);
+      x[9 + m] = ld(q.o1, m * fs4);
     }
-    // layer 1
+    x[18] = ld(q.o2 + fo8, 0);
+    x[19] = ld(q.o2 + fo8, fs4);
+    x[20] = ld(q.o2, 2 * fs4);
+  };
+  auto infer = [&](const Geo& q, int b, float* x) {
+    // the weights are read from LDS in every item: an offset the compiler
+    // cannot see through keeps it from hoisting all 22 operand tiles (88
+    // registers) out of the loop, which would halve the occupancy
+    int zo;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(zo));
+    const bf8* __restrict__ w1i = w1 + zo;
+    const bf8* __restrict__ w2i = w2 + zo;
+    const bf8* __restrict__ w3i = w3 + zo;
+    const f4* __restrict__ b2i = b2p + zo;
+#pragma unroll
+    for (int m = 0; m < 9; ++m) {
+      x[m] = q.k0 ? x[m] : 0.f;
+      x[9 + m] = q.k1 ? x[9 + m] : 0.f;
+    }
+    x[18] = q.k2 ? x[18] : 0.f;
+    x[19] = q.k2 ? x[19] : 0.f;
+    x[20] = q.k2 && vel8 ? x[20] : 0.f;  // the third point-8 slot: group 0 only
+    // sum of squares with stresses divided by the impedance
+    float sv = 0.f, ss = 0.f;
+#pragma unroll
+    for (int m = 0; m < 9; ++m) {
+      if (m < 3) {
+        sv = fmaf(x[m], x[m], sv);
+        sv = fmaf(x[9 + m], x[9 + m], sv);
+      } else {
+        ss = fmaf(x[m], x[m], ss);
+        ss = fmaf(x[9 + m], x[9 + m], ss);
+      }
+    }
+    float s8 = x[18] * x[18];
+    s8 = fmaf(x[19], x[19], s8);
+    s8 = fmaf(x[20], x[20], s8);
+    float tot = fmaf(zinv * zinv, vel8 ? ss : ss + s8, vel8 ? sv + s8 : sv);
+    tot += __shfl_xor(tot, 16);
+    tot += __shfl_xor(tot, 32);
+    const float r = sqrtf(tot * (1.f / LB_IN)) + 1e-30f;
+    const float rv = __builtin_amdgcn_rcpf(r), rs = rv * zinv, r8 = vel8 ? rv : rs;
+    bf8 xb[3];
+#pragma unroll
+    for (int n = 0; n < 24; ++n) {
+      float v;
+      if (n < 18) v = x[n] * ((n % 9) < 3 ? rv : rs);
+      else if (n < 21) v = x[n] * r8;
+      else v = 0.f;
+      xb[n >> 3][n & 7] = (__bf16)v;
+    }
+    // layer 1: bias of the cell's depth as the accumulator's start
     f4 acc[4];
+    const f4* __restrict__ b1p = reinterpret_cast<const f4*>(wsh + LB_PK_B1) + q.depth * 16 + g;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+      acc[t] = b1p[4 * t];
 #pragma unroll
-      for (int s = 0; s < 6; ++s) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[s], b1op(s, t), acc[t], 0, 0, 0);
+      for (int s = 0; s < 3; ++s)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1i[(t * 3 + s) * 64], xb[s], acc[t], 0, 0, 0);
     }
+    bf8 h[2];
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        hs[w][4 * grp + j][t * 16 + row] = to_bf16(silu(acc[t][j] + b1[(fm[j] & 0xff) * LB_H + t * 16 + row]));
-    wave_sync();
-    bf4 a2[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) a2[s][j] = (short)hs[w][row][s * 16 + 4 * grp + j];
-    wave_sync();
+      for (int e = 0; e < 4; ++e) h[t >> 1][(t & 1) * 4 + e] = (__bf16)silu_hw(acc[t][e]);
     // layer 2
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+      acc[t] = b2i[4 * t];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[s], b2op(s, t), acc[t], 0, 0, 0);
+      for (int s = 0; s < 2; ++s)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2i[(t * 2 + s) * 64], h[s], acc[t], 0, 0, 0);
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) hs[w][4 * grp + j][t * 16 + row] = to_bf16(silu(acc[t][j] + b2v[t]));
-    wave_sync();
+      for (int e = 0; e < 4; ++e) h[t >> 1][(t & 1) * 4 + e] = (__bf16)silu_hw(acc[t][e]);
+    // layer 3: outputs 4g .. 4g+3 of cell j
+    f4 y = b3v;
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
+    for (int s = 0; s < 2; ++s) y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3i[s * 64], h[s], y, 0, 0, 0);
+    // every lane stores (no branch, so the compiler can count the stores
+    // and wait for the next gathers only): lanes without an output, past
+    // the face's end or at outputs 9..15, write to the trash slot
+    float* __restrict__ Db = a.D + (int64_t)b * LB_OUT * drow + q.cell;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) a2[s][j] = (short)hs[w][row][s * 16 + 4 * grp + j];
-    wave_sync();
-    // layer 3: lane holds y[m = 4*grp + j][o = row]
-    f4 y = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int s = 0; s < 4; ++s) y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[s], b3op(s), y, 0, 0, 0);
-    if (row < LB_OUT) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int64_t sm = base + 4 * grp + j;
-        if (sm < n) {
-          const LbCell cm{(fm[j] >> 8) & 1, (fm[j] >> 9) & 1, (fm[j] >> 10) & 1, 0};
-          D[sm * LB_OUT + row] = lb_comp_live(cm, row) ? (y[j] + b3v) * rm[j] * (row >= 3 ? z : 1.f) : 0.f;
-        }
-      }
+    for (int e = 0; e < 4; ++e) {
+      const int o = 4 * g + e;
+      const bool keep = q.valid && o < LB_OUT;
+      const bool live = o < 6 || ((q.live >> (min(o, 8) - 6)) & 1);
+      *(keep ? Db + (int64_t)o * drow : a.trash + lane) = live ? y[e] * r * (o >= 3 ? z : 1.f) : 0.f;
     }
+  };
+
+  int grp = (blk - a.blk_off[f]) * 4 + w;
+  if (grp >= ngrp) return;
+  Geo cur = geo(grp, cell_of(grp));
+  float x[21];
+  gather(cur, 0, x);
+  for (;;) {
+    const int gn = grp + nw;
+    const bool more = gn < ngrp;
+    // the next group's cell records, in flight during this group
+    const LbCell cn = cell_of(more ? gn : grp);
+    Geo nxt = cur;
+    for (int b = 0; b < a.nbatch; ++b) {
+      float xn[21];
+      if (b + 1 < a.nbatch) {
+        gather(cur, b + 1, xn);
+      } else if (more) {
+        // opaque to the compiler, so the records are not used (and waited
+        // for) right where they are loaded, in the loop's header
+        LbCell cq = cn;
+        asm volatile("" : "+v"(cq.i), "+v"(cq.j), "+v"(cq.k), "+v"(cq.fd));
+        nxt = geo(gn, cq);
+        gather(nxt, 0, xn);
+      }
+      infer(cur, b, x);
+#pragma unroll
+      for (int m = 0; m < 21; ++m) x[m] = xn[m];
+    }
+    if (!more) break;
+    cur = nxt;
+    grp = gn;
   }
 }
 
-// Inference, after the step: fields <- shrink(fields, D).
-__global__ __launch_bounds__(256) void k_lb_add(const LbGeom g, const LbCell* __restrict__ cells, int32_t n,
-                                               const float* __restrict__ D) {
-  const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (s >= n) return;
-  const LbCell c = cells[s];
-  const int64_t a = c.i + c.j * g.sy + (int64_t)c.k * g.sz;
+// After the step: fields <- shrink(fields, D), every wavefield (blockIdx.y).
+__global__ __launch_bounds__(256) void k_lb_add(const LbInfer a) {
+  const int nc = a.face_off[6];
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= nc) return;
+  const LbCell c = a.cells[s];
+  float* F = a.F + (int64_t)blockIdx.y * a.wstride + ((uint32_t)c.i + (uint32_t)c.j * (uint32_t)a.sy +
+                                                      (uint32_t)c.k * (uint32_t)a.sz);
+  const float* D = a.D + (int64_t)blockIdx.y * LB_OUT * nc + s;
   // all loads first: the nine fields are distinct arrays, but the compiler
   // cannot know it and would otherwise wait for each load after each store
   float v[LB_OUT], d[LB_OUT];
 #pragma unroll
-  for (int f = 0; f < LB_OUT; ++f) {
-    v[f] = g.F[f][a];
-    d[f] = D[s * LB_OUT + f];
+  for (int q = 0; q < LB_OUT; ++q) {
+    v[q] = F[(int64_t)q * a.fstride];
+    d[q] = D[(int64_t)q * nc];
   }
 #pragma unroll
-  for (int f = 0; f < LB_OUT; ++f) const_cast<float*>(g.F[f])[a] = lb_shrink(v[f], d[f]);
+  for (int q = 0; q < LB_OUT; ++q) F[(int64_t)q * a.fstride] = lb_shrink(v[q], d[q]);
 }
 
 // Training rollout: the corrections predicted before the update, from Y
-// [n][9] and the stencil RMS R, applied as k_lb_predict_mfma + k_lb_add do.
+// [n][9] and the stencil RMS R, applied as k_lb_infer + k_lb_add do.
 __global__ __launch_bounds__(256) void k_lb_apply_y(const LbGeom g, const LbCell* __restrict__ cells, int32_t n,
                                                    const float* __restrict__ Y, const float* __restrict__ R) {
   const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -551,17 +687,34 @@ hipError_t lb_launch_targets(const LbGeom& thin, const LbGeom& thick, int off, c
 }
 
 
-hipError_t lb_launch_predict_mfma(const float* P, int L, const LbGeom& g, const LbCell* cells, int n, float* D,
-                                  hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  const int nblk = (int)std::min<int64_t>(2048, (n + 16 * LB_WAVES - 1) / (16 * LB_WAVES));
-  hipLaunchKernelGGL(k_lb_predict_mfma, dim3(nblk), dim3(256), 0, s, P, L, g, cells, n, D);
+hipError_t lb_launch_pack(const float* P, int np, int L, uint8_t* pk, hipStream_t s) {
+  constexpr int n = 4 * 3 * 64 * 8 + 4 * 2 * 64 * 8 + 2 * 64 * 8 + LB_MAXL * LB_H + LB_H + 16;
+  hipLaunchKernelGGL(k_lb_pack, dim3(lb_blocks(n, 256), 6), dim3(256), 0, s, P, np, L, pk);
   return hipGetLastError();
 }
 
-hipError_t lb_launch_add(const LbGeom& g, const LbCell* cells, int n, const float* D, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_lb_add, dim3(lb_blocks(n, 256)), dim3(256), 0, s, g, cells, n, D);
+hipError_t lb_launch_infer(LbInfer& a, hipStream_t s) {
+  // workgroups per face in proportion to its 16-cell groups, about 8 per CU
+  // in all; every wavefront then takes the same share of groups
+  int64_t ng[6], tot = 0;
+  for (int f = 0; f < 6; ++f) {
+    ng[f] = (a.face_off[f + 1] - a.face_off[f] + 15) / 16;
+    tot += ng[f];
+  }
+  a.blk_off[0] = 0;
+  if (tot == 0 || a.nbatch <= 0) return hipSuccess;
+  const int64_t want = std::min<int64_t>(2048, (tot + 3) / 4);
+  for (int f = 0; f < 6; ++f) {
+    const int64_t nb = ng[f] ? std::max<int64_t>(1, (want * ng[f] + tot / 2) / tot) : 0;
+    a.blk_off[f + 1] = a.blk_off[f] + (int32_t)nb;
+  }
+  hipLaunchKernelGGL(k_lb_infer, dim3(a.blk_off[6]), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t lb_launch_add(const LbInfer& a, hipStream_t s) {
+  if (a.face_off[6] <= 0 || a.nbatch <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lb_add, dim3(lb_blocks(a.face_off[6], 256), a.nbatch), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
