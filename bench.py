@@ -18,7 +18,8 @@ Other configurations:
               linear combination and 10 of them checked against direct runs
   --config 3  32 isotropic random-field cubes (edges 96..256) advanced as
               one group (one launch per half-step over all models), against
-              the same models run one after the other
+              the same models run one after the other; also 64 cubes of
+              24..48 and 32 of 48..96 cells (--small-steps steps) both ways
   --config 4  the learned 6-cell boundary on a 512x512x256 grid: receiver
               errors against a 2x-padded reference (with the plain 6-cell
               and 24-cell layers beside it), and six elementary tensors with
@@ -430,6 +431,7 @@ def bench_uneven(args, steps):
               "kernel": "two-pass group", "tiling": list(tiling), "medium": _medium_desc(wls[0], enc, 0),
               "alg_bytes_per_cell": ba + bb,
               "autotune_ms_per_step": None if tune is None else round(tune, 4), "parallelism": "single GPU"}
+    small = [_small_group(n, e, args.small_steps) for n, e in ((64, (24, 32, 40, 48)), (32, (48, 64, 80, 96)))]
     # the job's throughput is that of the faster way to run it; both are reported
     el = el_g
     if el_seq < el_g:
@@ -440,7 +442,46 @@ def bench_uneven(args, steps):
                  group_vs_sequential={"group_s": round(el_g, 3), "sequential_s": round(el_seq, 3),
                                       "speedup": round(el_seq / el_g, 3), "traces_identical": same,
                                       "sequential_stress_ms": round(ms_grp[0], 2),
-                                      "sequential_velocity_ms": round(ms_grp[1], 2)})
+                                      "sequential_velocity_ms": round(ms_grp[1], 2)},
+                 group_vs_sequential_small_models=small)
+
+
+def _small_group(nmodels, edges, steps):
+    """Many small isotropic random-field cubes (6-cell separable damping) as
+    one group against the same models one after the other, each autotuned
+    and replaying a captured graph when that is faster: where one model
+    cannot fill the GPU, one launch per half-step for all of them pays."""
+    from elanfmt_amd import fwe, workloads as W
+
+    wls = W.uneven_batch(nmodels=nmodels, seed=12, edges=edges, nt=steps + 60, L=6)
+    cells = sum(w.cells for w in wls)
+    print(f"# uneven, small: {nmodels} models, edges {edges}, {steps} steps", file=sys.stderr, flush=True)
+    sol = [W.make_solver(w) for w in wls]
+    g = fwe.Group(sol)
+    g.autotune(3)
+    g.run(20)
+    for s in sol:
+        s.reset()
+        s.synchronize()
+    t0 = time.perf_counter()
+    g.run(steps)
+    for s in sol:
+        s.synchronize()
+    el_g = time.perf_counter() - t0
+    tr_g = [s.traces() for s in sol]
+    g.close()
+    el_s, graphs = 0.0, 0
+    for s in sol:
+        el, _, _, _, gsteps = _timed_run(s, steps, 20, True)
+        el_s += el
+        graphs += gsteps > 0
+    same = all(np.array_equal(s.traces(), t) for s, t in zip(sol, tr_g))
+    for s in sol:
+        s.close()
+    return {"models": nmodels, "edges": list(edges), "cells": cells, "steps": steps,
+            "group_s": round(el_g, 4), "sequential_s": round(el_s, 4), "speedup": round(el_s / el_g, 2),
+            "group_gcell_s": round(cells * steps / el_g / 1e9, 2), "sequential_graph_replays": graphs,
+            "traces_identical": same}
 
 
 def bench_learned(args, steps):
@@ -544,6 +585,7 @@ def main():
     ap.add_argument("--eval-steps", type=int, default=1500)
     ap.add_argument("--kernel", default="auto", choices=["auto", "two_pass", "fused"])
     ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--small-steps", type=int, default=400, help="config 3: steps of the small-model groups")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--medium", default="auto", choices=sorted(MEDIUM),

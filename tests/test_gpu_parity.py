@@ -514,6 +514,39 @@ def test_group_equals_separate_runs(gpu):
         s.close()
 
 
+def test_group_of_many_small_models_equals_separate_runs(gpu):
+    """48 cubes of 13..40 cells (ragged against every tile shape, some
+    smaller than one tile) in one group: every member's traces and fields
+    equal its separate run bit for bit, after the group's autotune too."""
+    from elanfmt_amd import fwe, workloads as W
+
+    wls = W.uneven_batch(nmodels=48, seed=5, edges=(13, 17, 24, 31, 40), nt=60, L=4)
+    print(f"group of {len(wls)} models, edges {sorted(set(w.nx for w in wls))}", flush=True)
+    ref = []
+    for wl in wls:
+        s = W.make_solver(wl)
+        s.run(wl.nt)
+        ref.append((s.traces(), [s.get_field(q) for q in range(9)]))
+        s.close()
+    sol = [W.make_solver(wl) for wl in wls]
+    g = fwe.Group(sol)
+    for tune in (False, True):
+        if tune:
+            assert g.autotune(2) > 0
+        g.run(25)
+        g.run(35)
+        for s, (tr, f) in zip(sol, ref):
+            assert np.abs(tr).max() > 0
+            assert np.array_equal(s.traces(), tr)
+            for q in range(9):
+                assert np.array_equal(s.get_field(q), f[q]), q
+        for s in sol:
+            s.reset()
+    g.close()
+    for s in sol:
+        s.close()
+
+
 def test_combine_elementary_traces_equals_direct_runs(gpu):
     """Six elementary-tensor wavefields in one batched run, combined with the
     weights of a general tensor, reproduce the direct single-tensor run
