@@ -260,8 +260,8 @@ __global__ __launch_bounds__(256) void k_lb_gather(const LbGeom g, const LbCell*
   depth[s] = c.fd & 0xff;
 }
 
-// Shear components that the stress pass keeps at zero for this cell are
-// masked (a correction there would be overwritten).
+// Shear components outside the stress update's range at this cell (the
+// reference never updates them) are masked.
 __device__ __forceinline__ bool lb_comp_live(const LbCell& c, int f) {
   if (f == 6) return c.i > 0 && c.j > 0;
   if (f == 7) return c.i > 0 && c.k > 0;

@@ -417,10 +417,11 @@ __global__ __launch_bounds__(64 * SWEEP_WPG, R == 2 ? 2 : 1) void k_sweep(const 
           nxz = ortho_shear(w, oxz[r], m7, dxz);
           nyz = ortho_shear(w, oyz[r], m6, dyz);
         }
-        // shear cells outside the reference update ranges stay zero
-        if (!(im && jm)) nxy = 0.f;
-        if (!(im && km)) nxz = 0.f;
-        if (!(jm && km)) nyz = 0.f;
+        // shear cells outside the reference update ranges keep their value
+        // (zero unless a point injection adds to them)
+        if (!(im && jm)) nxy = oxy[r];
+        if (!(im && km)) nxz = oxz[r];
+        if (!(jm && km)) nyz = oyz[r];
         if (!a.src_after_bc) {
           nxx += src[r][0]; nyy += src[r][1]; nzz += src[r][2];
           nxy += src[r][3]; nxz += src[r][4]; nyz += src[r][5];

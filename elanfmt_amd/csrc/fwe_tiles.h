@@ -199,9 +199,8 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb, const int
       const float vz_im = ldz(Vz, im, c - 1);
       const float vz_jm = ldz(Vz, jm, c - sy);
       float sxx = ld(Sxx + c), syy = ld(Syy + c), szz = ld(Szz + c);
-      float sxy = ldz(Sxy, im && jm, c);
-      float sxz = ldz(Sxz, im && km, c);
-      float syz = ldz(Syz, jm && km, c);
+      const float oxy = ld(Sxy + c), oxz = ld(Sxz + c), oyz = ld(Syz + c);
+      float sxy = oxy, sxz = oxz, syz = oyz;
       cvx[b] = vx; cvy[b] = vy; cvz[b] = vz_kp;
       const float ex = (vx_ip - vx) * a.rdx, ey = (vy_jp - vy) * a.rdy, ez = (vz_kp - vz) * a.rdz;
       const float dxy = rate2(vy - vy_im, a.rdx, vx - vx_jm, a.rdy);
@@ -224,10 +223,12 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb, const int
         sxz = ortho_shear(w, sxz, m7, dxz);
         syz = ortho_shear(w, syz, m6, dyz);
       }
-      // shear cells outside the reference update ranges stay zero
-      if (!(im && jm)) sxy = 0.f;
-      if (!(im && km)) sxz = 0.f;
-      if (!(jm && km)) syz = 0.f;
+      // shear cells outside the reference update ranges keep their value
+      // (zero unless a point injection adds to them, as in the reference,
+      // which never writes them otherwise)
+      if (!(im && jm)) sxy = oxy;
+      if (!(im && km)) sxz = oxz;
+      if (!(jm && km)) syz = oyz;
       // point injections of this cell (OK4.cu:336-342 / ANISO.cu:228-252)
       float src[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       float src_xz1 = 0.f, src_yz1 = 0.f;  // injections at (i,j,1), for BC1

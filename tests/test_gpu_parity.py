@@ -89,6 +89,32 @@ def test_ortho_free_surface_all_damping_forms(gpu, pml):
         assert rel_l2(gf[f], rf[f]) < TOL, f
 
 
+@pytest.mark.parametrize("law,fs", [("ortho", 0), ("ortho", 1), ("iso", 0), ("iso", 1)])
+def test_injections_into_shear_cells_outside_the_update_range_accumulate(gpu, law, fs):
+    """The reference updates Sxy only at i, j >= 1 (Sxz at i, k >= 1, Syz at
+    j, k >= 1) and never writes the other shear cells, so injections there
+    accumulate from step to step; the kernels keep those cells' values."""
+    from elanfmt_amd import fwe, workloads as W
+
+    if law == "ortho":
+        wl = W.ortho_layered(nx=40, ny=34, nz=30, L=4, nt=40, nrec_side=2)
+    else:
+        wl = W.iso_homogeneous(n=36, L=4, nt=40, rec_depth=2, nrec_side=2)
+    wl.free_surface = fs
+    wl.injections = [(fwe.SXY, 0, 9, 9, 0, 2e14), (fwe.SXY, 9, 0, 9, 0, -1e14), (fwe.SXZ, 0, 9, 10, 0, 1e14),
+                     (fwe.SXZ, 11, 8, 0, 0, 3e14), (fwe.SYZ, 8, 0, 11, 0, 2e14), (fwe.SYZ, 8, 11, 0, 0, -2e14),
+                     (fwe.SXX, 12, 12, 12, 0, 1e15)]
+    wl.receivers = [(q, i, j, k, 0) for q in range(3) for (i, j, k) in ((1, 9, 9), (9, 1, 9), (11, 8, 1), (20, 20, 20))]
+    g, r, gf, rf = _run_both(wl)
+    for comp, (i, j, k) in ((fwe.SXY, (0, 9, 9)), (fwe.SXZ, (11, 8, 0)), (fwe.SYZ, (8, 0, 11))):
+        f = 3 + comp  # fields: Vx Vy Vz, then the six stresses
+        assert abs(rf[f][i, j, k]) > 0
+        assert abs(gf[f][i, j, k] - rf[f][i, j, k]) <= 1e-5 * abs(rf[f][i, j, k])
+    assert rel_l2(g, r) < TOL
+    for f in range(9):
+        assert rel_l2(gf[f], rf[f]) < TOL, f
+
+
 def test_iso_vacuum_layer_top(gpu):
     """Vacuum-layer top (free_surface=2, the BC2 variant,
     diff_BC2_OK3/FastWaveED_GPU3D_v1.cu:309-314): empty top cell layer and

@@ -16,11 +16,14 @@ TOL = 1e-5
 
 
 def _edge_points(wl, rows, kc, seed=0):
-    """Injections and receivers at sweep-tile columns 61..63 and 123..125,
-    rows around multiples of `rows`, planes around multiples of `kc`."""
+    """Injections and receivers at the grid's first and last columns (shear
+    cells there are outside the update range and only accumulate their
+    injections), at the sweep tiles' edge and halo columns (61..63,
+    123..125) and at 64-column boundaries, rows around multiples of `rows`,
+    planes around multiples of `kc`."""
     rng = np.random.default_rng(seed)
     nx, ny, nz = wl.nx, wl.ny, wl.nz
-    ic = [i for i in (1, 61, 62, 63, 123, 124, 125, nx - 2) if 0 <= i < nx]
+    ic = [i for i in (0, 1, 61, 62, 63, 64, 123, 124, 125, 128, nx - 2, nx - 1) if 0 <= i < nx]
     jc = [j for j in (1, rows - 1, rows, rows + 1, 2 * rows, ny - 2) if 0 <= j < ny]
     kcs = [k for k in (1, kc - 1, kc, kc + 1, 2 * kc - 1, nz - 2) if 0 <= k < nz]
     pts = sorted({(i, j, k) for i in ic for j in jc for k in kcs})
@@ -125,9 +128,10 @@ def test_sweep_free_surface(gpu, fs_mode, after, rows, kc):
         for comp in range(6):
             inj.append((comp, 30 + comp, 10 + k, k, 0, 1e14 * (1 + comp + k)))
     inj.append((fwe.SXZ, 62, rows, 1, 0, 2e14))
+    inj.append((fwe.SXZ, 63, rows, 1, 0, 1e14))
     inj.append((fwe.SYZ, 61, rows - 1, 1, 0, -2e14))
     wl.injections = inj
-    wl.receivers = [(q, i, j, k, 0) for q in range(3) for i in (30, 61, 62, 63) for j in (rows, 11)
+    wl.receivers = [(q, i, j, k, 0) for q in range(3) for i in (30, 61, 62, 63, 64) for j in (rows, 11)
                     for k in (1, 2, 3)]
     print(f"sweep free surface {fs_mode} after_bc={after} rows={rows} kchunk={kc}", flush=True)
 
