@@ -320,7 +320,10 @@ def bench_batched(args, steps):
     order = ORDER_NAMES[s.wg_order()]
     enc, ntab = s.medium_encoding()
     bt = s.batch_tile()
-    kernels, dom, bstep = _kernel_entries(wl, False, tiling, ms_a, ms_b, steps, cells, enc, bt)
+    # the autotuner may keep the single-pass sweep (one launch per step)
+    fused = s.kernel() == fwe.KERNEL_FUSED
+    sweep = s.sweep_tiling()
+    kernels, dom, bstep = _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells, enc, bt, sweep=sweep)
     # 1000 lune tensors by linear combination of the six elementary runs
     mts = np.array([W.mt_lune_random(50000 + q, m0)[0] for q in range(1000)])
     t0 = time.perf_counter()
@@ -330,7 +333,7 @@ def bench_batched(args, steps):
 
     # six plain runs, one per elementary tensor
     nr = len(wl.receivers) // 6
-    el_p, plain_tiling = 0.0, None
+    el_p, plain_tiling, plain_kernel = 0.0, None, None
     xs, ys, zs = wl.nx // 2 - 1, wl.ny // 2 - 1, wl.nz // 2 - 1
     base = W.config(2, nt=max(steps, 1), m0=m0)
     base.nbatch, base.stf = 1, base.stf[:1]
@@ -352,6 +355,7 @@ def bench_batched(args, steps):
         el, _, _, _, _ = _timed_run(p, steps, args.warmup if e == 0 else 0, False)
         el_p += el
         plain_tiling = (p.tile_cols(),) + tuple(p.tiling())
+        plain_kernel = "single-pass sweep" if p.kernel() == fwe.KERNEL_FUSED else "two-pass"
         p.close()
     print(f"# batched {el_b:.2f} s, six plain {el_p:.2f} s", file=sys.stderr, flush=True)
 
@@ -369,16 +373,19 @@ def bench_batched(args, steps):
         errs.append(float(np.linalg.norm(sy - tr) / max(np.linalg.norm(tr), 1e-300)))
     config = {"workload": f"{wl.name}: layered orthorhombic FP32, 24-cell separable damping, six elementary "
                           f"moment tensors (Mxx..Myz) as one batch of wavefields, {nr} receiver traces each",
-              "grid": [wl.nx, wl.ny, wl.nz], "wavefields": 6, "nt": wl.nt, "kernel": "two-pass batched",
-              "tiling": list(tiling), "wg_order": order, "medium": _medium_desc(wl, enc, ntab),
-              "wavefields_per_workgroup": bt,
+              "grid": [wl.nx, wl.ny, wl.nz], "wavefields": 6, "nt": wl.nt,
+              "kernel": "single-pass sweep, all six wavefields per launch" if fused else "two-pass batched",
+              "tiling": ({"rows_per_lane": sweep[0], "zchunk": sweep[1]} if fused else list(tiling)),
+              "wg_order": None if fused else order, "medium": _medium_desc(wl, enc, ntab),
+              "wavefields_per_workgroup": None if fused else bt,
               "autotune_ms_per_step": None if tune is None else round(tune, 4), "graph_steps": gsteps,
               "parallelism": "single GPU"}
     return _line(cells * 6 * steps / el_b / 1e9, steps, args.warmup, el_b, config,
                  hbm_pct_step=round(100 * bstep * cells * steps / el_b / 1e9 / PEAK_HBM_GBS, 2),
                  kernels=kernels, roofline=_roofline(dom, steps, cells, wl.name + "x6"),
                  batched_vs_plain={"batched_s": round(el_b, 3), "six_plain_runs_s": round(el_p, 3),
-                                   "speedup": round(el_p / el_b, 3), "plain_tiling": list(plain_tiling)},
+                                   "speedup": round(el_p / el_b, 3), "plain_tiling": list(plain_tiling),
+                                   "plain_kernel": plain_kernel},
                  synthesis={"tensors": len(mts), "seconds": round(el_syn, 3),
                             "checked_direct_runs": len(errs), "max_rel_l2": float(f"{max(errs):.3e}")})
 
