@@ -45,17 +45,31 @@ def test_network_forward_and_gradient_match_cpu(gpu):
     lb.close()
 
 
-def test_attached_boundary_matches_oracle_plus_cpu_corrections(gpu):
+@pytest.mark.parametrize("dims,L", [((22, 20, 18), 3), ((26, 17, 9), 5), ((11, 23, 14), 6), ((31, 12, 12), 6)])
+def test_attached_boundary_matches_oracle_plus_cpu_corrections(gpu, dims, L):
     """Every step: corrections from the fields before the step, computed by
     the CPU network on the CPU stencil features, applied (dissipative part)
-    after the oracle step; the HIP path must follow within FP32 round-off."""
+    after the oracle step; the HIP path must follow within FP32 round-off.
+    Besides a plain case, grids no thicker than two layers on one or two
+    axes: opposite faces meet, every cell is a layer cell, faces hold few
+    cells and stencil points leave the grid on several axes.  There every
+    cell is corrected every step, by corrections of 10 to 60% of the field
+    near the source, and a bf16 activation whose FP32 (GPU) and FP64
+    (reference) pre-activations round to neighbouring values moves its
+    cell's output by up to ~5e-4: measured against an FP32 evaluation of
+    the same network the GPU outputs differ by at most 9e-6, against the
+    FP64 reference by up to 5e-4 at a few cells.  Over four steps that
+    reaches 4e-4 of the fields, so grids with no interior along some axis
+    are held to 1e-3, which a wrong weight, stencil point or depth (errors
+    of a percent or more in the corrections) still exceeds."""
     from elanfmt_amd import fwe, workloads as W
     from oracle import Oracle, lb_ref
 
     rng = np.random.default_rng(12)
-    L = 3
-    wl = W.ortho_layered(nx=22, ny=20, nz=18, L=L, nt=30, f0=25.0, nrec_side=2)
-    c = (10, 9, 8)
+    nx, ny, nz = dims
+    wl = W.ortho_layered(nx=nx, ny=ny, nz=nz, L=min(L, 4), nt=30, f0=25.0, nrec_side=2)
+    print(f"learned boundary L={L} on {nx}x{ny}x{nz}", flush=True)
+    c = (nx // 2, ny // 2, nz // 2)
     wl.injections = fwe.mt_injections(fwe.SRC_LAYOUT_OK4, *c, [1e15, -5e14, 2e14, 3e14, -1e14, 2e14])
     z = 1.5e7
     params = [_params(rng, L) for _ in range(6)]
@@ -87,7 +101,7 @@ def test_attached_boundary_matches_oracle_plus_cpu_corrections(gpu):
         s.run(1)
         errs = [rel_l2(s.get_field(f), new[f]) for f in range(9)]
         print(f"step {step}: max field rel-L2 {max(errs):.2e}", flush=True)
-        assert max(errs) < 1e-4, errs
+        assert max(errs) < (1e-4 if min(nx, ny, nz) > 2 * L else 1e-3), errs
     # corrections were not negligible, and some were cut to their dissipative part
     assert max(abs(v) for *_, v in corr) > 0 and cut > 0
     lb.detach(s)
