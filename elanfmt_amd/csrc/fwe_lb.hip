@@ -335,6 +335,26 @@ __host__ __device__ constexpr int lb_slot1(int g, int n) {
 }
 __host__ __device__ constexpr int lb_k2(int g, int n) { return 16 * (n >> 2) + 4 * g + (n & 3); }
 
+// every input lands in exactly one layer-1 slot and every hidden unit in
+// exactly one layer-2/3 slot
+constexpr bool lb_slots_are_bijections() {
+  int seen1[LB_IN] = {}, seen2[LB_H] = {};
+  for (int g = 0; g < 4; ++g)
+    for (int n = 0; n < 24; ++n) {
+      const int e = lb_slot1(g, n);
+      if (e >= LB_IN) return false;
+      if (e >= 0) ++seen1[e];
+    }
+  for (int g = 0; g < 4; ++g)
+    for (int n = 0; n < 16; ++n) ++seen2[lb_k2(g, n)];
+  for (int e = 0; e < LB_IN; ++e)
+    if (seen1[e] != 1) return false;
+  for (int h = 0; h < LB_H; ++h)
+    if (seen2[h] != 1) return false;
+  return true;
+}
+static_assert(lb_slots_are_bijections(), "inference operand order must cover each input and hidden unit once");
+
 __global__ __launch_bounds__(256) void k_lb_pack(const float* __restrict__ P, int32_t np, int32_t L,
                                                 uint8_t* __restrict__ pk) {
   constexpr int NW1 = 4 * 3 * 64 * 8, NW2 = 4 * 2 * 64 * 8, NW3 = 2 * 64 * 8;
