@@ -96,6 +96,14 @@ struct fwe_ctx {
   // wavefields per workgroup of a batch (a divisor of nbatch; 0 = default)
   int batch_tile = 0;
   float* w3 = nullptr;      // nelem (W3D)
+  // convolutional PML: layer cells per face, the three axis tables (6 rows
+  // of n+1 each, x then y then z) and the memory variables (pstride floats
+  // per wavefield: x slab, y slab, z slab, six variables each)
+  int cpml_L = 0;
+  float* ctab = nullptr;
+  float* psi = nullptr;
+  int64_t pvar[3] = {0, 0, 0};
+  int64_t pstride = 0;
   float* prof = nullptr;    // wx wy wz whx why whz (profile / taper tables)
   float* stf = nullptr;     // nb * nstf
   float* traces = nullptr;  // nrec * nt
@@ -231,11 +239,11 @@ static fwe_status check_cfg(const fwe_config* c) {
   if (c->mode != FWE_MODE_ISO && c->mode != FWE_MODE_ORTHO) return fail(FWE_ERR_ARG, "bad mode");
   if (c->nbatch < 1 || c->nbatch > FWE_MAX_BATCH) return fail(FWE_ERR_ARG, "nbatch %d out of 1..%d", c->nbatch, FWE_MAX_BATCH);
   if (!(c->dx > 0 && c->dy > 0 && c->dz > 0 && c->dt > 0)) return fail(FWE_ERR_ARG, "dx,dy,dz,dt must be > 0");
-  if (c->mode == FWE_MODE_ISO && c->pml != FWE_PML_NONE && c->pml != FWE_PML_PROFILE)
-    return fail(FWE_ERR_UNSUPPORTED, "iso law takes PROFILE or NONE damping");
+  if (c->mode == FWE_MODE_ISO && c->pml != FWE_PML_NONE && c->pml != FWE_PML_PROFILE && c->pml != FWE_PML_CPML)
+    return fail(FWE_ERR_UNSUPPORTED, "iso law takes PROFILE, CPML or NONE damping");
   if (c->mode == FWE_MODE_ORTHO && c->pml == FWE_PML_PROFILE)
-    return fail(FWE_ERR_UNSUPPORTED, "ortho law takes WEIGHTS3D, SEPARABLE or NONE damping");
-  if (c->pml < 0 || c->pml > 3) return fail(FWE_ERR_ARG, "bad pml");
+    return fail(FWE_ERR_UNSUPPORTED, "ortho law takes WEIGHTS3D, SEPARABLE, CPML or NONE damping");
+  if (c->pml < 0 || c->pml > 4) return fail(FWE_ERR_ARG, "bad pml");
   if (c->free_surface < FWE_FS_NONE || c->free_surface > FWE_FS_VACUUM)
     return fail(FWE_ERR_ARG, "free_surface must be 0 (none), 1 (BC1) or 2 (vacuum layer)");
   return FWE_OK;
@@ -295,6 +303,7 @@ extern "C" fwe_status fwe_destroy(fwe_ctx* x) {
   hipFree(x->it_dev);
   for (auto ev : x->events) hipEventDestroy(ev);
   hipFree(x->fbuf[0]); hipFree(x->fbuf[1]); hipFree(x->medium); hipFree(x->w3); hipFree(x->prof);
+  hipFree(x->ctab); hipFree(x->psi);
   hipFree(x->mid); hipFree(x->mtab);
   hipFree(x->lb_cells); hipFree(x->lb_D);
   hipFree(x->stf); hipFree(x->traces); hipFree(x->inj_off); hipFree(x->inj);
@@ -440,6 +449,70 @@ extern "C" fwe_status fwe_set_pml_weights(fwe_ctx* x, const float* w) {
 // ---------------------------------------------------------------------------
 // source and receivers
 // ---------------------------------------------------------------------------
+extern "C" fwe_status fwe_set_cpml(fwe_ctx* x, int32_t layer, const float* tx, const float* ty, const float* tz) {
+  if (!x || !tx || !ty || !tz) return fail(FWE_ERR_ARG, "NULL argument");
+  const fwe_config& c = x->cfg;
+  if (c.pml != FWE_PML_CPML) return fail(FWE_ERR_STATE, "context damping is not CPML");
+  if (layer < 1 || 2 * layer > c.nx || 2 * layer > c.ny || 2 * layer > c.nz)
+    return fail(FWE_ERR_ARG, "CPML layer %d: need 1 <= layer and 2*layer <= nx, ny, nz", layer);
+  HIPCHK(hipSetDevice(c.device));
+  HIPCHK(hipStreamSynchronize(x->stream));  // steps still queued read the old state
+  drop_graph(x);
+  const int64_t L2 = 2 * (int64_t)layer;
+  const int64_t pv[3] = {L2 * c.ny * c.nz, (int64_t)c.nx * L2 * c.nz, (int64_t)c.nx * c.ny * L2};
+  const int64_t ps = 6 * (pv[0] + pv[1] + pv[2]);
+  if (layer != x->cpml_L || !x->psi) {
+    hipFree(x->psi);
+    x->psi = nullptr;
+    x->cpml_L = 0;
+    hipError_t e = hipMalloc(&x->psi, sizeof(float) * (size_t)ps * c.nbatch);
+    if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? FWE_ERR_NOMEM : FWE_ERR_HIP, "CPML memory: %s",
+                                     hipGetErrorString(e));
+  }
+  if (!x->ctab) HIPCHK(hipMalloc(&x->ctab, sizeof(float) * 6 * ((size_t)c.nx + c.ny + c.nz + 3)));
+  std::vector<float> h;
+  h.reserve(6 * ((size_t)c.nx + c.ny + c.nz + 3));
+  h.insert(h.end(), tx, tx + 6 * ((size_t)c.nx + 1));
+  h.insert(h.end(), ty, ty + 6 * ((size_t)c.ny + 1));
+  h.insert(h.end(), tz, tz + 6 * ((size_t)c.nz + 1));
+  fwe_status st = upload(x, x->ctab, h);
+  if (st) return st;
+  // the memory variables start from zero with the new coefficients
+  HIPCHK(hipMemsetAsync(x->psi, 0, sizeof(float) * (size_t)ps * c.nbatch, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  x->cpml_L = layer;
+  for (int d = 0; d < 3; ++d) x->pvar[d] = pv[d];
+  x->pstride = ps;
+  x->have_damp = true;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_cpml_state_size(const fwe_ctx* x, int64_t* n) {
+  if (!x || !n) return fail(FWE_ERR_ARG, "NULL argument");
+  if (!x->psi) return fail(FWE_ERR_STATE, "no CPML set");
+  *n = x->pstride;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_get_cpml_state(fwe_ctx* x, int32_t batch, float* out) {
+  if (!x || !out) return fail(FWE_ERR_ARG, "NULL argument");
+  if (!x->psi) return fail(FWE_ERR_STATE, "no CPML set");
+  if (batch < 0 || batch >= x->cfg.nbatch) return fail(FWE_ERR_ARG, "bad batch");
+  HIPCHK(hipSetDevice(x->cfg.device));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  HIPCHK(hipMemcpy(out, x->psi + (int64_t)batch * x->pstride, sizeof(float) * (size_t)x->pstride,
+                   hipMemcpyDeviceToHost));
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_set_cpml_state(fwe_ctx* x, int32_t batch, const float* in) {
+  if (!x || !in) return fail(FWE_ERR_ARG, "NULL argument");
+  if (!x->psi) return fail(FWE_ERR_STATE, "no CPML set");
+  if (batch < 0 || batch >= x->cfg.nbatch) return fail(FWE_ERR_ARG, "bad batch");
+  std::vector<float> h(in, in + x->pstride);
+  return upload(x, x->psi + (int64_t)batch * x->pstride, h);
+}
+
 extern "C" fwe_status fwe_set_stf(fwe_ctx* x, int32_t batch, const float* stf, int32_t n) {
   if (!x || (!stf && n > 0) || n < 0) return fail(FWE_ERR_ARG, "bad stf arguments");
   if (batch < 0 || batch >= x->cfg.nbatch) return fail(FWE_ERR_ARG, "batch %d out of range", batch);
@@ -653,6 +726,17 @@ static StepArgs make_args(fwe_ctx* x, int from) {
   a.wx = x->prof + ox; a.wy = x->prof + oy; a.wz = x->prof + oz;
   a.whx = x->prof + oh + ox; a.why = x->prof + oh + oy; a.whz = x->prof + oh + oz;
   a.w3 = x->w3;
+  if (c.pml == FWE_PML_CPML) {
+    a.cL = x->cpml_L;
+    a.ctab[0] = x->ctab;
+    a.ctab[1] = x->ctab + 6 * ((int64_t)c.nx + 1);
+    a.ctab[2] = a.ctab[1] + 6 * ((int64_t)c.ny + 1);
+    a.psi[0] = x->psi;
+    a.psi[1] = x->psi + 6 * x->pvar[0];
+    a.psi[2] = a.psi[1] + 6 * x->pvar[1];
+    for (int d = 0; d < 3; ++d) a.pvar[d] = x->pvar[d];
+    a.pstride = x->pstride;
+  }
   a.rdx = (float)(1.0 / c.dx); a.rdy = (float)(1.0 / c.dy); a.rdz = (float)(1.0 / c.dz);
   a.dtdx = (float)(c.dt / c.dx); a.dtdy = (float)(c.dt / c.dy); a.dtdz = (float)(c.dt / c.dz);
   a.ty_rows = x->ty; a.ntx = x->ntx; a.nty = x->nty; a.nkc = x->nkc; a.kchunk = x->kchunk;
@@ -760,6 +844,7 @@ static fwe::SweepArgs make_sweep(fwe_ctx* x, int from) {
 // the single-pass sweep kernel runs when asked for, or under AUTO when
 // fwe_autotune measured it faster than the two passes
 static bool use_fused(const fwe_ctx* x) {
+  if (x->cfg.pml == FWE_PML_CPML) return false;  // the two-pass kernels only
   return x->kernel == FWE_KERNEL_FUSED || (x->kernel == FWE_KERNEL_AUTO && x->auto_sweep);
 }
 
@@ -841,6 +926,7 @@ extern "C" fwe_status fwe_reset(fwe_ctx* x) {
   x->cur = 0;
   if (x->traces)
     HIPCHK(hipMemsetAsync(x->traces, 0, sizeof(float) * std::max<size_t>(1, (size_t)x->nrec * std::max(1, x->cfg.nt)), x->stream));
+  if (x->psi) HIPCHK(hipMemsetAsync(x->psi, 0, sizeof(float) * (size_t)x->pstride * x->cfg.nbatch, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   x->it = 0;
   return FWE_OK;
@@ -962,6 +1048,8 @@ extern "C" fwe_status fwe_set_kernel(fwe_ctx* x, int32_t kind) {
     return fail(FWE_ERR_ARG, "bad kernel kind %d", kind);
   if (kind == FWE_KERNEL_FUSED && x->lb)
     return fail(FWE_ERR_UNSUPPORTED, "the single-pass kernel has no learned boundary");
+  if (kind == FWE_KERNEL_FUSED && x->cfg.pml == FWE_PML_CPML)
+    return fail(FWE_ERR_UNSUPPORTED, "the single-pass kernel has no convolutional PML");
   if (kind == FWE_KERNEL_FUSED && !sweep_fits(x))
     return fail(FWE_ERR_UNSUPPORTED, "the single-pass kernel needs arrays under 4 GiB");
   drop_graph(x);
@@ -1094,7 +1182,7 @@ extern "C" fwe_status fwe_autotune(fwe_ctx* x, int32_t trial_steps, double* best
   };
   std::vector<Cand> cands;
   const bool try_two = x->kernel != FWE_KERNEL_FUSED;
-  const bool try_sweep = x->kernel != FWE_KERNEL_TWO_PASS && !x->lb && sweep_fits(x);
+  const bool try_sweep = x->kernel != FWE_KERNEL_TWO_PASS && !x->lb && sweep_fits(x) && x->cfg.pml != FWE_PML_CPML;
   // batch layouts: all wavefields per workgroup, or one
   std::vector<int> bts(1, x->cfg.nbatch);
   if (x->cfg.nbatch > 1) bts.push_back(1);
@@ -1631,6 +1719,8 @@ extern "C" fwe_status fwe_group_create(int32_t n, fwe_ctx* const* members, fwe_g
       return fail(FWE_ERR_ARG, "member %d differs in device, law, damping or free surface", q);
     if (c.nbatch != 1) return fail(FWE_ERR_UNSUPPORTED, "member %d has %d wavefields; groups take 1", q, c.nbatch);
     if (x->lb) return fail(FWE_ERR_UNSUPPORTED, "member %d has a learned boundary; groups run plain damping", q);
+    if (c.pml == FWE_PML_CPML)
+      return fail(FWE_ERR_UNSUPPORTED, "member %d uses the convolutional PML; groups run the profile forms", q);
     if (x->it != members[0]->it) return fail(FWE_ERR_STATE, "member %d is at another step", q);
     fwe_status st = ready(x);
     if (st) return st;
@@ -2135,6 +2225,8 @@ extern "C" fwe_status fwe_lb_eval(fwe_lb* lb, int32_t face, int32_t n, const flo
 
 static fwe_status lb_plain_ctx(const fwe_ctx* x, const char* what) {
   if (x->cfg.nbatch != 1) return fail(FWE_ERR_UNSUPPORTED, "%s: one wavefield only", what);
+  if (x->cfg.pml == FWE_PML_CPML)
+    return fail(FWE_ERR_UNSUPPORTED, "%s: training crops fields only; no convolutional PML", what);
   if (x->cfg.free_surface) return fail(FWE_ERR_UNSUPPORTED, "%s: the learned boundary replaces damping on all six faces; no free surface", what);
   return FWE_OK;
 }

@@ -141,6 +141,36 @@ void fwe_gen_taper(int32_t n, int32_t m_lo, int32_t m_hi, double rate, float* t)
   }
 }
 
+/* normalised depth of a node at position x into the layer of one axis end */
+static double cpml_depth(double x, int32_t n, int32_t layer, int32_t lo, int32_t hi) {
+  double q = 0.0;
+  if (lo) q = fmax(q, ((double)layer - 0.5 - x) / layer);
+  if (hi) q = fmax(q, (x - ((double)n - 0.5 - layer)) / layer);
+  return fmin(q, 1.0);
+}
+
+int32_t fwe_gen_cpml_table(int32_t n, int32_t layer, double h, double dt, double vmax, double R,
+                           double kappa_max, double alpha_max, double power, int32_t lo, int32_t hi,
+                           float* out) {
+  if (!out || n < 1 || layer < 1 || 2 * layer > n || !(h > 0) || !(dt > 0) || !(vmax > 0) || !(R > 0 && R < 1) ||
+      !(kappa_max >= 1) || !(alpha_max >= 0) || !(power >= 0))
+    return -1;
+  const int64_t m = (int64_t)n + 1;
+  const double d0 = -(power + 1.0) * vmax * log(R) / (2.0 * layer * h);
+  for (int32_t s = 0; s < 2; ++s)  /* 0: integer nodes x = i, 1: half nodes x = i - 1/2 */
+    for (int64_t i = 0; i < m; ++i) {
+      const double q = cpml_depth((double)i - 0.5 * s, n, layer, lo, hi);
+      const double qp = q > 0 ? pow(q, power) : 0.0;
+      const double d = d0 * qp, kappa = 1.0 + (kappa_max - 1.0) * qp, alpha = alpha_max * (1.0 - q);
+      const double b = exp(-(d / kappa + alpha) * dt);
+      const double a = d > 0 ? d * (b - 1.0) / (kappa * (d + kappa * alpha)) : 0.0;
+      out[(3 * s + 0) * m + i] = (float)b;
+      out[(3 * s + 1) * m + i] = (float)a;
+      out[(3 * s + 2) * m + i] = (float)(1.0 / kappa);
+    }
+  return 0;
+}
+
 void fwe_gen_weights3d(int32_t nx, int32_t ny, int32_t nz, int32_t mx_lo, int32_t mx_hi,
                        int32_t my_lo, int32_t my_hi, int32_t mz_lo, int32_t mz_hi, double rate,
                        float* w) {

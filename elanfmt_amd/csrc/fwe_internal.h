@@ -22,7 +22,9 @@ constexpr int SWEEP_X = 62;
 enum Mode { MODE_ISO = 0, MODE_ORTHO = 1, MODE_ORTHO_TAB = 2, MODE_ORTHO_ISO = 3, MODE_ISO_TAB = 4 };
 // the iso law's update formulas (OK4.cu) rather than the ortho law's
 __host__ __device__ constexpr bool iso_law(int mode) { return mode == MODE_ISO || mode == MODE_ISO_TAB; }
-enum Pml { PML_NONE = 0, PML_PROFILE = 1, PML_W3D = 2, PML_SEP = 3 };
+// PML_CPML: convolutional PML (memory variables in slabs along each face;
+// StepArgs::cL and below), the two-pass kernels only
+enum Pml { PML_NONE = 0, PML_PROFILE = 1, PML_W3D = 2, PML_SEP = 3, PML_CPML = 4 };
 
 // One injection or receiver, sorted by xy tile on the host.
 struct PointEntry {
@@ -66,6 +68,20 @@ struct StepArgs {
   const float* wx; const float* wy; const float* wz;
   const float* whx; const float* why; const float* whz;
   const float* w3;
+  // convolutional PML (PML_CPML): every derivative across a face's slab
+  // (the cL cells at either end of an axis) becomes g/kappa + psi with
+  // psi <- b psi + a g.  ctab[d]: 6 rows of n_d+1 floats (b, a, 1/kappa at
+  // integer nodes, then at half nodes) for axis d.  psi[d]: the memory
+  // variables of axis d's two slabs for wavefield 0, six variables of
+  // pvar[d] elements each (0..2 read by the stress pass, 3..5 by the
+  // velocity pass); slab element of cell (i,j,k): x: xi + 2cL (j + ny k),
+  // y: i + nx (yj + 2cL k), z: i + nx (j + ny zk), where xi = i (i < cL) or
+  // i - nx + 2cL (i >= nx - cL), likewise yj, zk.  Wavefield w at + w*pstride.
+  int32_t cL;
+  const float* ctab[3];
+  float* psi[3];
+  int64_t pvar[3];
+  int64_t pstride;
   float rdx, rdy, rdz;   // 1/dx ...
   float dtdx, dtdy, dtdz; // dt/dx ... (iso stress)
   // tiling

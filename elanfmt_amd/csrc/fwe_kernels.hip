@@ -57,6 +57,16 @@ __global__ __launch_bounds__(256) void k_combine(const float* __restrict__ tr, i
 // dispatch: law x damping form -> instantiation unit
 // ---------------------------------------------------------------------------
 hipError_t launch_stress(const StepArgs& a, int mode, int pml, int fs, hipStream_t s) {
+  if (pml == PML_CPML) {
+    switch (mode) {
+      case MODE_ISO: return stress_mp<MODE_ISO, PML_CPML>(a, fs, s);
+      case MODE_ISO_TAB: return stress_mp<MODE_ISO_TAB, PML_CPML>(a, fs, s);
+      case MODE_ORTHO: return stress_mp<MODE_ORTHO, PML_CPML>(a, fs, s);
+      case MODE_ORTHO_TAB: return stress_mp<MODE_ORTHO_TAB, PML_CPML>(a, fs, s);
+      case MODE_ORTHO_ISO: return stress_mp<MODE_ORTHO_ISO, PML_CPML>(a, fs, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
   if (mode == MODE_ISO_TAB) {
     if (pml == PML_PROFILE) return stress_mp<MODE_ISO_TAB, PML_PROFILE>(a, fs, s);
     if (pml == PML_NONE) return stress_mp<MODE_ISO_TAB, PML_NONE>(a, fs, s);
@@ -92,6 +102,16 @@ hipError_t launch_stress(const StepArgs& a, int mode, int pml, int fs, hipStream
 }
 
 hipError_t launch_velocity(const StepArgs& a, int mode, int pml, hipStream_t s) {
+  if (pml == PML_CPML) {
+    switch (mode) {
+      case MODE_ISO: return velocity_mp<MODE_ISO, PML_CPML>(a, s);
+      case MODE_ISO_TAB: return velocity_mp<MODE_ISO_TAB, PML_CPML>(a, s);
+      case MODE_ORTHO: return velocity_mp<MODE_ORTHO, PML_CPML>(a, s);
+      case MODE_ORTHO_TAB: return velocity_mp<MODE_ORTHO_TAB, PML_CPML>(a, s);
+      case MODE_ORTHO_ISO: return velocity_mp<MODE_ORTHO_ISO, PML_CPML>(a, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
   if (mode == MODE_ISO_TAB) {
     if (pml == PML_PROFILE) return velocity_mp<MODE_ISO_TAB, PML_PROFILE>(a, s);
     if (pml == PML_NONE) return velocity_mp<MODE_ISO_TAB, PML_NONE>(a, s);

@@ -11,8 +11,9 @@
 namespace fwe {
 template hipError_t stress_mp<FWE_MODE, FWE_PML>(const StepArgs&, int, hipStream_t);
 template hipError_t velocity_mp<FWE_MODE, FWE_PML>(const StepArgs&, hipStream_t);
-#if FWE_MODE != 2 && FWE_MODE != 4
-// groups read the medium arrays (no material table)
+#if FWE_MODE != 2 && FWE_MODE != 4 && FWE_PML != 4
+// groups read the medium arrays (no material table) and take no
+// convolutional PML
 template hipError_t group_mp<FWE_MODE, FWE_PML>(const StepArgs*, const int32_t*, int, int, int, int, int, bool,
                                                 hipStream_t);
 #endif

@@ -98,6 +98,34 @@ __device__ __forceinline__ float ldz(const float* p, bool use, int64_t idx) {
   return use ? v : 0.f;
 }
 
+// Convolutional PML: derivative g at a node of axis index q through the
+// memory variable *p; t holds one node kind's rows b, a, 1/kappa of m
+// entries each.  psi <- b psi + a g, returns g/kappa + psi.
+__device__ __forceinline__ float cpml_g(float g, float* p, const float* t, int m, int q) {
+  const float ps = fmaf(t[q], *p, t[m + q] * g);
+  *p = ps;
+  return fmaf(g, t[2 * m + q], ps);
+}
+
+// Slab membership and element offsets of cell (i,j,k) for the convolutional
+// PML (see StepArgs::psi).
+struct CpmlCell {
+  bool x, y, z;
+  int64_t ox, oy, oz;
+};
+__device__ __forceinline__ CpmlCell cpml_cell(const StepArgs& a, int i, int j, int k) {
+  const int L = a.cL, L2 = 2 * a.cL;
+  CpmlCell c;
+  c.x = i < L || i >= a.nx - L;
+  c.y = j < L || j >= a.ny - L;
+  c.z = k < L || k >= a.nz - L;
+  const int xi = i < L ? i : i - a.nx + L2, yj = j < L ? j : j - a.ny + L2, zk = k < L ? k : k - a.nz + L2;
+  c.ox = xi + (int64_t)L2 * (j + (int64_t)a.ny * k);
+  c.oy = i + (int64_t)a.nx * (yj + (int64_t)L2 * k);
+  c.oz = i + (int64_t)a.nx * (j + (int64_t)a.ny * zk);
+  return c;
+}
+
 // ---------------------------------------------------------------------------
 // Stress pass.  For cell (i,j,k) of the column tile:
 //   normal:  S_ii <- damp * S_ii + sum_j C_ij * dt * dV_j/dx_j
@@ -141,6 +169,8 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb, const int
     constexpr bool FIRST = decltype(first_tag)::value;
     const int64_t c = (int64_t)i + j * sy + k * sz;
     const bool km = k >= 1;
+    CpmlCell cc{};
+    if (PML == PML_CPML) cc = cpml_cell(a, i, j, k);
     float wzk = 1.f, fzh = 1.f;
     if (PML == PML_PROFILE) { wzk = a.wz[k]; fzh = a.whz[k]; }
     if (PML == PML_SEP) wzk = a.wz[k];
@@ -202,10 +232,46 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb, const int
       const float oxy = ld(Sxy + c), oxz = ld(Sxz + c), oyz = ld(Syz + c);
       float sxy = oxy, sxz = oxz, syz = oyz;
       cvx[b] = vx; cvy[b] = vy; cvz[b] = vz_kp;
-      const float ex = (vx_ip - vx) * a.rdx, ey = (vy_jp - vy) * a.rdy, ez = (vz_kp - vz) * a.rdz;
-      const float dxy = rate2(vy - vy_im, a.rdx, vx - vx_jm, a.rdy);
-      const float dxz = rate2(vz - vz_im, a.rdx, vx - vx_km, a.rdz);
-      const float dyz = rate2(vy - vy_km, a.rdz, vz - vz_jm, a.rdy);
+      float ex, ey, ez, dxy, dxz, dyz;
+      if (PML == PML_CPML) {
+        // the nine strain-rate derivatives, each through its memory
+        // variable where the cell lies in that axis' slab
+        float gxx = (vx_ip - vx) * a.rdx, gyy = (vy_jp - vy) * a.rdy, gzz = (vz_kp - vz) * a.rdz;
+        float gyx = (vy - vy_im) * a.rdx, gxy = (vx - vx_jm) * a.rdy;  // Sxy node
+        float gzx = (vz - vz_im) * a.rdx, gxz = (vx - vx_km) * a.rdz;  // Sxz node
+        float gyz = (vy - vy_km) * a.rdz, gzy = (vz - vz_jm) * a.rdy;  // Syz node
+        if (cc.x || cc.y || cc.z) {
+          const int64_t po = (int64_t)(wb + b) * a.pstride;
+          if (cc.x) {
+            float* p = a.psi[0] + po + cc.ox;
+            const int m = a.nx + 1;
+            gxx = cpml_g(gxx, p, a.ctab[0], m, i);
+            gyx = cpml_g(gyx, p + a.pvar[0], a.ctab[0] + 3 * m, m, i);
+            gzx = cpml_g(gzx, p + 2 * a.pvar[0], a.ctab[0] + 3 * m, m, i);
+          }
+          if (cc.y) {
+            float* p = a.psi[1] + po + cc.oy;
+            const int m = a.ny + 1;
+            gyy = cpml_g(gyy, p, a.ctab[1], m, j);
+            gxy = cpml_g(gxy, p + a.pvar[1], a.ctab[1] + 3 * m, m, j);
+            gzy = cpml_g(gzy, p + 2 * a.pvar[1], a.ctab[1] + 3 * m, m, j);
+          }
+          if (cc.z) {
+            float* p = a.psi[2] + po + cc.oz;
+            const int m = a.nz + 1;
+            gzz = cpml_g(gzz, p, a.ctab[2], m, k);
+            gxz = cpml_g(gxz, p + a.pvar[2], a.ctab[2] + 3 * m, m, k);
+            gyz = cpml_g(gyz, p + 2 * a.pvar[2], a.ctab[2] + 3 * m, m, k);
+          }
+        }
+        ex = gxx; ey = gyy; ez = gzz;
+        dxy = gyx + gxy; dxz = gzx + gxz; dyz = gyz + gzy;
+      } else {
+        ex = (vx_ip - vx) * a.rdx; ey = (vy_jp - vy) * a.rdy; ez = (vz_kp - vz) * a.rdz;
+        dxy = rate2(vy - vy_im, a.rdx, vx - vx_jm, a.rdy);
+        dxz = rate2(vz - vz_im, a.rdx, vx - vx_km, a.rdz);
+        dyz = rate2(vy - vy_km, a.rdz, vz - vz_jm, a.rdy);
+      }
       if (iso_law(MODE)) {
         // OK4.cu:307-309,323-329 with c11,c12 arrays holding c*dt
         sxx = iso_normal(fxh, sxx, m0, ex, m1, ey, m1, ez);
@@ -316,6 +382,8 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb, const i
   auto plane = [&](const int k, auto first_tag) __attribute__((always_inline)) {
     constexpr bool FIRST = decltype(first_tag)::value;
     const int64_t c = (int64_t)i + j * sy + k * sz;
+    CpmlCell cc{};
+    if (PML == PML_CPML) cc = cpml_cell(a, i, j, k);
     const bool ux = ux_ij && k >= 1 && k <= nz - 2;
     const bool uy = uy_ij && k >= 1 && k <= nz - 2;
     const bool uz = uz_ij && k >= 1;
@@ -363,9 +431,45 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb, const i
       // every load above is issued
       if (NB == 1) __builtin_amdgcn_sched_barrier(0);
       if (PML == PML_SEP && b == 0) w = txi * tyj * wzk;
-      const float dSx = rate3(sxx - sxx_im, a.rdx, sxy_jp - sxy, a.rdy, sxz_kp - sxz, a.rdz);
-      const float dSy = rate3(syy - syy_jm, a.rdy, sxy_ip - sxy, a.rdx, syz_kp - syz, a.rdz);
-      const float dSz = rate3(szz - szz_km, a.rdz, sxz_ip - sxz, a.rdx, syz_jp - syz, a.rdy);
+      float dSx, dSy, dSz;
+      if (PML == PML_CPML) {
+        // the nine stress derivatives, each through its memory variable
+        // where the cell lies in that axis' slab
+        float hxx = (sxx - sxx_im) * a.rdx, hxyy = (sxy_jp - sxy) * a.rdy, hxzz = (sxz_kp - sxz) * a.rdz;
+        float hyy = (syy - syy_jm) * a.rdy, hxyx = (sxy_ip - sxy) * a.rdx, hyzz = (syz_kp - syz) * a.rdz;
+        float hzz = (szz - szz_km) * a.rdz, hxzx = (sxz_ip - sxz) * a.rdx, hyzy = (syz_jp - syz) * a.rdy;
+        if (cc.x || cc.y || cc.z) {
+          const int64_t po = (int64_t)(wb + b) * a.pstride;
+          if (cc.x) {
+            float* p = a.psi[0] + po + cc.ox + 3 * a.pvar[0];
+            const int m = nx + 1;
+            hxx = cpml_g(hxx, p, a.ctab[0] + 3 * m, m, i);
+            hxyx = cpml_g(hxyx, p + a.pvar[0], a.ctab[0], m, i);
+            hxzx = cpml_g(hxzx, p + 2 * a.pvar[0], a.ctab[0], m, i);
+          }
+          if (cc.y) {
+            float* p = a.psi[1] + po + cc.oy + 3 * a.pvar[1];
+            const int m = ny + 1;
+            hyy = cpml_g(hyy, p, a.ctab[1] + 3 * m, m, j);
+            hxyy = cpml_g(hxyy, p + a.pvar[1], a.ctab[1], m, j);
+            hyzy = cpml_g(hyzy, p + 2 * a.pvar[1], a.ctab[1], m, j);
+          }
+          if (cc.z) {
+            float* p = a.psi[2] + po + cc.oz + 3 * a.pvar[2];
+            const int m = nz + 1;
+            hzz = cpml_g(hzz, p, a.ctab[2] + 3 * m, m, k);
+            hxzz = cpml_g(hxzz, p + a.pvar[2], a.ctab[2], m, k);
+            hyzz = cpml_g(hyzz, p + 2 * a.pvar[2], a.ctab[2], m, k);
+          }
+        }
+        dSx = hxx + hxyy + hxzz;
+        dSy = hyy + hxyx + hyzz;
+        dSz = hzz + hxzx + hyzy;
+      } else {
+        dSx = rate3(sxx - sxx_im, a.rdx, sxy_jp - sxy, a.rdy, sxz_kp - sxz, a.rdz);
+        dSy = rate3(syy - syy_jm, a.rdy, sxy_ip - sxy, a.rdx, syz_kp - syz, a.rdz);
+        dSz = rate3(szz - szz_km, a.rdz, sxz_ip - sxz, a.rdx, syz_jp - syz, a.rdy);
+      }
       if (ux) vx = iso_law(MODE) ? iso_vel(wxm, vx, bx, dSx) : ortho_vel(w, vx, bx, dSx);
       if (uy) vy = iso_law(MODE) ? iso_vel(wym, vy, by, dSy) : ortho_vel(w, vy, by, dSy);
       if (uz) vz = iso_law(MODE) ? iso_vel(wzm, vz, bz, dSz) : ortho_vel(w, vz, bz, dSz);

@@ -22,7 +22,7 @@ ORDER_LINEAR, ORDER_XCD, ORDER_PLANE_SPLIT = 0, 1, 2
 MEDIUM_AUTO, MEDIUM_ARRAYS, MEDIUM_TABLE, MEDIUM_ISOTROPIC = 0, 1, 2, 3
 FS_NONE, FS_BC1, FS_VACUUM = 0, 1, 2
 MODE_ISO, MODE_ORTHO = 0, 1
-PML_NONE, PML_PROFILE, PML_WEIGHTS3D, PML_SEPARABLE = 0, 1, 2, 3
+PML_NONE, PML_PROFILE, PML_WEIGHTS3D, PML_SEPARABLE, PML_CPML = 0, 1, 2, 3, 4
 SXX, SYY, SZZ, SXY, SXZ, SYZ = range(6)
 F_VX, F_VY, F_VZ, F_SXX, F_SYY, F_SZZ, F_SXY, F_SXZ, F_SYZ = range(9)
 REC_VX, REC_VY, REC_VZ = range(3)
@@ -84,6 +84,10 @@ SIGNATURES = {
     "fwe_set_material_ortho": (C.c_int, [C.c_void_p, C.POINTER(_FP), C.POINTER(_FP)]),
     "fwe_set_pml_profiles": (C.c_int, [C.c_void_p, _FP, _FP, _FP]),
     "fwe_set_pml_weights": (C.c_int, [C.c_void_p, _FP]),
+    "fwe_set_cpml": (C.c_int, [C.c_void_p, C.c_int32, _FP, _FP, _FP]),
+    "fwe_cpml_state_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "fwe_get_cpml_state": (C.c_int, [C.c_void_p, C.c_int32, _FP]),
+    "fwe_set_cpml_state": (C.c_int, [C.c_void_p, C.c_int32, _FP]),
     "fwe_set_stf": (C.c_int, [C.c_void_p, C.c_int32, _FP, C.c_int32]),
     "fwe_set_injections": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Injection)]),
     "fwe_set_receivers": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Receiver)]),
@@ -152,6 +156,7 @@ SIGNATURES = {
                                    C.c_int32, _FP]),
     "fwe_gen_cell_profile": (None, [C.c_int32, _FP, _FP]),
     "fwe_gen_taper": (None, [C.c_int32, C.c_int32, C.c_int32, C.c_double, _FP]),
+    "fwe_gen_cpml_table": (C.c_int32, [C.c_int32, C.c_int32] + [C.c_double] * 7 + [C.c_int32, C.c_int32, _FP]),
     "fwe_gen_weights3d": (None, [C.c_int32] * 9 + [C.c_double, _FP]),
     "fwe_gen_stagger_buoyancy": (None, [C.c_int32] * 3 + [_FP] * 4),
     "fwe_gen_harmonic_edges": (None, [C.c_int32] * 3 + [_FP] * 6),
@@ -304,6 +309,31 @@ class Solver:
     def set_pml_weights(self, w):
         a = f32(w)
         _check(lib().fwe_set_pml_weights(self._h, fptr(a)))
+
+    def set_cpml(self, layer, tx, ty, tz):
+        """Convolutional PML of `layer` cells per face; per axis the 6 x (n+1)
+        table of workloads.cpml_table (fwe_gen_cpml_table)."""
+        t = [np.ascontiguousarray(np.asarray(a, np.float32).ravel()) for a in (tx, ty, tz)]
+        for a, n in zip(t, (self.cfg.nx, self.cfg.ny, self.cfg.nz)):
+            if a.size != 6 * (n + 1):
+                raise ValueError("a CPML table holds 6 x (n+1) floats")
+        _check(lib().fwe_set_cpml(self._h, layer, *[fptr(a) for a in t]))
+
+    def cpml_state(self, batch=0):
+        """Memory variables of one wavefield (flat, floats)."""
+        n = C.c_int64(0)
+        _check(lib().fwe_cpml_state_size(self._h, C.byref(n)))
+        out = np.zeros(n.value, np.float32)
+        _check(lib().fwe_get_cpml_state(self._h, batch, fptr(out)))
+        return out
+
+    def set_cpml_state(self, state, batch=0):
+        a = np.ascontiguousarray(np.asarray(state, np.float32).ravel())
+        n = C.c_int64(0)
+        _check(lib().fwe_cpml_state_size(self._h, C.byref(n)))
+        if a.size != n.value:
+            raise ValueError(f"CPML state holds {n.value} floats")
+        _check(lib().fwe_set_cpml_state(self._h, batch, fptr(a)))
 
     # source and receivers
     def set_stf(self, stf, batch=0):

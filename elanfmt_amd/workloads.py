@@ -48,6 +48,9 @@ class Workload:
     # damping
     prof: tuple | None = None
     w3: np.ndarray | None = None
+    # convolutional PML: layer cells per face and the three axis tables
+    cpml_layer: int = 0
+    cpml: tuple | None = None
     # source and receivers
     stf: list = field(default_factory=list)  # one per batch
     injections: list = field(default_factory=list)
@@ -101,6 +104,42 @@ def taper(n, m_lo, m_hi, rate):
     t = _fa(n)
     fwe.lib().fwe_gen_taper(n, m_lo, m_hi, rate, fwe.fptr(t))
     return t
+
+
+def cpml_table(n, L, h, dt, vmax, R=1e-3, kappa_max=1.0, alpha_max=0.0, power=2.0, lo=1, hi=1):
+    """Convolutional PML coefficients of one axis (fwe_gen_cpml_table):
+    6 x (n+1) floats, b, a, 1/kappa at integer then half nodes."""
+    t = _fa(6 * (n + 1))
+    if fwe.lib().fwe_gen_cpml_table(n, L, h, dt, vmax, R, kappa_max, alpha_max, power, lo, hi, fwe.fptr(t)) != 0:
+        raise ValueError("bad CPML parameters")
+    return t.reshape(6, n + 1)
+
+
+def medium_vmax(wl: Workload):
+    """Fastest wave speed bound of a workload's medium: sqrt(max modulus x
+    max buoyancy) (moduli and buoyancy are the physical values)."""
+    if wl.mode == fwe.MODE_ISO:
+        return float(np.sqrt(np.max(wl.c11) * max(wl.bscal)))
+    return float(np.sqrt(max(float(np.max(c)) for c in wl.cij[:3]) * max(float(np.max(b)) for b in wl.buoy)))
+
+
+def use_cpml(wl: Workload, layer, R=1e-4, kappa_max=1.0, alpha_max=None, power=2.0, vmax=None):
+    """Switch a workload's damping to a convolutional PML of `layer` cells on
+    every face (the top end of z undamped under a free surface).  alpha_max
+    defaults to pi f0 of the workload's wavelet (0 when unknown)."""
+    if alpha_max is None:
+        alpha_max = np.pi * float(wl.info.get("f0", 0.0))
+    if vmax is None:
+        vmax = wl.info.get("vmax") or medium_vmax(wl)
+    top = 0 if wl.free_surface else 1
+    wl.cpml = (cpml_table(wl.nx, layer, wl.dx, wl.dt, vmax, R, kappa_max, alpha_max, power),
+               cpml_table(wl.ny, layer, wl.dy, wl.dt, vmax, R, kappa_max, alpha_max, power),
+               cpml_table(wl.nz, layer, wl.dz, wl.dt, vmax, R, kappa_max, alpha_max, power, lo=top))
+    wl.cpml_layer = layer
+    wl.pml = fwe.PML_CPML
+    wl.prof = None
+    wl.w3 = None
+    return wl
 
 
 def weights3d(nx, ny, nz, mx, my, mz_lo, mz_hi, rate):
@@ -166,6 +205,8 @@ def apply(solver, wl: Workload):
         solver.set_pml_profiles(*wl.prof)
     elif wl.pml == fwe.PML_WEIGHTS3D:
         solver.set_pml_weights(wl.w3)
+    elif wl.pml == fwe.PML_CPML:
+        solver.set_cpml(wl.cpml_layer, *wl.cpml)
     for b, s in enumerate(wl.stf):
         solver.set_stf(s, b)
     solver.set_injections(wl.injections)

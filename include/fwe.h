@@ -66,12 +66,17 @@ typedef enum { FWE_MODE_ISO = 0, FWE_MODE_ORTHO = 1 } fwe_mode;
  * weights.dat multiplying the old value (ANISO.cu:306-352,374-386; ortho
  * only).  SEPARABLE = the same ANISO law with w(i,j,k)=tx[i]*ty[j]*tz[k]
  * rebuilt in the kernel from three 1-D cell-centred profiles of length n
- * (ortho only; the ANISO taper of ANIS.m:466-512 is separable). */
+ * (ortho only; the ANISO taper of ANIS.m:466-512 is separable).
+ * CPML = a convolutional PML in a thin layer on every face (either law; no
+ * reference counterpart, see fwe_set_cpml): it replaces the multiplicative
+ * damping of ANISO.cu:306-352,374-386 / OK4.cu:307-329,361-375 and absorbs
+ * far better per cell; the two-pass kernels only. */
 typedef enum {
   FWE_PML_NONE = 0,
   FWE_PML_PROFILE = 1,
   FWE_PML_WEIGHTS3D = 2,
-  FWE_PML_SEPARABLE = 3
+  FWE_PML_SEPARABLE = 3,
+  FWE_PML_CPML = 4
 } fwe_pml;
 
 /* Stress components (injection targets). */
@@ -185,6 +190,23 @@ fwe_status fwe_set_pml_profiles(fwe_ctx* ctx, const float* wx, const float* wy,
                                 const float* wz);
 /* WEIGHTS3D: weights.dat, nx*ny*nz (ANISO.cu:456-457). */
 fwe_status fwe_set_pml_weights(fwe_ctx* ctx, const float* w);
+/* CPML: `layer` cells at both ends of every axis (1 <= layer, 2 layer <= n)
+ * and per axis the coefficient table of fwe_gen_cpml_table (6 rows of n+1
+ * floats: b, a, 1/kappa at integer nodes, then at half nodes).  Every
+ * derivative across a slab becomes g/kappa + psi with psi <- b psi + a g:
+ * in the stress pass dVx/dx, dVy/dx, dVz/dx (x slabs; likewise y, z), in the
+ * velocity pass dSxx/dx, dSxy/dx, dSxz/dx (likewise).  Resets the memory
+ * variables to zero.  A face without damping has zero entries (its table's
+ * end disabled in the generator); with BC1 the top end of z is normally
+ * disabled.  No reference counterpart (the reference damps with the
+ * profiles above); replaces them for thin layers. */
+fwe_status fwe_set_cpml(fwe_ctx* ctx, int32_t layer, const float* tx, const float* ty,
+                        const float* tz);
+/* Memory variables of one wavefield (floats per wavefield in *n): saved and
+ * restored with the fields for an exact restart of a CPML run. */
+fwe_status fwe_cpml_state_size(const fwe_ctx* ctx, int64_t* n);
+fwe_status fwe_get_cpml_state(fwe_ctx* ctx, int32_t batch, float* out);
+fwe_status fwe_set_cpml_state(fwe_ctx* ctx, int32_t batch, const float* in);
 
 /* ---- source and receivers ------------------------------------------------ */
 /* Source time function of wavefield `batch` (Src.dat, OK4.cu:454); steps at

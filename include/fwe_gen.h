@@ -58,6 +58,29 @@ void fwe_gen_cell_profile(int32_t n, const float* w, float* t);
 /* ANISO taper factor of one axis (ANIS.m:466-506): depth p into a margin of
  * width m_lo (low end) / m_hi (high end); t[i] = exp(-(rate*p)^2). */
 void fwe_gen_taper(int32_t n, int32_t m_lo, int32_t m_hi, double rate, float* t);
+
+/* Coefficients of a convolutional PML (no reference counterpart: the
+ * reference damps with the multiplicative profiles above).  Recursive
+ * convolution of Roden & Gedney (2000) with the complex-frequency-shifted
+ * stretching s = kappa + d/(alpha + i w) of Kuzuoglu & Mittra (1996), in the
+ * form of Komatitsch & Martin (2007): every derivative g across the layer is
+ * replaced by g/kappa + psi with psi <- b psi + a g.  On one axis of n
+ * cells with `layer` cells at the low (lo != 0) and/or high (hi != 0) end,
+ * a node at normalised depth q in [0, 1] gets
+ *   d = d0 q^power, d0 = -(power+1) vmax ln(R) / (2 layer h),
+ *   kappa = 1 + (kappa_max - 1) q^power, alpha = alpha_max (1 - q),
+ *   b = exp(-(d/kappa + alpha) dt), a = d (b - 1) / (kappa (d + kappa alpha)).
+ * Integer nodes (index i at x = i: normal stresses, and the velocities
+ * across the axis) and half nodes (x = i - 1/2: the velocity along the
+ * axis, the shear stresses with that axis) get q = (layer - 1/2 - x)/layer
+ * at the low end and (x - (n - 1/2 - layer))/layer at the high end, clipped
+ * to [0, 1] (0 on a disabled end).  out holds 6 rows of n+1 entries:
+ * b, a, 1/kappa at the integer nodes, then the same at the half nodes.
+ * Returns 0, or -1 for bad arguments (layer < 1, 2 layer > n, R not in
+ * (0,1), kappa_max < 1, alpha_max < 0, power < 0). */
+int32_t fwe_gen_cpml_table(int32_t n, int32_t layer, double h, double dt, double vmax, double R,
+                           double kappa_max, double alpha_max, double power, int32_t lo, int32_t hi,
+                           float* out);
 /* ANISO 3-D weights.dat: w = exp(-rate^2 (p^2 + q^2 + r^2)) computed in
  * double and rounded once (ANIS.m:466-512). */
 void fwe_gen_weights3d(int32_t nx, int32_t ny, int32_t nz, int32_t mx_lo, int32_t mx_hi,

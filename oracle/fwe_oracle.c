@@ -47,6 +47,13 @@ struct oracle {
   int nrec;
   int32_t *rec_comp, *rec_ijk;
   double* traces;
+  /* convolutional PML (pml = 4): layer cells per face, coefficient tables
+   * per axis (6 rows of n+1: b, a, 1/kappa at integer then half nodes) and
+   * 18 memory variables on the cell grid: axis d owns psi[6d .. 6d+5]
+   * (0..2 stress pass, 3..5 velocity pass) */
+  int cL;
+  float* ct[3];
+  REAL* psi[18];
 };
 
 /* Reference flat indices (OK4.cu:290-292,323-329,351-353). */
@@ -89,6 +96,8 @@ void oracle_destroy(oracle_t* o) {
   free(o->wx); free(o->wy); free(o->wz); free(o->w3);
   free(o->stf); free(o->inj_comp); free(o->inj_ijk); free(o->inj_coef);
   free(o->rec_comp); free(o->rec_ijk); free(o->traces);
+  for (int d = 0; d < 3; ++d) free(o->ct[d]);
+  for (int v = 0; v < 18; ++v) free(o->psi[v]);
   free(o);
 }
 
@@ -140,6 +149,44 @@ int oracle_set_weights3d(oracle_t* o, const float* w) {
   return 0;
 }
 
+/* Convolutional PML of the HIP library (its own addition: the reference
+ * damps with multiplicative profiles only).  The tables are the float
+ * coefficients the device receives, so both sides use the same numbers. */
+int oracle_set_cpml(oracle_t* o, int L, const float* tx, const float* ty, const float* tz) {
+  const int n[3] = {o->nx, o->ny, o->nz};
+  const float* t[3] = {tx, ty, tz};
+  if (o->pml != 4 || L < 1) return -1;
+  for (int d = 0; d < 3; ++d)
+    if (2 * L > n[d] || !t[d]) return -1;
+  o->cL = L;
+  for (int d = 0; d < 3; ++d) {
+    free(o->ct[d]);
+    o->ct[d] = (float*)malloc(sizeof(float) * 6 * (size_t)(n[d] + 1));
+    if (!o->ct[d]) return -1;
+    memcpy(o->ct[d], t[d], sizeof(float) * 6 * (size_t)(n[d] + 1));
+  }
+  for (int v = 0; v < 18; ++v) {
+    free(o->psi[v]);
+    o->psi[v] = (REAL*)zalloc((int64_t)o->nx * o->ny * o->nz);
+    if (!o->psi[v]) return -1;
+  }
+  return 0;
+}
+
+/* Derivative g along axis d at a node with axis index q (h = 0 integer
+ * node, 1 half node) of cell c, through memory variable v of that axis:
+ * inside the axis' slab (the cL cells at either end) psi <- b psi + a g and
+ * the result is g/kappa + psi; elsewhere g. */
+static inline REAL cpml(oracle_t* o, int d, int v, int h, int q, int64_t c, REAL g) {
+  const int n = d == 0 ? o->nx : (d == 1 ? o->ny : o->nz);
+  if (o->pml != 4 || (q >= o->cL && q < n - o->cL)) return g;
+  const int64_t m = (int64_t)n + 1;
+  const float* t = o->ct[d] + 3 * h * m;
+  REAL* p = o->psi[6 * d + v] + c;
+  *p = (REAL)t[q] * *p + (REAL)t[m + q] * g;
+  return g * (REAL)t[2 * m + q] + *p;
+}
+
 int oracle_set_stf(oracle_t* o, const float* stf, int n) {
   free(o->stf);
   o->stf = copy_in(stf, n);
@@ -176,6 +223,8 @@ int oracle_set_receivers(oracle_t* o, int n, const int32_t* comp, const int32_t*
 void oracle_reset(oracle_t* o) {
   for (int f = 0; f < F_N; ++f)
     memset(o->f[f], 0, sizeof(REAL) * (size_t)oracle_field_size(o, f));
+  for (int v = 0; v < 18; ++v)
+    if (o->psi[v]) memset(o->psi[v], 0, sizeof(REAL) * (size_t)o->nx * o->ny * o->nz);
   if (o->traces) memset(o->traces, 0, sizeof(double) * (size_t)o->nrec * o->nt);
   o->it = 0;
 }
@@ -219,9 +268,9 @@ static void stress_iso(oracle_t* o) {
       for (int i = 0; i < nx; ++i) {
         const int64_t c = IC(i, j, k);
         const REAL c66 = shear_modulus(o, o->c66, k);
-        const REAL exx = (Vx[IVX(i + 1, j, k)] - Vx[IVX(i, j, k)]) * rdx;
-        const REAL eyy = (Vy[IVY(i, j + 1, k)] - Vy[IVY(i, j, k)]) * rdy;
-        const REAL ezz = (Vz[IVZ(i, j, k + 1)] - Vz[IVZ(i, j, k)]) * rdz;
+        const REAL exx = cpml(o, 0, 0, 0, i, c, (Vx[IVX(i + 1, j, k)] - Vx[IVX(i, j, k)]) * rdx);
+        const REAL eyy = cpml(o, 1, 0, 0, j, c, (Vy[IVY(i, j + 1, k)] - Vy[IVY(i, j, k)]) * rdy);
+        const REAL ezz = cpml(o, 2, 0, 0, k, c, (Vz[IVZ(i, j, k + 1)] - Vz[IVZ(i, j, k)]) * rdz);
         const REAL a = o->c11[c], b = o->c12[c];
         const REAL fx = prof ? (REAL)0.5 * (wx[i + 1] + wx[i]) : (REAL)1;
         const REAL fy = prof ? (REAL)0.5 * (wy[j + 1] + wy[j]) : (REAL)1;
@@ -232,20 +281,20 @@ static void stress_iso(oracle_t* o) {
         if (i >= 1 && j >= 1) {
           const int64_t e = IXY(i, j, k);
           const REAL w = prof ? wz[k] * wy[j] * wx[i] : (REAL)1;
-          sxy[e] = w * (sxy[e] + c66 * dt * ((Vy[IVY(i, j, k)] - Vy[IVY(i - 1, j, k)]) * rdx +
-                                             (Vx[IVX(i, j, k)] - Vx[IVX(i, j - 1, k)]) * rdy));
+          sxy[e] = w * (sxy[e] + c66 * dt * (cpml(o, 0, 1, 1, i, c, (Vy[IVY(i, j, k)] - Vy[IVY(i - 1, j, k)]) * rdx) +
+                                             cpml(o, 1, 1, 1, j, c, (Vx[IVX(i, j, k)] - Vx[IVX(i, j - 1, k)]) * rdy)));
         }
         if (i >= 1 && k >= 1) {
           const int64_t e = IXZ(i, j, k);
           const REAL w = prof ? wy[j] * wz[k] * wx[i] : (REAL)1;
-          sxz[e] = w * (sxz[e] + c66 * dt * ((Vz[IVZ(i, j, k)] - Vz[IVZ(i - 1, j, k)]) * rdx +
-                                             (Vx[IVX(i, j, k)] - Vx[IVX(i, j, k - 1)]) * rdz));
+          sxz[e] = w * (sxz[e] + c66 * dt * (cpml(o, 0, 2, 1, i, c, (Vz[IVZ(i, j, k)] - Vz[IVZ(i - 1, j, k)]) * rdx) +
+                                             cpml(o, 2, 1, 1, k, c, (Vx[IVX(i, j, k)] - Vx[IVX(i, j, k - 1)]) * rdz)));
         }
         if (j >= 1 && k >= 1) {
           const int64_t e = IYZ(i, j, k);
           const REAL w = prof ? wx[i] * wy[j] * wz[k] : (REAL)1;
-          syz[e] = w * (syz[e] + c66 * dt * ((Vy[IVY(i, j, k)] - Vy[IVY(i, j, k - 1)]) * rdz +
-                                             (Vz[IVZ(i, j, k)] - Vz[IVZ(i, j - 1, k)]) * rdy));
+          syz[e] = w * (syz[e] + c66 * dt * (cpml(o, 2, 2, 1, k, c, (Vy[IVY(i, j, k)] - Vy[IVY(i, j, k - 1)]) * rdz) +
+                                             cpml(o, 1, 2, 1, j, c, (Vz[IVZ(i, j, k)] - Vz[IVZ(i, j - 1, k)]) * rdy)));
         }
       }
 }
@@ -266,28 +315,28 @@ static void stress_ortho(oracle_t* o) {
       for (int i = 0; i < nx; ++i) {
         const int64_t c = IC(i, j, k);
         const REAL w = w3(o, i, j, k);
-        const REAL exx = (Vx[IVX(i + 1, j, k)] - Vx[IVX(i, j, k)]) * rdx;
-        const REAL eyy = (Vy[IVY(i, j + 1, k)] - Vy[IVY(i, j, k)]) * rdy;
-        const REAL ezz = (Vz[IVZ(i, j, k + 1)] - Vz[IVZ(i, j, k)]) * rdz;
+        const REAL exx = cpml(o, 0, 0, 0, i, c, (Vx[IVX(i + 1, j, k)] - Vx[IVX(i, j, k)]) * rdx);
+        const REAL eyy = cpml(o, 1, 0, 0, j, c, (Vy[IVY(i, j + 1, k)] - Vy[IVY(i, j, k)]) * rdy);
+        const REAL ezz = cpml(o, 2, 0, 0, k, c, (Vz[IVZ(i, j, k + 1)] - Vz[IVZ(i, j, k)]) * rdz);
         sxx[c] = sxx[c] * w + dt * (C11[c] * exx + C12[c] * eyy + C13[c] * ezz);
         syy[c] = syy[c] * w + dt * (C12[c] * exx + C22[c] * eyy + C23[c] * ezz);
         szz[c] = szz[c] * w + dt * (C13[c] * exx + C23[c] * eyy + C33[c] * ezz);
         if (i >= 1 && j >= 1) { /* ANISO.cu:327-334 */
           const int64_t e = IXY(i, j, k);
-          const REAL dVx_dy = (Vx[IVX(i, j, k)] - Vx[IVX(i, j - 1, k)]) / dy;
-          const REAL dVy_dx = (Vy[IVY(i, j, k)] - Vy[IVY(i - 1, j, k)]) / dx;
+          const REAL dVx_dy = cpml(o, 1, 1, 1, j, c, (Vx[IVX(i, j, k)] - Vx[IVX(i, j - 1, k)]) / dy);
+          const REAL dVy_dx = cpml(o, 0, 1, 1, i, c, (Vy[IVY(i, j, k)] - Vy[IVY(i - 1, j, k)]) / dx);
           sxy[e] = sxy[e] * w + dt * shear_modulus(o, C66[e], k) * (dVy_dx + dVx_dy);
         }
         if (i >= 1 && k >= 1) { /* ANISO.cu:336-343 */
           const int64_t e = IXZ(i, j, k);
-          const REAL dVx_dz = (Vx[IVX(i, j, k)] - Vx[IVX(i, j, k - 1)]) / dz;
-          const REAL dVz_dx = (Vz[IVZ(i, j, k)] - Vz[IVZ(i - 1, j, k)]) / dx;
+          const REAL dVx_dz = cpml(o, 2, 1, 1, k, c, (Vx[IVX(i, j, k)] - Vx[IVX(i, j, k - 1)]) / dz);
+          const REAL dVz_dx = cpml(o, 0, 2, 1, i, c, (Vz[IVZ(i, j, k)] - Vz[IVZ(i - 1, j, k)]) / dx);
           sxz[e] = sxz[e] * w + dt * shear_modulus(o, C55[e], k) * (dVz_dx + dVx_dz);
         }
         if (j >= 1 && k >= 1) { /* ANISO.cu:345-352 */
           const int64_t e = IYZ(i, j, k);
-          const REAL dVy_dz = (Vy[IVY(i, j, k)] - Vy[IVY(i, j, k - 1)]) / dz;
-          const REAL dVz_dy = (Vz[IVZ(i, j, k)] - Vz[IVZ(i, j - 1, k)]) / dy;
+          const REAL dVy_dz = cpml(o, 2, 2, 1, k, c, (Vy[IVY(i, j, k)] - Vy[IVY(i, j, k - 1)]) / dz);
+          const REAL dVz_dy = cpml(o, 1, 2, 1, j, c, (Vz[IVZ(i, j, k)] - Vz[IVZ(i, j - 1, k)]) / dy);
           syz[e] = syz[e] * w + dt * shear_modulus(o, C44[e], k) * (dVy_dz + dVz_dy);
         }
       }
@@ -345,11 +394,12 @@ static void velocity(oracle_t* o) {
   for (int k = 1; k < nz; ++k)
     for (int j = 1; j < ny; ++j)
       for (int i = 1; i < nx; ++i) {
+        const int64_t c = IC(i, j, k);
         if (k <= nz - 2 && j <= ny - 2) {
           const int64_t e = IVX(i, j, k);
-          const REAL dS = (sxx[IC(i, j, k)] - sxx[IC(i - 1, j, k)]) * rdx +
-                          (sxy[IXY(i, j + 1, k)] - sxy[IXY(i, j, k)]) * rdy +
-                          (sxz[IXZ(i, j, k + 1)] - sxz[IXZ(i, j, k)]) * rdz;
+          const REAL dS = cpml(o, 0, 3, 1, i, c, (sxx[IC(i, j, k)] - sxx[IC(i - 1, j, k)]) * rdx) +
+                          cpml(o, 1, 4, 0, j, c, (sxy[IXY(i, j + 1, k)] - sxy[IXY(i, j, k)]) * rdy) +
+                          cpml(o, 2, 4, 0, k, c, (sxz[IXZ(i, j, k + 1)] - sxz[IXZ(i, j, k)]) * rdz);
           if (iso) {
             const REAL w = prof ? o->wx[i - 1] : (REAL)1;
             Vx[e] = w * (Vx[e] * rdt + o->bx * dS) * dt;
@@ -359,9 +409,9 @@ static void velocity(oracle_t* o) {
         }
         if (k <= nz - 2 && i <= nx - 2) {
           const int64_t e = IVY(i, j, k);
-          const REAL dS = (syy[IC(i, j, k)] - syy[IC(i, j - 1, k)]) * rdy +
-                          (sxy[IXY(i + 1, j, k)] - sxy[IXY(i, j, k)]) * rdx +
-                          (syz[IYZ(i, j, k + 1)] - syz[IYZ(i, j, k)]) * rdz;
+          const REAL dS = cpml(o, 1, 3, 1, j, c, (syy[IC(i, j, k)] - syy[IC(i, j - 1, k)]) * rdy) +
+                          cpml(o, 0, 4, 0, i, c, (sxy[IXY(i + 1, j, k)] - sxy[IXY(i, j, k)]) * rdx) +
+                          cpml(o, 2, 5, 0, k, c, (syz[IYZ(i, j, k + 1)] - syz[IYZ(i, j, k)]) * rdz);
           if (iso) {
             const REAL w = prof ? o->wy[j - 1] : (REAL)1;
             Vy[e] = w * (Vy[e] * rdt + o->by * dS) * dt;
@@ -371,9 +421,9 @@ static void velocity(oracle_t* o) {
         }
         if (j <= ny - 2 && i <= nx - 2) {
           const int64_t e = IVZ(i, j, k);
-          const REAL dS = (szz[IC(i, j, k)] - szz[IC(i, j, k - 1)]) * rdz +
-                          (sxz[IXZ(i + 1, j, k)] - sxz[IXZ(i, j, k)]) * rdx +
-                          (syz[IYZ(i, j + 1, k)] - syz[IYZ(i, j, k)]) * rdy;
+          const REAL dS = cpml(o, 2, 3, 1, k, c, (szz[IC(i, j, k)] - szz[IC(i, j, k - 1)]) * rdz) +
+                          cpml(o, 0, 5, 0, i, c, (sxz[IXZ(i + 1, j, k)] - sxz[IXZ(i, j, k)]) * rdx) +
+                          cpml(o, 1, 5, 0, j, c, (syz[IYZ(i, j + 1, k)] - syz[IYZ(i, j, k)]) * rdy);
           if (iso) {
             const REAL w = prof ? o->wz[k - 1] : (REAL)1;
             Vz[e] = w * (Vz[e] * rdt + o->bz * dS) * dt;
@@ -409,6 +459,7 @@ void oracle_run(oracle_t* o, int nsteps, int nthreads) {
 #else
   (void)nthreads;
 #endif
+  if (o->pml == 4 && !o->ct[0]) return; /* convolutional PML without its tables */
   for (int s = 0; s < nsteps; ++s) {
     if (o->mode == 0) stress_iso(o); else stress_ortho(o);
     if (!o->src_after_bc) inject(o);

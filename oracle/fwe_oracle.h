@@ -26,7 +26,9 @@ extern "C" {
 typedef struct oracle oracle_t;
 
 /* mode 0 = iso (OK4), 1 = ortho (ANISO); pml 0 none, 1 profile (iso),
- * 2 weights3d (ortho), 3 separable (ortho); free_surface 0 none, 1 BC1,
+ * 2 weights3d (ortho), 3 separable (ortho), 4 convolutional PML (either law;
+ * the HIP library's own thin-layer damping, no reference counterpart: see
+ * oracle_set_cpml); free_surface 0 none, 1 BC1,
  * 2 vacuum layer (shear modulus 1e-5 on plane k = 1, no BC1); src_after_bc
  * as in fwe_config. */
 oracle_t* oracle_create(int mode, int nx, int ny, int nz, double dx, double dy,
@@ -39,6 +41,12 @@ int oracle_set_iso(oracle_t* o, const float* c11, const float* c12, double c66,
 int oracle_set_ortho(oracle_t* o, const float* const cij[9], const float* const b[3]);
 int oracle_set_profiles(oracle_t* o, const float* wx, const float* wy, const float* wz);
 int oracle_set_weights3d(oracle_t* o, const float* w);
+/* pml = 4: memory variables in the L cells at both ends of every axis; per
+ * axis 6 rows of n+1 floats (b, a, 1/kappa at integer nodes x = i, then at
+ * half nodes x = i - 1/2), as fwe_gen_cpml_table writes them.  Every
+ * derivative g across a slab becomes g/kappa + psi with psi <- b psi + a g
+ * (no reference counterpart; restated from Komatitsch & Martin 2007). */
+int oracle_set_cpml(oracle_t* o, int L, const float* tx, const float* ty, const float* tz);
 int oracle_set_stf(oracle_t* o, const float* stf, int n);
 /* comp/ijk/coef arrays of length n (comp as FWE_SXX..FWE_SYZ) */
 int oracle_set_injections(oracle_t* o, int n, const int32_t* comp, const int32_t* ijk,

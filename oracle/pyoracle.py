@@ -33,6 +33,7 @@ def _lib(fp32=False):
             "oracle_set_ortho": (C.c_int, [C.c_void_p, C.POINTER(_FP), C.POINTER(_FP)]),
             "oracle_set_profiles": (C.c_int, [C.c_void_p, _FP, _FP, _FP]),
             "oracle_set_weights3d": (C.c_int, [C.c_void_p, _FP]),
+            "oracle_set_cpml": (C.c_int, [C.c_void_p, C.c_int, _FP, _FP, _FP]),
             "oracle_set_stf": (C.c_int, [C.c_void_p, _FP, C.c_int]),
             "oracle_set_injections": (C.c_int, [C.c_void_p, C.c_int, _IP, _IP, _FP]),
             "oracle_set_receivers": (C.c_int, [C.c_void_p, C.c_int, _IP, _IP]),
@@ -107,6 +108,16 @@ class Oracle:
     def set_pml_weights(self, w):
         a = _f32(w)
         self.L.oracle_set_weights3d(self.h, a.ctypes.data_as(_FP))
+
+    def set_cpml(self, layer, tx, ty, tz):
+        """pml = 4: convolutional PML of `layer` cells per face; per axis the
+        6 x (n+1) coefficient table of fwe_gen_cpml_table."""
+        t = [np.ascontiguousarray(np.asarray(a, np.float32).ravel()) for a in (tx, ty, tz)]
+        for a, n in zip(t, (self.nx, self.ny, self.nz)):
+            if a.size != 6 * (n + 1):
+                raise ValueError("a CPML table holds 6 x (n+1) floats")
+        if self.L.oracle_set_cpml(self.h, layer, *[a.ctypes.data_as(_FP) for a in t]) != 0:
+            raise ValueError("oracle_set_cpml failed")
 
     def set_stf(self, stf, batch=0):
         a = _f32(stf)

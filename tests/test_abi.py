@@ -95,7 +95,7 @@ def test_legacy_params_reader(fwe, tmp_path):
 @pytest.mark.parametrize("field,value,status", [
     ("nx", 2, "ERR_ARG"), ("nz", 2, "ERR_ARG"), ("nt", -1, "ERR_ARG"), ("nbatch", 0, "ERR_ARG"),
     ("nbatch", 9, "ERR_ARG"), ("dx", 0.0, "ERR_ARG"), ("dt", -1.0, "ERR_ARG"), ("free_surface", 3, "ERR_ARG"),
-    ("pml", 4, "ERR_ARG"), ("mode", 5, "ERR_ARG")])
+    ("pml", 5, "ERR_ARG"), ("mode", 5, "ERR_ARG")])
 def test_create_rejects_bad_configs(fwe, field, value, status):
     """Configuration errors are reported before any device is touched, with
     the status the header documents and no context returned."""
