@@ -224,7 +224,8 @@ MEASURES = {0: "single-wavefield Gcell-updates/s and % of 8 TB/s HBM (a cache-re
             1: "single-wavefield Gcell-updates/s and % of 8 TB/s HBM (the plain-stencil half of the metric)",
             2: "batched six elementary-MT wavefields against six plain runs (wavefield Gcell-updates/s)",
             3: "uneven batch of 32 models: Gcell-updates/s of the whole batch",
-            4: "learned thin boundary: batched-6 interior cell-updates/s and errors against a padded reference"}
+            4: "thin absorbing layer: batched-6 interior cell-updates/s on the 6-cell CPML grid, errors of every "
+               "layer form against a padded reference, and six plain runs at equal error"}
 
 
 def _line(value, steps, warmup, el, config, **extra):
@@ -492,27 +493,40 @@ def _small_group(nmodels, edges, steps):
 
 
 def bench_learned(args, steps):
-    """Learned thin boundary at full size: a 512x512x256 grid whose outer 6
-    cells are the learned layer (interior 500x500x244), against the same grid
-    with its plain 6-cell layer, the 24-cell plain layer over the same
-    interior (548x548x292), and a 2x-padded reference (1024x1024x512, 24-cell
-    layer); receiver errors over `eval_steps`.  Throughput: six elementary
-    tensors as one batch on the learned grid against six plain 24-cell runs,
-    as interior cell-updates per second."""
+    """Thin absorbing layers at full size.  A 512x512x256 grid whose outer 6
+    cells are the thin layer (interior 500x500x244) with: the plain 6-cell
+    layer (the reference's damping profile), the learned networks over it,
+    and the convolutional PML; plain layers of 24 and more cells over the
+    same interior; all against a 2x-padded reference (1024x1024x512) bounded
+    by a 32-cell convolutional PML, receiver errors over `eval_steps`.
+    Throughput: the six elementary tensors as one batch on the thin grid
+    against six plain runs (timed as one plain run x 6) at 24 cells and at
+    the width whose error first reaches the thin layer's."""
     from elanfmt_amd import learned as Lr
-
-    L, fb = 6, 24
-    thin, inner = (512, 512, 256), (500, 500, 244)
-    fall = tuple(q + 2 * fb for q in inner)
-    ref = (1024, 1024, 512)
-    lb, meta = Lr.load(os.path.join(ROOT, "elanfmt_amd", "data", "learned_boundary_L6.npz"))
-    seed, f0, nt = 4242, 3.0, args.eval_steps
-    print(f"# learned: medium {ref} on the host", file=sys.stderr, flush=True)
-    med = Lr.layered_medium(ref, seed)
     from elanfmt_amd import workloads as W
 
+    L, L8, fb, Lref = 6, 8, 24, 32
+    thin, inner = (512, 512, 256), (500, 500, 244)
+    thin8 = tuple(q + 2 * L8 for q in inner)
+    ref = (1024, 1024, 512)
+    widths = sorted({fb} | {int(w) for w in args.plain_widths.split(",") if w})
+    lb, meta = Lr.load(os.path.join(ROOT, "elanfmt_amd", "data", "learned_boundary_L6.npz"))
+    seed, f0, nt = 4242, 3.0, args.eval_steps
+    # CPML coefficients (fwe_gen_cpml_table): the 6-cell layer as measured
+    # best at this size over R, power, the frequency shift and kappa
+    cp_thin = dict(R=1e-4, kappa_max=1.0, alpha_max=0.0, power=2.0)
+    cp_8 = dict(R=1e-4, kappa_max=1.0, alpha_max=np.pi * f0, power=2.0)
+    cp_ref = dict(R=1e-6, kappa_max=1.0, alpha_max=np.pi * f0, power=2.0)
+    # one medium for every run: generated on the interior and continued
+    # unchanged outward (each absorbing layer emulates that continuation),
+    # on a grid that holds the reference and the widest plain grid; each run
+    # takes the window centred on the interior
+    big = tuple(max(r, q + 2 * max(widths)) for r, q in zip(ref, inner))
+    print(f"# learned: medium {inner} continued to {big} on the host", file=sys.stderr, flush=True)
+    med = Lr.extended_medium(inner, big, seed)
     dt = W.cfl_dt(med["vmax"], Lr.DX, Lr.DX, Lr.DX, 0.5)
-    z = Lr.impedance([c[256:768, 256:768, 128:384] for c in med["cells"][:1]], med["bcell"][256:768, 256:768, 128:384])
+    tw = tuple(slice((b - t) // 2, (b - t) // 2 + t) for b, t in zip(big, thin))
+    z = Lr.impedance([med["cells"][0][tw]], med["bcell"][tw])
     m, _, _ = W.mt_lune_random(seed, 1e15)
     # 128 receivers (8 x 16) on the plane 10 cells inside the thin layer, x/y/z velocity
     xs = np.linspace(L + 10, thin[0] - L - 11, 8).round().astype(int).tolist()
@@ -520,18 +534,23 @@ def bench_learned(args, steps):
     rec_thin = [(c, x, y, L + 10, 0) for y in ys for x in xs for c in range(3)]
     src_thin = (thin[0] // 2 - 1, thin[1] // 2 - 1, thin[2] // 2 - 1)
 
-    def grid_run(g, damp, R=1e-8, with_lb=False, nsteps=nt, nbatch=1, timed=False):
-        o = tuple((r - q) // 2 for r, q in zip(ref, g))
+    def grid_run(g, damp, kind, R=1e-8, nsteps=nt, nbatch=1, timed=False):
+        """kind: "plain" (the reference's profile), "learned" (networks over
+        the plain layer) or "cpml"."""
+        o = tuple((b - q) // 2 for b, q in zip(big, g))
         sh = tuple((q - t) // 2 for q, t in zip(g, thin))
         rec = [(c, i + sh[0], j + sh[1], k + sh[2], b) for (c, i, j, k, b) in rec_thin]
         src = tuple(a + b for a, b in zip(src_thin, sh))
         mm = m if nbatch == 1 else [np.eye(6)[e] * 1e15 for e in range(nbatch)]
         wl = Lr.workload(f"g{g}", g, damp, Lr.crop(med, o, g), dt, src, mm, f0, nsteps, rec, R=R, nbatch=nbatch)
+        if kind == "cpml":
+            W.use_cpml(wl, damp, vmax=med["vmax"], **(cp_ref if g == ref else (cp_8 if damp == L8 else cp_thin)))
         sv = W.make_solver(wl)
-        if with_lb:
+        if kind == "learned":
             lb.attach(sv, z)
         if timed:
-            # median of three timed runs from rest, after a 10-step warmup
+            # autotuned, then the median of three timed runs from rest
+            sv.autotune(3)
             sv.run(10)
             ts = []
             for _ in range(3):
@@ -548,39 +567,73 @@ def bench_learned(args, steps):
         sv.close()
         return tr
 
+    def plain_grid(w):
+        return tuple(q + 2 * w for q in inner)
+
     errs = None
+    w_eq = w_bar = None
     if nt > 0:  # --eval-steps 0: timing only
-        print(f"# learned: reference run {ref} x {nt} steps", file=sys.stderr, flush=True)
-        tref = grid_run(ref, fb)
+        print(f"# learned: reference run {ref} (32-cell CPML) x {nt} steps", file=sys.stderr, flush=True)
+        tref = grid_run(ref, Lref, "cpml")
+        nref = np.linalg.norm(tref)
         errs = {}
-        for name, g, damp, R, use in (("thin_plain_6", thin, L, meta["R_thin"], False),
-                                      ("thin_learned_6", thin, L, meta["R_thin"], True),
-                                      ("plain_24", fall, fb, 1e-8, False)):
+        runs = [("thin_plain_6", thin, L, "plain", meta["R_thin"]), ("thin_learned_6", thin, L, "learned", meta["R_thin"]),
+                ("thin_cpml_6", thin, L, "cpml", 1e-8), ("thin_cpml_8", thin8, L8, "cpml", 1e-8)]
+        runs += [(f"plain_{w}", plain_grid(w), w, "plain", 1e-8) for w in widths]
+        for name, g, damp, kind, R in runs:
             print(f"# learned: {name} {g}", file=sys.stderr, flush=True)
-            t = grid_run(g, damp, R, use)
-            errs[name] = float(np.linalg.norm(t - tref) / np.linalg.norm(tref))
+            t = grid_run(g, damp, kind, R)
+            errs[name] = float(np.linalg.norm(t - tref) / nref)
+        ok = [w for w in widths if errs[f"plain_{w}"] <= errs["thin_cpml_6"]]
+        w_eq = min(ok) if ok else None
+        ok3 = [w for w in widths if errs[f"plain_{w}"] <= 1e-3]
+        w_bar = min(ok3) if ok3 else None
     print(f"# learned: timing {steps} steps", file=sys.stderr, flush=True)
-    t_b6 = grid_run(thin, L, meta["R_thin"], True, nsteps=steps, nbatch=6, timed=True)
-    t_p1 = grid_run(fall, fb, 1e-8, False, nsteps=steps, timed=True)
-    t_l1 = grid_run(thin, L, meta["R_thin"], True, nsteps=steps, timed=True)
-    t_t1 = grid_run(thin, L, meta["R_thin"], False, nsteps=steps, timed=True)
+    t_b6 = grid_run(thin, L, "cpml", nsteps=steps, nbatch=6, timed=True)
+    t_b8 = grid_run(thin8, L8, "cpml", nsteps=steps, nbatch=6, timed=True)
+    t_b6_lb = grid_run(thin, L, "learned", meta["R_thin"], nsteps=steps, nbatch=6, timed=True)
+    t_c1 = grid_run(thin, L, "cpml", nsteps=steps, timed=True)
+    t_t1 = grid_run(thin, L, "plain", meta["R_thin"], nsteps=steps, timed=True)
+    w_cmp = w_eq if w_eq is not None else (max(widths) if errs is not None else fb)
+    w_cmp3 = w_bar if w_bar is not None else (max(widths) if errs is not None else fb)
+    t_p = {w: grid_run(plain_grid(w), w, "plain", nsteps=steps, timed=True) for w in sorted({fb, w_cmp, w_cmp3})}
     lb.close()
     cells_in = inner[0] * inner[1] * inner[2]
     value = cells_in * 6 * steps / t_b6 / 1e9
-    config = {"workload": f"learned 6-cell layer on 512x512x256 (interior 500x500x244), layered orthorhombic FP32, "
-                          f"lune MT, Ricker {f0} Hz, 128 receivers x 3 components; networks trained by "
-                          f"elanfmt_amd/train_lb.py ({meta['runs']} runs)",
+    config = {"workload": f"6-cell convolutional PML on 512x512x256 (interior 500x500x244), layered orthorhombic "
+                          f"FP32 (generated on the interior, continued unchanged outward), six elementary moment "
+                          f"tensors in one batch, Ricker {f0} Hz, 128 receivers x 3 components; also the 8-cell "
+                          f"CPML and the 6-cell plain layer with and without the learned networks "
+                          f"(elanfmt_amd/train_lb.py, {meta['runs']} runs)",
               "grid": list(thin), "interior": list(inner), "wavefields": 6, "eval_steps": nt,
-              "reference_grid": list(ref), "parallelism": "single GPU"}
+              "reference": {"grid": list(ref), "boundary": f"{Lref}-cell convolutional PML", **cp_ref},
+              "thin_cpml": cp_thin, "thin_cpml_8": cp_8, "parallelism": "single GPU"}
+    eq = None
+    if errs is not None:
+        eq = {"plain_width_at_thin_cpml_error": w_eq,
+              "compared_width": w_cmp,
+              "note": ("smallest plain width whose error is at most the thin CPML's" if w_eq is not None else
+                       f"no plain width up to {max(widths)} reaches the thin CPML's error; the widest is "
+                       f"compared, so its speedup is a lower bound"),
+              "bar_1e-3": {k: v <= 1e-3 for k, v in errs.items()},
+              "plain_width_at_1e-3": w_bar, "compared_width_at_1e-3": w_cmp3}
     return _line(value, steps, 0, t_b6, config, unit_note="interior cell-updates/s (6 wavefields)",
-                 trace_error_vs_padded_reference=errs,
-                 throughput={"batched6_learned_s": round(t_b6, 3), "six_plain24_runs_s": round(6 * t_p1, 3),
-                             "speedup": round(6 * t_p1 / t_b6, 3),
-                             "note": "six plain runs timed as 6 x one measured 24-cell run; every run the "
-                                     "median of three timings",
-                             "learned_thin_ms_per_step": round(t_l1 / steps * 1e3, 3),
+                 trace_error_vs_padded_reference=errs, equal_error=eq,
+                 throughput={"batched6_cpml_s": round(t_b6, 3),
+                             "six_plain24_runs_s": round(6 * t_p[fb], 3),
+                             "speedup_vs_six_plain24": round(6 * t_p[fb] / t_b6, 3),
+                             f"six_plain{w_cmp}_runs_s": round(6 * t_p[w_cmp], 3),
+                             "speedup_at_equal_error": round(6 * t_p[w_cmp] / t_b6, 3),
+                             "batched6_cpml8_s": round(t_b8, 3),
+                             f"six_plain{w_cmp3}_runs_s": round(6 * t_p[w_cmp3], 3),
+                             "speedup_cpml8_at_1e-3": round(6 * t_p[w_cmp3] / t_b8, 3),
+                             "batched6_learned_plain6_s": round(t_b6_lb, 3),
+                             "speedup_learned_vs_six_plain24": round(6 * t_p[fb] / t_b6_lb, 3),
+                             "note": "six plain runs timed as 6 x one measured run; every run autotuned, then "
+                                     "the median of three timings",
+                             "cpml_thin_ms_per_step": round(t_c1 / steps * 1e3, 3),
                              "plain_thin_ms_per_step": round(t_t1 / steps * 1e3, 3),
-                             "plain24_ms_per_step": round(t_p1 / steps * 1e3, 3)})
+                             **{f"plain{w}_ms_per_step": round(t / steps * 1e3, 3) for w, t in t_p.items()}})
 
 
 def main():
@@ -589,7 +642,9 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 3, 4])
-    ap.add_argument("--eval-steps", type=int, default=1500)
+    ap.add_argument("--eval-steps", type=int, default=4000, help="config 4: steps of the error runs")
+    ap.add_argument("--plain-widths", default="48,96,144,192,240",
+                    help="config 4: plain-layer widths (besides 24) whose errors bracket the thin layer's")
     ap.add_argument("--kernel", default="auto", choices=["auto", "two_pass", "fused"])
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--small-steps", type=int, default=400, help="config 3: steps of the small-model groups")

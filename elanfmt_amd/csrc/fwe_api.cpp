@@ -2225,8 +2225,6 @@ extern "C" fwe_status fwe_lb_eval(fwe_lb* lb, int32_t face, int32_t n, const flo
 
 static fwe_status lb_plain_ctx(const fwe_ctx* x, const char* what) {
   if (x->cfg.nbatch != 1) return fail(FWE_ERR_UNSUPPORTED, "%s: one wavefield only", what);
-  if (x->cfg.pml == FWE_PML_CPML)
-    return fail(FWE_ERR_UNSUPPORTED, "%s: training crops fields only; no convolutional PML", what);
   if (x->cfg.free_surface) return fail(FWE_ERR_UNSUPPORTED, "%s: the learned boundary replaces damping on all six faces; no free surface", what);
   return FWE_OK;
 }
@@ -2244,6 +2242,12 @@ extern "C" fwe_status fwe_lb_train_step(fwe_lb* lb, fwe_ctx* thick, fwe_ctx* thi
   if (a.nx != b.nx + 2 * offset || a.ny != b.ny + 2 * offset || a.nz != b.nz + 2 * offset)
     return fail(FWE_ERR_ARG, "thick grid must be the thin grid plus %d cells on every face", offset);
   if ((st = ready(thick)) || (st = ready(thin))) return st;
+  const bool cp = thin->cfg.pml == FWE_PML_CPML;
+  if (cp != (thick->cfg.pml == FWE_PML_CPML))
+    return fail(FWE_ERR_ARG, "thick and thin runs must both use the convolutional PML, or neither");
+  if (cp && thick->cpml_L < thin->cpml_L + offset)
+    return fail(FWE_ERR_ARG, "the thick run's CPML layer (%d) must cover the thin layer plus the offset (%d)",
+                thick->cpml_L, thin->cpml_L + offset);
   if (!lb_fits(thick) || !lb_fits(thin))
     return fail(FWE_ERR_UNSUPPORTED, "learned boundary: the nine fields of a wavefield exceed 4 GiB");
   HIPCHK(hipSetDevice(lb->device));
@@ -2264,8 +2268,16 @@ extern "C" fwe_status fwe_lb_train_step(fwe_lb* lb, fwe_ctx* thick, fwe_ctx* thi
   HIPCHK(hipStreamSynchronize(thin->stream));
   // thin window of the thick run at step n -> thin fields (resync), or the
   // thin run's own state; features
-  if (resync || thin->it != thick->it)
+  if (resync || thin->it != thick->it) {
     HIPCHK(fwe::lb_launch_crop(lb_geom(thick, thick->cur, zinv), lb_geom(thin, thin->cur, zinv), offset, lb->stream));
+    if (cp) {
+      auto pg = [](const fwe_ctx* x) {
+        fwe::PsiGeom g{x->psi, x->cfg.nx, x->cfg.ny, x->cfg.nz, x->cpml_L, {x->pvar[0], x->pvar[1], x->pvar[2]}};
+        return g;
+      };
+      HIPCHK(fwe::lb_launch_crop_psi(pg(thick), pg(thin), offset, lb->stream));
+    }
+  }
   HIPCHK(fwe::lb_launch_gather(lb_geom(thin, thin->cur, zinv), lb->cells, (int)ncell, lb->X, lb->R, lb->depth,
                                lb->stream));
   // RMS level of the batch (weights of the loss)

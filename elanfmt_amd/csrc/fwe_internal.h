@@ -261,4 +261,15 @@ hipError_t lb_launch_apply_y(const LbGeom& g, const LbCell* cells, int n, const 
                              hipStream_t s);
 hipError_t lb_launch_crop(const LbGeom& thick, const LbGeom& thin, int off, hipStream_t s);
 
+// the convolutional PML memory variables of one wavefield (StepArgs::psi
+// layout: x, y, z slabs of six variables, pvar[d] elements each)
+struct PsiGeom {
+  float* psi;
+  int32_t nx, ny, nz, L;
+  int64_t pvar[3];
+};
+// thin slabs <- the same cells of the thick run's (wider) slabs, which must
+// hold them: thick L >= thin L + off
+hipError_t lb_launch_crop_psi(const PsiGeom& thick, const PsiGeom& thin, int off, hipStream_t s);
+
 }  // namespace fwe

@@ -659,6 +659,57 @@ __global__ __launch_bounds__(256) void k_lb_crop(const LbGeom thick, const LbGeo
   for (int f = 0; f < 9; ++f) const_cast<float*>(thin.F[f])[a] = thick.F[f][b];
 }
 
+// Memory variables of the convolutional PML, thin window of the thick run
+// -> thin run: thin slab element e (all three slabs, six variables each)
+// names cell (i,j,k); the thick run holds the same variable at cell
+// (i+off, j+off, k+off) of its own, wider slab.
+__device__ __forceinline__ int64_t psi_slab_index(const PsiGeom& g, int d, int i, int j, int k) {
+  const int L2 = 2 * g.L;
+  if (d == 0) {
+    const int xi = i < g.L ? i : i - g.nx + L2;
+    return xi + (int64_t)L2 * (j + (int64_t)g.ny * k);
+  }
+  if (d == 1) {
+    const int yj = j < g.L ? j : j - g.ny + L2;
+    return i + (int64_t)g.nx * (yj + (int64_t)L2 * k);
+  }
+  const int zk = k < g.L ? k : k - g.nz + L2;
+  return i + (int64_t)g.nx * (j + (int64_t)g.ny * zk);
+}
+
+__global__ __launch_bounds__(256) void k_lb_crop_psi(const PsiGeom thick, const PsiGeom thin, int32_t off) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t n0 = 6 * thin.pvar[0], n1 = 6 * thin.pvar[1];
+  if (e >= n0 + n1 + 6 * thin.pvar[2]) return;
+  const int L2 = 2 * thin.L;
+  int d, i, j, k;
+  int64_t r, v;
+  if (e < n0) {
+    d = 0; v = e / thin.pvar[0]; r = e - v * thin.pvar[0];
+    const int xi = (int)(r % L2);
+    const int64_t t = r / L2;
+    j = (int)(t % thin.ny); k = (int)(t / thin.ny);
+    i = xi < thin.L ? xi : xi - L2 + thin.nx;
+  } else if (e < n0 + n1) {
+    d = 1; v = (e - n0) / thin.pvar[1]; r = e - n0 - v * thin.pvar[1];
+    i = (int)(r % thin.nx);
+    const int64_t t = r / thin.nx;
+    const int yj = (int)(t % L2);
+    k = (int)(t / L2);
+    j = yj < thin.L ? yj : yj - L2 + thin.ny;
+  } else {
+    d = 2; v = (e - n0 - n1) / thin.pvar[2]; r = e - n0 - n1 - v * thin.pvar[2];
+    i = (int)(r % thin.nx);
+    const int64_t t = r / thin.nx;
+    j = (int)(t % thin.ny);
+    const int zk = (int)(t / thin.ny);
+    k = zk < thin.L ? zk : zk - L2 + thin.nz;
+  }
+  int64_t tb = 0;
+  for (int q = 0; q < d; ++q) tb += 6 * thick.pvar[q];
+  thin.psi[e] = thick.psi[tb + v * thick.pvar[d] + psi_slab_index(thick, d, i + off, j + off, k + off)];
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -742,6 +793,12 @@ hipError_t lb_launch_apply_y(const LbGeom& g, const LbCell* cells, int n, const 
                              hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_lb_apply_y, dim3(lb_blocks(n, 256)), dim3(256), 0, s, g, cells, n, Y, R);
+  return hipGetLastError();
+}
+
+hipError_t lb_launch_crop_psi(const PsiGeom& thick, const PsiGeom& thin, int off, hipStream_t s) {
+  const int64_t n = 6 * (thin.pvar[0] + thin.pvar[1] + thin.pvar[2]);
+  hipLaunchKernelGGL(k_lb_crop_psi, dim3(lb_blocks(n, 256)), dim3(256), 0, s, thick, thin, off);
   return hipGetLastError();
 }
 

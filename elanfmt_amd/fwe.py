@@ -208,6 +208,15 @@ def f32(a) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(a, dtype=np.float32).ravel(order="F"))
 
 
+def _sized(a, n, what):
+    """a as a float32 x-fastest vector of exactly n elements (the library
+    reads n floats through the pointer)."""
+    v = f32(a)
+    if v.size != n:
+        raise ValueError(f"{what}: {v.size} values, the grid needs {n}")
+    return v
+
+
 def device_count() -> int:
     n = C.c_int32(0)
     _check(lib().fwe_device_count(C.byref(n)))
@@ -291,23 +300,35 @@ class Solver:
         self.close()
 
     # medium and damping
+    def _dims(self):
+        return self.cfg.nx, self.cfg.ny, self.cfg.nz
+
     def set_material_iso(self, c11, c12, c66, bx, by, bz):
-        a, b = f32(c11), f32(c12)
+        nx, ny, nz = self._dims()
+        a, b = _sized(c11, nx * ny * nz, "c11"), _sized(c12, nx * ny * nz, "c12")
         _check(lib().fwe_set_material_iso(self._h, fptr(a), fptr(b), c66, bx, by, bz))
 
     def set_material_ortho(self, cij, buoy):
-        cs = [f32(c) for c in cij]
-        bs = [f32(b) for b in buoy]
+        nx, ny, nz = self._dims()
+        nc = nx * ny * nz
+        csz = [nc] * 6 + [nx * (ny + 1) * (nz + 1), (nx + 1) * ny * (nz + 1), (nx + 1) * (ny + 1) * nz]
+        bsz = [(nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1)]
+        cs = [_sized(c, n, f"cij[{q}]") for q, (c, n) in enumerate(zip(cij, csz))]
+        bs = [_sized(b, n, f"buoy[{q}]") for q, (b, n) in enumerate(zip(buoy, bsz))]
+        if len(cs) != 9 or len(bs) != 3:
+            raise ValueError("nine moduli and three buoyancy arrays")
         carr = (_FP * 9)(*[fptr(c) for c in cs])
         barr = (_FP * 3)(*[fptr(b) for b in bs])
         _check(lib().fwe_set_material_ortho(self._h, carr, barr))
 
     def set_pml_profiles(self, wx, wy, wz):
-        a, b, c = f32(wx), f32(wy), f32(wz)
+        add = 1 if self.cfg.pml == PML_PROFILE else 0
+        a, b, c = (_sized(w, n + add, "profile") for w, n in zip((wx, wy, wz), self._dims()))
         _check(lib().fwe_set_pml_profiles(self._h, fptr(a), fptr(b), fptr(c)))
 
     def set_pml_weights(self, w):
-        a = f32(w)
+        nx, ny, nz = self._dims()
+        a = _sized(w, nx * ny * nz, "weights")
         _check(lib().fwe_set_pml_weights(self._h, fptr(a)))
 
     def set_cpml(self, layer, tx, ty, tz):
@@ -475,7 +496,8 @@ class Solver:
         return out.reshape(shp, order="F")
 
     def set_field(self, field, arr, batch=0):
-        a = f32(arr)
+        shp = field_shape(self.cfg.nx, self.cfg.ny, self.cfg.nz, field)
+        a = _sized(arr, int(np.prod(shp)), f"field {field}")
         _check(lib().fwe_set_field(self._h, batch, field, fptr(a)))
 
 

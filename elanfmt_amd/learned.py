@@ -23,6 +23,10 @@ from . import fwe, workloads as W
 
 THICK = 32  # damping cells of the teacher runs
 DX = 50.0
+# convolutional PML of the thin layer (fwe_gen_cpml_table arguments), and of
+# the references the thin layers are judged against
+CPML_THIN = dict(R=1e-4, kappa_max=1.0, alpha_max=0.0, power=2.0)
+CPML_REF = dict(R=1e-6, kappa_max=1.0, power=2.0)  # alpha_max = pi f0
 
 
 def impedance(cells, bcell):
@@ -44,6 +48,27 @@ def layered_medium(n, seed, nl=5):
     return dict(cells=cells, bcell=bcell, cij=cij, buoy=buoy, vmax=vmax)
 
 
+def extended_medium(core, grid, seed, nl=5):
+    """A layered medium generated on the `core` grid (an interior) and
+    continued unchanged outward to `grid` (centred): every cell outside the
+    core takes the value of the nearest core cell, then the staggered
+    arrays are built on the whole grid.  Every absorbing layer emulates the
+    grid's own medium continued to infinity, so a reference run on the
+    larger grid measures the boundaries alone, not the medium the smaller
+    grids leave out."""
+    core, grid = _dims(core), _dims(grid)
+    m = layered_medium(core, seed, nl)
+    pad = [((g - c) // 2, g - c - (g - c) // 2) for g, c in zip(grid, core)]
+    if any(a < 0 or b < 0 for a, b in pad):
+        raise ValueError(f"grid {grid} smaller than the core {core}")
+    cells = [np.asfortranarray(np.pad(c, pad, mode="edge")) for c in m["cells"]]
+    bcell = np.asfortranarray(np.pad(m["bcell"], pad, mode="edge"))
+    vmax = m["vmax"]  # no new material outside the core
+    del m
+    cij, buoy = W.ortho_from_cells(cells, bcell)
+    return dict(cells=cells, bcell=bcell, cij=cij, buoy=buoy, vmax=vmax)
+
+
 _CIJ_EXTRA = [(0, 0, 0)] * 6 + [(0, 1, 1), (1, 0, 1), (1, 1, 0)]
 _B_EXTRA = [(1, 0, 0), (0, 1, 0), (0, 0, 1)]
 
@@ -52,6 +77,9 @@ def crop(med, o, n):
     """Window [o, o+n) per axis of a medium (staggered arrays keep their +1);
     o and n are scalars or per-axis triples."""
     o3, n3 = _dims(o), _dims(n)
+    full = med["bcell"].shape
+    if any(a < 0 or a + b > f for a, b, f in zip(o3, n3, full)):
+        raise ValueError(f"window {n3} at {o3} leaves the medium's grid {full}")
 
     def sl(a, ex):
         return a[o3[0]:o3[0] + n3[0] + ex[0], o3[1]:o3[1] + n3[1] + ex[1], o3[2]:o3[2] + n3[2] + ex[2]]
@@ -102,6 +130,36 @@ def workload(name, n, damp, med, dt, src, m, f0, nt, receivers=(), R=1e-8, taper
                       receivers=rec)
 
 
+def teacher_cpml_table(thin_table, off):
+    """CPML table of a teacher axis (the thin axis plus `off` cells at both
+    ends): the thin table over the thin window, and beyond it the values of
+    the thin layer's outer edge (normalised depth 1, the thin table's half
+    node 0), so the teacher's layer is the thin one continued at full
+    damping.  Teacher and thin steps then differ only where the thin grid
+    ends."""
+    t = np.asarray(thin_table, np.float32)
+    n = t.shape[1] - 1
+    edge = t[3:6, 0]
+    out = np.empty((6, n + 2 * off + 1), np.float32)
+    out[:, off:off + n + 1] = t
+    for r in range(3):
+        out[r, :off] = edge[r]
+        out[3 + r, :off] = edge[r]
+        out[r, off + n + 1:] = edge[r]
+        out[3 + r, off + n + 1:] = edge[r]
+    return out
+
+
+def cpml_workload(wl, layer, vmax, tables=None, **cp):
+    """Switch a workload to the convolutional PML: `layer` cells with the
+    generator's tables (cp: fwe_gen_cpml_table arguments), or the given
+    per-axis tables."""
+    if tables is None:
+        return W.use_cpml(wl, layer, vmax=vmax, **cp)
+    wl.cpml, wl.cpml_layer, wl.pml, wl.prof, wl.w3 = tuple(tables), layer, fwe.PML_CPML, None, None
+    return wl
+
+
 @dataclass
 class TrainLog:
     runs: int = 0
@@ -112,14 +170,18 @@ class TrainLog:
 
 
 def train(lb: fwe.LearnedBoundary, runs=200, n_int=40, warm=None, steps=None, lr=5e-4, seed0=1000, f0=(3.0, 8.0),
-          R_thin=1e-3, resync_every=8, log_every=0, progress_every=0):
+          R_thin=1e-3, resync_every=8, log_every=0, progress_every=0, base="plain", extend=False):
     """Training on `runs` thick-damping runs over seeded random layered
     media and lune moment tensors.  n_int is the interior edge, or a
     sequence of edges drawn per run.  Each run: `warm` plain steps (default
     1.5 n_int, until the first wavefront reaches the layer), then `steps`
     training steps (default 5 n_int; one Adam step per face each); the thin
     run is re-synchronised to the thick run every `resync_every` steps and
-    rolls out with its own corrections in between."""
+    rolls out with its own corrections in between.  base: the thin layer's
+    own damping, "plain" (the reference's profile) or "cpml" (CPML_THIN;
+    the teacher continues it, teacher_cpml_table).  extend: media generated
+    on the interior and continued unchanged outward (always with "cpml"),
+    instead of one medium generated on the teacher grid."""
     L = lb.layer
     off = THICK - L
     sizes = [int(n_int)] if np.isscalar(n_int) else [int(v) for v in n_int]
@@ -131,7 +193,7 @@ def train(lb: fwe.LearnedBoundary, runs=200, n_int=40, warm=None, steps=None, lr
         n_thin, n_thick = ni + 2 * L, ni + 2 * THICK
         wu = warm if warm is not None else max(60, int(1.5 * ni))
         nsteps = steps if steps is not None else 5 * ni
-        med = layered_medium(n_thick, seed)
+        med = extended_medium(ni, n_thick, seed) if (extend or base == "cpml") else layered_medium(n_thick, seed)
         dt = W.cfl_dt(med["vmax"], DX, DX, DX, 0.5)
         z = impedance(med["cells"], med["bcell"])
         ff = float(rng.uniform(*f0))
@@ -139,10 +201,16 @@ def train(lb: fwe.LearnedBoundary, runs=200, n_int=40, warm=None, steps=None, lr
         c = ni // 2 + rng.integers(-ni // 6, ni // 6 + 1, 3)
         nt = wu + nsteps + 2
         taper = teacher_taper(n_thin, L, off, dt, med["vmax"], R_thin)
-        thick = W.make_solver(workload("thick", n_thick, THICK, med, dt, tuple(int(v) + THICK for v in c), m, ff, nt,
-                                       taper=taper))
-        thin = W.make_solver(workload("thin", n_thin, L, crop(med, off, n_thin), dt, tuple(int(v) + L for v in c),
-                                      m, ff, nt, R=R_thin))
+        wthick = workload("thick", n_thick, THICK, med, dt, tuple(int(v) + THICK for v in c), m, ff, nt, taper=taper)
+        wthin = workload("thin", n_thin, L, crop(med, off, n_thin), dt, tuple(int(v) + L for v in c), m, ff, nt,
+                         R=R_thin)
+        if base == "cpml":
+            tt = W.cpml_table(n_thin, L, DX, dt, med["vmax"], **CPML_THIN)
+            cpml_workload(wthin, L, med["vmax"], tables=(tt, tt, tt))
+            tk = teacher_cpml_table(tt, off)
+            cpml_workload(wthick, THICK, med["vmax"], tables=(tk, tk, tk))
+        thick = W.make_solver(wthick)
+        thin = W.make_solver(wthin)
         thick.run(wu)
         for t in range(nsteps):
             loss = lb.train_step(thick, thin, off, z, lr, resync=(t % resync_every == 0))
@@ -174,10 +242,13 @@ def face_receivers(n_thin, L, inset=2, per_side=2):
 
 
 def evaluate(lb: fwe.LearnedBoundary | None, seed=7, n_int=40, nt=400, f0=5.0, fallback=24, R_thin=1e-3,
-             cfl=0.5):
+             cfl=0.5, base="plain", extend=False):
     """Relative L2 error of receiver traces near the boundary against a
-    padded reference, for the thin plain layer, the thin layer with the
-    networks (if lb), and a `fallback`-cell plain layer."""
+    padded reference, for the thin layer (base "plain" or "cpml"), the thin
+    layer with the networks (if lb), and a `fallback`-cell plain layer.
+    extend (always with "cpml"): the medium generated on the interior and
+    continued unchanged outward, and the reference bounded by a 32-cell
+    CPML, so the errors measure the boundaries alone."""
     L = lb.layer if lb is not None else 6
     n_thin = n_int + 2 * L
     rng = np.random.default_rng(seed)
@@ -185,19 +256,22 @@ def evaluate(lb: fwe.LearnedBoundary | None, seed=7, n_int=40, nt=400, f0=5.0, f
     cells_per_step = cfl / np.sqrt(3.0)
     pad = int(np.ceil(cells_per_step * nt / 2 + 24 + 8))
     n_ref = n_int + 2 * pad
-    med = layered_medium(n_ref, seed)
+    ext = extend or base == "cpml"
+    med = extended_medium(n_int, n_ref, seed) if ext else layered_medium(n_ref, seed)
     dt = W.cfl_dt(med["vmax"], DX, DX, DX, cfl)
     z = impedance(med["cells"], med["bcell"])
     m, _, _ = W.mt_lune_random(seed * 31 + 1, 1e15)
     c = n_int // 2 + rng.integers(-n_int // 8, n_int // 8 + 1, 3)
     rec_thin = face_receivers(n_thin, L)
 
-    def run(n, damp, R=1e-8, with_lb=False):
+    def run(n, damp, R=1e-8, with_lb=False, cp=None):
         o = (n_ref - n) // 2
         shift = (n - n_thin) // 2
         rec = [(q, i + shift, j + shift, k + shift, b) for (q, i, j, k, b) in rec_thin]
         wl = workload(f"g{n}", n, damp, crop(med, o, n), dt, tuple(int(v) + (n - n_int) // 2 for v in c), m, f0,
                       nt, rec, R=R)
+        if cp is not None:
+            cpml_workload(wl, damp, med["vmax"], **cp)
         s = W.make_solver(wl)
         if with_lb:
             lb.attach(s, z)
@@ -206,15 +280,16 @@ def evaluate(lb: fwe.LearnedBoundary | None, seed=7, n_int=40, nt=400, f0=5.0, f
         s.close()
         return tr
 
-    ref = run(n_ref, 24)
-    out = {"reference_grid": n_ref, "thin_grid": n_thin, "steps": nt}
+    ref = run(n_ref, 32, cp=dict(CPML_REF, alpha_max=np.pi * f0)) if ext else run(n_ref, 24)
+    out = {"reference_grid": n_ref, "thin_grid": n_thin, "steps": nt, "base": base}
 
     def err(t):
         return float(np.linalg.norm(t - ref) / np.linalg.norm(ref))
 
-    out["thin_plain"] = err(run(n_thin, L, R=R_thin))
+    cp = CPML_THIN if base == "cpml" else None
+    out["thin_plain" if base == "plain" else "thin_cpml"] = err(run(n_thin, L, R=R_thin, cp=cp))
     if lb is not None:
-        out["thin_learned"] = err(run(n_thin, L, R=R_thin, with_lb=True))
+        out["thin_learned"] = err(run(n_thin, L, R=R_thin, with_lb=True, cp=cp))
     out[f"plain_{fallback}"] = err(run(n_int + 2 * fallback, fallback))
     return out
 

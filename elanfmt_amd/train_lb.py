@@ -32,8 +32,16 @@ def main():
     ap.add_argument("--sizes", default="40", help="interior edges drawn per run, comma separated")
     ap.add_argument("--f0", default="3,8", help="Ricker peak-frequency range (Hz) drawn per run, lo,hi")
     ap.add_argument("--init", default=None, help="start from the parameters of this .npz instead of a fresh init")
-    ap.add_argument("--out", default=os.path.join(ROOT, "elanfmt_amd", "data", f"learned_boundary_L6.npz"))
+    ap.add_argument("--base", default="plain", choices=["plain", "cpml"],
+                    help="the thin layer's own damping: the reference's profile, or the convolutional PML")
+    ap.add_argument("--extend", action="store_true",
+                    help="media generated on the interior and continued outward (always with --base cpml)")
+    ap.add_argument("--eval-sizes", default="80:500:5,120:700:4", help="interior:steps:f0 of the larger evaluations")
+    ap.add_argument("--out", default=None)
     a = ap.parse_args()
+    if a.out is None:
+        a.out = os.path.join(ROOT, "elanfmt_amd", "data",
+                             "learned_boundary_L6.npz" if a.base == "plain" else "learned_boundary_cpml_L6.npz")
     if a.init:
         lb, _ = learned.load(a.init)
     else:
@@ -42,20 +50,23 @@ def main():
     sizes = [int(v) for v in a.sizes.split(",")]
     f0_range = tuple(float(v) for v in a.f0.split(","))
     log = learned.train(lb, runs=a.runs, n_int=sizes, lr=a.lr, resync_every=a.resync, R_thin=a.r_thin, f0=f0_range,
-                        seed0=a.seed0, log_every=200, progress_every=10)
+                        seed0=a.seed0, log_every=200, progress_every=10, base=a.base, extend=a.extend)
     t_train = time.time() - t0
     print(f"trained {log.runs} runs, {log.steps} steps in {t_train:.1f} s; loss {log.first_loss:.3e} -> "
           f"{log.last_loss:.3e}", flush=True)
     evals = {}
+    ev = dict(R_thin=a.r_thin, base=a.base, extend=a.extend)
     for sd in (7, 8, 9):
-        evals[sd] = learned.evaluate(lb, seed=sd, R_thin=a.r_thin)
+        evals[sd] = learned.evaluate(lb, seed=sd, **ev)
         print(f"held-out medium {sd}: {evals[sd]}", flush=True)
-    for n_int, nt, f0 in ((80, 500, 5.0), (120, 700, 4.0)):
-        evals[f"interior{n_int}"] = learned.evaluate(lb, seed=7, n_int=n_int, nt=nt, f0=f0, R_thin=a.r_thin)
+    for spec in a.eval_sizes.split(","):
+        n_int, nt, f0 = (t(v) for t, v in zip((int, int, float), spec.split(":")))
+        evals[f"interior{n_int}"] = learned.evaluate(lb, seed=7, n_int=n_int, nt=nt, f0=f0, **ev)
         print(f"held-out medium 7, interior {n_int}: {evals[f'interior{n_int}']}", flush=True)
     learned.save(lb, a.out, runs=a.runs, layer=a.layer, lr=a.lr, resync_every=a.resync, R_thin=a.r_thin,
                  seed0=a.seed0, sizes=sizes, f0=list(f0_range), train_seconds=round(t_train, 1), steps=log.steps,
-                 loss_curve=[(s, float(f"{l:.4e}")) for s, l in log.losses], evaluation=evals)
+                 loss_curve=[(s, float(f"{l:.4e}")) for s, l in log.losses], evaluation=evals, base=a.base,
+                 extend=bool(a.extend or a.base == "cpml"), cpml=learned.CPML_THIN if a.base == "cpml" else None)
     print(json.dumps({"out": a.out, "evaluation": evals}), flush=True)
     lb.close()
 

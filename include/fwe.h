@@ -421,8 +421,10 @@ fwe_status fwe_lb_eval(fwe_lb* lb, int32_t face, int32_t n, const float* x, cons
 /* One training step.  `thick` is a run whose grid is the thin grid plus
  * `offset` cells on every face (thick damping).  With resync (or when the
  * step counters differ) its fields at step n are first copied into `thin`
- * (same medium, `layer`-cell plain damping); otherwise `thin` continues from
- * its own state.  Both advance one step; each face's network is fitted (one
+ * (same medium, `layer`-cell plain damping or convolutional PML); otherwise
+ * `thin` continues from its own state.  With the convolutional PML both runs
+ * use it, the thick run's layer covers the thin one plus the offset, and the
+ * resync also copies the thin slabs' memory variables.  Both advance one step; each face's network is fitted (one
  * Adam step, rate lr) to the difference between the thick step and the
  * thin plain step in the layer cells, and `thin` then receives the
  * corrections predicted before the update, as with the boundary attached.  Each cell's error is weighted by (r / r_ref)^2, r its

@@ -131,3 +131,21 @@ def test_create_rejects_unsupported_law_damping_pairs(fwe):
     cfg.nz = 1024  # 2^34 cells
     cfg.pml = fwe.PML_SEPARABLE
     assert fwe.lib().fwe_create(C.byref(cfg), C.byref(h)) == fwe.ERR_ARG
+
+
+def test_host_arrays_are_checked_before_the_library_reads_them(fwe):
+    """The Python mirror refuses arrays of the wrong length (the C ABI reads
+    as many floats as the grid needs through the pointer), and media windows
+    outside the generated grid."""
+    import numpy as np
+
+    from elanfmt_amd import learned as Lr
+
+    assert fwe._sized(np.zeros((3, 4, 5)), 60, "a").size == 60
+    with pytest.raises(ValueError):
+        fwe._sized(np.zeros((3, 4, 5)), 61, "a")
+    m = Lr.layered_medium((20, 20, 10), 1)
+    assert Lr.crop(m, (2, 3, 1), (10, 12, 8))["bcell"].shape == (10, 12, 8)
+    for o, n in (((0, 0, -1), (20, 20, 10)), ((0, 0, 0), (20, 20, 11)), ((5, 0, 0), (16, 20, 10))):
+        with pytest.raises(ValueError):
+            Lr.crop(m, o, n)

@@ -243,3 +243,48 @@ def test_learned_members_are_refused_by_groups_and_attached_boundaries_stay_aliv
     sol[0].close()  # closing the solver detaches the boundary
     lb.close()
     sol[1].close()
+
+
+def test_cpml_training_resync_copies_fields_and_memory_variables(gpu):
+    """Training over a convolutional-PML thin layer: after a resync the thin
+    run steps from the teacher's window, fields and memory variables alike,
+    with the teacher's continued CPML table equal to the thin one over the
+    window.  One thin step (networks that correct nothing, rate 0) then
+    equals the teacher's step bit for bit wherever the stencil stays inside
+    the thin grid, layer cells included; a training loop over the CPML base
+    runs and changes the networks."""
+    from elanfmt_amd import fwe, learned as Lr, workloads as W
+
+    L, n_int, off = 6, 30, Lr.THICK - 6
+    n_thin, n_thick = n_int + 2 * L, n_int + 2 * Lr.THICK
+    med = Lr.extended_medium(n_int, n_thick, 3)
+    dt = W.cfl_dt(med["vmax"], Lr.DX, Lr.DX, Lr.DX, 0.5)
+    m, _, _ = W.mt_lune_random(11, 1e15)
+    c = (n_int // 2 - 2, n_int // 2 + 1, n_int // 2)
+    nt = 200
+    tt = W.cpml_table(n_thin, L, Lr.DX, dt, med["vmax"], **Lr.CPML_THIN)
+    tk = Lr.teacher_cpml_table(tt, off)
+    np.testing.assert_array_equal(tk[:, off:off + n_thin + 1], tt)
+    wk = Lr.cpml_workload(Lr.workload("thick", n_thick, Lr.THICK, med, dt, tuple(v + Lr.THICK for v in c), m, 6.0, nt),
+                          Lr.THICK, med["vmax"], tables=(tk, tk, tk))
+    wt = Lr.cpml_workload(Lr.workload("thin", n_thin, L, Lr.crop(med, off, n_thin), dt, tuple(v + L for v in c), m,
+                                      6.0, nt), L, med["vmax"], tables=(tt, tt, tt))
+    thick, thin = W.make_solver(wk), W.make_solver(wt)
+    lb = fwe.LearnedBoundary(L)
+    thick.run(45)  # the wavefront is inside the thin layer
+    print("case cpml resync: teacher at step 45, one training step at rate 0", flush=True)
+    lb.train_step(thick, thin, off, 1e7, 0.0, resync=True)
+    assert thin.step() == thick.step() == 46
+    for f in range(9):
+        a = thin.get_field(f)
+        b = thick.get_field(f)[off:off + a.shape[0], off:off + a.shape[1], off:off + a.shape[2]]
+        assert np.abs(b).max() > 0, f
+        sl = (slice(2, a.shape[0] - 2), slice(2, a.shape[1] - 2), slice(2, a.shape[2] - 2))
+        assert np.array_equal(a[sl], b[sl]), f
+    thick.close()
+    thin.close()
+    p0 = [lb.get_params(f) for f in range(6)]
+    log = Lr.train(lb, runs=1, n_int=24, warm=40, steps=80, base="cpml", f0=(5.0, 7.0))
+    assert log.steps == 80 and np.isfinite(log.last_loss) and log.last_loss > 0
+    assert any(not np.array_equal(lb.get_params(f), p0[f]) for f in range(6))
+    lb.close()
