@@ -549,8 +549,10 @@ def bench_learned(args, steps):
         if kind == "learned":
             lb.attach(sv, z)
         if timed:
-            # autotuned, then the median of three timed runs from rest
-            sv.autotune(3)
+            # autotuned (unless --no-autotune), then the median of three
+            # timed runs from rest
+            if not args.no_autotune:
+                sv.autotune(3)
             sv.run(10)
             ts = []
             for _ in range(3):

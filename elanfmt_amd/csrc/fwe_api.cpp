@@ -1903,6 +1903,7 @@ struct fwe_lb {
   float* Av = nullptr;    // Adam second moments
   float* grad = nullptr;  // np (one face at a time)
   int adam_t[6] = {0, 0, 0, 0, 0, 0};
+  int form = FWE_LB_DISSIPATIVE;  // how corrections are applied
   // cell lists of the last thin geometry trained on
   int gx = -1, gy = -1, gz = -1;
   LbCell* cells = nullptr;
@@ -2002,6 +2003,7 @@ static fwe::LbInfer lb_infer_args(const fwe_ctx* x, int from) {
   a.nbatch = x->cfg.nbatch;
   a.zinv = x->lb_zinv;
   a.D = x->lb_D;
+  a.free_form = x->lb->form == FWE_LB_FREE;
   a.trash = x->lb_D + (int64_t)x->lb_face_off[6] * fwe::LB_OUT * x->cfg.nbatch;
   return a;
 }
@@ -2122,6 +2124,25 @@ extern "C" fwe_status fwe_lb_create(int32_t device, int32_t layer, fwe_lb** out)
   fwe_status st = fwe_lb_init(lb, 1);
   if (st) { fwe_lb_destroy(lb); return st; }
   *out = lb;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_lb_set_correction_form(fwe_lb* lb, int32_t form) {
+  if (!lb || (form != FWE_LB_DISSIPATIVE && form != FWE_LB_FREE)) return fail(FWE_ERR_ARG, "bad correction form");
+  HIPCHK(hipSetDevice(lb->device));
+  // steps queued on attached contexts apply corrections in the old form;
+  // their captured time loops hold it as a launch argument
+  for (fwe_ctx* x : lb->attached) {
+    HIPCHK(hipStreamSynchronize(x->stream));
+    drop_graph(x);
+  }
+  lb->form = form;
+  return FWE_OK;
+}
+
+extern "C" fwe_status fwe_lb_get_correction_form(const fwe_lb* lb, int32_t* form) {
+  if (!lb || !form) return fail(FWE_ERR_ARG, "NULL argument");
+  *form = lb->form;
   return FWE_OK;
 }
 
@@ -2317,7 +2338,8 @@ extern "C" fwe_status fwe_lb_train_step(fwe_lb* lb, fwe_ctx* thick, fwe_ctx* thi
     }
   }
   // the thin run continues as it would with the boundary attached
-  HIPCHK(fwe::lb_launch_apply_y(lb_geom(thin, thin->cur, zinv), lb->cells, (int)ncell, lb->Y, lb->R, lb->stream));
+  HIPCHK(fwe::lb_launch_apply_y(lb_geom(thin, thin->cur, zinv), lb->cells, (int)ncell, lb->Y, lb->R,
+                                 lb->form == FWE_LB_FREE, lb->stream));
   if (lr > 0.f) HIPCHK(fwe::lb_launch_pack(lb->P, lb->np, lb->L, lb->pk, lb->stream));
   HIPCHK(hipStreamSynchronize(lb->stream));
   if (loss) *loss = lsum / (9.0 * (double)std::max<int64_t>(1, ncell));

@@ -249,6 +249,7 @@ struct LbInfer {
   int32_t nbatch;
   float zinv;
   float* D;               // corrections [nbatch][9][face_off[6]]
+  int32_t free_form;      // 1: fields + D; 0: their dissipative part
   float* trash;           // 64 floats, written by lanes that have no output
 };
 hipError_t lb_launch_pack(const float* P, int np, int L, uint8_t* pk, hipStream_t s);
@@ -258,7 +259,7 @@ hipError_t lb_launch_infer(LbInfer& a, hipStream_t s);
 // fields <- shrink(fields, D) after the step
 hipError_t lb_launch_add(const LbInfer& a, hipStream_t s);
 hipError_t lb_launch_apply_y(const LbGeom& g, const LbCell* cells, int n, const float* Y, const float* R,
-                             hipStream_t s);
+                             int free_form, hipStream_t s);
 hipError_t lb_launch_crop(const LbGeom& thick, const LbGeom& thin, int off, hipStream_t s);
 
 // the convolutional PML memory variables of one wavefield (StepArgs::psi

@@ -610,7 +610,8 @@ __global__ __launch_bounds__(256) void k_lb_infer(const LbInfer a) {
   }
 }
 
-// After the step: fields <- shrink(fields, D), every wavefield (blockIdx.y).
+// After the step: fields <- shrink(fields, D), or fields + D in the free
+// form, every wavefield (blockIdx.y).
 __global__ __launch_bounds__(256) void k_lb_add(const LbInfer a) {
   const int nc = a.face_off[6];
   const int s = blockIdx.x * 256 + threadIdx.x;
@@ -628,13 +629,14 @@ __global__ __launch_bounds__(256) void k_lb_add(const LbInfer a) {
     d[q] = D[(int64_t)q * nc];
   }
 #pragma unroll
-  for (int q = 0; q < LB_OUT; ++q) F[(int64_t)q * a.fstride] = lb_shrink(v[q], d[q]);
+  for (int q = 0; q < LB_OUT; ++q) F[(int64_t)q * a.fstride] = a.free_form ? v[q] + d[q] : lb_shrink(v[q], d[q]);
 }
 
 // Training rollout: the corrections predicted before the update, from Y
 // [n][9] and the stencil RMS R, applied as k_lb_infer + k_lb_add do.
 __global__ __launch_bounds__(256) void k_lb_apply_y(const LbGeom g, const LbCell* __restrict__ cells, int32_t n,
-                                                   const float* __restrict__ Y, const float* __restrict__ R) {
+                                                   const float* __restrict__ Y, const float* __restrict__ R,
+                                                   int32_t free_form) {
   const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (s >= n) return;
   const LbCell c = cells[s];
@@ -643,7 +645,8 @@ __global__ __launch_bounds__(256) void k_lb_apply_y(const LbGeom g, const LbCell
   for (int f = 0; f < LB_OUT; ++f)
     if (lb_comp_live(c, f)) {
       float* p = const_cast<float*>(g.F[f]) + a;
-      *p = lb_shrink(*p, Y[s * LB_OUT + f] * r * (f >= 3 ? z : 1.f));
+      const float d = Y[s * LB_OUT + f] * r * (f >= 3 ? z : 1.f);
+      *p = free_form ? *p + d : lb_shrink(*p, d);
     }
 }
 
@@ -790,9 +793,9 @@ hipError_t lb_launch_add(const LbInfer& a, hipStream_t s) {
 }
 
 hipError_t lb_launch_apply_y(const LbGeom& g, const LbCell* cells, int n, const float* Y, const float* R,
-                             hipStream_t s) {
+                             int free_form, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_lb_apply_y, dim3(lb_blocks(n, 256)), dim3(256), 0, s, g, cells, n, Y, R);
+  hipLaunchKernelGGL(k_lb_apply_y, dim3(lb_blocks(n, 256)), dim3(256), 0, s, g, cells, n, Y, R, free_form);
   return hipGetLastError();
 }
 

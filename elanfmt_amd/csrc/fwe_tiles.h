@@ -140,7 +140,10 @@ __device__ __forceinline__ CpmlCell cpml_cell(const StepArgs& a, int i, int j, i
 // ---------------------------------------------------------------------------
 template <int MODE, int PML, bool FS, int TX, int TY, int NB>
 __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb, const int wb) {
-  const int lane = threadIdx.x, row = threadIdx.y;
+  // a wavefront holds one tile row (64 lanes of a 64- or 128-column row), so
+  // the row is uniform per wavefront: saying so keeps row addressing in
+  // scalar registers
+  const int lane = threadIdx.x, row = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int tx = lb % a.ntx;
   lb /= a.ntx;
   const int tyi = lb % a.nty;
@@ -348,7 +351,10 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb, const int
 // ---------------------------------------------------------------------------
 template <int MODE, int PML, int TX, int TY, int NB>
 __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb, const int wb) {
-  const int lane = threadIdx.x, row = threadIdx.y;
+  // a wavefront holds one tile row (64 lanes of a 64- or 128-column row), so
+  // the row is uniform per wavefront: saying so keeps row addressing in
+  // scalar registers
+  const int lane = threadIdx.x, row = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int tx = lb % a.ntx;
   lb /= a.ntx;
   const int tyi = lb % a.nty;

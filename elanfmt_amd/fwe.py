@@ -23,6 +23,7 @@ MEDIUM_AUTO, MEDIUM_ARRAYS, MEDIUM_TABLE, MEDIUM_ISOTROPIC = 0, 1, 2, 3
 FS_NONE, FS_BC1, FS_VACUUM = 0, 1, 2
 MODE_ISO, MODE_ORTHO = 0, 1
 PML_NONE, PML_PROFILE, PML_WEIGHTS3D, PML_SEPARABLE, PML_CPML = 0, 1, 2, 3, 4
+LB_DISSIPATIVE, LB_FREE = 0, 1
 SXX, SYY, SZZ, SXY, SXZ, SYZ = range(6)
 F_VX, F_VY, F_VZ, F_SXX, F_SYY, F_SZZ, F_SXY, F_SXZ, F_SYZ = range(9)
 REC_VX, REC_VY, REC_VZ = range(3)
@@ -128,6 +129,8 @@ SIGNATURES = {
     "fwe_lb_create": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "fwe_lb_destroy": (C.c_int, [C.c_void_p]),
     "fwe_lb_init": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "fwe_lb_set_correction_form": (C.c_int, [C.c_void_p, C.c_int32]),
+    "fwe_lb_get_correction_form": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "fwe_lb_set_params": (C.c_int, [C.c_void_p, C.c_int32, _FP]),
     "fwe_lb_get_params": (C.c_int, [C.c_void_p, C.c_int32, _FP]),
     "fwe_lb_eval": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _FP, _IP, _FP, _FP, _FP, _FP, _DP]),
@@ -612,6 +615,15 @@ class LearnedBoundary:
         _check(lib().fwe_lb_eval(self._h, face, n, fptr(x), d.ctypes.data_as(_IP), fptr(t), fptr(m), fptr(y),
                                  fptr(g), C.byref(loss)))
         return y, loss.value, g
+
+    def set_correction_form(self, form):
+        """LB_DISSIPATIVE (the default) or LB_FREE."""
+        _check(lib().fwe_lb_set_correction_form(self._h, form))
+
+    def correction_form(self):
+        f = C.c_int32()
+        _check(lib().fwe_lb_get_correction_form(self._h, C.byref(f)))
+        return f.value
 
     def train_step(self, thick, thin, offset, impedance, lr, resync=True):
         loss = C.c_double()

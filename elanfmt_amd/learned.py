@@ -303,11 +303,14 @@ def save(lb: fwe.LearnedBoundary, path, **meta):
 
 
 def load(path, device=0):
-    """A LearnedBoundary with the parameters saved by `save`, and its metadata."""
+    """A LearnedBoundary with the parameters saved by `save` (and the
+    correction form recorded there), and its metadata."""
     import json
 
     with np.load(path, allow_pickle=False) as d:
         lb = fwe.LearnedBoundary(int(d["layer"]), device=device)
         for f in range(6):
             lb.set_params(f, d["params"][f])
-        return lb, json.loads(str(d["meta"]))
+        meta = json.loads(str(d["meta"]))
+        lb.set_correction_form(int(meta.get("form", fwe.LB_DISSIPATIVE)))
+        return lb, meta

@@ -36,6 +36,8 @@ def main():
                     help="the thin layer's own damping: the reference's profile, or the convolutional PML")
     ap.add_argument("--extend", action="store_true",
                     help="media generated on the interior and continued outward (always with --base cpml)")
+    ap.add_argument("--form", default="dissipative", choices=["dissipative", "free"],
+                    help="corrections applied in their dissipative part only, or as predicted")
     ap.add_argument("--eval-sizes", default="80:500:5,120:700:4", help="interior:steps:f0 of the larger evaluations")
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
@@ -46,6 +48,8 @@ def main():
         lb, _ = learned.load(a.init)
     else:
         lb = fwe.LearnedBoundary(a.layer)
+    form = fwe.LB_FREE if a.form == "free" else fwe.LB_DISSIPATIVE
+    lb.set_correction_form(form)
     t0 = time.time()
     sizes = [int(v) for v in a.sizes.split(",")]
     f0_range = tuple(float(v) for v in a.f0.split(","))
@@ -66,7 +70,8 @@ def main():
     learned.save(lb, a.out, runs=a.runs, layer=a.layer, lr=a.lr, resync_every=a.resync, R_thin=a.r_thin,
                  seed0=a.seed0, sizes=sizes, f0=list(f0_range), train_seconds=round(t_train, 1), steps=log.steps,
                  loss_curve=[(s, float(f"{l:.4e}")) for s, l in log.losses], evaluation=evals, base=a.base,
-                 extend=bool(a.extend or a.base == "cpml"), cpml=learned.CPML_THIN if a.base == "cpml" else None)
+                 extend=bool(a.extend or a.base == "cpml"), cpml=learned.CPML_THIN if a.base == "cpml" else None,
+                 form=form)
     print(json.dumps({"out": a.out, "evaluation": evals}), flush=True)
     lb.close()
 

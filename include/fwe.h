@@ -410,6 +410,14 @@ fwe_status fwe_lb_create(int32_t device, int32_t layer, fwe_lb** out);
 fwe_status fwe_lb_destroy(fwe_lb* lb);
 /* He-normal hidden layers, zero output layer (no correction); resets Adam */
 fwe_status fwe_lb_init(fwe_lb* lb, uint64_t seed);
+/* How corrections reach the fields after each step: DISSIPATIVE (the
+ * default) keeps only the part that moves a value toward zero, by at most
+ * its own magnitude; FREE adds them as predicted (for a layer whose own
+ * damping is already absorbing, the convolutional PML).  Waits for the steps
+ * queued on attached contexts. */
+enum { FWE_LB_DISSIPATIVE = 0, FWE_LB_FREE = 1 };
+fwe_status fwe_lb_set_correction_form(fwe_lb* lb, int32_t form);
+fwe_status fwe_lb_get_correction_form(const fwe_lb* lb, int32_t* form);
 fwe_status fwe_lb_set_params(fwe_lb* lb, int32_t face, const float* params);
 fwe_status fwe_lb_get_params(fwe_lb* lb, int32_t face, float* params);
 /* Network of one face on n explicit samples (host arrays): x [n][81],

@@ -45,8 +45,9 @@ def test_network_forward_and_gradient_match_cpu(gpu):
     lb.close()
 
 
-@pytest.mark.parametrize("dims,L", [((22, 20, 18), 3), ((26, 17, 9), 5), ((11, 23, 14), 6), ((31, 12, 12), 6)])
-def test_attached_boundary_matches_oracle_plus_cpu_corrections(gpu, dims, L):
+@pytest.mark.parametrize("dims,L,form", [((22, 20, 18), 3, 0), ((26, 17, 9), 5, 0), ((11, 23, 14), 6, 0),
+                                         ((31, 12, 12), 6, 0), ((22, 20, 18), 3, 1)])
+def test_attached_boundary_matches_oracle_plus_cpu_corrections(gpu, dims, L, form):
     """Every step: corrections from the fields before the step, computed by
     the CPU network on the CPU stencil features, applied (dissipative part)
     after the oracle step; the HIP path must follow within FP32 round-off.
@@ -61,7 +62,8 @@ def test_attached_boundary_matches_oracle_plus_cpu_corrections(gpu, dims, L):
     FP64 reference by up to 5e-4 at a few cells.  Over four steps that
     reaches 4e-4 of the fields, so grids with no interior along some axis
     are held to 1e-3, which a wrong weight, stencil point or depth (errors
-    of a percent or more in the corrections) still exceeds."""
+    of a percent or more in the corrections) still exceeds.  form 1: the
+    free form, corrections added as predicted."""
     from elanfmt_amd import fwe, workloads as W
     from oracle import Oracle, lb_ref
 
@@ -76,6 +78,8 @@ def test_attached_boundary_matches_oracle_plus_cpu_corrections(gpu, dims, L):
     lb = fwe.LearnedBoundary(L)
     for f in range(6):
         lb.set_params(f, params[f])
+    lb.set_correction_form(form)
+    assert lb.correction_form() == form
     s = W.make_solver(wl)
     o = Oracle(wl.nx, wl.ny, wl.nz, wl.dx, wl.dy, wl.dz, wl.dt, wl.nt, mode=wl.mode, pml=wl.pml)
     W.apply(o, wl)
@@ -94,7 +98,7 @@ def test_attached_boundary_matches_oracle_plus_cpu_corrections(gpu, dims, L):
         new = [o.get_field(f) for f in range(9)]
         for i, j, k, comp, v in corr:
             old = new[comp][i, j, k]
-            new[comp][i, j, k] = lb_ref.shrink(old, v)
+            new[comp][i, j, k] = lb_ref.shrink(old, v) if form == 0 else old + v
             cut += new[comp][i, j, k] != old + v
         for f in range(9):
             o.set_field(f, new[f])
@@ -102,8 +106,9 @@ def test_attached_boundary_matches_oracle_plus_cpu_corrections(gpu, dims, L):
         errs = [rel_l2(s.get_field(f), new[f]) for f in range(9)]
         print(f"step {step}: max field rel-L2 {max(errs):.2e}", flush=True)
         assert max(errs) < (1e-4 if min(nx, ny, nz) > 2 * L else 1e-3), errs
-    # corrections were not negligible, and some were cut to their dissipative part
-    assert max(abs(v) for *_, v in corr) > 0 and cut > 0
+    # corrections were not negligible, and (dissipative form) some were cut
+    # to their dissipative part
+    assert max(abs(v) for *_, v in corr) > 0 and (cut > 0) == (form == 0)
     lb.detach(s)
     s.close()
     o.close()
