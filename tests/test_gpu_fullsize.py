@@ -212,3 +212,72 @@ def test_learned_corrections_only_shrink_values_at_512x512x256(gpu):
         moved += int(np.count_nonzero(a != b))
     print(f"c4 shrink: {moved} layer values corrected", flush=True)
     assert moved > 0
+
+
+@pytest.mark.timeout(900)
+def test_cpml_thin_layer_at_512x512x256(gpu):
+    """The 6-cell convolutional PML on the full 512x512x256 grid (interior
+    500x500x244, medium generated on the interior and continued outward):
+    the six elementary wavefields batched equal their single runs bit for
+    bit (0, 3, 5) and under graph replay; over 1200 steps the 128 receivers
+    10 cells inside the top layer stay within 3e-3 of the 2x-padded
+    reference (1024x1024x512, 32-cell CPML), where the plain 24-cell layer
+    over the same interior is off by more than 0.02."""
+    from elanfmt_amd import learned as Lr, workloads as W
+
+    L, inner, thin, ref = 6, (500, 500, 244), (512, 512, 256), (1024, 1024, 512)
+    f0, nt = 3.0, 1200
+    print("c4 cpml: medium on the host", flush=True)
+    med = Lr.extended_medium(inner, ref, 4242)
+    dt = W.cfl_dt(med["vmax"], Lr.DX, Lr.DX, Lr.DX, 0.5)
+    m, _, _ = W.mt_lune_random(4242, 1e15)
+    xs = np.linspace(L + 10, thin[0] - L - 11, 8).round().astype(int).tolist()
+    ys = np.linspace(L + 10, thin[1] - L - 11, 16).round().astype(int).tolist()
+    rec_thin = [(c, x, y, L + 10, 0) for y in ys for x in xs for c in range(3)]
+    src_thin = (thin[0] // 2 - 1, thin[1] // 2 - 1, thin[2] // 2 - 1)
+
+    def wl_for(g, damp, kind, nsteps, nbatch=1, src0=src_thin, rec0=rec_thin):
+        o = tuple((r - q) // 2 for r, q in zip(ref, g))
+        sh = tuple((q - t) // 2 for q, t in zip(g, thin))
+        rec = [(c, i + sh[0], j + sh[1], k + sh[2], 0) for (c, i, j, k, b) in rec0]
+        src = tuple(a + b for a, b in zip(src0, sh))
+        mm = m if nbatch == 1 else [np.eye(6)[e] * 1e15 for e in range(nbatch)]
+        wl = Lr.workload(f"g{g}", g, damp, Lr.crop(med, o, g), dt, src, mm, f0, nsteps, rec, nbatch=nbatch)
+        if kind == "cpml":
+            cp = Lr.CPML_THIN if g != ref else dict(Lr.CPML_REF, alpha_max=np.pi * f0)
+            W.use_cpml(wl, damp, vmax=med["vmax"], **cp)
+        return wl
+
+    def traces(wl, graph=0):
+        s = W.make_solver(wl)
+        s.set_graph_steps(graph)
+        s.run(wl.nt)
+        t = s.traces().astype(np.float64)
+        s.close()
+        return t
+
+    # the batch: a source 6 cells inside the x- layer and receivers in and
+    # beside the layer, so the memory variables are at work within the run
+    steps = 60
+    src_b = (12, 256, 128)
+    rec_b = [(c, i, 256 + dj, 128 + dk, 0) for c in range(3) for i in (3, 7, 9) for (dj, dk) in ((0, 0), (4, -3))]
+    print(f"c4 cpml: 512x512x256 x6 wavefields, {steps} steps, direct and graph replay", flush=True)
+    wb = wl_for(thin, L, "cpml", steps, nbatch=6, src0=src_b, rec0=rec_b)
+    batch = traces(wb)
+    assert np.isfinite(batch).all()
+    assert np.array_equal(traces(wb, graph=16), batch)
+    nr = len(batch) // 6
+    for b in (0, 3, 5):
+        w1 = wl_for(thin, L, "cpml", steps, src0=src_b, rec0=rec_b)
+        w1.injections = [(q, i, j, k, 0, c) for (q, i, j, k, bb, c) in wb.injections if bb == b]
+        w1.stf = [wb.stf[b]]
+        single = traces(w1)
+        assert np.abs(single[[0, 1, 6, 7]]).max() > 0, b  # receivers inside the layer (i = 3) are reached
+        assert np.array_equal(single, batch[b * nr:(b + 1) * nr]), b
+    print(f"c4 cpml: reference {ref} and thin layers, {nt} steps", flush=True)
+    r = traces(wl_for(ref, 32, "cpml", nt))
+    e_cpml = rel_l2(traces(wl_for(thin, L, "cpml", nt)), r)
+    e_p24 = rel_l2(traces(wl_for(tuple(q + 48 for q in inner), 24, "plain", nt)), r)
+    print(f"c4 cpml: errors cpml6 {e_cpml:.2e} plain24 {e_p24:.2e}", flush=True)
+    assert np.abs(r).max() > 0
+    assert e_cpml < 3e-3 and e_p24 > 0.02
