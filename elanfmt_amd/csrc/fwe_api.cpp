@@ -100,6 +100,7 @@ struct fwe_ctx {
   // of n+1 each, x then y then z) and the memory variables (pstride floats
   // per wavefield: x slab, y slab, z slab, six variables each)
   int cpml_L = 0;
+  double vmax = 0;  // sqrt(largest normal modulus x largest buoyancy) of the medium
   float* ctab = nullptr;
   float* psi = nullptr;
   int64_t pvar[3] = {0, 0, 0};
@@ -374,8 +375,20 @@ extern "C" fwe_status fwe_set_material_iso(fwe_ctx* x, const float* c11, const f
   x->bxdt = (float)((double)bx * dt);
   x->bydt = (float)((double)by * dt);
   x->bzdt = (float)((double)bz * dt);
+  {
+    double cmax = 0;
+    for (int64_t q = 0; q < d[0] * d[1] * d[2]; ++q) cmax = std::max(cmax, (double)c11[q]);
+    x->vmax = sqrt(cmax * std::max({(double)bx, (double)by, (double)bz}));
+  }
   x->have_medium = true;
   return build_material_table(x, h);
+}
+
+extern "C" fwe_status fwe_get_medium_vmax(const fwe_ctx* x, double* vmax) {
+  if (!x || !vmax) return fail(FWE_ERR_ARG, "NULL argument");
+  if (!x->have_medium) return fail(FWE_ERR_STATE, "medium not set");
+  *vmax = x->vmax;
+  return FWE_OK;
 }
 
 extern "C" fwe_status fwe_set_material_ortho(fwe_ctx* x, const float* const cij[9],
@@ -388,6 +401,18 @@ extern "C" fwe_status fwe_set_material_ortho(fwe_ctx* x, const float* const cij[
   static const int shape_of[12] = {FWE_F_SXX, FWE_F_SXX, FWE_F_SXX, FWE_F_SXX, FWE_F_SXX, FWE_F_SXX,
                                    FWE_F_SYZ, FWE_F_SXZ, FWE_F_SXY, FWE_F_VX,  FWE_F_VY,  FWE_F_VZ};
   drop_graph(x);
+  {
+    double cmax = 0, bmax = 0;
+    const size_t nc = (size_t)x->cfg.nx * x->cfg.ny * x->cfg.nz;
+    for (int c = 0; c < 3; ++c)
+      for (size_t q = 0; q < nc; ++q) cmax = std::max(cmax, (double)cij[c][q]);
+    for (int c = 0; c < 3; ++c) {
+      int64_t d[3];
+      field_shape(x->cfg, c, d);
+      for (int64_t q = 0; q < d[0] * d[1] * d[2]; ++q) bmax = std::max(bmax, (double)buoy[c][q]);
+    }
+    x->vmax = sqrt(cmax * bmax);
+  }
   std::vector<std::vector<float>> h(12);
   for (int c = 0; c < 12; ++c) {
     int64_t d[3];

@@ -22,7 +22,11 @@
  * fields; traces before the restart step stay zero), tune=<n> (autotune the
  * tiling with n-step trials before the run, default 3; 0 keeps the default
  * tiling), graph=<g> (replay g-step HIP graphs instead of launching every
- * kernel, default 0).
+ * kernel, default 0), cpml=<L> (a convolutional PML of L cells on every face
+ * but a free-surface top instead of the weight files: no reference
+ * counterpart; coefficients from fwe_gen_cpml_table with R = 1e-4, power 2,
+ * no kappa or frequency shift, sized by the medium's fastest speed; its
+ * memory variables join the snapshots as <s>_0_cpml.res).
  * Outputs: 0_infos.inf and 0_0_{Src,Rec,Rec_x,Rec_y,Rec_y2,Vx,Vy,Vz}.res.
  */
 #include <math.h>
@@ -32,6 +36,7 @@
 #include <time.h>
 
 #include "../../include/fwe.h"
+#include "../../include/fwe_gen.h"
 
 #define MAXREC 1024
 
@@ -40,7 +45,7 @@ static int usage(void) {
           "usage: fwe_run [NBX=.. NBY=.. NBZ=.. OVERX=.. OVERY=.. OVERZ=..] [nx= ny= nz=] [nt=]\n"
           "               [mode=iso|ortho|isohet] [pml=profile|weights|taper|none] [fs=0|1|2]\n"
           "               [xsrc= ysrc= zsrc=] [rec=x:i:j:k:name,...] [dir=.] [device=0] [nout=200]\n"
-          "               [snap=<steps>] [restart=<snapshot>] [tune=3] [graph=0]\n");
+          "               [snap=<steps>] [restart=<snapshot>] [tune=3] [graph=0] [cpml=<cells>]\n");
   return 2;
 }
 
@@ -94,6 +99,18 @@ static int write_snapshot(fwe_ctx* ctx, const char* dir, int s, const fwe_config
     free(h);
     if (!ok) { fprintf(stderr, "fwe_run: cannot write snapshot %d\n", s); return 1; }
   }
+  if (c->pml == FWE_PML_CPML) {  /* the memory variables are part of the state */
+    int64_t n = 0;
+    if (fwe_cpml_state_size(ctx, &n) != FWE_OK) { fprintf(stderr, "fwe_run: %s\n", fwe_last_error()); return 1; }
+    float* h = (float*)malloc(sizeof(float) * (size_t)n);
+    if (!h) { fprintf(stderr, "fwe_run: out of host memory\n"); return 1; }
+    if (fwe_get_cpml_state(ctx, 0, h) != FWE_OK) { fprintf(stderr, "fwe_run: %s\n", fwe_last_error()); free(h); return 1; }
+    FILE* fc = fopen(snap_path(dir, s, "cpml", "res"), "wb");
+    const int ok = fc && fwrite(h, sizeof(float), (size_t)n, fc) == (size_t)n;
+    if (fc) fclose(fc);
+    free(h);
+    if (!ok) { fprintf(stderr, "fwe_run: cannot write snapshot %d\n", s); return 1; }
+  }
   FILE* fp = fopen(snap_path(dir, s, "step", "inf"), "w");
   if (!fp) { fprintf(stderr, "fwe_run: cannot write snapshot %d\n", s); return 1; }
   fprintf(fp, "%d\n", step);
@@ -121,13 +138,26 @@ static int read_snapshot(fwe_ctx* ctx, const char* dir, int s, const fwe_config*
     free(h);
     if (st != FWE_OK) { fprintf(stderr, "fwe_run: %s\n", fwe_last_error()); return 1; }
   }
+  if (c->pml == FWE_PML_CPML) {
+    int64_t n = 0;
+    if (fwe_cpml_state_size(ctx, &n) != FWE_OK) { fprintf(stderr, "fwe_run: %s\n", fwe_last_error()); return 1; }
+    float* h = (float*)malloc(sizeof(float) * (size_t)n);
+    if (!h) { fprintf(stderr, "fwe_run: out of host memory\n"); return 1; }
+    fp = fopen(snap_path(dir, s, "cpml", "res"), "rb");
+    const int ok = fp && fread(h, sizeof(float), (size_t)n, fp) == (size_t)n;
+    if (fp) fclose(fp);
+    if (!ok) { fprintf(stderr, "fwe_run: snapshot %d: missing or short %s\n", s, snap_path(dir, s, "cpml", "res")); free(h); return 1; }
+    const fwe_status st = fwe_set_cpml_state(ctx, 0, h);
+    free(h);
+    if (st != FWE_OK) { fprintf(stderr, "fwe_run: %s\n", fwe_last_error()); return 1; }
+  }
   if (fwe_set_step(ctx, *step) != FWE_OK) { fprintf(stderr, "fwe_run: %s\n", fwe_last_error()); return 1; }
   return 0;
 }
 
 int main(int argc, char** argv) {
   int nbx = -1, nby = -1, nbz = -1, ovx = 0, ovy = 0, ovz = 0, nx = -1, ny = -1, nz = -1;
-  int nt = -1, mode = FWE_MODE_ISO, pml = -1, fs = 1, device = 0, nout = 200, snap = 0, restart = -1;
+  int nt = -1, mode = FWE_MODE_ISO, pml = -1, fs = 1, device = 0, nout = 200, snap = 0, restart = -1, cpml = 0;
   int tune = 3, graph = 0;
   int xsrc = -1, ysrc = -1, zsrc = -1;
   int npars[3] = {-1, -1, -1};
@@ -170,6 +200,7 @@ int main(int argc, char** argv) {
     else if (KEY("restart")) restart = atoi(v);
     else if (KEY("tune")) tune = atoi(v);
     else if (KEY("graph")) graph = atoi(v);
+    else if (KEY("cpml")) cpml = atoi(v);
     else { fprintf(stderr, "fwe_run: unknown key in '%s'\n", argv[a]); return usage(); }
 #undef KEY
   }
@@ -193,9 +224,28 @@ int main(int argc, char** argv) {
   cfg.free_surface = fs;
   cfg.device = device;
   if (pml >= 0) cfg.pml = pml;
+  if (cpml > 0) cfg.pml = FWE_PML_CPML;
   fwe_ctx* ctx = NULL;
   CHECK(fwe_create(&cfg, &ctx));
   CHECK(fwe_load_legacy_inputs(ctx, dir, &p));
+  if (cpml > 0) {
+    /* thin convolutional PML on every face but a free-surface top */
+    double vmax = 0;
+    CHECK(fwe_get_medium_vmax(ctx, &vmax));
+    const int n[3] = {nx, ny, nz};
+    const double h[3] = {cfg.dx, cfg.dy, cfg.dz};
+    float* t[3];
+    for (int a = 0; a < 3; ++a) {
+      t[a] = (float*)malloc(sizeof(float) * 6 * (size_t)(n[a] + 1));
+      if (!t[a] || fwe_gen_cpml_table(n[a], cpml, h[a], cfg.dt, vmax, 1e-4, 1.0, 0.0, 2.0,
+                                      a == 2 && fs != FWE_FS_NONE ? 0 : 1, 1, t[a]) != 0) {
+        fprintf(stderr, "fwe_run: cpml=%d does not fit the grid (2 x cells <= nx, ny, nz)\n", cpml);
+        return 1;
+      }
+    }
+    CHECK(fwe_set_cpml(ctx, cpml, t[0], t[1], t[2]));
+    for (int a = 0; a < 3; ++a) free(t[a]);
+  }
 
   /* source: OK4.cu:450-452 (234, round(Ly/2/dy), 100) and MTS.dat;
    * ANISO.cu:483-485 (180, round(Ly/2/dy), 200) with the coefficients of
@@ -262,6 +312,7 @@ int main(int argc, char** argv) {
   printf("fwe_run: %s law, %dx%dx%d cells, %d steps, dt=%.6g, free surface %s (%s)\n",
          mode == FWE_MODE_ISO ? "iso" : "ortho", nx, ny, nz, nt, cfg.dt,
          fs == FWE_FS_BC1 ? "BC1" : (fs == FWE_FS_VACUUM ? "vacuum layer" : "off"), fwe_version());
+  if (cpml > 0) printf("fwe_run: %d-cell convolutional PML instead of the weight files\n", cpml);
   /* tiling: autotuned (the context is reset afterwards, before any restart) */
   if (tune > 0) CHECK(fwe_autotune(ctx, tune, NULL));
   if (graph > 0) CHECK(fwe_set_graph_steps(ctx, graph));

@@ -86,6 +86,7 @@ SIGNATURES = {
     "fwe_set_pml_profiles": (C.c_int, [C.c_void_p, _FP, _FP, _FP]),
     "fwe_set_pml_weights": (C.c_int, [C.c_void_p, _FP]),
     "fwe_set_cpml": (C.c_int, [C.c_void_p, C.c_int32, _FP, _FP, _FP]),
+    "fwe_get_medium_vmax": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "fwe_cpml_state_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "fwe_get_cpml_state": (C.c_int, [C.c_void_p, C.c_int32, _FP]),
     "fwe_set_cpml_state": (C.c_int, [C.c_void_p, C.c_int32, _FP]),
@@ -342,6 +343,12 @@ class Solver:
             if a.size != 6 * (n + 1):
                 raise ValueError("a CPML table holds 6 x (n+1) floats")
         _check(lib().fwe_set_cpml(self._h, layer, *[fptr(a) for a in t]))
+
+    def medium_vmax(self):
+        """sqrt(largest normal modulus x largest buoyancy) of the medium."""
+        v = C.c_double(0)
+        _check(lib().fwe_get_medium_vmax(self._h, C.byref(v)))
+        return v.value
 
     def cpml_state(self, batch=0):
         """Memory variables of one wavefield (flat, floats)."""

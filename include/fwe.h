@@ -184,6 +184,11 @@ fwe_status fwe_set_material_iso(fwe_ctx* ctx, const float* c11, const float* c12
  * buoy[0..2] = vrho_{x,y,z}_11_hc (staggered with Vx, Vy, Vz). */
 fwe_status fwe_set_material_ortho(fwe_ctx* ctx, const float* const cij[9],
                                   const float* const buoy[3]);
+/* sqrt(largest normal modulus x largest buoyancy) of the medium set last:
+ * a bound of its fastest (P) wave speed, for sizing damping profiles
+ * (fwe_gen_cpml_table's vmax).  No reference counterpart (the reference's
+ * driver computes its profiles from its own velocity model, OK4.m:297-341). */
+fwe_status fwe_get_medium_vmax(const fwe_ctx* ctx, double* vmax);
 /* PROFILE: weights_{x,y,z}.dat, lengths nx+1, ny+1, nz+1 (OK4.cu:419-424).
  * SEPARABLE: cell-centred tapers of lengths nx, ny, nz. */
 fwe_status fwe_set_pml_profiles(fwe_ctx* ctx, const float* wx, const float* wy,

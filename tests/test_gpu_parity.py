@@ -717,3 +717,62 @@ def test_eight_wavefield_iso_batch_equals_single_runs(gpu):
     n0 = len(rec0)
     for b in range(8):
         assert np.array_equal(tb[b * n0:(b + 1) * n0], singles[b]), b
+
+
+def test_legacy_binary_with_a_thin_cpml(gpu, tmp_path):
+    """fwe_run cpml=6: the legacy input set with a 6-cell convolutional PML
+    (sized by the medium's fastest speed, no damping on the BC1 top) in
+    place of the weight files, against the oracle's CPML with the same
+    coefficients; snapshots carry the memory variables, so a restart
+    continues bit for bit, and a snapshot without them is refused."""
+    import shutil
+    import subprocess
+
+    from elanfmt_amd import fwe, legacy, workloads as W
+    from oracle import Oracle
+
+    n, nt, L = 40, 80, 6
+    wl = W.iso_homogeneous(n=n, L=6, nt=nt, free_surface=1)
+    m = np.array(wl.info["mt"], np.float32)
+    a, b = tmp_path / "a", tmp_path / "b"
+    for d in (a, b):
+        d.mkdir()
+        legacy.write_inputs(str(d), wl, mts=m)
+    xs = 18
+    wl.injections = fwe.mt_injections(fwe.SRC_LAYOUT_OK4, xs, xs, xs, m)
+    recs = [(2, 25, 21, 1, 0), (0, 36, 21, 1, 0), (1, 25, 38, 1, 0), (1, 2, 23, 30, 0)]
+    wl.receivers = recs
+    spec = ",".join(f"{'xyz'[c]}:{i}:{j}:{k}:{nm}" for (c, i, j, k, _), nm in
+                    zip(recs, ["Rec", "Rec_x", "Rec_y", "Rec_y2"]))
+    args = [fwe.RUN_PATH, f"nx={n}", f"ny={n}", f"nz={n}", f"nt={nt}", "mode=iso", "fs=1", f"xsrc={xs}",
+            f"ysrc={xs}", f"zsrc={xs}", f"rec={spec}", "nout=0", f"cpml={L}"]
+    r = subprocess.run(args + ["snap=40"], cwd=str(a), capture_output=True, text=True, timeout=120)
+    print(r.stdout[-400:], r.stderr[-300:], flush=True)
+    assert r.returncode == 0 and "convolutional PML" in r.stdout
+    dt32 = float(np.float32(wl.dt))
+    vmax = float(np.sqrt(np.float64(np.max(wl.c11)) * np.float64(max(np.float32(v) for v in wl.bscal))))
+    t = [W.cpml_table(n, L, wl.dx, dt32, vmax, R=1e-4, kappa_max=1.0, alpha_max=0.0, power=2.0, lo=lo)
+         for lo in (1, 1, 0)]
+    o = Oracle(n, n, n, wl.dx, wl.dy, wl.dz, dt32, nt, mode=0, pml=fwe.PML_CPML, free_surface=1)
+    wl.pml = fwe.PML_CPML
+    wl.cpml_layer, wl.cpml = L, tuple(t)
+    W.apply(o, wl)
+    o.run(nt)
+    got = np.stack([legacy.read_res(str(a), nm) for nm in ["Rec", "Rec_x", "Rec_y", "Rec_y2"]])
+    assert np.abs(o.traces()).max() > 0
+    assert rel_l2(got, o.traces()) < TOL
+    assert rel_l2(legacy.read_res(str(a), "Vx", (n + 1, n, n)), o.get_field(0)) < TOL
+    assert (a / "1_0_cpml.res").exists()
+    for name in ["Vx", "Vy", "Vz", "Sxx", "Syy", "Szz", "Sxy", "Sxz", "Syz", "step"]:
+        ext = "inf" if name == "step" else "res"
+        shutil.copy(a / f"1_0_{name}.{ext}", b / f"1_0_{name}.{ext}")
+    r = subprocess.run(args + ["restart=1"], cwd=str(b), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "cpml" in r.stderr  # the memory variables are missing
+    shutil.copy(a / "1_0_cpml.res", b / "1_0_cpml.res")
+    r = subprocess.run(args + ["restart=1"], cwd=str(b), capture_output=True, text=True, timeout=120)
+    print(r.stdout[-300:], r.stderr[-300:], flush=True)
+    assert r.returncode == 0
+    for nm in ["Rec", "Rec_x", "Rec_y", "Rec_y2"]:
+        ta, tb = legacy.read_res(str(a), nm), legacy.read_res(str(b), nm)
+        assert np.array_equal(ta[40:], tb[40:]), nm
+    assert np.array_equal(legacy.read_res(str(a), "Vx", (n + 1, n, n)), legacy.read_res(str(b), "Vx", (n + 1, n, n)))
