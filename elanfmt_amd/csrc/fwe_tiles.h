@@ -107,6 +107,23 @@ __device__ __forceinline__ float cpml_g(float g, float* p, const float* t, int m
   return fmaf(g, t[2 * m + q], ps);
 }
 
+// The same with the coefficients already in registers.
+__device__ __forceinline__ float cpml_v(float g, float* p, float b, float a, float kinv) {
+  const float ps = fmaf(b, *p, a * g);
+  *p = ps;
+  return fmaf(g, kinv, ps);
+}
+
+// Coefficients of one axis index: b, a, 1/kappa at the integer node, then at
+// the half node.  A lane's x index and a wavefront's row are fixed for its
+// whole z-march, so those are loaded once, before the march.
+struct CpmlCoef {
+  float bi, ai, ki, bh, ah, kh;
+};
+__device__ __forceinline__ CpmlCoef cpml_coef(const float* t, int m, int q) {
+  return CpmlCoef{t[q], t[m + q], t[2 * m + q], t[3 * m + q], t[4 * m + q], t[5 * m + q]};
+}
+
 // Slab membership and element offsets of cell (i,j,k) for the convolutional
 // PML (see StepArgs::psi).
 struct CpmlCell {
@@ -160,6 +177,11 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb, const int
 
   float fxh = 1.f, fyh = 1.f, wxi = 1.f, wyj = 1.f;
   if (PML == PML_PROFILE) { fxh = a.whx[i]; fyh = a.why[j]; }
+  CpmlCoef qx{}, qy{};  // convolutional PML: this lane's x and this row's y coefficients
+  if (PML == PML_CPML) {
+    if (i < a.cL || i >= a.nx - a.cL) qx = cpml_coef(a.ctab[0], a.nx + 1, i);
+    if (j < a.cL || j >= a.ny - a.cL) qy = cpml_coef(a.ctab[1], a.ny + 1, j);
+  }
   if (PML == PML_PROFILE || PML == PML_SEP) { wxi = a.wx[i]; wyj = a.wy[j]; }
 
   // carried z-neighbours, per wavefield: Vx(k-1), Vy(k-1), Vz(k)
@@ -247,17 +269,15 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb, const int
           const int64_t po = (int64_t)(wb + b) * a.pstride;
           if (cc.x) {
             float* p = a.psi[0] + po + cc.ox;
-            const int m = a.nx + 1;
-            gxx = cpml_g(gxx, p, a.ctab[0], m, i);
-            gyx = cpml_g(gyx, p + a.pvar[0], a.ctab[0] + 3 * m, m, i);
-            gzx = cpml_g(gzx, p + 2 * a.pvar[0], a.ctab[0] + 3 * m, m, i);
+            gxx = cpml_v(gxx, p, qx.bi, qx.ai, qx.ki);
+            gyx = cpml_v(gyx, p + a.pvar[0], qx.bh, qx.ah, qx.kh);
+            gzx = cpml_v(gzx, p + 2 * a.pvar[0], qx.bh, qx.ah, qx.kh);
           }
           if (cc.y) {
             float* p = a.psi[1] + po + cc.oy;
-            const int m = a.ny + 1;
-            gyy = cpml_g(gyy, p, a.ctab[1], m, j);
-            gxy = cpml_g(gxy, p + a.pvar[1], a.ctab[1] + 3 * m, m, j);
-            gzy = cpml_g(gzy, p + 2 * a.pvar[1], a.ctab[1] + 3 * m, m, j);
+            gyy = cpml_v(gyy, p, qy.bi, qy.ai, qy.ki);
+            gxy = cpml_v(gxy, p + a.pvar[1], qy.bh, qy.ah, qy.kh);
+            gzy = cpml_v(gzy, p + 2 * a.pvar[1], qy.bh, qy.ah, qy.kh);
           }
           if (cc.z) {
             float* p = a.psi[2] + po + cc.oz;
@@ -373,6 +393,11 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb, const i
   const bool uz_ij = xin && i <= nx - 2 && j <= ny - 2;
 
   float wxm = 1.f, wym = 1.f, txi = 1.f, tyj = 1.f;
+  CpmlCoef qx{}, qy{};  // convolutional PML: this lane's x and this row's y coefficients
+  if (PML == PML_CPML) {
+    if (i < a.cL || i >= nx - a.cL) qx = cpml_coef(a.ctab[0], nx + 1, i);
+    if (j < a.cL || j >= ny - a.cL) qy = cpml_coef(a.ctab[1], ny + 1, j);
+  }
   if (PML == PML_PROFILE) {
     wxm = im ? a.wx[i - 1] : 1.f;
     wym = jm ? a.wy[j - 1] : 1.f;
@@ -448,17 +473,15 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb, const i
           const int64_t po = (int64_t)(wb + b) * a.pstride;
           if (cc.x) {
             float* p = a.psi[0] + po + cc.ox + 3 * a.pvar[0];
-            const int m = nx + 1;
-            hxx = cpml_g(hxx, p, a.ctab[0] + 3 * m, m, i);
-            hxyx = cpml_g(hxyx, p + a.pvar[0], a.ctab[0], m, i);
-            hxzx = cpml_g(hxzx, p + 2 * a.pvar[0], a.ctab[0], m, i);
+            hxx = cpml_v(hxx, p, qx.bh, qx.ah, qx.kh);
+            hxyx = cpml_v(hxyx, p + a.pvar[0], qx.bi, qx.ai, qx.ki);
+            hxzx = cpml_v(hxzx, p + 2 * a.pvar[0], qx.bi, qx.ai, qx.ki);
           }
           if (cc.y) {
             float* p = a.psi[1] + po + cc.oy + 3 * a.pvar[1];
-            const int m = ny + 1;
-            hyy = cpml_g(hyy, p, a.ctab[1] + 3 * m, m, j);
-            hxyy = cpml_g(hxyy, p + a.pvar[1], a.ctab[1], m, j);
-            hyzy = cpml_g(hyzy, p + 2 * a.pvar[1], a.ctab[1], m, j);
+            hyy = cpml_v(hyy, p, qy.bh, qy.ah, qy.kh);
+            hxyy = cpml_v(hxyy, p + a.pvar[1], qy.bi, qy.ai, qy.ki);
+            hyzy = cpml_v(hyzy, p + 2 * a.pvar[1], qy.bi, qy.ai, qy.ki);
           }
           if (cc.z) {
             float* p = a.psi[2] + po + cc.oz + 3 * a.pvar[2];
