@@ -76,13 +76,15 @@ def test_cpml_isotropic_arrays_against_the_oracle(gpu):
     _check(g, r, gf, rf)
 
 
-def test_cpml_ragged_grid_with_a_free_surface(gpu):
-    """BC1 on top (the z end at k = 0 undamped), CPML on the other faces, a
-    grid that fills no tile evenly, source and receivers near the top."""
+@pytest.mark.parametrize("fs", [1, 2])
+def test_cpml_ragged_grid_with_a_free_surface(gpu, fs):
+    """BC1 (fs 1) or the vacuum-layer top (fs 2) with the z end at k = 0
+    undamped, CPML on the other faces, a grid that fills no tile evenly,
+    source and receivers near the top."""
     from elanfmt_amd import fwe, workloads as W
 
     wl = W.ortho_layered(nx=37, ny=29, nz=23, L=5, nt=70, nrec_side=2)
-    wl.free_surface = 1
+    wl.free_surface = fs
     W.use_cpml(wl, 5)
     wl.injections = fwe.mt_injections(fwe.SRC_LAYOUT_OK4, 17, 13, 3, [1e15, -5e14, 2e14, 3e14, -1e14, 4e14])
     wl.receivers = [(q, i, j, k, 0) for q in range(3) for (i, j, k) in ((17, 13, 1), (3, 4, 2), (33, 26, 20))]
