@@ -98,18 +98,11 @@ __device__ __forceinline__ float ldz(const float* p, bool use, int64_t idx) {
   return use ? v : 0.f;
 }
 
-// Convolutional PML: derivative g at a node of axis index q through the
-// memory variable *p; t holds one node kind's rows b, a, 1/kappa of m
-// entries each.  psi <- b psi + a g, returns g/kappa + psi.
-__device__ __forceinline__ float cpml_g(float g, float* p, const float* t, int m, int q) {
-  const float ps = fmaf(t[q], *p, t[m + q] * g);
-  *p = ps;
-  return fmaf(g, t[2 * m + q], ps);
-}
-
-// The same with the coefficients already in registers.
-__device__ __forceinline__ float cpml_v(float g, float* p, float b, float a, float kinv) {
-  const float ps = fmaf(b, *p, a * g);
+// Convolutional PML: derivative g through its memory variable, whose old
+// value psi0 was loaded with the plane's other values and whose new value
+// is stored through p: psi <- b psi + a g, returns g/kappa + psi.
+__device__ __forceinline__ float cpml_v(float g, float psi0, float* p, float b, float a, float kinv) {
+  const float ps = fmaf(b, psi0, a * g);
   *p = ps;
   return fmaf(g, kinv, ps);
 }
@@ -136,7 +129,11 @@ __device__ __forceinline__ CpmlCell cpml_cell(const StepArgs& a, int i, int j, i
   c.x = i < L || i >= a.nx - L;
   c.y = j < L || j >= a.ny - L;
   c.z = k < L || k >= a.nz - L;
-  const int xi = i < L ? i : i - a.nx + L2, yj = j < L ? j : j - a.ny + L2, zk = k < L ? k : k - a.nz + L2;
+  // slab indices clamped into the slab, so cells outside it name a valid
+  // element too (their loads are issued with the slab lanes' and unused)
+  const int xi = min(max(i < L ? i : i - a.nx + L2, 0), L2 - 1);
+  const int yj = min(max(j < L ? j : j - a.ny + L2, 0), L2 - 1);
+  const int zk = min(max(k < L ? k : k - a.nz + L2, 0), L2 - 1);
   c.ox = xi + (int64_t)L2 * (j + (int64_t)a.ny * k);
   c.oy = i + (int64_t)a.nx * (yj + (int64_t)L2 * k);
   c.oz = i + (int64_t)a.nx * (j + (int64_t)a.ny * zk);
@@ -178,9 +175,11 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb, const int
   float fxh = 1.f, fyh = 1.f, wxi = 1.f, wyj = 1.f;
   if (PML == PML_PROFILE) { fxh = a.whx[i]; fyh = a.why[j]; }
   CpmlCoef qx{}, qy{};  // convolutional PML: this lane's x and this row's y coefficients
+  bool xw = false;       // this wavefront holds x-slab cells
   if (PML == PML_CPML) {
     if (i < a.cL || i >= a.nx - a.cL) qx = cpml_coef(a.ctab[0], a.nx + 1, i);
     if (j < a.cL || j >= a.ny - a.cL) qy = cpml_coef(a.ctab[1], a.ny + 1, j);
+    xw = __builtin_amdgcn_ballot_w64(i < a.cL || i >= a.nx - a.cL) != 0;
   }
   if (PML == PML_PROFILE || PML == PML_SEP) { wxi = a.wx[i]; wyj = a.wy[j]; }
 
@@ -255,6 +254,19 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb, const int
       const float vz_jm = ldz(Vz, jm, c - sy);
       float sxx = ld(Sxx + c), syy = ld(Syy + c), szz = ld(Szz + c);
       const float oxy = ld(Sxy + c), oxz = ld(Sxz + c), oyz = ld(Syz + c);
+      // memory variables of the slabs this wavefront touches, loaded with the
+      // plane's other values (one memory round trip per plane)
+      float px[3] = {0.f, 0.f, 0.f}, py[3] = {0.f, 0.f, 0.f}, pz[3] = {0.f, 0.f, 0.f};
+      int64_t po = 0;
+      if (PML == PML_CPML) {
+        po = (int64_t)(wb + b) * a.pstride;
+        if (xw)
+          for (int v = 0; v < 3; ++v) px[v] = a.psi[0][po + cc.ox + v * a.pvar[0]];
+        if (cc.y)
+          for (int v = 0; v < 3; ++v) py[v] = a.psi[1][po + cc.oy + v * a.pvar[1]];
+        if (cc.z)
+          for (int v = 0; v < 3; ++v) pz[v] = a.psi[2][po + cc.oz + v * a.pvar[2]];
+      }
       float sxy = oxy, sxz = oxz, syz = oyz;
       cvx[b] = vx; cvy[b] = vy; cvz[b] = vz_kp;
       float ex, ey, ez, dxy, dxz, dyz;
@@ -266,25 +278,25 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb, const int
         float gzx = (vz - vz_im) * a.rdx, gxz = (vx - vx_km) * a.rdz;  // Sxz node
         float gyz = (vy - vy_km) * a.rdz, gzy = (vz - vz_jm) * a.rdy;  // Syz node
         if (cc.x || cc.y || cc.z) {
-          const int64_t po = (int64_t)(wb + b) * a.pstride;
           if (cc.x) {
             float* p = a.psi[0] + po + cc.ox;
-            gxx = cpml_v(gxx, p, qx.bi, qx.ai, qx.ki);
-            gyx = cpml_v(gyx, p + a.pvar[0], qx.bh, qx.ah, qx.kh);
-            gzx = cpml_v(gzx, p + 2 * a.pvar[0], qx.bh, qx.ah, qx.kh);
+            gxx = cpml_v(gxx, px[0], p, qx.bi, qx.ai, qx.ki);
+            gyx = cpml_v(gyx, px[1], p + a.pvar[0], qx.bh, qx.ah, qx.kh);
+            gzx = cpml_v(gzx, px[2], p + 2 * a.pvar[0], qx.bh, qx.ah, qx.kh);
           }
           if (cc.y) {
             float* p = a.psi[1] + po + cc.oy;
-            gyy = cpml_v(gyy, p, qy.bi, qy.ai, qy.ki);
-            gxy = cpml_v(gxy, p + a.pvar[1], qy.bh, qy.ah, qy.kh);
-            gzy = cpml_v(gzy, p + 2 * a.pvar[1], qy.bh, qy.ah, qy.kh);
+            gyy = cpml_v(gyy, py[0], p, qy.bi, qy.ai, qy.ki);
+            gxy = cpml_v(gxy, py[1], p + a.pvar[1], qy.bh, qy.ah, qy.kh);
+            gzy = cpml_v(gzy, py[2], p + 2 * a.pvar[1], qy.bh, qy.ah, qy.kh);
           }
           if (cc.z) {
             float* p = a.psi[2] + po + cc.oz;
+            const float* t = a.ctab[2];
             const int m = a.nz + 1;
-            gzz = cpml_g(gzz, p, a.ctab[2], m, k);
-            gxz = cpml_g(gxz, p + a.pvar[2], a.ctab[2] + 3 * m, m, k);
-            gyz = cpml_g(gyz, p + 2 * a.pvar[2], a.ctab[2] + 3 * m, m, k);
+            gzz = cpml_v(gzz, pz[0], p, t[k], t[m + k], t[2 * m + k]);
+            gxz = cpml_v(gxz, pz[1], p + a.pvar[2], t[3 * m + k], t[4 * m + k], t[5 * m + k]);
+            gyz = cpml_v(gyz, pz[2], p + 2 * a.pvar[2], t[3 * m + k], t[4 * m + k], t[5 * m + k]);
           }
         }
         ex = gxx; ey = gyy; ez = gzz;
@@ -394,9 +406,11 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb, const i
 
   float wxm = 1.f, wym = 1.f, txi = 1.f, tyj = 1.f;
   CpmlCoef qx{}, qy{};  // convolutional PML: this lane's x and this row's y coefficients
+  bool xw = false;       // this wavefront holds x-slab cells
   if (PML == PML_CPML) {
     if (i < a.cL || i >= nx - a.cL) qx = cpml_coef(a.ctab[0], nx + 1, i);
     if (j < a.cL || j >= ny - a.cL) qy = cpml_coef(a.ctab[1], ny + 1, j);
+    xw = __builtin_amdgcn_ballot_w64(i < a.cL || i >= nx - a.cL) != 0;
   }
   if (PML == PML_PROFILE) {
     wxm = im ? a.wx[i - 1] : 1.f;
@@ -452,6 +466,19 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb, const i
       const float sxz_kp = Sxz[c + sz], sxz_ip = Sxz[c + 1];
       const float syz = FIRST ? Syz[c] : csyz[b];
       const float syz_kp = Syz[c + sz], syz_jp = Syz[c + sy];
+      // memory variables of the slabs this wavefront touches, loaded with the
+      // plane's other values (one memory round trip per plane)
+      float px[3] = {0.f, 0.f, 0.f}, py[3] = {0.f, 0.f, 0.f}, pz[3] = {0.f, 0.f, 0.f};
+      int64_t po = 0;
+      if (PML == PML_CPML) {
+        po = (int64_t)(wb + b) * a.pstride;
+        if (xw)
+          for (int v = 0; v < 3; ++v) px[v] = a.psi[0][po + cc.ox + (3 + v) * a.pvar[0]];
+        if (cc.y)
+          for (int v = 0; v < 3; ++v) py[v] = a.psi[1][po + cc.oy + (3 + v) * a.pvar[1]];
+        if (cc.z)
+          for (int v = 0; v < 3; ++v) pz[v] = a.psi[2][po + cc.oz + (3 + v) * a.pvar[2]];
+      }
       cszz[b] = szz; csxz[b] = sxz_kp; csyz[b] = syz_kp;
       if (MODE == MODE_ORTHO_TAB && b == 0) {
         // entry words 8..11: C66xy, bx, by, bz
@@ -470,25 +497,25 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb, const i
         float hyy = (syy - syy_jm) * a.rdy, hxyx = (sxy_ip - sxy) * a.rdx, hyzz = (syz_kp - syz) * a.rdz;
         float hzz = (szz - szz_km) * a.rdz, hxzx = (sxz_ip - sxz) * a.rdx, hyzy = (syz_jp - syz) * a.rdy;
         if (cc.x || cc.y || cc.z) {
-          const int64_t po = (int64_t)(wb + b) * a.pstride;
           if (cc.x) {
             float* p = a.psi[0] + po + cc.ox + 3 * a.pvar[0];
-            hxx = cpml_v(hxx, p, qx.bh, qx.ah, qx.kh);
-            hxyx = cpml_v(hxyx, p + a.pvar[0], qx.bi, qx.ai, qx.ki);
-            hxzx = cpml_v(hxzx, p + 2 * a.pvar[0], qx.bi, qx.ai, qx.ki);
+            hxx = cpml_v(hxx, px[0], p, qx.bh, qx.ah, qx.kh);
+            hxyx = cpml_v(hxyx, px[1], p + a.pvar[0], qx.bi, qx.ai, qx.ki);
+            hxzx = cpml_v(hxzx, px[2], p + 2 * a.pvar[0], qx.bi, qx.ai, qx.ki);
           }
           if (cc.y) {
             float* p = a.psi[1] + po + cc.oy + 3 * a.pvar[1];
-            hyy = cpml_v(hyy, p, qy.bh, qy.ah, qy.kh);
-            hxyy = cpml_v(hxyy, p + a.pvar[1], qy.bi, qy.ai, qy.ki);
-            hyzy = cpml_v(hyzy, p + 2 * a.pvar[1], qy.bi, qy.ai, qy.ki);
+            hyy = cpml_v(hyy, py[0], p, qy.bh, qy.ah, qy.kh);
+            hxyy = cpml_v(hxyy, py[1], p + a.pvar[1], qy.bi, qy.ai, qy.ki);
+            hyzy = cpml_v(hyzy, py[2], p + 2 * a.pvar[1], qy.bi, qy.ai, qy.ki);
           }
           if (cc.z) {
             float* p = a.psi[2] + po + cc.oz + 3 * a.pvar[2];
+            const float* t = a.ctab[2];
             const int m = nz + 1;
-            hzz = cpml_g(hzz, p, a.ctab[2] + 3 * m, m, k);
-            hxzz = cpml_g(hxzz, p + a.pvar[2], a.ctab[2], m, k);
-            hyzz = cpml_g(hyzz, p + 2 * a.pvar[2], a.ctab[2], m, k);
+            hzz = cpml_v(hzz, pz[0], p, t[3 * m + k], t[4 * m + k], t[5 * m + k]);
+            hxzz = cpml_v(hxzz, pz[1], p + a.pvar[2], t[k], t[m + k], t[2 * m + k]);
+            hyzz = cpml_v(hyzz, pz[2], p + 2 * a.pvar[2], t[k], t[m + k], t[2 * m + k]);
           }
         }
         dSx = hxx + hxyy + hxzz;
