@@ -1,7 +1,8 @@
-"""Size-independent properties of the HIP path at the benchmark sizes, where
-the FP64 oracle would take too long: linearity in the moment tensor on the
-384x384x192 layered orthorhombic case, and batch members equal to single
-runs on the 512x512x256 six-wavefield case."""
+"""The HIP path at the benchmark sizes: the headline 384x384x192 case and
+the 512x512x256 CPML grid against the FP64 oracle over a few hundred steps,
+and, where the oracle would take too long, size-independent properties:
+linearity in the moment tensor, batch members equal to single runs, the
+uneven batch as one group, the thin layers."""
 import numpy as np
 import pytest
 
@@ -29,6 +30,68 @@ def test_linearity_in_the_moment_tensor_at_384x384x192(gpu):
         s.close()
     assert np.abs(tr[0]).max() > 0 and np.abs(tr[1]).max() > 0
     assert rel_l2(tr[2], tr[0] + tr[1]) < 1e-5
+
+
+@pytest.mark.timeout(600)
+def test_config1_against_the_oracle_at_384x384x192(gpu):
+    """The headline workload itself (384x384x192 layered orthorhombic,
+    24-cell separable layer, material table, autotuned) against the FP64
+    oracle on every host core for 400 steps: all receiver traces and all
+    nine fields within 1e-5."""
+    from elanfmt_amd import workloads as W
+    from oracle import Oracle
+
+    wl = W.config(1, nt=400)
+    print(f"case {wl.name} against the oracle, {wl.nt} steps", flush=True)
+    s = W.make_solver(wl)
+    s.autotune(2)
+    s.run(wl.nt)
+    g = s.traces().astype(np.float64)
+    gf = [s.get_field(f) for f in range(9)]
+    s.close()
+    o = Oracle(wl.nx, wl.ny, wl.nz, wl.dx, wl.dy, wl.dz, wl.dt, wl.nt, mode=wl.mode, pml=wl.pml,
+               free_surface=wl.free_surface)
+    W.apply(o, wl)
+    o.run(wl.nt)
+    r = o.traces()
+    assert np.abs(r).max() > 0
+    assert rel_l2(g, r) < 1e-5
+    for f in range(9):
+        assert rel_l2(gf[f], o.get_field(f)) < 1e-5, f
+    o.close()
+
+
+@pytest.mark.timeout(600)
+def test_cpml_against_the_oracle_at_512x512x256(gpu):
+    """The 6-cell convolutional PML on the full 512x512x256 grid against
+    the oracle's CPML for 150 steps, with the source 6 cells inside the x-
+    layer so the wave is in the x, y and z slabs: receivers in and beside
+    the layer and all nine fields within 1e-5."""
+    from elanfmt_amd import learned as Lr, workloads as W
+    from oracle import Oracle
+
+    inner, thin, L, nt = (500, 500, 244), (512, 512, 256), 6, 150
+    med = Lr.extended_medium(inner, thin, 4242)
+    dt = W.cfl_dt(med["vmax"], Lr.DX, Lr.DX, Lr.DX, 0.5)
+    rec = [(c, i, 256 + dj, 128 + dk, 0) for c in range(3) for i in (3, 7, 9) for (dj, dk) in ((0, 0), (4, -3))]
+    wl = Lr.workload("c4cpml", thin, L, med, dt, (12, 256, 128), np.array([1e15, -4e14, 2e14, 3e14, -2e14, 1e14]),
+                     15.0, nt, rec)
+    W.use_cpml(wl, L, vmax=med["vmax"], **Lr.CPML_THIN)
+    print(f"case c4 cpml 512x512x256 against the oracle, {nt} steps", flush=True)
+    s = W.make_solver(wl)
+    s.run(nt)
+    g = s.traces().astype(np.float64)
+    gf = [s.get_field(f) for f in range(9)]
+    s.close()
+    o = Oracle(wl.nx, wl.ny, wl.nz, wl.dx, wl.dy, wl.dz, wl.dt, wl.nt, mode=wl.mode, pml=wl.pml)
+    W.apply(o, wl)
+    o.run(nt)
+    r = o.traces()
+    assert np.abs(r[[0, 1, 6, 7]]).max() > 0  # receivers inside the layer are reached
+    assert rel_l2(g, r) < 1e-5
+    for f in range(9):
+        assert rel_l2(gf[f], o.get_field(f)) < 1e-5, f
+    o.close()
 
 
 def test_batch_members_equal_single_runs_at_512x512x256(gpu):
