@@ -105,6 +105,12 @@ struct fwe_ctx {
   float* psi = nullptr;
   int64_t pvar[3] = {0, 0, 0};
   int64_t pstride = 0;
+  // the tiles that touch no slab (a box, StepArgs::bx0..) run through the
+  // plain kernel and the others (listed in cpml_tiles) through the CPML one;
+  // rebuilt when the tiling or the layer changes (cpml_box[1] = 0: no box)
+  int32_t cpml_box[6] = {0, 0, 0, 0, 0, 0};
+  int32_t* cpml_tiles = nullptr;
+  int32_t cpml_ntiles = 0;
   float* prof = nullptr;    // wx wy wz whx why whz (profile / taper tables)
   float* stf = nullptr;     // nb * nstf
   float* traces = nullptr;  // nrec * nt
@@ -136,6 +142,8 @@ struct fwe_ctx {
   int64_t lb_face_off[7] = {0, 0, 0, 0, 0, 0, 0};
   float* lb_D = nullptr;            // corrections [cells][9]
 };
+
+static fwe_status cpml_split_update(fwe_ctx* x);
 
 static void drop_graph(fwe_ctx* x) {
   if (x->gexec) hipGraphExecDestroy(x->gexec);
@@ -304,7 +312,7 @@ extern "C" fwe_status fwe_destroy(fwe_ctx* x) {
   hipFree(x->it_dev);
   for (auto ev : x->events) hipEventDestroy(ev);
   hipFree(x->fbuf[0]); hipFree(x->fbuf[1]); hipFree(x->medium); hipFree(x->w3); hipFree(x->prof);
-  hipFree(x->ctab); hipFree(x->psi);
+  hipFree(x->ctab); hipFree(x->psi); hipFree(x->cpml_tiles);
   hipFree(x->mid); hipFree(x->mtab);
   hipFree(x->lb_cells); hipFree(x->lb_D);
   hipFree(x->stf); hipFree(x->traces); hipFree(x->inj_off); hipFree(x->inj);
@@ -509,7 +517,7 @@ extern "C" fwe_status fwe_set_cpml(fwe_ctx* x, int32_t layer, const float* tx, c
   for (int d = 0; d < 3; ++d) x->pvar[d] = pv[d];
   x->pstride = ps;
   x->have_damp = true;
-  return FWE_OK;
+  return cpml_split_update(x);
 }
 
 extern "C" fwe_status fwe_cpml_state_size(const fwe_ctx* x, int64_t* n) {
@@ -765,6 +773,8 @@ static StepArgs make_args(fwe_ctx* x, int from) {
   a.rdx = (float)(1.0 / c.dx); a.rdy = (float)(1.0 / c.dy); a.rdz = (float)(1.0 / c.dz);
   a.dtdx = (float)(c.dt / c.dx); a.dtdy = (float)(c.dt / c.dy); a.dtdz = (float)(c.dt / c.dz);
   a.ty_rows = x->ty; a.ntx = x->ntx; a.nty = x->nty; a.nkc = x->nkc; a.kchunk = x->kchunk;
+  a.bx0 = 0; a.bnx = x->ntx; a.by0 = 0; a.bny = x->nty; a.bk0 = 0; a.bnk = x->nkc;
+  a.tiles = nullptr; a.ntiles = 0;
   a.inj_off = x->inj_off; a.inj = x->inj; a.stf = x->stf; a.nstf = x->nstf;
   a.src_after_bc = c.src_after_bc;
   a.rec_off = x->rec_off; a.rec_koff = x->rec_koff; a.rec = x->rec; a.traces = x->traces; a.nt = c.nt;
@@ -894,6 +904,36 @@ static int kernel_mode(const fwe_ctx* x) {
   return x->cfg.mode;
 }
 
+// The stress and velocity passes of one step, `mid` (optional) recorded
+// between them.  With the convolutional PML and a box of slab-free tiles
+// (not in plane-split order, which covers the whole grid), each pass is two
+// launches: the box through the plain kernel, the listed tiles through the
+// CPML one; cells in no slab compute the same bits in either.
+static hipError_t two_pass(fwe_ctx* x, const StepArgs& a, hipEvent_t mid) {
+  const fwe_config& c = x->cfg;
+  const int km = kernel_mode(x);
+  const int fs = c.free_surface == FWE_FS_BC1;
+  hipError_t e = hipSuccess;
+  if (c.pml == FWE_PML_CPML && x->cpml_box[1] > 0 && x->wg_order != FWE_ORDER_PLANE_SPLIT) {
+    StepArgs in = a, sh = a;
+    in.bx0 = x->cpml_box[0]; in.bnx = x->cpml_box[1];
+    in.by0 = x->cpml_box[2]; in.bny = x->cpml_box[3];
+    in.bk0 = x->cpml_box[4]; in.bnk = x->cpml_box[5];
+    sh.tiles = x->cpml_tiles;
+    sh.ntiles = x->cpml_ntiles;
+    e = fwe::launch_stress(in, km, FWE_PML_NONE, fs, x->stream);
+    if (e == hipSuccess && sh.ntiles > 0) e = fwe::launch_stress(sh, km, FWE_PML_CPML, fs, x->stream);
+    if (e == hipSuccess && mid) e = hipEventRecord(mid, x->stream);
+    if (e == hipSuccess) e = fwe::launch_velocity(in, km, FWE_PML_NONE, x->stream);
+    if (e == hipSuccess && sh.ntiles > 0) e = fwe::launch_velocity(sh, km, FWE_PML_CPML, x->stream);
+    return e;
+  }
+  e = fwe::launch_stress(a, km, c.pml, fs, x->stream);
+  if (e == hipSuccess && mid) e = hipEventRecord(mid, x->stream);
+  if (e == hipSuccess) e = fwe::launch_velocity(a, km, c.pml, x->stream);
+  return e;
+}
+
 static hipError_t step_launch(fwe_ctx* x, int from, int it, const int32_t* it_base, hipEvent_t mid) {
   const fwe_config& c = x->cfg;
   const int km = kernel_mode(x);
@@ -906,9 +946,7 @@ static hipError_t step_launch(fwe_ctx* x, int from, int it, const int32_t* it_ba
       StepArgs a = make_args(x, from);
       a.it = it;
       a.it_base = it_base;
-      e = fwe::launch_stress(a, km, c.pml, c.free_surface == FWE_FS_BC1, x->stream);
-      if (e == hipSuccess && mid) e = hipEventRecord(mid, x->stream);
-      if (e == hipSuccess) e = fwe::launch_velocity(a, km, c.pml, x->stream);
+      e = two_pass(x, a, mid);
     }
     if (e == hipSuccess) e = lb_after(x, from);
     return e;
@@ -925,9 +963,7 @@ static hipError_t step_launch(fwe_ctx* x, int from, int it, const int32_t* it_ba
     StepArgs a = make_args(x, from);
     a.it = it;
     a.it_base = it_base;
-    e = fwe::launch_stress(a, km, c.pml, c.free_surface == FWE_FS_BC1, x->stream);
-    if (e == hipSuccess && mid) e = hipEventRecord(mid, x->stream);
-    if (e == hipSuccess) e = fwe::launch_velocity(a, km, c.pml, x->stream);
+    e = two_pass(x, a, mid);
   }
   return e;
 }
@@ -1048,6 +1084,45 @@ extern "C" fwe_status fwe_set_graph_steps(fwe_ctx* x, int32_t g) {
   return FWE_OK;
 }
 
+// The convolutional PML's tile split for the current tiling and layer: the
+// box of tiles whose cells all lie outside every slab, and the list of the
+// others (x fastest, then y, then chunk).  No box when the layer or the
+// tiling leaves no such tile.
+static fwe_status cpml_split_update(fwe_ctx* x) {
+  for (int q = 0; q < 6; ++q) x->cpml_box[q] = 0;
+  x->cpml_ntiles = 0;
+  const fwe_config& c = x->cfg;
+  const int L = x->cpml_L;
+  if (c.pml != FWE_PML_CPML || L <= 0 || x->ntx <= 0) return FWE_OK;
+  auto range = [](int n, int t, int L, int& lo, int& cnt) {
+    lo = (L + t - 1) / t;           // first tile with every cell at index >= L
+    const int hi = (n - L) / t;     // tiles below hi have every cell below n - L
+    cnt = std::max(0, hi - lo);
+  };
+  int b[6];
+  range(c.nx, x->tx, L, b[0], b[1]);
+  range(c.ny, x->ty, L, b[2], b[3]);
+  range(c.nz, x->kchunk, L, b[4], b[5]);
+  if (b[1] <= 0 || b[3] <= 0 || b[5] <= 0) return FWE_OK;
+  std::vector<int32_t> list;
+  for (int kc = 0; kc < x->nkc; ++kc)
+    for (int ty = 0; ty < x->nty; ++ty)
+      for (int tx = 0; tx < x->ntx; ++tx) {
+        const bool in = tx >= b[0] && tx < b[0] + b[1] && ty >= b[2] && ty < b[2] + b[3] && kc >= b[4] &&
+                        kc < b[4] + b[5];
+        if (!in) list.push_back(tx + (ty + kc * x->nty) * x->ntx);
+      }
+  HIPCHK(hipSetDevice(c.device));
+  HIPCHK(hipStreamSynchronize(x->stream));  // queued steps read the old list
+  hipFree(x->cpml_tiles);
+  x->cpml_tiles = nullptr;
+  HIPCHK(hipMalloc(&x->cpml_tiles, sizeof(int32_t) * std::max<size_t>(1, list.size())));
+  HIPCHK(hipMemcpy(x->cpml_tiles, list.data(), sizeof(int32_t) * list.size(), hipMemcpyHostToDevice));
+  x->cpml_ntiles = (int32_t)list.size();
+  for (int q = 0; q < 6; ++q) x->cpml_box[q] = b[q];
+  return FWE_OK;
+}
+
 static fwe_status apply_tiling(fwe_ctx* x, int tx, int ty, int kchunk) {
   const fwe_config& c = x->cfg;
   if (ty != 2 && ty != 4 && ty != 8) return fail(FWE_ERR_ARG, "tile rows must be 2, 4 or 8");
@@ -1064,6 +1139,7 @@ static fwe_status apply_tiling(fwe_ctx* x, int tx, int ty, int kchunk) {
   x->nkc = (c.nz + kchunk - 1) / kchunk;
   fwe_status st = upload_points(x, x->inj_host, &x->inj_off, &x->inj);
   if (st) return st;
+  if ((st = cpml_split_update(x))) return st;
   return upload_receivers(x);
 }
 

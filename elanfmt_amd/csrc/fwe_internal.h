@@ -91,6 +91,15 @@ struct StepArgs {
   int32_t split;        // plane split: tiles per XCD part of a chunk, ceil(ntx*nty/8)
   int32_t ty_rows;      // TILE_Y
   int32_t ntx, nty, nkc, kchunk;
+  // the tiles one launch covers (tile = tx + ty*ntx + kc*ntx*nty): a box of
+  // bnx x bny x bnk tiles from (bx0, by0, bk0), the whole grid by default,
+  // or, when `tiles` is set, the ntiles tiles listed there.  The
+  // convolutional PML runs the tiles that touch no slab as a box through
+  // the plain kernel and lists the others for its own kernel.  Plane-split
+  // order always covers the whole grid.
+  int32_t bx0, bnx, by0, bny, bk0, bnk;
+  const int32_t* tiles;
+  int32_t ntiles;
   // sources
   const int32_t* inj_off;   // ntx*nty+1
   const PointEntry* inj;

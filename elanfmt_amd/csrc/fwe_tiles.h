@@ -63,16 +63,28 @@ __device__ __forceinline__ int plane_split(int b, int P, int s) {
   return t < P ? t + kc * P : -1;
 }
 
+// logical tile t of the launch -> tile of the grid (StepArgs::tiles, box)
+__device__ __forceinline__ int box_tile(const StepArgs& a, int t) {
+  if (a.tiles) return a.tiles[t];
+  if (a.bnx == a.ntx && a.bny == a.nty) return t + (a.bk0 * a.nty) * a.ntx;  // whole planes of tiles
+  const int bx = t % a.bnx;
+  t /= a.bnx;
+  const int by = t % a.bny;
+  const int bk = t / a.bny;
+  return (a.bx0 + bx) + (a.by0 + by + (a.bk0 + bk) * a.nty) * a.ntx;
+}
+
 __device__ __forceinline__ int tile_of(const StepArgs& a, int b, int nb) {
-  if (a.xcd_order == 1) return xcd_remap(b, nb);
+  if (a.xcd_order == 1) return box_tile(a, xcd_remap(b, nb));
   if (a.xcd_order == 2) return plane_split(b, a.ntx * a.nty, a.split);
-  return b;
+  return box_tile(a, b);
 }
 
 // workgroups of one group of NB wavefields (plane split pads every chunk to
 // 8*split tiles); a launch covers nbg such groups, one after the other
 __host__ __device__ inline int launch_blocks(const StepArgs& a) {
-  return a.xcd_order == 2 ? a.nkc * 8 * a.split : a.ntx * a.nty * a.nkc;
+  if (a.xcd_order == 2) return a.nkc * 8 * a.split;
+  return a.tiles ? a.ntiles : a.bnx * a.bny * a.bnk;
 }
 
 // workgroup -> (wavefield group g, tile lb or -1 for a padding slot).  Each
@@ -270,14 +282,15 @@ __device__ __forceinline__ void stress_tile(const StepArgs& a, int lb, const int
       float sxy = oxy, sxz = oxz, syz = oyz;
       cvx[b] = vx; cvy[b] = vy; cvz[b] = vz_kp;
       float ex, ey, ez, dxy, dxz, dyz;
-      if (PML == PML_CPML) {
+      if (PML == PML_CPML && (cc.x || cc.y || cc.z)) {
         // the nine strain-rate derivatives, each through its memory
-        // variable where the cell lies in that axis' slab
+        // variable where the cell lies in that axis' slab (cells in no slab
+        // take the plain form below, bit for bit as the plain kernel)
         float gxx = (vx_ip - vx) * a.rdx, gyy = (vy_jp - vy) * a.rdy, gzz = (vz_kp - vz) * a.rdz;
         float gyx = (vy - vy_im) * a.rdx, gxy = (vx - vx_jm) * a.rdy;  // Sxy node
         float gzx = (vz - vz_im) * a.rdx, gxz = (vx - vx_km) * a.rdz;  // Sxz node
         float gyz = (vy - vy_km) * a.rdz, gzy = (vz - vz_jm) * a.rdy;  // Syz node
-        if (cc.x || cc.y || cc.z) {
+        {
           if (cc.x) {
             float* p = a.psi[0] + po + cc.ox;
             gxx = cpml_v(gxx, px[0], p, qx.bi, qx.ai, qx.ki);
@@ -490,13 +503,14 @@ __device__ __forceinline__ void velocity_tile(const StepArgs& a, int lb, const i
       if (NB == 1) __builtin_amdgcn_sched_barrier(0);
       if (PML == PML_SEP && b == 0) w = txi * tyj * wzk;
       float dSx, dSy, dSz;
-      if (PML == PML_CPML) {
+      if (PML == PML_CPML && (cc.x || cc.y || cc.z)) {
         // the nine stress derivatives, each through its memory variable
-        // where the cell lies in that axis' slab
+        // where the cell lies in that axis' slab (cells in no slab take the
+        // plain form below, bit for bit as the plain kernel)
         float hxx = (sxx - sxx_im) * a.rdx, hxyy = (sxy_jp - sxy) * a.rdy, hxzz = (sxz_kp - sxz) * a.rdz;
         float hyy = (syy - syy_jm) * a.rdy, hxyx = (sxy_ip - sxy) * a.rdx, hyzz = (syz_kp - syz) * a.rdz;
         float hzz = (szz - szz_km) * a.rdz, hxzx = (sxz_ip - sxz) * a.rdx, hyzy = (syz_jp - syz) * a.rdy;
-        if (cc.x || cc.y || cc.z) {
+        {
           if (cc.x) {
             float* p = a.psi[0] + po + cc.ox + 3 * a.pvar[0];
             hxx = cpml_v(hxx, px[0], p, qx.bh, qx.ah, qx.kh);
