@@ -494,14 +494,24 @@ extern "C" fwe_status fwe_set_cpml(fwe_ctx* x, int32_t layer, const float* tx, c
   const int64_t L2 = 2 * (int64_t)layer;
   const int64_t pv[3] = {L2 * c.ny * c.nz, (int64_t)c.nx * L2 * c.nz, (int64_t)c.nx * c.ny * L2};
   const int64_t ps = 6 * (pv[0] + pv[1] + pv[2]);
+  // runs are refused until the new layer is complete, so a failure below
+  // never leaves a context that steps with a missing or mis-sized state
+  x->have_damp = false;
   if (layer != x->cpml_L || !x->psi) {
     hipFree(x->psi);
     x->psi = nullptr;
     x->cpml_L = 0;
+    x->pstride = 0;
     hipError_t e = hipMalloc(&x->psi, sizeof(float) * (size_t)ps * c.nbatch);
-    if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? FWE_ERR_NOMEM : FWE_ERR_HIP, "CPML memory: %s",
-                                     hipGetErrorString(e));
+    if (e != hipSuccess) {
+      x->psi = nullptr;
+      return fail(e == hipErrorOutOfMemory ? FWE_ERR_NOMEM : FWE_ERR_HIP, "CPML memory: %s", hipGetErrorString(e));
+    }
   }
+  // the sizes always describe the allocation (reset and the state calls use them)
+  x->cpml_L = layer;
+  for (int d = 0; d < 3; ++d) x->pvar[d] = pv[d];
+  x->pstride = ps;
   if (!x->ctab) HIPCHK(hipMalloc(&x->ctab, sizeof(float) * 6 * ((size_t)c.nx + c.ny + c.nz + 3)));
   std::vector<float> h;
   h.reserve(6 * ((size_t)c.nx + c.ny + c.nz + 3));
@@ -513,11 +523,9 @@ extern "C" fwe_status fwe_set_cpml(fwe_ctx* x, int32_t layer, const float* tx, c
   // the memory variables start from zero with the new coefficients
   HIPCHK(hipMemsetAsync(x->psi, 0, sizeof(float) * (size_t)ps * c.nbatch, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
-  x->cpml_L = layer;
-  for (int d = 0; d < 3; ++d) x->pvar[d] = pv[d];
-  x->pstride = ps;
+  if ((st = cpml_split_update(x))) return st;
   x->have_damp = true;
-  return cpml_split_update(x);
+  return FWE_OK;
 }
 
 extern "C" fwe_status fwe_cpml_state_size(const fwe_ctx* x, int64_t* n) {

@@ -203,8 +203,9 @@ fwe_status fwe_set_pml_weights(fwe_ctx* ctx, const float* w);
  * velocity pass dSxx/dx, dSxy/dx, dSxz/dx (likewise).  Resets the memory
  * variables to zero.  A face without damping has zero entries (its table's
  * end disabled in the generator); with BC1 the top end of z is normally
- * disabled.  No reference counterpart (the reference damps with the
- * profiles above); replaces them for thin layers. */
+ * disabled.  If the call fails, runs are refused (FWE_ERR_STATE) until a
+ * later call succeeds.  No reference counterpart (the reference damps with
+ * the profiles above); replaces them for thin layers. */
 fwe_status fwe_set_cpml(fwe_ctx* ctx, int32_t layer, const float* tx, const float* ty,
                         const float* tz);
 /* Memory variables of one wavefield (floats per wavefield in *n): saved and
