@@ -91,7 +91,7 @@ def _cpu_entry(args, wl_single, name):
     """CPU baseline beside the GPU figure: the FP64 oracle on `cpu_steps`
     steps of this workload (median of three timings) and its FP32 build;
     for the 384x384x192 case also the 128^3 isotropic case in full (600
-    steps), as the baseline plan asks."""
+    steps)."""
     if args.no_cpu or args.cpu_steps <= 0:
         return None
     from elanfmt_amd import workloads as W
@@ -657,7 +657,7 @@ def main():
                          "apply (auto); always the twelve arrays; the table; the isotropic arrays")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
               "(models and wavefields would be placed on GPUs whole, a single large model split in z-slabs); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)

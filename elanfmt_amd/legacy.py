@@ -9,7 +9,8 @@ Inputs (raw float32, x fastest; OK4.m:348-429, ANIS.m:378-533):
   iso:   c11u_het.dat c12u_het.dat weights_x.dat weights_y.dat weights_z.dat
   ortho: C11m C22m C33m C12m C13m C23m C44myz C55mxz C66mxy .dat,
          vrho_x_11_hc vrho_y_11_hc vrho_z_11_hc .dat, weights.dat
-         (or taper_x/y/z.dat for the separable damping of this build)
+         (or taper_x/y/z.dat for the separable damping, which has no
+         reference counterpart)
 Outputs (OK4.cu:153-164,521-532): 0_infos.inf, 0_0_<name>.res.
 """
 from __future__ import annotations

@@ -103,7 +103,7 @@ void fwe_gen_harmonic_edges(int32_t nx, int32_t ny, int32_t nz, const float* c44
  * C55 C66}.  Returns 0 if the tensor is positive definite, -1 otherwise. */
 int fwe_gen_ortho_stiffness(double vp, double vs, double rho, const double p[7], double c[9]);
 
-/* Layered orthorhombic medium (a workload of this build, not a reference
+/* Layered orthorhombic medium (a benchmark workload, not a reference
  * input): nl horizontal layers with seeded random thicknesses;
  * per layer vp~U[vp_lo,vp_hi], vp/vs~U[r_lo,r_hi], rho~U[rho_lo,rho_hi],
  * Tsvankin parameters ~U[0,aniso_hi].  Writes the 9 cell Cij arrays
