@@ -45,9 +45,10 @@ def test_network_forward_and_gradient_match_cpu(gpu):
     lb.close()
 
 
-@pytest.mark.parametrize("dims,L,form", [((22, 20, 18), 3, 0), ((26, 17, 9), 5, 0), ((11, 23, 14), 6, 0),
-                                         ((31, 12, 12), 6, 0), ((22, 20, 18), 3, 1)])
-def test_attached_boundary_matches_oracle_plus_cpu_corrections(gpu, dims, L, form):
+@pytest.mark.parametrize("dims,L,form,base", [((22, 20, 18), 3, 0, "plain"), ((26, 17, 9), 5, 0, "plain"),
+                                              ((11, 23, 14), 6, 0, "plain"), ((31, 12, 12), 6, 0, "plain"),
+                                              ((22, 20, 18), 3, 1, "plain"), ((22, 20, 18), 3, 0, "cpml")])
+def test_attached_boundary_matches_oracle_plus_cpu_corrections(gpu, dims, L, form, base):
     """Every step: corrections from the fields before the step, computed by
     the CPU network on the CPU stencil features, applied (dissipative part)
     after the oracle step; the HIP path must follow within FP32 round-off.
@@ -63,14 +64,18 @@ def test_attached_boundary_matches_oracle_plus_cpu_corrections(gpu, dims, L, for
     reaches 4e-4 of the fields, so grids with no interior along some axis
     are held to 1e-3, which a wrong weight, stencil point or depth (errors
     of a percent or more in the corrections) still exceeds.  form 1: the
-    free form, corrections added as predicted."""
+    free form, corrections added as predicted.  base "cpml": the layer over
+    a 3-cell convolutional PML (the oracle's memory variables follow its own
+    run; they match the GPU's to round-off when the fields are copied)."""
     from elanfmt_amd import fwe, workloads as W
     from oracle import Oracle, lb_ref
 
     rng = np.random.default_rng(12)
     nx, ny, nz = dims
     wl = W.ortho_layered(nx=nx, ny=ny, nz=nz, L=min(L, 4), nt=30, f0=25.0, nrec_side=2)
-    print(f"learned boundary L={L} on {nx}x{ny}x{nz}", flush=True)
+    if base == "cpml":
+        W.use_cpml(wl, min(L, 4))
+    print(f"learned boundary L={L} form={form} base={base} on {nx}x{ny}x{nz}", flush=True)
     c = (nx // 2, ny // 2, nz // 2)
     wl.injections = fwe.mt_injections(fwe.SRC_LAYOUT_OK4, *c, [1e15, -5e14, 2e14, 3e14, -1e14, 2e14])
     z = 1.5e7
