@@ -41,19 +41,27 @@ METRIC = "Gcell-updates/s + % of 8 TB/s HBM; batched-6-MT vs 6 plain runs at equ
 DEFAULT_STEPS = {0: 600, 1: 3000, 2: 1000, 3: 1500, 4: 200}
 
 
-def _traffic(kernel_name, workload):
+def _traffic(kernel_name, geom, workload):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE, each scaled by the same counter on the copy probe), kept in
-    profiles/traffic.json, for exactly this kernel (template arguments
-    included) on this workload; None when that kernel was not profiled."""
+    profiles/traffic.json under "<kernel name> <geometry>", for exactly this
+    kernel (template arguments included) launched with exactly this
+    workgroup order and z chunk (geometry "linear kc2"; the sweep: "rows2
+    kc12") on this workload; None when that launch was not profiled."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p) as f:
             d = json.load(f)[workload]
     except Exception:
         return None
-    v = d.get(kernel_name)
+    v = d.get(f"{kernel_name} {geom}")
     return None if v is None else v["bytes_per_launch"]
+
+
+def _geom(fused, order, tiling, sweep):
+    """Launch geometry that traffic.json keys carry besides the kernel name:
+    workgroup order and z chunk (two-pass), rows per lane and z chunk (sweep)."""
+    return f"rows{sweep[0]} kc{sweep[1]}" if fused else f"{order} kc{tiling[2]}"
 
 
 def cpu_baseline(wl, steps, threads, fp32=False, repeats=3):
@@ -187,12 +195,12 @@ def _kernel_entries(wl, fused, tiling, ms_a, ms_b, steps, cells, medium=1, batch
     return kernels, dom, ba + bb
 
 
-def _roofline(dom, steps, cells, wname):
+def _roofline(dom, steps, cells, wname, geom):
     name, ms, b, kname = dom
     ach = b * cells / (ms / steps * 1e-3) / 1e9
-    traffic = _traffic(kname, wname)
+    traffic = _traffic(kname, geom, wname)
     return {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel_name": kname,
+            "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel_name": kname, "launch": geom,
             "alg_bytes_per_launch": b * cells}
 
 
@@ -296,7 +304,7 @@ def bench_plain(args, steps):
               "parallelism": "single GPU"}
     return _line(cells * wl.nbatch * steps / el / 1e9, steps, args.warmup, el, config,
                  hbm_pct_step=round(100 * bstep * cells * steps / el / 1e9 / PEAK_HBM_GBS, 2),
-                 kernels=kernels, roofline=_roofline(dom, steps, cells, wl.name),
+                 kernels=kernels, roofline=_roofline(dom, steps, cells, wl.name, _geom(fused, order, tiling, sweep)),
                  copy_probe_GBs=round(copy_gbs, 1), traces_finite=finite, medium_arrays_run=arrays_run,
                  published_reference=PUBLISHED_A100,
                  cpu_baseline=_cpu_entry(args, W.config(args.config, nt=args.cpu_steps + 2), wl.name))
@@ -383,7 +391,7 @@ def bench_batched(args, steps):
               "parallelism": "single GPU"}
     return _line(cells * 6 * steps / el_b / 1e9, steps, args.warmup, el_b, config,
                  hbm_pct_step=round(100 * bstep * cells * steps / el_b / 1e9 / PEAK_HBM_GBS, 2),
-                 kernels=kernels, roofline=_roofline(dom, steps, cells, wl.name + "x6"),
+                 kernels=kernels, roofline=_roofline(dom, steps, cells, wl.name + "x6", _geom(fused, order, tiling, sweep)),
                  batched_vs_plain={"batched_s": round(el_b, 3), "six_plain_runs_s": round(el_p, 3),
                                    "speedup": round(el_p / el_b, 3), "plain_tiling": list(plain_tiling),
                                    "plain_kernel": plain_kernel},
