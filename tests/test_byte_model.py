@@ -2,7 +2,8 @@
 (workloads.alg_bytes): each streamed 3-D array counts 4 B once per pass
 (read and/or write), the material-table index 1 B, 1-D tables, the table
 entries, sources and receivers 0.  Counted here array by array from the
-kernels' loads and stores."""
+kernels' loads and stores.  Also the lookup of the measured traffic that
+bench.py reports beside it."""
 from elanfmt_amd import fwe, workloads as W
 
 F = 4  # bytes per FP32 element
@@ -39,3 +40,30 @@ def test_single_pass():
     # fields R9 W9 per wavefield, medium (C9 b3 [w]) once
     assert W.alg_bytes(fwe.MODE_ORTHO, fwe.PML_SEPARABLE, fused=True) == (F * (18 + 12), 0)
     assert W.alg_bytes(fwe.MODE_ISO, fwe.PML_PROFILE, fused=True) == (F * (18 + 2), 0)
+
+
+def _bench():
+    import importlib.util
+    import os
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(root, "bench.py"))
+    b = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(b)
+    return b
+
+
+def test_roofline_traffic_needs_the_exact_launch():
+    """roofline.traffic is the counter traffic of exactly the kernel, order and
+    z chunk the run used: another order or z chunk of the same kernel, or a
+    kernel never profiled, gives None (no neighbouring entry stands in)."""
+    b = _bench()
+    k = "k_stress<2, 3, false, 64, 4, 1>"
+    assert b._geom(False, "linear", (64, 4, 2), (2, 12)) == "linear kc2"
+    assert b._geom(True, "linear", (64, 4, 2), (2, 12)) == "rows2 kc12"
+    t = b._traffic(k, "linear kc2", "ortho384x384x192")
+    assert t is not None and 1.0 < t / (61 * 384 * 384 * 192) < 1.5
+    assert b._traffic(k, "xcd kc2", "ortho384x384x192") is None
+    assert b._traffic(k, "linear kc4", "ortho384x384x192") is None
+    assert b._traffic("k_stress<2, 3, false, 64, 8, 1>", "linear kc1", "ortho384x384x192") is None
+    assert b._traffic(k, "linear kc2", "ortho512x512x256") is None
