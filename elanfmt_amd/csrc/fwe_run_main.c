@@ -26,7 +26,13 @@
  * but a free-surface top instead of the weight files: no reference
  * counterpart; coefficients from fwe_gen_cpml_table with R = 1e-4, power 2,
  * no kappa or frequency shift, sized by the medium's fastest speed; its
- * memory variables join the snapshots as <s>_0_cpml.res).
+ * memory variables join the snapshots as <s>_0_cpml.res), elementary=1 (the
+ * six unit moment tensors Mxx Myy Mzz Mxy Mxz Myz advance as one batch of
+ * wavefields in this one process, instead of one source per process as in
+ * the reference (OK4.cu:336-342): the usual outputs then hold the MTS.dat
+ * tensor synthesised from the six by linearity, and 0_0_<name>_<Mc>.res the
+ * traces of each unit tensor, from which any tensor's traces are the same
+ * linear combination; takes neither snap= nor restart=).
  * Outputs: 0_infos.inf and 0_0_{Src,Rec,Rec_x,Rec_y,Rec_y2,Vx,Vy,Vz}.res.
  */
 #include <math.h>
@@ -45,7 +51,8 @@ static int usage(void) {
           "usage: fwe_run [NBX=.. NBY=.. NBZ=.. OVERX=.. OVERY=.. OVERZ=..] [nx= ny= nz=] [nt=]\n"
           "               [mode=iso|ortho|isohet] [pml=profile|weights|taper|none] [fs=0|1|2]\n"
           "               [xsrc= ysrc= zsrc=] [rec=x:i:j:k:name,...] [dir=.] [device=0] [nout=200]\n"
-          "               [snap=<steps>] [restart=<snapshot>] [tune=3] [graph=0] [cpml=<cells>]\n");
+          "               [snap=<steps>] [restart=<snapshot>] [tune=3] [graph=0] [cpml=<cells>]\n"
+          "               [elementary=0|1]\n");
   return 2;
 }
 
@@ -78,6 +85,18 @@ static size_t field_elems(const fwe_config* c, int f) {
     case 8: return nx * (ny + 1) * (nz + 1);
     default: return nx * ny * nz;
   }
+}
+
+/* raw float32 output 0_0_<name>.res (the "%d_%d_%s.res" convention of
+ * OK4.cu:159-164 with isave = 0, rank 0) */
+static int write_res(const char* dir, const char* name, const float* a, size_t n) {
+  char fn[4096];
+  snprintf(fn, sizeof(fn), "%s/0_0_%s.res", dir, name);
+  FILE* fp = fopen(fn, "wb");
+  const int ok = fp && fwrite(a, sizeof(float), n, fp) == n;
+  if (fp) fclose(fp);
+  if (!ok) fprintf(stderr, "fwe_run: cannot write %s\n", fn);
+  return !ok;
 }
 
 static char* snap_path(const char* dir, int s, const char* name, const char* ext) {
@@ -155,10 +174,52 @@ static int read_snapshot(fwe_ctx* ctx, const char* dir, int s, const fwe_config*
   return 0;
 }
 
+/* Outputs of elementary=1 (see the header): traces of the tensor m by
+ * fwe_combine_traces, velocities sum_b m_b V_b (b in order), and the traces
+ * of each unit tensor as 0_0_<name>_<Mc>.res. */
+static int write_elementary(fwe_ctx* ctx, const char* dir, const fwe_config* c, const char* const* np, int nrec,
+                            const float m[6]) {
+  static const char* const MC[6] = {"Mxx", "Myy", "Mzz", "Mxy", "Mxz", "Myz"};
+  const size_t nt = (size_t)c->nt;
+  int bad = 0;
+  float* all = (float*)malloc(sizeof(float) * 6 * (size_t)nrec * (nt ? nt : 1));
+  float* comb = (float*)malloc(sizeof(float) * (size_t)nrec * (nt ? nt : 1));
+  if (!all || !comb) { fprintf(stderr, "fwe_run: out of host memory\n"); bad = 1; }
+  if (!bad && nrec > 0 && nt > 0) {
+    fwe_status st = fwe_get_traces(ctx, all);
+    if (st == FWE_OK || st == FWE_ERR_NONFINITE) st = fwe_combine_traces(ctx, 1, m, comb);
+    if (st != FWE_OK && st != FWE_ERR_NONFINITE) { fprintf(stderr, "fwe_run: %s\n", fwe_last_error()); bad = 1; }
+    char nm[160];
+    for (int r = 0; r < nrec && !bad; ++r) {
+      bad |= write_res(dir, np[r], comb + (size_t)r * nt, nt);
+      for (int b = 0; b < 6 && !bad; ++b) {
+        snprintf(nm, sizeof(nm), "%s_%s", np[r], MC[b]);
+        bad |= write_res(dir, nm, all + ((size_t)b * nrec + r) * nt, nt);
+      }
+    }
+  }
+  free(all);
+  free(comb);
+  for (int f = 0; f < 3 && !bad; ++f) {
+    const size_t n = field_elems(c, f);
+    float* acc = (float*)calloc(n, sizeof(float));
+    float* h = (float*)malloc(sizeof(float) * n);
+    if (!acc || !h) { fprintf(stderr, "fwe_run: out of host memory\n"); bad = 1; }
+    for (int b = 0; b < 6 && !bad; ++b) {
+      if (fwe_get_field(ctx, b, f, h) != FWE_OK) { fprintf(stderr, "fwe_run: %s\n", fwe_last_error()); bad = 1; break; }
+      for (size_t q = 0; q < n; ++q) acc[q] = fmaf(m[b], h[q], acc[q]);
+    }
+    if (!bad) bad = write_res(dir, FIELD_NAMES[f], acc, n);
+    free(acc);
+    free(h);
+  }
+  return bad;
+}
+
 int main(int argc, char** argv) {
   int nbx = -1, nby = -1, nbz = -1, ovx = 0, ovy = 0, ovz = 0, nx = -1, ny = -1, nz = -1;
   int nt = -1, mode = FWE_MODE_ISO, pml = -1, fs = 1, device = 0, nout = 200, snap = 0, restart = -1, cpml = 0;
-  int tune = 3, graph = 0;
+  int tune = 3, graph = 0, elementary = 0;
   int xsrc = -1, ysrc = -1, zsrc = -1;
   int npars[3] = {-1, -1, -1};
   const char* dir = ".";
@@ -201,6 +262,7 @@ int main(int argc, char** argv) {
     else if (KEY("tune")) tune = atoi(v);
     else if (KEY("graph")) graph = atoi(v);
     else if (KEY("cpml")) cpml = atoi(v);
+    else if (KEY("elementary")) elementary = atoi(v) != 0;
     else { fprintf(stderr, "fwe_run: unknown key in '%s'\n", argv[a]); return usage(); }
 #undef KEY
   }
@@ -210,6 +272,11 @@ int main(int argc, char** argv) {
   if (nz < 0 && nbz > 0) nz = 64 * nbz - ovz;
   if (nx < 3 || ny < 3 || nz < 3) { fprintf(stderr, "fwe_run: grid size not given\n"); return usage(); }
   if (nt < 0) nt = mode == FWE_MODE_ISO ? 3000 : 4500;
+  if (elementary && (snap > 0 || restart >= 0)) {
+    fprintf(stderr, "fwe_run: snapshots hold one wavefield; elementary=1 takes neither snap= nor restart=\n");
+    return 2;
+  }
+  const int nb = elementary ? 6 : 1;
 
   fwe_legacy_params p;
   CHECK(fwe_read_legacy_params(dir, &p));
@@ -223,6 +290,7 @@ int main(int argc, char** argv) {
   cfg.dx = p.pa1[0]; cfg.dy = p.pa1[1]; cfg.dz = p.pa1[2]; cfg.dt = p.pa1[3];
   cfg.free_surface = fs;
   cfg.device = device;
+  cfg.nbatch = nb;
   if (pml >= 0) cfg.pml = pml;
   if (cpml > 0) cfg.pml = FWE_PML_CPML;
   fwe_ctx* ctx = NULL;
@@ -265,13 +333,18 @@ int main(int argc, char** argv) {
     fwe_destroy(ctx);
     return 1;
   }
-  fwe_injection inj[6];
-  CHECK(fwe_mt_injections(mode == FWE_MODE_ISO ? FWE_SRC_LAYOUT_OK4 : FWE_SRC_LAYOUT_ANISO, xsrc, ysrc,
-                          zsrc, m, 0, inj));
-  CHECK(fwe_set_injections(ctx, 6, inj));
+  /* one tensor, or the six unit tensors on wavefields 0..5 */
+  fwe_injection inj[36];
+  for (int b = 0; b < nb; ++b) {
+    float e[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    e[b] = 1.f;
+    CHECK(fwe_mt_injections(mode == FWE_MODE_ISO ? FWE_SRC_LAYOUT_OK4 : FWE_SRC_LAYOUT_ANISO, xsrc, ysrc, zsrc,
+                            elementary ? e : m, b, inj + 6 * b));
+  }
+  CHECK(fwe_set_injections(ctx, 6 * nb, inj));
 
   /* receivers: OK4.cu:383-386 or ANISO.cu:264-267 unless rec= is given */
-  static fwe_receiver rec[MAXREC];
+  static fwe_receiver rec[6 * MAXREC];
   static char names[MAXREC][64];
   const char* np[MAXREC];
   int nrec = 0;
@@ -307,11 +380,18 @@ int main(int argc, char** argv) {
     }
   }
   for (int r = 0; r < nrec; ++r) np[r] = names[r];
-  CHECK(fwe_set_receivers(ctx, nrec, rec));
+  /* the same receivers on every wavefield of the batch */
+  for (int b = 1; b < nb; ++b)
+    for (int r = 0; r < nrec; ++r) {
+      rec[b * nrec + r] = rec[r];
+      rec[b * nrec + r].batch = b;
+    }
+  CHECK(fwe_set_receivers(ctx, nb * nrec, rec));
 
   printf("fwe_run: %s law, %dx%dx%d cells, %d steps, dt=%.6g, free surface %s (%s)\n",
          mode == FWE_MODE_ISO ? "iso" : "ortho", nx, ny, nz, nt, cfg.dt,
          fs == FWE_FS_BC1 ? "BC1" : (fs == FWE_FS_VACUUM ? "vacuum layer" : "off"), fwe_version());
+  if (elementary) printf("fwe_run: the six unit moment tensors as one batch of wavefields\n");
   if (cpml > 0) printf("fwe_run: %d-cell convolutional PML instead of the weight files\n", cpml);
   /* tiling: autotuned (the context is reset afterwards, before any restart) */
   if (tune > 0) CHECK(fwe_autotune(ctx, tune, NULL));
@@ -356,10 +436,14 @@ int main(int argc, char** argv) {
   const double el = now_s() - t0;
   const double cells = (double)nx * ny * nz;
   printf("time %.3f s after %d iterations, %.2f it/s, %.3f Gcell-updates/s\n", el, nt - first,
-         (nt - first) / el, cells * (nt - first) / el * 1e-9);
+         (nt - first) / el, cells * nb * (nt - first) / el * 1e-9);
   fwe_status st = fwe_write_legacy_outputs(ctx, dir, np, nrec);
   if (st == FWE_ERR_NONFINITE) fprintf(stderr, "fwe_run: warning: %s\n", fwe_last_error());
   else if (st != FWE_OK) { fprintf(stderr, "fwe_run: %s\n", fwe_last_error()); fwe_destroy(ctx); return 1; }
+  /* elementary=1: the files above hold wavefield 0 (unit Mxx); replace the
+   * traces and velocities by the MTS.dat tensor's, the same linear
+   * combination of the six wavefields, and add the six unit tensors' traces */
+  if (elementary && write_elementary(ctx, dir, &cfg, np, nrec, m)) { fwe_destroy(ctx); return 1; }
   fwe_destroy(ctx);
   return 0;
 }

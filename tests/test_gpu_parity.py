@@ -345,6 +345,60 @@ def test_legacy_drop_in_binary(gpu, tmp_path):
         assert np.array_equal(legacy.read_res(str(tmp_path), n), v), n
 
 
+def test_legacy_binary_elementary_tensors(gpu, tmp_path):
+    """fwe_run elementary=1: the six unit tensors advance as one batch; the
+    usual outputs then hold the MTS.dat tensor synthesised from them (as a
+    direct run of that tensor, to FP32 round-off), and 0_0_<name>_<Mc>.res
+    each unit tensor's traces (against the oracle run of that tensor)."""
+    import subprocess
+
+    from elanfmt_amd import fwe, legacy, workloads as W
+    from oracle import Oracle
+
+    wl = W.iso_homogeneous(n=40, L=6, nt=70, free_surface=1)
+    m = np.array(wl.info["mt"], np.float32)
+    a, b = tmp_path / "a", tmp_path / "b"
+    for d in (a, b):
+        d.mkdir()
+        legacy.write_inputs(str(d), wl, mts=m)
+    xs = 18
+    recs = [(2, 25, 21, 1, 0), (0, 25, 21, 1, 0), (1, 25, 24, 1, 0), (1, 2, 23, 30, 0)]
+    names = ["Rec", "Rec_x", "Rec_y", "Rec_y2"]
+    spec = ",".join(f"{'xyz'[c]}:{i}:{j}:{k}:{n}" for (c, i, j, k, _), n in zip(recs, names))
+    args = [fwe.RUN_PATH, "nx=40", "ny=40", "nz=40", "nt=70", "mode=iso", "fs=1", f"xsrc={xs}", f"ysrc={xs}",
+            f"zsrc={xs}", f"rec={spec}"]
+    r = subprocess.run(args, cwd=str(a), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    print("case fwe_run elementary=1 on iso40", flush=True)
+    r = subprocess.run(args + ["elementary=1"], cwd=str(b), capture_output=True, text=True, timeout=120)
+    print(r.stdout[-400:], r.stderr[-300:], flush=True)
+    assert r.returncode == 0 and "six unit moment tensors" in r.stdout
+    direct = np.stack([legacy.read_res(str(a), n) for n in names])
+    synth = np.stack([legacy.read_res(str(b), n) for n in names])
+    assert np.abs(direct).max() > 0
+    assert rel_l2(synth, direct) < 1e-5
+    assert rel_l2(legacy.read_res(str(b), "Vx", (41, 40, 40)), legacy.read_res(str(a), "Vx", (41, 40, 40))) < 1e-5
+    assert np.array_equal(legacy.read_res(str(b), "Src"), legacy.read_res(str(a), "Src"))
+    assert legacy.read_infos(str(b)) == legacy.read_infos(str(a))
+    unit = {c: np.stack([legacy.read_res(str(b), f"{n}_{c}") for n in names])
+            for c in ("Mxx", "Myy", "Mzz", "Mxy", "Mxz", "Myz")}
+    comb = sum(np.float64(m[q]) * unit[c].astype(np.float64) for q, c in enumerate(unit))
+    assert rel_l2(comb, synth) < 1e-6
+    # one unit tensor against the oracle
+    e = np.zeros(6, np.float32)
+    e[3] = 1.0
+    wl.injections = fwe.mt_injections(fwe.SRC_LAYOUT_OK4, xs, xs, xs, e)
+    wl.receivers = recs
+    o = Oracle(40, 40, 40, wl.dx, wl.dy, wl.dz, np.float32(wl.dt), 70, mode=0, pml=1, free_surface=1)
+    W.apply(o, wl)
+    o.run(70)
+    assert np.abs(o.traces()).max() > 0
+    assert rel_l2(unit["Mxy"], o.traces()) < TOL
+    # snapshots hold one wavefield: refused
+    r = subprocess.run(args + ["elementary=1", "snap=30"], cwd=str(b), capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "elementary" in r.stderr
+
+
 def test_legacy_binary_snapshots_and_restart(gpu, tmp_path):
     """fwe_run snap=30 writes all nine fields every 30 steps; a second run
     with restart=1 in a directory holding the inputs and snapshot 1 ends
