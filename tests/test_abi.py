@@ -149,3 +149,19 @@ def test_host_arrays_are_checked_before_the_library_reads_them(fwe):
     for o, n in (((0, 0, -1), (20, 20, 10)), ((0, 0, 0), (20, 20, 11)), ((5, 0, 0), (16, 20, 10))):
         with pytest.raises(ValueError):
             Lr.crop(m, o, n)
+
+
+def test_drop_in_binary_refuses_bad_arguments_before_any_device_call(fwe, tmp_path):
+    """fwe_run checks its arguments before touching inputs or the GPU:
+    unknown keys, a missing grid size, and elementary=1 with snapshots or a
+    restart (snapshots hold one wavefield) exit with status 2."""
+    import subprocess
+
+    cases = [(["nx=8", "ny=8", "nz=8", "bogus=1"], "unknown key"),
+             (["nt=10"], "grid size not given"),
+             (["nx=8", "ny=8", "nz=8", "elementary=1", "snap=5"], "elementary"),
+             (["nx=8", "ny=8", "nz=8", "elementary=1", "restart=1"], "elementary")]
+    for args, msg in cases:
+        r = subprocess.run([fwe.RUN_PATH] + args, cwd=str(tmp_path), capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2, (args, r.stderr)
+        assert msg in r.stderr, (args, r.stderr)
