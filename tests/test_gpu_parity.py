@@ -345,11 +345,13 @@ def test_legacy_drop_in_binary(gpu, tmp_path):
         assert np.array_equal(legacy.read_res(str(tmp_path), n), v), n
 
 
-def test_legacy_binary_elementary_tensors(gpu, tmp_path):
+@pytest.mark.parametrize("cpml", [0, 6])
+def test_legacy_binary_elementary_tensors(gpu, tmp_path, cpml):
     """fwe_run elementary=1: the six unit tensors advance as one batch; the
     usual outputs then hold the MTS.dat tensor synthesised from them (as a
     direct run of that tensor, to FP32 round-off), and 0_0_<name>_<Mc>.res
-    each unit tensor's traces (against the oracle run of that tensor)."""
+    each unit tensor's traces (against the oracle run of that tensor, with
+    the weight files).  cpml=6: the same with a thin convolutional PML."""
     import subprocess
 
     from elanfmt_amd import fwe, legacy, workloads as W
@@ -366,10 +368,10 @@ def test_legacy_binary_elementary_tensors(gpu, tmp_path):
     names = ["Rec", "Rec_x", "Rec_y", "Rec_y2"]
     spec = ",".join(f"{'xyz'[c]}:{i}:{j}:{k}:{n}" for (c, i, j, k, _), n in zip(recs, names))
     args = [fwe.RUN_PATH, "nx=40", "ny=40", "nz=40", "nt=70", "mode=iso", "fs=1", f"xsrc={xs}", f"ysrc={xs}",
-            f"zsrc={xs}", f"rec={spec}"]
+            f"zsrc={xs}", f"rec={spec}"] + ([f"cpml={cpml}"] if cpml else [])
     r = subprocess.run(args, cwd=str(a), capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    print("case fwe_run elementary=1 on iso40", flush=True)
+    print(f"case fwe_run elementary=1 on iso40, cpml={cpml}", flush=True)
     r = subprocess.run(args + ["elementary=1"], cwd=str(b), capture_output=True, text=True, timeout=120)
     print(r.stdout[-400:], r.stderr[-300:], flush=True)
     assert r.returncode == 0 and "six unit moment tensors" in r.stdout
@@ -384,6 +386,8 @@ def test_legacy_binary_elementary_tensors(gpu, tmp_path):
             for c in ("Mxx", "Myy", "Mzz", "Mxy", "Mxz", "Myz")}
     comb = sum(np.float64(m[q]) * unit[c].astype(np.float64) for q, c in enumerate(unit))
     assert rel_l2(comb, synth) < 1e-6
+    if cpml:
+        return  # (the thin layer's traces against the oracle: test_legacy_binary_with_a_thin_cpml)
     # one unit tensor against the oracle
     e = np.zeros(6, np.float32)
     e[3] = 1.0
