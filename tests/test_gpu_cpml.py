@@ -229,3 +229,47 @@ def test_cpml_absorbs_on_the_gpu(gpu):
     print(f"errors: cpml6 {e_c:.2e} plain6 {e_p6:.2e} plain24 {e_p24:.2e}", flush=True)
     assert e_c < 3e-3
     assert e_p6 > 0.05 and e_p24 > 0.015
+
+
+def test_cpml_layer_set_refused_and_changed(gpu):
+    """A CPML context runs only once its layer is set; a layer that does not
+    fit the grid is refused and leaves the set layer in place; setting a
+    layer of another width reallocates the memory variables (the state size
+    follows) and the run then equals a fresh context with that layer, bit
+    for bit."""
+    import copy
+
+    from elanfmt_amd import fwe, workloads as W
+
+    def wl_of(layer):
+        return W.use_cpml(W.ortho_layered(nx=40, ny=36, nz=30, L=5, nt=40, nrec_side=2), layer)
+
+    def traces(wl):
+        s = W.make_solver(wl)
+        s.run(wl.nt)
+        t = s.traces()
+        s.close()
+        return t
+
+    wl5, wl7 = wl_of(5), wl_of(7)
+    t5, t7 = traces(wl5), traces(wl7)
+    print("case cpml layer checks 40x36x30", flush=True)
+    s = fwe.Solver(wl5.nx, wl5.ny, wl5.nz, wl5.dx, wl5.dy, wl5.dz, wl5.dt, wl5.nt, mode=wl5.mode, pml=fwe.PML_CPML)
+    bare = copy.copy(wl5)
+    bare.pml = fwe.PML_NONE  # everything but the damping
+    W.apply(s, bare)
+    with pytest.raises(fwe.FweError):
+        s.run(1)
+    s.set_cpml(5, *wl5.cpml)
+    with pytest.raises(fwe.FweError):
+        s.set_cpml(16, *wl5.cpml)  # 2 x 16 > nz
+    s.reset()
+    s.run(wl5.nt)
+    assert np.array_equal(s.traces(), t5)
+    s.set_cpml(7, *wl7.cpml)
+    n7 = 6 * 14 * (wl7.ny * wl7.nz + wl7.nx * wl7.nz + wl7.nx * wl7.ny)
+    assert s.cpml_state().size == n7
+    s.reset()
+    s.run(wl7.nt)
+    assert np.array_equal(s.traces(), t7)
+    s.close()
